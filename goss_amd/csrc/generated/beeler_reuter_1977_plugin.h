@@ -1,0 +1,66 @@
+// Generated by goss_amd.codegen from beeler_reuter_1977.ode -- do not edit.
+#ifndef BEELER_REUTER_1977_PLUGIN_H_IS_INCLUDED
+#define BEELER_REUTER_1977_PLUGIN_H_IS_INCLUDED
+#include <memory>
+#include <stdexcept>
+#include <cmath>
+#include "goss/ParameterizedODE.h"
+#include "beeler_reuter_1977.h"
+
+namespace goss {
+
+class Beeler_reuter_1977 : public ParameterizedODE
+{
+  struct Acc { const double* v; template <int k> double get() const { return v[k]; } };
+public:
+  Beeler_reuter_1977() : ParameterizedODE(8, 10, 1, 0, 0)
+  {
+    for (int i = 0; i < 8; ++i) _state_names[i] = goss_models::beeler_reuter_1977::state_names[i];
+    for (int i = 0; i < 10; ++i) {
+      _parameter_names[i] = goss_models::beeler_reuter_1977::parameter_names[i];
+      _p[i] = goss_models::beeler_reuter_1977::default_parameters[i];
+    }
+    _field_state_names[0] = "V";
+    _field_state_indices[0] = 7;
+    _bind();
+    for (int i = 0; i < 8; ++i) _linear_terms[i] = goss_models::beeler_reuter_1977::linear_terms[i];
+  }
+  Beeler_reuter_1977(const Beeler_reuter_1977& ode) : ParameterizedODE(ode)
+  {
+    for (int i = 0; i < 10; ++i) _p[i] = ode._p[i];
+    _bind();
+  }
+  void eval(const double* states, double time, double* values)
+  { goss_models::beeler_reuter_1977::eval<double>(states, time, Acc{_p}, values); }
+  void linearized_eval(const double* states, double time, double* linear, double* rhs,
+                       bool only_linear) const
+  { goss_models::beeler_reuter_1977::linearized_eval<double>(states, time, Acc{_p}, linear, rhs, only_linear); }
+  void get_ic(goss::DoubleVector* values) const
+  {
+    values->n = _num_states;
+    values->data.reset(new double[_num_states]);
+    for (int i = 0; i < 8; ++i) values->data[i] = goss_models::beeler_reuter_1977::initial_conditions[i];
+  }
+  std::shared_ptr<ODE> copy() const
+  { return std::make_shared<Beeler_reuter_1977>(*this); }
+  void eval_monitored(const double*, double, double*) const
+  { throw std::runtime_error("No monitored in the 'Beeler_reuter_1977' model."); }
+  void set_field_parameters(const double* field_params)
+  {
+    (void)field_params;
+  }
+private:
+  double _p[10];
+  void _bind()
+  {
+    for (int i = 0; i < 10; ++i) _param_to_value[_parameter_names[i]] = &_p[i];
+  }
+};
+
+}  // namespace goss
+
+extern "C" __attribute__((visibility("default"))) goss::ParameterizedODE* create_Beeler_reuter_1977()
+{
+  return new goss::Beeler_reuter_1977;
+}
+#endif

@@ -1,0 +1,66 @@
+// Generated by goss_amd.codegen from lorentz.ode -- do not edit.
+#ifndef LORENTZ_PLUGIN_H_IS_INCLUDED
+#define LORENTZ_PLUGIN_H_IS_INCLUDED
+#include <memory>
+#include <stdexcept>
+#include <cmath>
+#include "goss/ParameterizedODE.h"
+#include "lorentz.h"
+
+namespace goss {
+
+class Lorentz : public ParameterizedODE
+{
+  struct Acc { const double* v; template <int k> double get() const { return v[k]; } };
+public:
+  Lorentz() : ParameterizedODE(3, 3, 1, 0, 0)
+  {
+    for (int i = 0; i < 3; ++i) _state_names[i] = goss_models::lorentz::state_names[i];
+    for (int i = 0; i < 3; ++i) {
+      _parameter_names[i] = goss_models::lorentz::parameter_names[i];
+      _p[i] = goss_models::lorentz::default_parameters[i];
+    }
+    _field_state_names[0] = "x";
+    _field_state_indices[0] = 0;
+    _bind();
+    for (int i = 0; i < 3; ++i) _linear_terms[i] = goss_models::lorentz::linear_terms[i];
+  }
+  Lorentz(const Lorentz& ode) : ParameterizedODE(ode)
+  {
+    for (int i = 0; i < 3; ++i) _p[i] = ode._p[i];
+    _bind();
+  }
+  void eval(const double* states, double time, double* values)
+  { goss_models::lorentz::eval<double>(states, time, Acc{_p}, values); }
+  void linearized_eval(const double* states, double time, double* linear, double* rhs,
+                       bool only_linear) const
+  { goss_models::lorentz::linearized_eval<double>(states, time, Acc{_p}, linear, rhs, only_linear); }
+  void get_ic(goss::DoubleVector* values) const
+  {
+    values->n = _num_states;
+    values->data.reset(new double[_num_states]);
+    for (int i = 0; i < 3; ++i) values->data[i] = goss_models::lorentz::initial_conditions[i];
+  }
+  std::shared_ptr<ODE> copy() const
+  { return std::make_shared<Lorentz>(*this); }
+  void eval_monitored(const double*, double, double*) const
+  { throw std::runtime_error("No monitored in the 'Lorentz' model."); }
+  void set_field_parameters(const double* field_params)
+  {
+    (void)field_params;
+  }
+private:
+  double _p[3];
+  void _bind()
+  {
+    for (int i = 0; i < 3; ++i) _param_to_value[_parameter_names[i]] = &_p[i];
+  }
+};
+
+}  // namespace goss
+
+extern "C" __attribute__((visibility("default"))) goss::ParameterizedODE* create_Lorentz()
+{
+  return new goss::Lorentz;
+}
+#endif

@@ -1,0 +1,71 @@
+// Generated by goss_amd.codegen from oscilator.ode -- do not edit.
+#ifndef OSCILATOR_PLUGIN_H_IS_INCLUDED
+#define OSCILATOR_PLUGIN_H_IS_INCLUDED
+#include <memory>
+#include <stdexcept>
+#include <cmath>
+#include "goss/ParameterizedODE.h"
+#include "oscilator.h"
+
+namespace goss {
+
+class Oscilator : public ParameterizedODE
+{
+  struct Acc { const double* v; template <int k> double get() const { return v[k]; } };
+public:
+  Oscilator() : ParameterizedODE(2, 2, 2, 2, 0)
+  {
+    for (int i = 0; i < 2; ++i) _state_names[i] = goss_models::oscilator::state_names[i];
+    for (int i = 0; i < 2; ++i) {
+      _parameter_names[i] = goss_models::oscilator::parameter_names[i];
+      _p[i] = goss_models::oscilator::default_parameters[i];
+    }
+    _field_state_names[0] = "x";
+    _field_state_indices[0] = 0;
+    _field_state_names[1] = "y";
+    _field_state_indices[1] = 1;
+    _field_parameter_names[0] = "a";
+    _field_parameter_names[1] = "b";
+    _bind();
+    for (int i = 0; i < 2; ++i) _linear_terms[i] = goss_models::oscilator::linear_terms[i];
+  }
+  Oscilator(const Oscilator& ode) : ParameterizedODE(ode)
+  {
+    for (int i = 0; i < 2; ++i) _p[i] = ode._p[i];
+    _bind();
+  }
+  void eval(const double* states, double time, double* values)
+  { goss_models::oscilator::eval<double>(states, time, Acc{_p}, values); }
+  void linearized_eval(const double* states, double time, double* linear, double* rhs,
+                       bool only_linear) const
+  { goss_models::oscilator::linearized_eval<double>(states, time, Acc{_p}, linear, rhs, only_linear); }
+  void get_ic(goss::DoubleVector* values) const
+  {
+    values->n = _num_states;
+    values->data.reset(new double[_num_states]);
+    for (int i = 0; i < 2; ++i) values->data[i] = goss_models::oscilator::initial_conditions[i];
+  }
+  std::shared_ptr<ODE> copy() const
+  { return std::make_shared<Oscilator>(*this); }
+  void eval_monitored(const double*, double, double*) const
+  { throw std::runtime_error("No monitored in the 'Oscilator' model."); }
+  void set_field_parameters(const double* field_params)
+  {
+    _p[0] = field_params[0];
+    _p[1] = field_params[1];
+  }
+private:
+  double _p[2];
+  void _bind()
+  {
+    for (int i = 0; i < 2; ++i) _param_to_value[_parameter_names[i]] = &_p[i];
+  }
+};
+
+}  // namespace goss
+
+extern "C" __attribute__((visibility("default"))) goss::ParameterizedODE* create_Oscilator()
+{
+  return new goss::Oscilator;
+}
+#endif

@@ -1,0 +1,77 @@
+// Generated by goss_amd.codegen from tentusscher_panfilov_2006_M_cell.ode -- do not edit.
+#ifndef TP06_M_FP6_PLUGIN_H_IS_INCLUDED
+#define TP06_M_FP6_PLUGIN_H_IS_INCLUDED
+#include <memory>
+#include <stdexcept>
+#include <cmath>
+#include "goss/ParameterizedODE.h"
+#include "tp06_m_fp6.h"
+
+namespace goss {
+
+class Tp06_m_fp6 : public ParameterizedODE
+{
+  struct Acc { const double* v; template <int k> double get() const { return v[k]; } };
+public:
+  Tp06_m_fp6() : ParameterizedODE(19, 53, 1, 6, 0)
+  {
+    for (int i = 0; i < 19; ++i) _state_names[i] = goss_models::tp06_m_fp6::state_names[i];
+    for (int i = 0; i < 53; ++i) {
+      _parameter_names[i] = goss_models::tp06_m_fp6::parameter_names[i];
+      _p[i] = goss_models::tp06_m_fp6::default_parameters[i];
+    }
+    _field_state_names[0] = "V";
+    _field_state_indices[0] = 17;
+    _field_parameter_names[0] = "g_Kr";
+    _field_parameter_names[1] = "g_Ks";
+    _field_parameter_names[2] = "g_CaL";
+    _field_parameter_names[3] = "g_Na";
+    _field_parameter_names[4] = "g_K1";
+    _field_parameter_names[5] = "g_to";
+    _bind();
+    for (int i = 0; i < 19; ++i) _linear_terms[i] = goss_models::tp06_m_fp6::linear_terms[i];
+  }
+  Tp06_m_fp6(const Tp06_m_fp6& ode) : ParameterizedODE(ode)
+  {
+    for (int i = 0; i < 53; ++i) _p[i] = ode._p[i];
+    _bind();
+  }
+  void eval(const double* states, double time, double* values)
+  { goss_models::tp06_m_fp6::eval<double>(states, time, Acc{_p}, values); }
+  void linearized_eval(const double* states, double time, double* linear, double* rhs,
+                       bool only_linear) const
+  { goss_models::tp06_m_fp6::linearized_eval<double>(states, time, Acc{_p}, linear, rhs, only_linear); }
+  void get_ic(goss::DoubleVector* values) const
+  {
+    values->n = _num_states;
+    values->data.reset(new double[_num_states]);
+    for (int i = 0; i < 19; ++i) values->data[i] = goss_models::tp06_m_fp6::initial_conditions[i];
+  }
+  std::shared_ptr<ODE> copy() const
+  { return std::make_shared<Tp06_m_fp6>(*this); }
+  void eval_monitored(const double*, double, double*) const
+  { throw std::runtime_error("No monitored in the 'Tp06_m_fp6' model."); }
+  void set_field_parameters(const double* field_params)
+  {
+    _p[2] = field_params[0];
+    _p[3] = field_params[1];
+    _p[6] = field_params[2];
+    _p[4] = field_params[3];
+    _p[1] = field_params[4];
+    _p[8] = field_params[5];
+  }
+private:
+  double _p[53];
+  void _bind()
+  {
+    for (int i = 0; i < 53; ++i) _param_to_value[_parameter_names[i]] = &_p[i];
+  }
+};
+
+}  // namespace goss
+
+extern "C" __attribute__((visibility("default"))) goss::ParameterizedODE* create_Tp06_m_fp6()
+{
+  return new goss::Tp06_m_fp6;
+}
+#endif

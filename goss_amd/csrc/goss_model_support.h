@@ -1,0 +1,70 @@
+// Math helpers behind the generated model code (goss_amd/csrc/generated/*.h).
+//
+// The generated expressions call m_exp, m_log, m_sq, ... so that one text
+// serves the CPU model class (g++, double, glibc) and the HIP kernels
+// (hipcc, float or double, ROCm device libraries).
+//
+// Device float: exp/log use the hardware exp2/log2 (v_exp_f32, v_log_f32)
+// with a scale, unless GOSS_F32_ACCURATE_MATH is defined, in which case the
+// correctly rounded-ish ocml routines are used.  Device double always uses
+// ocml (exp/log are software sequences on CDNA).
+#pragma once
+
+#include <cmath>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GOSS_HD __host__ __device__
+#define GOSS_INLINE __attribute__((always_inline)) inline
+#else
+#define GOSS_HD
+#define GOSS_INLINE inline
+#endif
+
+template <typename T> GOSS_HD GOSS_INLINE T m_sq(const T x) { return x * x; }
+template <typename T> GOSS_HD GOSS_INLINE T m_cube(const T x) { return x * x * x; }
+template <typename T> GOSS_HD GOSS_INLINE T m_sign(const T x) { return T((x > T(0)) - (x < T(0))); }
+template <typename T> GOSS_HD GOSS_INLINE T m_heaviside(const T x) { return x > T(0) ? T(1) : (x < T(0) ? T(0) : T(0.5)); }
+
+GOSS_HD GOSS_INLINE double m_exp(const double x) { return std::exp(x); }
+GOSS_HD GOSS_INLINE double m_log(const double x) { return std::log(x); }
+GOSS_HD GOSS_INLINE double m_sqrt(const double x) { return std::sqrt(x); }
+GOSS_HD GOSS_INLINE double m_fabs(const double x) { return std::fabs(x); }
+GOSS_HD GOSS_INLINE double m_floor(const double x) { return std::floor(x); }
+GOSS_HD GOSS_INLINE double m_ceil(const double x) { return std::ceil(x); }
+GOSS_HD GOSS_INLINE double m_pow(const double x, const double y) { return std::pow(x, y); }
+GOSS_HD GOSS_INLINE double m_fmax(const double x, const double y) { return std::fmax(x, y); }
+GOSS_HD GOSS_INLINE double m_fmin(const double x, const double y) { return std::fmin(x, y); }
+GOSS_HD GOSS_INLINE double m_sin(const double x) { return std::sin(x); }
+GOSS_HD GOSS_INLINE double m_cos(const double x) { return std::cos(x); }
+GOSS_HD GOSS_INLINE double m_tan(const double x) { return std::tan(x); }
+GOSS_HD GOSS_INLINE double m_asin(const double x) { return std::asin(x); }
+GOSS_HD GOSS_INLINE double m_acos(const double x) { return std::acos(x); }
+GOSS_HD GOSS_INLINE double m_atan(const double x) { return std::atan(x); }
+GOSS_HD GOSS_INLINE double m_sinh(const double x) { return std::sinh(x); }
+GOSS_HD GOSS_INLINE double m_cosh(const double x) { return std::cosh(x); }
+GOSS_HD GOSS_INLINE double m_tanh(const double x) { return std::tanh(x); }
+
+#if defined(__HIPCC__) && !defined(GOSS_F32_ACCURATE_MATH)
+GOSS_HD GOSS_INLINE float m_exp(const float x) { return __expf(x); }
+GOSS_HD GOSS_INLINE float m_log(const float x) { return __logf(x); }
+#else
+GOSS_HD GOSS_INLINE float m_exp(const float x) { return std::exp(x); }
+GOSS_HD GOSS_INLINE float m_log(const float x) { return std::log(x); }
+#endif
+GOSS_HD GOSS_INLINE float m_sqrt(const float x) { return std::sqrt(x); }
+GOSS_HD GOSS_INLINE float m_fabs(const float x) { return std::fabs(x); }
+GOSS_HD GOSS_INLINE float m_floor(const float x) { return std::floor(x); }
+GOSS_HD GOSS_INLINE float m_ceil(const float x) { return std::ceil(x); }
+GOSS_HD GOSS_INLINE float m_pow(const float x, const float y) { return std::pow(x, y); }
+GOSS_HD GOSS_INLINE float m_fmax(const float x, const float y) { return std::fmax(x, y); }
+GOSS_HD GOSS_INLINE float m_fmin(const float x, const float y) { return std::fmin(x, y); }
+GOSS_HD GOSS_INLINE float m_sin(const float x) { return std::sin(x); }
+GOSS_HD GOSS_INLINE float m_cos(const float x) { return std::cos(x); }
+GOSS_HD GOSS_INLINE float m_tan(const float x) { return std::tan(x); }
+GOSS_HD GOSS_INLINE float m_asin(const float x) { return std::asin(x); }
+GOSS_HD GOSS_INLINE float m_acos(const float x) { return std::acos(x); }
+GOSS_HD GOSS_INLINE float m_atan(const float x) { return std::atan(x); }
+GOSS_HD GOSS_INLINE float m_sinh(const float x) { return std::sinh(x); }
+GOSS_HD GOSS_INLINE float m_cosh(const float x) { return std::cosh(x); }
+GOSS_HD GOSS_INLINE float m_tanh(const float x) { return std::tanh(x); }

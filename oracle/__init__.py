@@ -1,0 +1,349 @@
+"""ctypes bindings of the CPU checkers.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of bench.py, never by the product package.
+
+* :class:`OracleSystem` -- our restatement of goss's ODESystemSolver and
+  solvers (``goss_oracle.cpp``), built into ``oracle/liboracle.so``;
+* :class:`RefSystem` -- the reference's own C++ (``cpp/goss/*.cpp``) built into
+  ``oracle/_ref/libgoss_ref.so`` when the reference tree was available at build
+  time.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libgoss_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_up = ctypes.POINTER(ctypes.c_uint)
+
+
+def build(ref: Optional[str] = None, jobs: int = 8) -> None:
+    """Build liboracle.so, and _ref/libgoss_ref.so when the reference exists."""
+    ref = ref or os.environ.get("GOSS_REFERENCE", "/root/reference")
+    targets = ["oracle"]
+    if os.path.isdir(os.path.join(ref, "cpp", "goss")):
+        targets.append("ref")
+    subprocess.check_call(["make", "-s", f"-j{jobs}", "-C", HERE, f"REF={ref}"] + targets)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} is not built (run oracle.build())")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_variant_name.restype = ctypes.c_char_p
+        L.oracle_system_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_void_p)]
+        L.oracle_system_forward.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.oracle_system_forward_times.argtypes = [ctypes.c_void_p, _dp, ctypes.c_ulong]
+        L.oracle_system_solve.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_ulong, ctypes.c_int]
+        L.oracle_system_set_parameter.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
+        for fn in ("get_states", "set_states"):
+            getattr(L, f"oracle_system_{fn}").argtypes = [ctypes.c_void_p, _dp]
+        for fn in ("get_field_states", "set_field_states", "get_field_parameters", "set_field_parameters"):
+            getattr(L, f"oracle_system_{fn}").argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        for fn in ("get_field_state_components", "set_field_state_components"):
+            getattr(L, f"oracle_system_{fn}").argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint, _up, ctypes.c_int]
+        L.oracle_system_set_num_threads.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        L.oracle_system_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_system_reset_default.argtypes = [ctypes.c_void_p]
+        L.oracle_system_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        L.oracle_model_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
+        L.oracle_model_linearized_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int]
+        L.oracle_model_analytic_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
+        L.oracle_model_fd_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
+        L.oracle_lu_factorize.argtypes = [ctypes.c_int, _dp]
+        L.oracle_forward_backward_subst.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"{REF_SO} is not built")
+        L = ctypes.CDLL(REF_SO)
+        L.ref_last_error.restype = ctypes.c_char_p
+        L.ref_system_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        L.ref_system_forward.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.ref_system_forward_times.argtypes = [ctypes.c_void_p, _dp, ctypes.c_ulong]
+        L.ref_system_solve.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_ulong, ctypes.c_int]
+        for fn in ("get_states", "set_states"):
+            getattr(L, f"ref_system_{fn}").argtypes = [ctypes.c_void_p, _dp]
+        for fn in ("get_field_states", "set_field_states", "get_field_parameters", "set_field_parameters"):
+            getattr(L, f"ref_system_{fn}").argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        for fn in ("get_field_state_components", "set_field_state_components"):
+            getattr(L, f"ref_system_{fn}").argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint, _up, ctypes.c_int]
+        L.ref_system_set_num_threads.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        L.ref_system_destroy.argtypes = [ctypes.c_void_p]
+        L.ref_system_reset_default.argtypes = [ctypes.c_void_p]
+        L.ref_model_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.ref_model_eval.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
+        L.ref_model_linearized_eval.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp, ctypes.c_int]
+        L.ref_model_compute_jacobian.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
+        L.ref_model_lu_factorize.argtypes = [ctypes.c_void_p, _dp]
+        L.ref_model_forward_backward_subst.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ref_model_set_parameter.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
+        L.ref_model_set_field_parameters.argtypes = [ctypes.c_void_p, _dp]
+        L.ref_model_linear_term.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        L.ref_model_destroy.argtypes = [ctypes.c_void_p]
+        _ref = L
+    return _ref
+
+
+class Variant:
+    """Sizes, defaults and field selection of a generated model variant."""
+
+    def __init__(self, name: str):
+        L = lib()
+        self.index = L.oracle_variant_find(name.encode())
+        if self.index < 0:
+            raise KeyError(name)
+        self.name = name
+        sz = (ctypes.c_int * 4)()
+        L.oracle_variant_sizes(self.index, sz)
+        self.num_states, self.num_parameters, self.num_field_states, self.num_field_parameters = list(sz)
+        self.default_parameters = np.zeros(self.num_parameters)
+        self.initial_conditions = np.zeros(self.num_states)
+        L.oracle_variant_defaults(self.index, _ptr(self.default_parameters), _ptr(self.initial_conditions))
+        fsi = (ctypes.c_int * max(self.num_field_states, 1))()
+        fpi = (ctypes.c_int * max(self.num_field_parameters, 1))()
+        L.oracle_variant_field_indices(self.index, fsi, fpi)
+        self.field_state_indices = list(fsi)[: self.num_field_states]
+        self.field_parameter_indices = list(fpi)[: self.num_field_parameters]
+        lt = (ctypes.c_ubyte * self.num_states)()
+        L.oracle_variant_linear_terms(self.index, lt)
+        self.linear_terms = list(lt)
+
+    def eval(self, y, t, params=None):
+        p = _f64(self.default_parameters if params is None else params)
+        y = _f64(y)
+        out = np.zeros(self.num_states)
+        lib().oracle_model_eval(self.index, _ptr(y), float(t), _ptr(p), _ptr(out))
+        return out
+
+    def linearized_eval(self, y, t, params=None, only_linear=False):
+        p = _f64(self.default_parameters if params is None else params)
+        y = _f64(y)
+        lin, rhs = np.zeros(self.num_states), np.zeros(self.num_states)
+        lib().oracle_model_linearized_eval(self.index, _ptr(y), float(t), _ptr(p), _ptr(lin), _ptr(rhs),
+                                           int(only_linear))
+        return lin, rhs
+
+    def analytic_jacobian(self, y, t, params=None):
+        p = _f64(self.default_parameters if params is None else params)
+        y = _f64(y)
+        jac = np.zeros(self.num_states * self.num_states)
+        lib().oracle_model_analytic_jacobian(self.index, _ptr(y), float(t), _ptr(p), _ptr(jac))
+        return jac.reshape(self.num_states, self.num_states)
+
+    def fd_jacobian(self, y, t, params=None):
+        p = _f64(self.default_parameters if params is None else params)
+        y = _f64(y).copy()
+        jac = np.zeros(self.num_states * self.num_states)
+        lib().oracle_model_fd_jacobian(self.index, _ptr(y), float(t), _ptr(p), _ptr(jac))
+        return jac.reshape(self.num_states, self.num_states)
+
+
+def _names(d: Dict[str, float]):
+    keys = list(d)
+    arr = (ctypes.c_char_p * max(len(keys), 1))(*[k.encode() for k in keys])
+    vals = np.array([float(d[k]) for k in keys] or [0.0])
+    return len(keys), arr, vals
+
+
+class _SystemBase:
+    _prefix = ""
+
+    def __init__(self, variant: str, solver: str, num_nodes: int):
+        self.variant = Variant(variant)
+        self.num_nodes = int(num_nodes)
+        self.solver = solver
+
+    @property
+    def S(self):
+        return self.variant.num_states
+
+    def states(self) -> np.ndarray:
+        out = np.zeros((self.num_nodes, self.S))
+        self._call("get_states", _ptr(out))
+        return out
+
+    def set_states(self, states) -> None:
+        a = _f64(states).reshape(self.num_nodes, self.S)
+        self._call("set_states", _ptr(a))
+
+    def field_parameters(self, tangled=True) -> np.ndarray:
+        F = self.variant.num_field_parameters
+        out = np.zeros((self.num_nodes, F) if tangled else (F, self.num_nodes))
+        self._call("get_field_parameters", _ptr(out), int(tangled))
+        return out
+
+    def set_field_parameters(self, fp, tangled=True) -> None:
+        a = _f64(fp)
+        self._call("set_field_parameters", _ptr(a), int(tangled))
+
+    def field_states(self, tangled=True) -> np.ndarray:
+        F = self.variant.num_field_states
+        out = np.zeros((self.num_nodes, F) if tangled else (F, self.num_nodes))
+        self._call("get_field_states", _ptr(out), int(tangled))
+        return out
+
+    def set_field_states(self, fs, tangled=True) -> None:
+        a = _f64(fs)
+        self._call("set_field_states", _ptr(a), int(tangled))
+
+    def set_num_threads(self, n: int) -> None:
+        self._call("set_num_threads", int(n))
+
+    def forward_times(self, t) -> None:
+        t = _f64(t)
+        rc = self._call("forward_times", _ptr(t), len(t))
+        self._check(rc)
+
+    def forward(self, t, interval) -> None:
+        self._check(self._call("forward", float(t), float(interval)))
+
+    def solve(self, t, tangled=True) -> np.ndarray:
+        t = _f64(t)
+        F = self.variant.num_field_states
+        out = np.zeros((len(t), self.num_nodes, F))
+        out[0] = self.field_states(True)
+        if not tangled:
+            out = np.zeros((len(t), F, self.num_nodes))
+            out[0] = self.field_states(False)
+        self._check(self._call("solve", _ptr(out), _ptr(t), len(t), int(tangled)))
+        return out
+
+    def reset_default(self):
+        self._call("reset_default")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._call("destroy")
+            self._h = None
+
+
+class OracleSystem(_SystemBase):
+    """Restated ODESystemSolver (oracle/goss_oracle.cpp)."""
+
+    def __init__(self, variant, solver, num_nodes, solver_params=None, per_node_state=False):
+        super().__init__(variant, solver, num_nodes)
+        self._L = lib()
+        n, names, vals = _names(solver_params or {})
+        self._h = ctypes.c_void_p()
+        rc = self._L.oracle_system_create(variant.encode(), solver.encode(), self.num_nodes, n, names, _ptr(vals),
+                                          int(per_node_state), ctypes.byref(self._h))
+        if rc != 0:
+            raise RuntimeError(self._L.oracle_last_error().decode())
+
+    def _call(self, fn, *args):
+        return getattr(self._L, f"oracle_system_{fn}")(self._h, *args)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self._L.oracle_last_error().decode())
+
+    def set_parameter(self, name: str, value: float) -> None:
+        self._check(self._call("set_parameter", name.encode(), float(value)))
+
+    def counters(self) -> np.ndarray:
+        out = (ctypes.c_long * 4)()
+        self._call("counters", out)
+        return np.array(list(out))
+
+
+class RefSystem(_SystemBase):
+    """The reference's ODESystemSolver compiled from its own sources."""
+
+    def __init__(self, variant, solver, num_nodes, solver_params=None, model_params=None):
+        super().__init__(variant, solver, num_nodes)
+        self._L = ref_lib()
+        n, names, vals = _names(solver_params or {})
+        m, mnames, mvals = _names(model_params or {})
+        self._h = ctypes.c_void_p()
+        rc = self._L.ref_system_create(variant.encode(), solver.encode(), self.num_nodes, n, names, _ptr(vals),
+                                       m, mnames, _ptr(mvals), ctypes.byref(self._h))
+        if rc != 0:
+            raise RuntimeError(self._L.ref_last_error().decode())
+
+    def _call(self, fn, *args):
+        return getattr(self._L, f"ref_system_{fn}")(self._h, *args)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self._L.ref_last_error().decode())
+
+
+class RefModel:
+    """A reference-side goss::ParameterizedODE instance (generated plugin)."""
+
+    def __init__(self, variant: str):
+        self._L = ref_lib()
+        self._h = ctypes.c_void_p()
+        if self._L.ref_model_create(variant.encode(), ctypes.byref(self._h)) != 0:
+            raise RuntimeError(self._L.ref_last_error().decode())
+        self.variant = Variant(variant)
+
+    def eval(self, y, t):
+        y = _f64(y)
+        out = np.zeros(len(y))
+        self._L.ref_model_eval(self._h, _ptr(y), float(t), _ptr(out))
+        return out
+
+    def linearized_eval(self, y, t, only_linear=False):
+        y = _f64(y)
+        lin, rhs = np.zeros(len(y)), np.zeros(len(y))
+        self._L.ref_model_linearized_eval(self._h, _ptr(y), float(t), _ptr(lin), _ptr(rhs), int(only_linear))
+        return lin, rhs
+
+    def compute_jacobian(self, y, t):
+        y = _f64(y).copy()
+        jac = np.zeros(len(y) * len(y))
+        self._L.ref_model_compute_jacobian(self._h, _ptr(y), float(t), _ptr(jac))
+        return jac.reshape(len(y), len(y))
+
+    def set_parameter(self, name, value):
+        self._L.ref_model_set_parameter(self._h, name.encode(), float(value))
+
+    def linear_term(self, i):
+        return self._L.ref_model_linear_term(self._h, i)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.ref_model_destroy(self._h)
+            self._h = None
