@@ -1,0 +1,285 @@
+#!/usr/bin/env python3
+"""Benchmark of the ODESystemSolver::forward hot path on one MI355X.
+
+Workload (the headline configuration): the ten Tusscher-Panfilov 2006 M-cell
+(19 states) advanced with GRL1 over 1,048,576 nodes at dt = 0.02 ms; per-node
+g_Kr, g_Ks, g_CaL = default * exp(N(0, 0.2^2)) and initial states
+default * (1 + U(-0.01, 0.01)), both seeded; default stimulus (-52 pA/pF,
+1 ms, every 1000 ms).  A step is one ODESystemSolver::forward(t, dt) over all
+nodes.  The default run times 50,000 steps (1000 ms) after 500 warm-up steps.
+
+Reported: FP32C node-steps/s (FP32 arithmetic with compensated V, Ca_i,
+Ca_SR, Ca_ss, Na_i, K_i) as `value`, the plain FP64 kernel beside it, the
+accuracy of FP32C against FP64 (max |dV| over all nodes at the end, and over
+time on a node subset), FP64 parity against the CPU oracle on a node subset,
+the roofline of the dominant kernel, and the reference CPU path timed on
+this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("node-steps/s TP06 GRL1 1M nodes, FP32c vs FP64 @max|ΔV|≤0.05mV; adjoint/fwd")
+VARIANT = "tp06_m_fp3"
+FP_DEFAULTS = np.array([0.153, 0.098, 3.98e-5])
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (the system's own stream)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return ms.value
+
+    def device_synchronize(self):
+        assert self.hip.hipDeviceSynchronize() == 0
+
+
+def inputs(N, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    from goss_amd.system import ModelVariant
+
+    ic = ModelVariant(VARIANT).initial_conditions
+    y0 = ic * (1.0 + rng.uniform(-0.01, 0.01, (N, ic.size)))
+    fp = FP_DEFAULTS * np.exp(rng.normal(0.0, 0.2, (N, 3)))
+    return y0, fp
+
+
+def flops_per_node_step():
+    """State-dependent operations of one GRL1 node-step: the generated
+    linearized_eval after CSE (each add, mul, div, exp, log, sqrt, pow is one
+    operation) plus the GRL1 update of each state (a*h, exp, -1, b/a, *, +)."""
+    with open(os.path.join(ROOT, "goss_amd", "csrc", "generated", "variants.json")) as f:
+        meta = json.load(f)[VARIANT]
+    c = meta["census"]["linearized_eval_state"]
+    ops = sum(c[k] for k in ("add", "mul", "div", "exp", "log", "sqrt", "pow"))
+    return ops + 6 * meta["num_states"], ops
+
+
+def timed_run(prec, N, y0, fp, t_warm, t_timed, spl, ev):
+    from goss_amd.system import DeviceSystem
+
+    dev = DeviceSystem(VARIANT, "GRL1", N, prec)
+    dev.set_steps_per_launch(spl)
+    dev.set_states(y0)
+    dev.set_field_parameters(fp)
+    if len(t_warm) > 1:
+        dev.forward_times(t_warm)
+    dev.synchronize()
+    ev.device_synchronize()
+    stream = dev.stream()
+    e0, e1 = ev.create(), ev.create()
+    w0 = time.perf_counter()
+    ev.record(e0, stream)
+    dev.forward_times(t_timed)
+    ev.record(e1, stream)
+    dev.synchronize()
+    ev.device_synchronize()
+    wall = time.perf_counter() - w0
+    kernel_ms = ev.elapsed_ms(e0, e1)
+    return dev, wall, kernel_ms
+
+
+def subset_trace(prec, y0, fp, t_end, interval=1.0, dt=0.02):
+    """V every `interval` ms on a node subset (sub-stepped at dt)."""
+    from goss_amd.system import DeviceSystem
+
+    n = y0.shape[0]
+    dev = DeviceSystem(VARIANT, "GRL1", n, prec, {"internal_time_step": dt})
+    dev.set_states(y0)
+    dev.set_field_parameters(fp)
+    t = np.arange(0.0, t_end + 1e-9, interval)
+    out = np.zeros((len(t), n, 1))
+    dev.solve(t, out)
+    return out[1:, :, 0]
+
+
+def cpu_baseline(N, y0, fp, dt, budget_s=12.0):
+    """The reference ODESystemSolver (cpp/goss compiled from its sources,
+    OpenMP over nodes) on this host's cores, on a window of the workload."""
+    import oracle
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    if oracle.ref_available():
+        sysc = oracle.RefSystem(VARIANT, "GRL1", N)
+        kind = "reference"
+    else:
+        sysc = oracle.OracleSystem(VARIANT, "GRL1", N)
+        kind = "port"
+    sysc.set_num_threads(threads)
+    sysc.set_states(y0)
+    sysc.set_field_parameters(fp)
+    # calibrate on 2 steps, then time a window of about budget_s seconds
+    w = time.perf_counter()
+    sysc.forward_times(np.arange(3) * dt)
+    per_step = (time.perf_counter() - w) / 2
+    k = int(max(2, min(1000, budget_s / max(per_step, 1e-6))))
+    t = (np.arange(k + 1) + 2) * dt
+    w = time.perf_counter()
+    sysc.forward_times(t)
+    el = time.perf_counter() - w
+    return {"value": N * k / el, "unit": "node-steps/s", "cores": threads, "kind": kind,
+            "sample": f"all {N:,} nodes x {k} GRL1 steps (FP64, dt {dt} ms) of the same workload, "
+                      f"{el:.1f} s, OpenMP {threads} threads"}
+
+
+def oracle_parity(y0, fp, t_all, gpu_final, nodes=64):
+    import oracle
+
+    o = oracle.OracleSystem(VARIANT, "GRL1", nodes)
+    o.set_num_threads(min(8, os.cpu_count() or 1))
+    o.set_states(y0[:nodes])
+    o.set_field_parameters(fp[:nodes])
+    o.forward_times(t_all)
+    b = o.states()
+    a = gpu_final[:nodes]
+    floor = 1e-6 * np.max(np.abs(b), axis=0, keepdims=True)
+    return float((np.abs(a - b) / np.maximum(np.abs(b), floor)).max())
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "grl1_tp06_fp32c_1M_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--nodes", type=int, default=1 << 20)
+    ap.add_argument("--dt", type=float, default=0.02)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--steps-per-launch", type=int, default=250)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-checks", action="store_true", help="skip accuracy/parity passes")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the ODE system path shards by node ranges with no "
+              "communication, and this build measures the single-GPU share (--gpus 1).", file=sys.stderr)
+        return 2
+
+    N, K, W, dt = args.nodes, args.steps, args.warmup, args.dt
+    t_all = np.arange(W + K + 1) * dt
+    t_warm, t_timed = t_all[: W + 1], t_all[W:]
+    y0, fp = inputs(N, args.seed)
+    ev = HipEvents()
+    print(f"[bench] {N} nodes, warmup {W}, steps {K}, dt {dt}", file=sys.stderr, flush=True)
+
+    res = {}
+    for prec in ("fp32c", "fp64"):
+        dev, wall, kms = timed_run(prec, N, y0, fp, t_warm, t_timed, args.steps_per_launch, ev)
+        launches = math.ceil(K / args.steps_per_launch)
+        res[prec] = {"dev": dev, "wall": wall, "kernel_ms": kms, "launches": launches,
+                     "value": N * K / wall}
+        print(f"[bench] {prec}: {wall:.3f} s wall, {kms:.1f} ms events, {N * K / wall:.4e} node-steps/s",
+              file=sys.stderr, flush=True)
+
+    final32 = res["fp32c"]["dev"].states()
+    final64 = res["fp64"]["dev"].states()
+    dv_end = float(np.abs(final32[:, 17] - final64[:, 17]).max())
+    floor = 1e-6 * np.max(np.abs(final64), axis=0, keepdims=True)
+    rel32 = (np.abs(final32 - final64) / np.maximum(np.abs(final64), floor)).max(axis=0)
+    for p in res.values():
+        p.pop("dev").close()
+
+    checks = {}
+    if not args.no_checks:
+        sub = 4096
+        tr32 = subset_trace("fp32c", y0[:sub], fp[:sub], (W + K) * dt, dt=dt)
+        tr64 = subset_trace("fp64", y0[:sub], fp[:sub], (W + K) * dt, dt=dt)
+        checks["max_abs_dV_over_time_subset_mV"] = float(np.abs(tr32 - tr64).max())
+        checks["subset"] = f"{sub} nodes, V every 1 ms over {(W + K) * dt:g} ms"
+        checks["fp64_vs_oracle_max_rel_err"] = oracle_parity(y0, fp, t_all, final64)
+        checks["fp64_vs_oracle_nodes"] = 64
+        print(f"[bench] checks {checks}", file=sys.stderr, flush=True)
+
+    flops, rhs_ops = flops_per_node_step()
+    spl = min(args.steps_per_launch, K)
+    avg_launch_s = res["fp32c"]["kernel_ms"] * 1e-3 / res["fp32c"]["launches"]
+    achieved = flops * N * spl / avg_launch_s / 1e12
+    traffic = load_traffic()
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "kernel": "goss_hip::explicit_kernel<tp06_m_fp3, GRL1, float, COMP=true>",
+                "flops_per_node_step": flops, "node_steps_per_launch": N * spl,
+                "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+    avg64 = res["fp64"]["kernel_ms"] * 1e-3 / res["fp64"]["launches"]
+    fp64_achieved = flops * N * spl / avg64 / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": res["fp32c"]["value"],
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": res["fp32c"]["wall"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "TP06 M-cell GRL1 ODESystemSolver::forward, 1,048,576 nodes, dt 0.02 ms, "
+                               "lognormal(0,0.2) g_Kr/g_Ks/g_CaL, stimulus -52 pA/pF every 1000 ms",
+                   "nodes": N, "dt_ms": dt, "solver": "GRL1", "precision": "fp32c",
+                   "steps_per_launch": args.steps_per_launch},
+        "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["wall"] / K * 1e3,
+                 "achieved_tflops": round(fp64_achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                 "frac": round(fp64_achieved / PEAK_FP64_TFLOPS, 4)},
+        "fp32c_over_fp64": res["fp32c"]["value"] / res["fp64"]["value"],
+        "accuracy": {"max_abs_dV_end_mV": dv_end, "max_rel_err_per_state_end": [float(x) for x in rel32],
+                     **checks},
+        "adjoint_over_forward": None,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(N, y0, fp, dt)
+        print(f"[bench] cpu {out['cpu_baseline']}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

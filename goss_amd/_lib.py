@@ -1,0 +1,136 @@
+"""ctypes binding of libgoss_hip.so (the C ABI declared in include/goss_hip.h).
+
+The library is loaded from the package directory, next to this file.  There
+is no fallback: if the library is missing, or no GPU is visible when a
+system is created, the call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgoss_hip.so")
+
+SOLVERS = {
+    "ExplicitEuler": 0,
+    "RK2": 1,
+    "RK4": 2,
+    "RL1": 3,
+    "RL2": 4,
+    "GRL1": 5,
+    "GRL2": 6,
+    "ImplicitEuler": 7,
+    "ESDIRK23a": 8,
+    "RKF32": 9,
+}
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp32c": 2}
+
+# Functions whose C signatures this module binds (checked by the tests against
+# the exported symbols).
+EXPORTS = (
+    "goss_hip_last_error", "goss_hip_device_count", "goss_hip_model_count", "goss_hip_model_name",
+    "goss_hip_model_info_get", "goss_hip_solver_params_default", "goss_hip_solver_is_adaptive",
+    "goss_hip_system_create", "goss_hip_system_destroy", "goss_hip_system_forward",
+    "goss_hip_system_forward_times", "goss_hip_system_solve", "goss_hip_system_get_field_states",
+    "goss_hip_system_set_field_states", "goss_hip_system_get_field_state_components",
+    "goss_hip_system_set_field_state_components", "goss_hip_system_get_field_parameters",
+    "goss_hip_system_set_field_parameters", "goss_hip_system_get_field_states_device",
+    "goss_hip_system_set_field_states_device", "goss_hip_system_get_states", "goss_hip_system_set_states",
+    "goss_hip_system_reset_default", "goss_hip_system_set_parameter", "goss_hip_system_get_parameter",
+    "goss_hip_system_set_num_threads", "goss_hip_system_get_num_threads", "goss_hip_system_num_nodes",
+    "goss_hip_system_set_steps_per_launch", "goss_hip_system_synchronize", "goss_hip_system_stream",
+)
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [
+        ("internal_time_step", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("eta_0", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("relative_tolerance", ctypes.c_double),
+        ("max_relative_previous_residual", ctypes.c_double),
+        ("min_dt", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("rtol", ctypes.c_double),
+        ("max_iterations", ctypes.c_int),
+        ("always_recompute_jacobian", ctypes.c_int),
+        ("num_refinements_without_always_recomputing_jacobian", ctypes.c_int),
+    ]
+
+
+class ModelInfo(ctypes.Structure):
+    _fields_ = [
+        ("num_states", ctypes.c_int),
+        ("num_parameters", ctypes.c_int),
+        ("num_field_states", ctypes.c_int),
+        ("num_field_parameters", ctypes.c_int),
+        ("state_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("parameter_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("field_state_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("field_parameter_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("field_state_indices", ctypes.POINTER(ctypes.c_int)),
+        ("field_parameter_indices", ctypes.POINTER(ctypes.c_int)),
+        ("default_parameters", ctypes.POINTER(ctypes.c_double)),
+        ("initial_conditions", ctypes.POINTER(ctypes.c_double)),
+        ("linear_terms", ctypes.POINTER(ctypes.c_ubyte)),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_lib: Optional[ctypes.CDLL] = None
+
+
+class GossHipError(RuntimeError):
+    """A failure reported by libgoss_hip.so (message from goss_hip_last_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C goss_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    L.goss_hip_last_error.restype = ctypes.c_char_p
+    L.goss_hip_model_name.restype = ctypes.c_char_p
+    L.goss_hip_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.goss_hip_model_info_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ModelInfo)]
+    L.goss_hip_solver_params_default.argtypes = [ctypes.c_int, ctypes.POINTER(SolverParams)]
+    L.goss_hip_system_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(SolverParams), ctypes.c_int,
+                                         ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+    V = ctypes.c_void_p
+    L.goss_hip_system_destroy.argtypes = [V]
+    L.goss_hip_system_forward.argtypes = [V, ctypes.c_double, ctypes.c_double]
+    L.goss_hip_system_forward_times.argtypes = [V, _dp, ctypes.c_uint64]
+    L.goss_hip_system_solve.argtypes = [V, _dp, _dp, ctypes.c_uint64, ctypes.c_int]
+    for fn in ("get_field_states", "set_field_states", "get_field_parameters", "set_field_parameters",
+               "get_field_states_device", "set_field_states_device"):
+        getattr(L, f"goss_hip_system_{fn}").argtypes = [V, ctypes.c_void_p, ctypes.c_int]
+    for fn in ("get_field_state_components", "set_field_state_components"):
+        getattr(L, f"goss_hip_system_{fn}").argtypes = [V, _dp, ctypes.c_uint32, _u32p, ctypes.c_int]
+    L.goss_hip_system_get_states.argtypes = [V, _dp]
+    L.goss_hip_system_set_states.argtypes = [V, _dp]
+    L.goss_hip_system_reset_default.argtypes = [V]
+    L.goss_hip_system_set_parameter.argtypes = [V, ctypes.c_char_p, ctypes.c_double]
+    L.goss_hip_system_get_parameter.argtypes = [V, ctypes.c_char_p, _dp]
+    L.goss_hip_system_set_num_threads.argtypes = [V, ctypes.c_uint32]
+    L.goss_hip_system_get_num_threads.argtypes = [V, _u32p]
+    L.goss_hip_system_num_nodes.argtypes = [V, _u32p]
+    L.goss_hip_system_set_steps_per_launch.argtypes = [V, ctypes.c_uint32]
+    L.goss_hip_system_synchronize.argtypes = [V]
+    L.goss_hip_system_stream.argtypes = [V, ctypes.POINTER(ctypes.c_void_p)]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise GossHipError(rc, lib().goss_hip_last_error().decode())

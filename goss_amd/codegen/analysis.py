@@ -217,7 +217,7 @@ def jacobian(model: ODEModel) -> Jacobian:
 _CENSUS_KEYS = ("add", "mul", "div", "exp", "log", "sqrt", "pow", "floor", "abs", "select", "cmp")
 
 
-def census_expr(e: sp.Expr, c: Counter) -> None:
+def census_expr(e: sp.Expr, c: Counter, keep=None) -> None:
     """Count the operations the C printer emits for ``e``.
 
     The counting mirrors :mod:`goss_amd.codegen.printer`: a ``Mul`` with
@@ -228,10 +228,12 @@ def census_expr(e: sp.Expr, c: Counter) -> None:
 
     if e.is_Atom:
         return
+    if keep is not None and not keep(e):
+        return
     if isinstance(e, sp.Add):
         c["add"] += len(e.args) - 1
         for a in e.args:
-            census_expr(a, c)
+            census_expr(a, c, keep)
         return
     if isinstance(e, sp.Mul):
         sign, num, den = split_mul(e)
@@ -242,11 +244,11 @@ def census_expr(e: sp.Expr, c: Counter) -> None:
             c["mul"] += max(len(den) - 1, 0)
             c["div"] += 1
         for a in num + den:
-            census_expr(a, c)
+            census_expr(a, c, keep)
         return
     if isinstance(e, sp.Pow):
         b, x = e.args
-        census_expr(b, c)
+        census_expr(b, c, keep)
         if x == -1:
             c["div"] += 1
         elif x.is_Integer and 2 <= abs(int(x)) <= 4:
@@ -260,7 +262,7 @@ def census_expr(e: sp.Expr, c: Counter) -> None:
             c["div"] += 1
         else:
             c["pow"] += 1
-            census_expr(x, c)
+            census_expr(x, c, keep)
         return
     if isinstance(e, sp.exp):
         c["exp"] += 1
@@ -273,18 +275,49 @@ def census_expr(e: sp.Expr, c: Counter) -> None:
     elif isinstance(e, sp.Piecewise):
         c["select"] += len(e.args) - 1
         for (v, cond) in e.args:
-            census_expr(v, c)
+            census_expr(v, c, keep)
             if cond is not sp.true:
-                census_expr(cond, c)
+                census_expr(cond, c, keep)
         return
     elif isinstance(e, sp.core.relational.Relational):
         c["cmp"] += 1
     for a in e.args:
-        census_expr(a, c)
+        if isinstance(a, sp.Basic):
+            census_expr(a, c, keep)
 
 
 def census_body(exprs: Sequence[sp.Expr]) -> Dict[str, int]:
     c: Counter = Counter()
     for e in exprs:
         census_expr(e, c)
+    return {k: int(c.get(k, 0)) for k in _CENSUS_KEYS}
+
+
+def census_state_ops(stmts: Sequence[Tuple[str, sp.Expr]], outputs: Sequence[sp.Expr],
+                     state_names: Sequence[str]) -> Dict[str, int]:
+    """Operation counts per evaluation after common-subexpression elimination,
+    counting only operations whose operands depend on a state.
+
+    Subexpressions of parameters and time alone are excluded: they are
+    constant per node (or per step) and any implementation evaluates them
+    outside the per-node-step work.  This is the algorithmic work of one
+    node-step that a roofline is priced on."""
+    dep: Set[str] = set(state_names)
+    for name, e in stmts:
+        if any(s.name in dep for s in e.free_symbols):
+            dep.add(name)
+    exprs = [e for _, e in stmts] + list(outputs)
+    rep, red = sp.cse(exprs)
+    for sym, e in rep:
+        if any(s.name in dep for s in e.free_symbols):
+            dep.add(sym.name)
+    c: Counter = Counter()
+
+    def keep(e):
+        return any(s.name in dep for s in e.free_symbols)
+
+    for _, e in rep:
+        census_expr(e, c, keep)
+    for e in red:
+        census_expr(e, c, keep)
     return {k: int(c.get(k, 0)) for k in _CENSUS_KEYS}

@@ -25,7 +25,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import sympy as sp
 
-from .analysis import Jacobian, Linearized, census_body, jacobian, linearize, needed_names
+from .analysis import Jacobian, Linearized, census_body, census_state_ops, jacobian, linearize, needed_names
 from .ode_model import ODEModel, TIME
 from .printer import CPrinter, assignments, c_name, fmt_double
 
@@ -91,14 +91,59 @@ class Emitted:
     jac_nnz: int
 
 
-def _body_lines(model: ODEModel, names: Sequence[str], printer: CPrinter, ind: str) -> List[str]:
-    bd = model.body_dict()
-    return assignments(printer, [(n, bd[n]) for n in names], ind)
+def time_only_names(model: ODEModel) -> List[str]:
+    """Intermediates that depend on time (and parameters) but on no state.
+
+    They are evaluated in the type of ``time`` (``treal``, double on every
+    target) so that a single-precision kernel sees the same discontinuities
+    in time (stimulus on/off) as the double-precision oracle."""
+    state_set = set(model.state_names)
+    dep_state: Dict[str, bool] = {}
+    dep_time: Dict[str, bool] = {}
+    out = []
+    for n, e in model.body:
+        ds = dt = False
+        for s in e.free_symbols:
+            if s.name in state_set:
+                ds = True
+            elif s == TIME:
+                dt = True
+            elif s.name in dep_state:
+                ds = ds or dep_state[s.name]
+                dt = dt or dep_time[s.name]
+        dep_state[n], dep_time[n] = ds, dt
+        if dt and not ds and n not in model.derivatives.values():
+            out.append(n)
+    return out
+
+
+class _Printers:
+    """The main printer (type ``real``) and the printer of time-only
+    statements (type ``treal``)."""
+
+    def __init__(self, model: ODEModel):
+        self.time_only = set(time_only_names(model))
+        main_map = {n: f"real({c_name(n)})" for n in self.time_only}
+        main_map["time"] = "real(time)"
+        self.main = CPrinter(main_map)
+        tmap = {p.name: f"treal({c_name(p.name)})" for p in model.parameters}
+        self.time = CPrinter(tmap, lit_type="treal")
+
+    def lines(self, model: ODEModel, names: Sequence[str], ind: str) -> List[str]:
+        bd = model.body_dict()
+        out = []
+        for n in names:
+            if n in self.time_only:
+                out += assignments(self.time, [(n, bd[n])], ind, decl="const treal ")
+            else:
+                out += assignments(self.main, [(n, bd[n])], ind)
+        return out
 
 
 def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     model = var.model
-    pr = CPrinter()
+    prs = _Printers(model)
+    pr = prs.main
     lin = linearize(model)
     S, P = model.num_states, model.num_parameters
     ind = "  "
@@ -136,15 +181,16 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
 
     # -- eval ---------------------------------------------------------------
     used = _used_symbols(rhs_exprs + [bd[model.derivatives[s]] for s in model.state_names])
-    L.append("// Right-hand side: values[i] = d states[i] / dt.")
-    L.append("template <typename real, typename PA>")
-    L.append("GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, const PA& pa,")
+    L.append("// Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on")
+    L.append("// no state are computed in the type of `time` (treal).")
+    L.append("template <typename real, typename PA, typename treal>")
+    L.append("GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time, const PA& pa,")
     L.append("                              real* __restrict__ values)")
     L.append("{")
     L.append(f"{ind}(void)time; (void)pa;")
     L += _bind_states(model, used, "states", ind)
     L += _bind_params(model, used, ind)
-    L += _body_lines(model, [n for n in rhs_names if n not in model.derivatives.values()], pr, ind)
+    L += prs.lines(model, [n for n in rhs_names if n not in model.derivatives.values()], ind)
     for i, s in enumerate(model.state_names):
         L.append(f"{ind}values[{i}] = {pr.p(bd[model.derivatives[s]])};")
     L.append("}")
@@ -159,8 +205,8 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     extra = [n for n in extra if n not in rhs_names]
     L.append("// Linearized right-hand side: rhs[i] as in eval and linear[i] = d rhs[i] / d states[i].")
     L.append("// Entries with linear_terms[i] == 1 are written before the only_linear return.")
-    L.append("template <typename real, typename PA>")
-    L.append("GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const real time, const PA& pa,")
+    L.append("template <typename real, typename PA, typename treal>")
+    L.append("GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const treal time, const PA& pa,")
     L.append("                                         real* __restrict__ linear, real* __restrict__ rhs,")
     L.append("                                         const bool only_linear)")
     L.append("{")
@@ -169,7 +215,7 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L += _bind_states(model, used_all, "states", ind)
     L += _bind_params(model, used_all, ind)
     all_inter = [n for n, _ in model.body if n in set(rhs_names) | set(extra) and n not in model.derivatives.values()]
-    L += _body_lines(model, all_inter, pr, ind)
+    L += prs.lines(model, all_inter, ind)
     for i, s in enumerate(model.state_names):
         L.append(f"{ind}rhs[{i}] = {pr.p(bd[model.derivatives[s]])};")
     L += assignments(pr, lin_first, ind)
@@ -184,7 +230,14 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("}")
     L.append("")
 
+    rhs_stmts = [(n, bd[n]) for n, _ in model.body if n in set(rhs_names)]
+    lin_stmts_all = [(n, bd[n]) for n, _ in model.body if n in set(rhs_names) | set(extra)] + lin.lin_stmts
     census = {
+        # Per evaluation, after common-subexpression elimination, operations
+        # whose operands depend on a state (the algorithmic work).
+        "eval_state": census_state_ops(rhs_stmts, [], model.state_names),
+        "linearized_eval_state": census_state_ops(lin_stmts_all, lin.linear, model.state_names),
+        # Operations as printed (before compiler CSE), all operations.
         "eval": census_body(rhs_exprs),
         "linearized_eval": census_body(
             [bd[n] for n in all_inter] + [bd[model.derivatives[s]] for s in model.state_names]
@@ -199,14 +252,14 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         jinter = [n for n, _ in model.body if n in set(jextra) and n not in model.derivatives.values()]
         jused |= _used_symbols([bd[n] for n in jinter])
         L.append("// Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].")
-        L.append("template <typename real, typename PA>")
-        L.append("GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const real time,")
+        L.append("template <typename real, typename PA, typename treal>")
+        L.append("GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,")
         L.append("                                          const PA& pa, real* __restrict__ jac)")
         L.append("{")
         L.append(f"{ind}(void)time; (void)pa;")
         L += _bind_states(model, jused, "states", ind)
         L += _bind_params(model, jused, ind)
-        L += _body_lines(model, jinter, pr, ind)
+        L += prs.lines(model, jinter, ind)
         L += assignments(pr, jac.stmts, ind)
         for i in range(S):
             for j in range(S):

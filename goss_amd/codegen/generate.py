@@ -34,6 +34,7 @@ VARIANTS: Dict[str, tuple] = {
     "fhn": ("fhn", ["v"], []),
     "beeler_reuter_1977": ("beeler_reuter_1977", ["V"], []),
     "oscilator": ("oscilator", ["x", "y"], ["a", "b"]),
+    "linear_decay": ("linear_decay", ["y"], ["a", "b"]),
 }
 
 
@@ -54,6 +55,13 @@ def generate(names: List[str] = None, out_dir: str = OUT_DIR) -> Dict[str, dict]
         _write(os.path.join(out_dir, f"{n}_plugin.h"), emit_parameterized_ode(var, f"{n}.h"))
         meta[n] = variant_metadata(var, em)
     _write(os.path.join(out_dir, "variants.json"), json.dumps(meta, indent=1, sort_keys=True) + "\n")
+    lines = ["// Generated by goss_amd.codegen.generate -- do not edit.",
+             "// X-macro list of the model variants compiled into the library.",
+             "#pragma once",
+             "#define GOSS_VARIANT_LIST(X) \\"]
+    lines += [f"  X({n}) \\" for n in names]
+    lines += ["", ""]
+    _write(os.path.join(out_dir, "variant_list.h"), "\n".join(lines))
     return meta
 
 

@@ -76,8 +76,9 @@ class CPrinter:
     PREC = {"atom": 100, "call": 100, "pow": 60, "mul": 50, "neg": 45, "add": 40, "cmp": 30,
             "and": 20, "or": 10, "select": 5}
 
-    def __init__(self, name_map: Optional[Dict[str, str]] = None):
+    def __init__(self, name_map: Optional[Dict[str, str]] = None, lit_type: str = "real"):
         self.name_map = name_map or {}
+        self.lit_type = lit_type
 
     def sym(self, s: sp.Symbol) -> str:
         if s.name in self.name_map:
@@ -85,7 +86,7 @@ class CPrinter:
         return c_name(s.name)
 
     def lit(self, x) -> str:
-        return f"real({fmt_double(float(x))})"
+        return f"{self.lit_type}({fmt_double(float(x))})"
 
     def p(self, e) -> str:
         return self._p(e)[0]
@@ -220,4 +221,4 @@ class CPrinter:
 
 def assignments(printer: CPrinter, stmts: Sequence[Tuple[str, sp.Expr]], indent: str,
                 decl: str = "const real ") -> List[str]:
-    return [f"{indent}{decl}{printer.sym(sp.Symbol(n))} = {printer.p(e)};" for n, e in stmts]
+    return [f"{indent}{decl}{c_name(n)} = {printer.p(e)};" for n, e in stmts]

@@ -19,9 +19,10 @@ constexpr double default_parameters[] = {50.0, 0.04, 3e-05, 0.0005, 0.5, 50000.0
 constexpr double initial_conditions[] = {0.011, 0.988, 0.975, 0.0001, 0.003, 0.994, 0.0001, -84.624};
 constexpr unsigned char linear_terms[] = {1, 1, 1, 0, 1, 1, 1, 0};
 
-// Right-hand side: values[i] = d states[i] / dt.
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, const PA& pa,
+// Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
+// no state are computed in the type of `time` (treal).
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time, const PA& pa,
                               real* __restrict__ values)
 {
   (void)time; (void)pa;
@@ -62,7 +63,7 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, 
   const real beta_x1 = real(0.0003916464405623223)*m_exp(-real(0.05998800239952009)*V)/(real(1.0) + m_exp(-real(0.8) - real(0.04)*V));
   const real V_eff_1 = (m_fabs(real(23.0) + V) < real(0.01) ? real(0.01) : real(23.0) + V);
   const real i_K1 = real(0.0035)*(real(4.6000000000000005) + real(0.2)*V)/(real(1.0) - m_exp(-real(0.04)*V_eff_1)) + real(0.0035)*(-real(4.0) + real(119.85640018958804)*m_exp(real(0.04)*V))/(real(8.331137487687693)*m_exp(real(0.04)*V) + real(69.4078518387552)*m_exp(real(0.08)*V));
-  const real Istim = (IstimEnd >= time && IstimStart <= time && IstimPulseDuration >= time - IstimStart - IstimPeriod*m_floor((time - IstimStart)/IstimPeriod) ? IstimAmplitude : real(0.0));
+  const treal Istim = (treal(IstimEnd) >= time && treal(IstimStart) <= time && treal(IstimPulseDuration) >= time - treal(IstimStart) - treal(IstimPeriod)*m_floor((time - treal(IstimStart))/treal(IstimPeriod)) ? treal(IstimAmplitude) : treal(0.0));
   values[0] = alpha_m*(real(1.0) - m) - beta_m*m;
   values[1] = alpha_h*(real(1.0) - h) - beta_h*h;
   values[2] = alpha_j*(real(1.0) - j) - beta_j*j;
@@ -70,13 +71,13 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, 
   values[4] = alpha_d*(real(1.0) - d) - beta_d*d;
   values[5] = alpha_f*(real(1.0) - f) - beta_f*f;
   values[6] = alpha_x1*(real(1.0) - x1) - beta_x1*x1;
-  values[7] = (Istim - i_K1 - i_Na - i_s - i_x1)/C;
+  values[7] = (real(Istim) - i_K1 - i_Na - i_s - i_x1)/C;
 }
 
 // Linearized right-hand side: rhs[i] as in eval and linear[i] = d rhs[i] / d states[i].
 // Entries with linear_terms[i] == 1 are written before the only_linear return.
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const real time, const PA& pa,
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const treal time, const PA& pa,
                                          real* __restrict__ linear, real* __restrict__ rhs,
                                          const bool only_linear)
 {
@@ -118,7 +119,7 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   const real beta_x1 = real(0.0003916464405623223)*m_exp(-real(0.05998800239952009)*V)/(real(1.0) + m_exp(-real(0.8) - real(0.04)*V));
   const real V_eff_1 = (m_fabs(real(23.0) + V) < real(0.01) ? real(0.01) : real(23.0) + V);
   const real i_K1 = real(0.0035)*(real(4.6000000000000005) + real(0.2)*V)/(real(1.0) - m_exp(-real(0.04)*V_eff_1)) + real(0.0035)*(-real(4.0) + real(119.85640018958804)*m_exp(real(0.04)*V))/(real(8.331137487687693)*m_exp(real(0.04)*V) + real(69.4078518387552)*m_exp(real(0.08)*V));
-  const real Istim = (IstimEnd >= time && IstimStart <= time && IstimPulseDuration >= time - IstimStart - IstimPeriod*m_floor((time - IstimStart)/IstimPeriod) ? IstimAmplitude : real(0.0));
+  const treal Istim = (treal(IstimEnd) >= time && treal(IstimStart) <= time && treal(IstimPulseDuration) >= time - treal(IstimStart) - treal(IstimPeriod)*m_floor((time - treal(IstimStart))/treal(IstimPeriod)) ? treal(IstimAmplitude) : treal(0.0));
   rhs[0] = alpha_m*(real(1.0) - m) - beta_m*m;
   rhs[1] = alpha_h*(real(1.0) - h) - beta_h*h;
   rhs[2] = alpha_j*(real(1.0) - j) - beta_j*j;
@@ -126,7 +127,7 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   rhs[4] = alpha_d*(real(1.0) - d) - beta_d*d;
   rhs[5] = alpha_f*(real(1.0) - f) - beta_f*f;
   rhs[6] = alpha_x1*(real(1.0) - x1) - beta_x1*x1;
-  rhs[7] = (Istim - i_K1 - i_Na - i_s - i_x1)/C;
+  rhs[7] = (real(Istim) - i_K1 - i_Na - i_s - i_x1)/C;
   const real ddm_dt_dm = -alpha_m - beta_m;
   const real ddh_dt_dh = -alpha_h - beta_h;
   const real ddj_dt_dj = -alpha_j - beta_j;
@@ -154,8 +155,8 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
 }
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const real time,
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,
                                           const PA& pa, real* __restrict__ jac)
 {
   (void)time; (void)pa;

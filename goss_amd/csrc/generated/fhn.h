@@ -19,9 +19,10 @@ constexpr double default_parameters[] = {0.13, 0.013, 0.26, 0.1, 1.0, 50.0, 1.0,
 constexpr double initial_conditions[] = {-85.0, 0.0};
 constexpr unsigned char linear_terms[] = {0, 1};
 
-// Right-hand side: values[i] = d states[i] / dt.
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, const PA& pa,
+// Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
+// no state are computed in the type of `time` (treal).
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time, const PA& pa,
                               real* __restrict__ values)
 {
   (void)time; (void)pa;
@@ -40,15 +41,15 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, 
   const real v_amp = v_peak - v_rest;
   const real v_th = v_rest + a*v_amp;
   const real I = -c_2*s*(v - v_rest)/v_amp + c_1*(v - v_rest)*(v - v_th)*(v_peak - v)/m_sq(v_amp);
-  const real i_Stim = stim_amplitude*(real(1.0) - real(1.0)/(real(1.0) + m_exp(real(5.0)*time - real(5.0)*stim_start)))/(real(1.0) + m_exp(real(5.0)*time - real(5.0)*stim_duration - real(5.0)*stim_start));
-  values[0] = I + i_Stim;
+  const treal i_Stim = treal(stim_amplitude)*(treal(1.0) - treal(1.0)/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_start))))/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_duration) - treal(5.0)*treal(stim_start)));
+  values[0] = I + real(i_Stim);
   values[1] = b*(v - v_rest - c_3*s);
 }
 
 // Linearized right-hand side: rhs[i] as in eval and linear[i] = d rhs[i] / d states[i].
 // Entries with linear_terms[i] == 1 are written before the only_linear return.
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const real time, const PA& pa,
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const treal time, const PA& pa,
                                          real* __restrict__ linear, real* __restrict__ rhs,
                                          const bool only_linear)
 {
@@ -68,8 +69,8 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   const real v_amp = v_peak - v_rest;
   const real v_th = v_rest + a*v_amp;
   const real I = -c_2*s*(v - v_rest)/v_amp + c_1*(v - v_rest)*(v - v_th)*(v_peak - v)/m_sq(v_amp);
-  const real i_Stim = stim_amplitude*(real(1.0) - real(1.0)/(real(1.0) + m_exp(real(5.0)*time - real(5.0)*stim_start)))/(real(1.0) + m_exp(real(5.0)*time - real(5.0)*stim_duration - real(5.0)*stim_start));
-  rhs[0] = I + i_Stim;
+  const treal i_Stim = treal(stim_amplitude)*(treal(1.0) - treal(1.0)/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_start))))/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_duration) - treal(5.0)*treal(stim_start)));
+  rhs[0] = I + real(i_Stim);
   rhs[1] = b*(v - v_rest - c_3*s);
   const real dds_dt_ds = -b*c_3;
   linear[1] = dds_dt_ds;
@@ -79,8 +80,8 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
 }
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const real time,
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,
                                           const PA& pa, real* __restrict__ jac)
 {
   (void)time; (void)pa;

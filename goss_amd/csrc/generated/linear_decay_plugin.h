@@ -1,0 +1,69 @@
+// Generated by goss_amd.codegen from linear_decay.ode -- do not edit.
+#ifndef LINEAR_DECAY_PLUGIN_H_IS_INCLUDED
+#define LINEAR_DECAY_PLUGIN_H_IS_INCLUDED
+#include <memory>
+#include <stdexcept>
+#include <cmath>
+#include "goss/ParameterizedODE.h"
+#include "linear_decay.h"
+
+namespace goss {
+
+class Linear_decay : public ParameterizedODE
+{
+  struct Acc { const double* v; template <int k> double get() const { return v[k]; } };
+public:
+  Linear_decay() : ParameterizedODE(1, 2, 1, 2, 0)
+  {
+    for (int i = 0; i < 1; ++i) _state_names[i] = goss_models::linear_decay::state_names[i];
+    for (int i = 0; i < 2; ++i) {
+      _parameter_names[i] = goss_models::linear_decay::parameter_names[i];
+      _p[i] = goss_models::linear_decay::default_parameters[i];
+    }
+    _field_state_names[0] = "y";
+    _field_state_indices[0] = 0;
+    _field_parameter_names[0] = "a";
+    _field_parameter_names[1] = "b";
+    _bind();
+    for (int i = 0; i < 1; ++i) _linear_terms[i] = goss_models::linear_decay::linear_terms[i];
+  }
+  Linear_decay(const Linear_decay& ode) : ParameterizedODE(ode)
+  {
+    for (int i = 0; i < 2; ++i) _p[i] = ode._p[i];
+    _bind();
+  }
+  void eval(const double* states, double time, double* values)
+  { goss_models::linear_decay::eval<double>(states, time, Acc{_p}, values); }
+  void linearized_eval(const double* states, double time, double* linear, double* rhs,
+                       bool only_linear) const
+  { goss_models::linear_decay::linearized_eval<double>(states, time, Acc{_p}, linear, rhs, only_linear); }
+  void get_ic(goss::DoubleVector* values) const
+  {
+    values->n = _num_states;
+    values->data.reset(new double[_num_states]);
+    for (int i = 0; i < 1; ++i) values->data[i] = goss_models::linear_decay::initial_conditions[i];
+  }
+  std::shared_ptr<ODE> copy() const
+  { return std::make_shared<Linear_decay>(*this); }
+  void eval_monitored(const double*, double, double*) const
+  { throw std::runtime_error("No monitored in the 'Linear_decay' model."); }
+  void set_field_parameters(const double* field_params)
+  {
+    _p[0] = field_params[0];
+    _p[1] = field_params[1];
+  }
+private:
+  double _p[2];
+  void _bind()
+  {
+    for (int i = 0; i < 2; ++i) _param_to_value[_parameter_names[i]] = &_p[i];
+  }
+};
+
+}  // namespace goss
+
+extern "C" __attribute__((visibility("default"))) goss::ParameterizedODE* create_Linear_decay()
+{
+  return new goss::Linear_decay;
+}
+#endif

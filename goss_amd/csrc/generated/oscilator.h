@@ -19,9 +19,10 @@ constexpr double default_parameters[] = {1.0, 1.0};
 constexpr double initial_conditions[] = {1.0, 0.0};
 constexpr unsigned char linear_terms[] = {1, 1};
 
-// Right-hand side: values[i] = d states[i] / dt.
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, const PA& pa,
+// Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
+// no state are computed in the type of `time` (treal).
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time, const PA& pa,
                               real* __restrict__ values)
 {
   (void)time; (void)pa;
@@ -35,8 +36,8 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const real time, 
 
 // Linearized right-hand side: rhs[i] as in eval and linear[i] = d rhs[i] / d states[i].
 // Entries with linear_terms[i] == 1 are written before the only_linear return.
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const real time, const PA& pa,
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const treal time, const PA& pa,
                                          real* __restrict__ linear, real* __restrict__ rhs,
                                          const bool only_linear)
 {
@@ -53,8 +54,8 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
 }
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
-template <typename real, typename PA>
-GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const real time,
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,
                                           const PA& pa, real* __restrict__ jac)
 {
   (void)time; (void)pa;

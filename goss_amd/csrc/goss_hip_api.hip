@@ -1,0 +1,679 @@
+// C ABI of libgoss_hip.so (declared in include/goss_hip.h).
+//
+// A goss_hip_system holds the device buffers of one ODESystemSolver:
+//   y    states, structure of arrays [S][ld] (double, or float hi parts)
+//   ylo  FP32C only: low parts of the compensated states [NCOMP][ld]
+//   fp   field parameters [NFP][ld] (double or float)
+// with ld = num_nodes rounded up to a multiple of 64, so that every state row
+// starts on a 512-byte (double) / 256-byte (float) boundary and one
+// wavefront's 64 lanes read one contiguous segment.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/goss_hip.h"
+#include "generated/variant_list.h"
+#include "goss_layout.h"
+#include "goss_registry.h"
+
+namespace goss_hip {
+
+#define GOSS_DECLARE_REGISTER(V) \
+  void goss_register_##V##_f64(VariantInfo&); \
+  void goss_register_##V##_f32(VariantInfo&);
+GOSS_VARIANT_LIST(GOSS_DECLARE_REGISTER)
+
+namespace {
+
+struct Registry {
+  std::vector<VariantInfo> v;
+  Registry()
+  {
+#define GOSS_DO_REGISTER(V)              \
+  {                                      \
+    VariantInfo info;                    \
+    std::memset(&info, 0, sizeof(info)); \
+    goss_register_##V##_f64(info);       \
+    goss_register_##V##_f32(info);       \
+    v.push_back(info);                   \
+  }
+    GOSS_VARIANT_LIST(GOSS_DO_REGISTER)
+  }
+};
+
+Registry& registry()
+{
+  static Registry r;
+  return r;
+}
+
+}  // namespace
+
+VariantInfo* variants(int* count)
+{
+  *count = int(registry().v.size());
+  return registry().v.data();
+}
+
+}  // namespace goss_hip
+
+using namespace goss_hip;
+
+namespace {
+
+thread_local std::string t_error;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) throw Error(GOSS_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f)
+{
+  try {
+    f();
+    return GOSS_HIP_OK;
+  } catch (Error& e) {
+    t_error = e.what();
+    return e.code;
+  } catch (std::exception& e) {
+    t_error = e.what();
+    return GOSS_HIP_EINVAL;
+  }
+}
+
+const VariantInfo* find_variant(const char* name)
+{
+  int n = 0;
+  VariantInfo* v = variants(&n);
+  for (int i = 0; i < n; ++i)
+    if (name && std::strcmp(v[i].name, name) == 0) return &v[i];
+  throw Error(GOSS_HIP_EINVAL, std::string("unknown model variant ") + (name ? name : "(null)"));
+}
+
+bool is_explicit(int solver) { return solver >= 0 && solver <= kGRL2; }
+
+void default_params(int solver, goss_hip_solver_params* p)
+{
+  std::memset(p, 0, sizeof(*p));
+  p->internal_time_step = -1.0;  // ODESolver.h:137
+  p->delta = 1e-8;               // GRL1.h:39
+  p->eta_0 = 1.0;                // ImplicitODESolver.h:36-41
+  p->kappa = 0.1;
+  p->relative_tolerance = 1e-12;
+  p->max_iterations = 30;
+  p->max_relative_previous_residual = 0.01;
+  p->always_recompute_jacobian = 0;
+  p->num_refinements_without_always_recomputing_jacobian = 2;  // ImplicitEuler.h:36
+  p->min_dt = solver == kESDIRK23a ? 0.001 : 0.0001;           // ESDIRK23a.h:38, ImplicitEuler.h:37
+  p->atol = 1e-5;  // AdaptiveExplicitSolver.cpp:52-53
+  p->rtol = 1e-8;
+}
+
+}  // namespace
+
+struct goss_hip_system {
+  const VariantInfo* v = nullptr;
+  int solver = 0;
+  int prec = 0;
+  goss_hip_solver_params sp{};
+  uint32_t N = 0, ld = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* y = nullptr;
+  void* ylo = nullptr;
+  void* fp = nullptr;
+  std::vector<double> shared;
+  // (t, interval) pairs: pinned host staging and device copy.
+  double2* h_steps = nullptr;
+  double2* d_steps = nullptr;
+  size_t steps_cap = 0;
+  hipEvent_t steps_free = nullptr;  // recorded after the last H2D of h_steps
+  double* d_stage = nullptr;  // caller-layout staging (doubles)
+  size_t stage_cap = 0;
+  uint32_t num_threads = 0;
+  uint32_t steps_per_launch = 200;
+  IndexMap map{};
+
+  size_t elem() const { return prec == kFP64 ? sizeof(double) : sizeof(float); }
+  bool is_f64() const { return prec == kFP64; }
+
+  void use_device() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
+
+  double* stage(size_t n)
+  {
+    if (n > stage_cap) {
+      if (d_stage) hip_check(hipFree(d_stage), "hipFree");
+      d_stage = nullptr;
+      hip_check(hipMalloc(&d_stage, std::max<size_t>(n, 1) * sizeof(double)), "hipMalloc(stage)");
+      stage_cap = n;
+    }
+    return d_stage;
+  }
+
+  // Upload n (t, interval) pairs; returns the device array.
+  const double2* upload_steps(const double2* src, size_t n)
+  {
+    hip_check(hipEventSynchronize(steps_free), "hipEventSynchronize");
+    if (n > steps_cap) {
+      if (h_steps) hip_check(hipHostFree(h_steps), "hipHostFree");
+      if (d_steps) hip_check(hipFree(d_steps), "hipFree");
+      h_steps = nullptr;
+      d_steps = nullptr;
+      const size_t cap = std::max<size_t>(n, 64);
+      hip_check(hipHostMalloc(&h_steps, cap * sizeof(double2), hipHostMallocDefault), "hipHostMalloc(steps)");
+      hip_check(hipMalloc(&d_steps, cap * sizeof(double2)), "hipMalloc(steps)");
+      steps_cap = cap;
+    }
+    std::memcpy(h_steps, src, n * sizeof(double2));
+    hip_check(hipMemcpyAsync(d_steps, h_steps, n * sizeof(double2), hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync(steps)");
+    hip_check(hipEventRecord(steps_free, stream), "hipEventRecord");
+    return d_steps;
+  }
+
+  template <class Fn>
+  void dispatch_layout(Fn&& fn)
+  {
+    if (is_f64()) fn(double());
+    else fn(float());
+  }
+
+  void launch_steps(const double2* d, size_t n, double* field_out, int tangled)
+  {
+    const ExplicitLaunch launch = v->explicit_launch[solver][prec];
+    if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not available for this model");
+    for (size_t off = 0; off < n; off += steps_per_launch) {
+      const int cnt = int(std::min<size_t>(steps_per_launch, n - off));
+      ExplicitArgs a;
+      a.y = y;
+      a.ylo = ylo;
+      a.fp = fp;
+      a.steps = d + off;
+      a.nouter = cnt;
+      a.shared = shared.data();
+      a.ldt_0 = sp.internal_time_step;
+      a.delta = sp.delta;
+      a.N = N;
+      a.ld = ld;
+      a.fo.out = field_out ? field_out + off * size_t(N) * v->NFS : nullptr;
+      a.fo.row_stride = size_t(N) * v->NFS;
+      a.fo.tangled = tangled;
+      a.stream = stream;
+      hip_check(launch(a), "kernel launch");
+    }
+  }
+
+  void reset_default()
+  {
+    DefaultValues d{};
+    for (int i = 0; i < v->S; ++i) d.ic[i] = v->ic[i];
+    for (int j = 0; j < v->NFP; ++j) d.fp[j] = shared[v->fpi[j]];
+    dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_fill_defaults<R>(y, ylo, fp, map, d, N, ld, stream), "fill_defaults");
+    });
+  }
+
+  void copy_out(double* host, const double* dev, size_t n)
+  {
+    hip_check(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(D2H)");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+  void copy_in(double* dev, const double* host, size_t n)
+  {
+    hip_check(hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, stream), "hipMemcpyAsync(H2D)");
+  }
+
+  ~goss_hip_system()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (y) (void)hipFree(y);
+    if (ylo) (void)hipFree(ylo);
+    if (fp) (void)hipFree(fp);
+    if (d_steps) (void)hipFree(d_steps);
+    if (h_steps) (void)hipHostFree(h_steps);
+    if (d_stage) (void)hipFree(d_stage);
+    if (steps_free) (void)hipEventDestroy(steps_free);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+ComponentList all_components(int F)
+{
+  ComponentList cl{};
+  cl.n = F;
+  for (int i = 0; i < F; ++i) cl.c[i] = i;
+  return cl;
+}
+
+ComponentList component_list(const goss_hip_system* s, uint32_t nc, const uint32_t* comp)
+{
+  if (nc > uint32_t(kMaxStates)) throw Error(GOSS_HIP_EINVAL, "too many components");
+  ComponentList cl{};
+  cl.n = int(nc);
+  for (uint32_t i = 0; i < nc; ++i) {
+    if (comp[i] >= uint32_t(s->v->NFS)) throw Error(GOSS_HIP_EINVAL, "field state component out of range");
+    cl.c[i] = int(comp[i]);
+  }
+  return cl;
+}
+
+void check_sys(const goss_hip_system* s)
+{
+  if (!s) throw Error(GOSS_HIP_EINVAL, "null system");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* goss_hip_last_error(void) { return t_error.c_str(); }
+
+int goss_hip_device_count(int* count)
+{
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int goss_hip_model_count(void)
+{
+  int n = 0;
+  variants(&n);
+  return n;
+}
+
+const char* goss_hip_model_name(int index)
+{
+  int n = 0;
+  VariantInfo* v = variants(&n);
+  return (index >= 0 && index < n) ? v[index].name : nullptr;
+}
+
+int goss_hip_model_info_get(const char* variant, goss_hip_model_info* out)
+{
+  return guarded([&] {
+    const VariantInfo* v = find_variant(variant);
+    out->num_states = v->S;
+    out->num_parameters = v->P;
+    out->num_field_states = v->NFS;
+    out->num_field_parameters = v->NFP;
+    out->state_names = v->state_names;
+    out->parameter_names = v->parameter_names;
+    out->field_state_names = v->field_state_names;
+    out->field_parameter_names = v->field_parameter_names;
+    out->field_state_indices = v->fsi;
+    out->field_parameter_indices = v->fpi;
+    out->default_parameters = v->defaults;
+    out->initial_conditions = v->ic;
+    out->linear_terms = v->linear_terms;
+  });
+}
+
+int goss_hip_solver_params_default(int solver, goss_hip_solver_params* out)
+{
+  return guarded([&] {
+    if (solver < 0 || solver >= kNumSolvers) throw Error(GOSS_HIP_EINVAL, "unknown solver");
+    default_params(solver, out);
+  });
+}
+
+int goss_hip_solver_is_adaptive(int solver)
+{
+  // AdaptiveExplicitSolver.h / AdaptiveImplicitSolver.h:116-119.
+  return solver == kESDIRK23a || solver == kRKF32;
+}
+
+int goss_hip_system_create(const char* variant, int solver, const goss_hip_solver_params* params, int precision,
+                           uint32_t num_nodes, goss_hip_system** out)
+{
+  goss_hip_system* s = nullptr;
+  const int rc = guarded([&] {
+    const VariantInfo* v = find_variant(variant);
+    if (solver < 0 || solver >= kNumSolvers) throw Error(GOSS_HIP_EINVAL, "unknown solver");
+    if (precision < 0 || precision >= kNumPrecisions) throw Error(GOSS_HIP_EINVAL, "unknown precision");
+    if (!is_explicit(solver)) throw Error(GOSS_HIP_EUNSUPPORTED, "solver not available on the device path");
+    if (!v->explicit_launch[solver][precision])
+      throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not compiled for this model");
+    if (v->S > kMaxStates || v->NFP > kMaxFieldParams) throw Error(GOSS_HIP_EUNSUPPORTED, "model too large");
+    s = new goss_hip_system;
+    s->v = v;
+    s->solver = solver;
+    s->prec = precision;
+    if (params) s->sp = *params;
+    else default_params(solver, &s->sp);
+    s->N = num_nodes;
+    s->ld = (num_nodes + 63u) & ~63u;
+    if (s->ld == 0) s->ld = 64;
+    hip_check(hipGetDevice(&s->device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&s->steps_free, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(s->steps_free, s->stream), "hipEventRecord");
+    s->shared.assign(v->defaults, v->defaults + v->P);
+    s->map.S = v->S;
+    s->map.F = v->NFS;
+    s->map.NFP = v->NFP;
+    int c = 0;
+    for (int i = 0; i < v->S; ++i) {
+      s->map.comp[i] = (precision == kFP32C && v->linear_terms[i] == 0) ? c++ : -1;
+    }
+    for (int i = 0; i < v->NFS; ++i) s->map.fsi[i] = v->fsi[i];
+    hip_check(hipMalloc(&s->y, size_t(v->S) * s->ld * s->elem()), "hipMalloc(states)");
+    if (precision == kFP32C && v->NCOMP > 0)
+      hip_check(hipMalloc(&s->ylo, size_t(v->NCOMP) * s->ld * sizeof(float)), "hipMalloc(states lo)");
+    hip_check(hipMalloc(&s->fp, size_t(std::max(v->NFP, 1)) * s->ld * s->elem()), "hipMalloc(field params)");
+    // The padding lanes of every row are never read by a kernel but start at zero.
+    hip_check(hipMemsetAsync(s->y, 0, size_t(v->S) * s->ld * s->elem(), s->stream), "hipMemset");
+    if (s->ylo) hip_check(hipMemsetAsync(s->ylo, 0, size_t(v->NCOMP) * s->ld * sizeof(float), s->stream), "hipMemset");
+    hip_check(hipMemsetAsync(s->fp, 0, size_t(std::max(v->NFP, 1)) * s->ld * s->elem(), s->stream), "hipMemset");
+    s->reset_default();
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+  if (rc != GOSS_HIP_OK) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return rc;
+}
+
+int goss_hip_system_destroy(goss_hip_system* sys)
+{
+  return guarded([&] {
+    if (sys) sys->use_device();
+    delete sys;
+  });
+}
+
+int goss_hip_system_forward(goss_hip_system* s, double t, double interval)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    const double2 st = make_double2(t, interval);
+    const double2* d = s->upload_steps(&st, 1);
+    s->launch_steps(d, 1, nullptr, 1);
+  });
+}
+
+int goss_hip_system_forward_times(goss_hip_system* s, const double* t, uint64_t n)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (n < 2) return;
+    s->use_device();
+    // interval = t[i+1] - t[i], computed as in ODESystemSolver::solve (ODESystemSolver.cpp:91-97).
+    std::vector<double2> st(n - 1);
+    for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
+    const double2* d = s->upload_steps(st.data(), st.size());
+    s->launch_steps(d, st.size(), nullptr, 1);
+  });
+}
+
+int goss_hip_system_solve(goss_hip_system* s, double* field_states, const double* t, uint64_t n, int tangled)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (n < 2) return;
+    s->use_device();
+    const size_t row = size_t(s->N) * s->v->NFS;
+    // Device rows are produced in chunks bounded by a staging budget (~256 MiB).
+    const size_t budget_rows = std::max<size_t>(1, (size_t(256) << 20) / std::max<size_t>(row * sizeof(double), 1));
+    std::vector<double2> st(n - 1);
+    for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
+    const double2* d = s->upload_steps(st.data(), st.size());
+    for (size_t off = 0; off < st.size(); off += budget_rows) {
+      const size_t cnt = std::min(budget_rows, st.size() - off);
+      double* dev = s->stage(cnt * row);
+      s->launch_steps(d + off, cnt, row ? dev : nullptr, tangled);
+      if (row) s->copy_out(field_states + (off + 1) * row, dev, cnt * row);
+    }
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_get_states(goss_hip_system* s, double* out)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    const size_t n = size_t(s->N) * s->v->S;
+    double* dev = s->stage(n);
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_states<R>(LayoutOp::kGet, s->y, s->ylo, dev, s->map, s->N, s->ld, s->stream), "states_to_aos");
+    });
+    s->copy_out(out, dev, n);
+  });
+}
+
+int goss_hip_system_set_states(goss_hip_system* s, const double* in)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    const size_t n = size_t(s->N) * s->v->S;
+    double* dev = s->stage(n);
+    s->copy_in(dev, in, n);
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_states<R>(LayoutOp::kSet, s->y, s->ylo, dev, s->map, s->N, s->ld, s->stream),
+                "states_from_aos");
+    });
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+static int field_states_io(goss_hip_system* s, double* host, const double* host_in, double* dev_ptr, bool device_io,
+                           LayoutOp op, const ComponentList& cl, int tangled)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    const size_t n = size_t(s->N) * s->v->NFS;
+    double* dev = device_io ? dev_ptr : s->stage(n);
+    if (op == LayoutOp::kSet && !device_io) s->copy_in(dev, host_in, n);
+    if (op == LayoutOp::kGet && !device_io && cl.n < s->v->NFS) {
+      // Components not selected keep the caller's values (ODESystemSolver.h:238-252).
+      s->copy_in(dev, host, n);
+    }
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_field_states<R>(op, s->y, s->ylo, dev, s->map, cl, s->N, s->ld, tangled, s->stream),
+                "field_states");
+    });
+    if (op == LayoutOp::kGet && !device_io) s->copy_out(host, dev, n);
+    else hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_get_field_states(goss_hip_system* s, double* out, int tangled)
+{
+  if (!s) { t_error = "null system"; return GOSS_HIP_EINVAL; }
+  return field_states_io(s, out, nullptr, nullptr, false, LayoutOp::kGet, all_components(s->v->NFS), tangled);
+}
+
+int goss_hip_system_set_field_states(goss_hip_system* s, const double* in, int tangled)
+{
+  if (!s) { t_error = "null system"; return GOSS_HIP_EINVAL; }
+  return field_states_io(s, nullptr, in, nullptr, false, LayoutOp::kSet, all_components(s->v->NFS), tangled);
+}
+
+int goss_hip_system_get_field_states_device(goss_hip_system* s, double* dev_out, int tangled)
+{
+  if (!s) { t_error = "null system"; return GOSS_HIP_EINVAL; }
+  return field_states_io(s, nullptr, nullptr, dev_out, true, LayoutOp::kGet, all_components(s->v->NFS), tangled);
+}
+
+int goss_hip_system_set_field_states_device(goss_hip_system* s, const double* dev_in, int tangled)
+{
+  if (!s) { t_error = "null system"; return GOSS_HIP_EINVAL; }
+  return field_states_io(s, nullptr, nullptr, const_cast<double*>(dev_in), true, LayoutOp::kSet,
+                         all_components(s->v->NFS), tangled);
+}
+
+int goss_hip_system_get_field_state_components(goss_hip_system* s, double* out, uint32_t nc, const uint32_t* comp,
+                                               int tangled)
+{
+  if (!s) { t_error = "null system"; return GOSS_HIP_EINVAL; }
+  ComponentList cl;
+  const int rc = guarded([&] { cl = component_list(s, nc, comp); });
+  if (rc) return rc;
+  return field_states_io(s, out, nullptr, nullptr, false, LayoutOp::kGet, cl, tangled);
+}
+
+int goss_hip_system_set_field_state_components(goss_hip_system* s, const double* in, uint32_t nc,
+                                               const uint32_t* comp, int tangled)
+{
+  if (!s) { t_error = "null system"; return GOSS_HIP_EINVAL; }
+  ComponentList cl;
+  int rc = guarded([&] { cl = component_list(s, nc, comp); });
+  if (rc) return rc;
+  // Only the selected components are read from `in`; the others keep their
+  // device values (ODESystemSolver.h:254-267).
+  return field_states_io(s, nullptr, in, nullptr, false, LayoutOp::kSet, cl, tangled);
+}
+
+int goss_hip_system_get_field_parameters(goss_hip_system* s, double* out, int tangled)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    const size_t n = size_t(s->N) * s->v->NFP;
+    if (n == 0) return;
+    double* dev = s->stage(n);
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_field_params<R>(LayoutOp::kGet, s->fp, dev, s->v->NFP, s->N, s->ld, tangled, s->stream),
+                "field_params_get");
+    });
+    s->copy_out(out, dev, n);
+  });
+}
+
+int goss_hip_system_set_field_parameters(goss_hip_system* s, const double* in, int tangled)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    const size_t n = size_t(s->N) * s->v->NFP;
+    if (n == 0) return;
+    double* dev = s->stage(n);
+    s->copy_in(dev, in, n);
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_field_params<R>(LayoutOp::kSet, s->fp, dev, s->v->NFP, s->N, s->ld, tangled, s->stream),
+                "field_params_set");
+    });
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_reset_default(goss_hip_system* s)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    s->reset_default();
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_set_parameter(goss_hip_system* s, const char* name, double value)
+{
+  return guarded([&] {
+    check_sys(s);
+    for (int k = 0; k < s->v->P; ++k) {
+      if (name && std::strcmp(name, s->v->parameter_names[k]) == 0) {
+        // Kernels read the shared values at launch; earlier launches already
+        // captured theirs.
+        s->shared[k] = value;
+        return;
+      }
+    }
+    // ParameterizedODE::_find_parameter (ParameterizedODE.h:185-195).
+    throw Error(GOSS_HIP_EINVAL, std::string("\"") + (name ? name : "") + "\" is not a parameter in  this ODE.");
+  });
+}
+
+int goss_hip_system_get_parameter(goss_hip_system* s, const char* name, double* value)
+{
+  return guarded([&] {
+    check_sys(s);
+    for (int k = 0; k < s->v->P; ++k) {
+      if (name && std::strcmp(name, s->v->parameter_names[k]) == 0) {
+        *value = s->shared[k];
+        return;
+      }
+    }
+    throw Error(GOSS_HIP_EINVAL, std::string("\"") + (name ? name : "") + "\" is not a parameter in  this ODE.");
+  });
+}
+
+int goss_hip_system_set_num_threads(goss_hip_system* s, uint32_t n)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->num_threads = n;
+  });
+}
+
+int goss_hip_system_get_num_threads(goss_hip_system* s, uint32_t* n)
+{
+  return guarded([&] {
+    check_sys(s);
+    *n = s->num_threads;
+  });
+}
+
+int goss_hip_system_num_nodes(goss_hip_system* s, uint32_t* n)
+{
+  return guarded([&] {
+    check_sys(s);
+    *n = s->N;
+  });
+}
+
+int goss_hip_system_set_steps_per_launch(goss_hip_system* s, uint32_t steps)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (steps == 0) throw Error(GOSS_HIP_EINVAL, "steps per launch must be positive");
+    s->steps_per_launch = steps;
+  });
+}
+
+int goss_hip_system_synchronize(goss_hip_system* s)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_stream(goss_hip_system* s, void** stream)
+{
+  return guarded([&] {
+    check_sys(s);
+    *stream = s->stream;
+  });
+}
+
+}  // extern "C"

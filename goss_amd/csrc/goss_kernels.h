@@ -1,0 +1,350 @@
+// Device kernels of the explicit solver family.
+//
+// One node per lane.  A node's states are loaded once from the structure-of-
+// arrays buffer (row i holds state i of every node, row stride `ld`, so each
+// wavefront reads 64 consecutive values of one state), kept in registers for
+// all fused outer steps of a launch, and stored once at the end.  Field
+// parameters are per-lane registers; shared parameters live in the kernel-
+// argument block and are read with scalar loads.
+//
+// Precision policies:
+//   double                 every operation in FP64 (the parity path);
+//   float, COMP = false    FP32 states and arithmetic;
+//   float, COMP = true     FP32 arithmetic; the states with linear_terms == 0
+//                          (TP06: Ca_i, Ca_SR, Ca_ss, Na_i, V, K_i) are carried
+//                          as unevaluated sums hi + lo, so that increments
+//                          below the FP32 spacing of the state accumulate.
+//
+// Time is always FP64: the outer (t, interval) pairs come from the host and
+// the sub-step clock lt += ldt is accumulated as in GRL1.cpp:71-81.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "goss_model_support.h"
+
+namespace goss_hip {
+
+enum SolverKind : int {
+  kExplicitEuler = 0,
+  kRK2 = 1,
+  kRK4 = 2,
+  kRL1 = 3,
+  kRL2 = 4,
+  kGRL1 = 5,
+  kGRL2 = 6,
+  kImplicitEuler = 7,
+  kESDIRK23a = 8,
+  kRKF32 = 9,
+  kNumSolvers = 10,
+};
+
+enum Precision : int { kFP64 = 0, kFP32 = 1, kFP32C = 2, kNumPrecisions = 3 };
+
+constexpr int kBlock = 256;
+
+// Shared (non-field) parameters, passed by value as a kernel argument.
+template <class R, int P>
+struct SharedParams {
+  R v[P > 0 ? P : 1];
+};
+
+template <class M>
+constexpr int field_slot(int k)
+{
+  for (int j = 0; j < M::num_field_parameters; ++j)
+    if (M::field_parameter_indices[j] == k) return j;
+  return -1;
+}
+
+// Parameter accessor handed to the generated model code.
+template <class M, class R>
+struct DevAcc {
+  const SharedParams<R, M::num_parameters>& sp;
+  const R* fpv;
+  template <int k>
+  __device__ GOSS_INLINE R get() const
+  {
+    constexpr int j = field_slot<M>(k);
+    if constexpr (j >= 0)
+      return fpv[j];
+    else
+      return sp.v[k];
+  }
+};
+
+// Slot of state i among the compensated states, or -1.
+template <class M>
+constexpr int comp_slot(int i)
+{
+  int c = 0;
+  for (int s = 0; s < i; ++s) c += M::linear_terms[s] == 0;
+  return M::linear_terms[i] == 0 ? c : -1;
+}
+
+template <class M>
+constexpr int num_comp()
+{
+  int c = 0;
+  for (int s = 0; s < M::num_states; ++s) c += M::linear_terms[s] == 0;
+  return c;
+}
+
+// exp(x) - 1 in FP32 without the cancellation of __expf(x) - 1 near 0:
+// Taylor polynomial up to x^7 for |x| < 0.25 (truncation below 2^-26
+// relative), hardware exp2 otherwise.
+__device__ GOSS_INLINE float expm1_f32(const float x)
+{
+  const float p = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x * (1.0f / 120.0f +
+                  x * (1.0f / 720.0f + x * (1.0f / 5040.0f)))))));
+  return fabsf(x) < 0.25f ? p : __expf(x) - 1.0f;
+}
+
+// Exact sum hi + lo += inc (Knuth's TwoSum; 6 additions).
+__device__ GOSS_INLINE void comp_add(float& hi, float& lo, const float inc)
+{
+  const float b = inc + lo;
+  const float s = hi + b;
+  const float bb = s - hi;
+  const float err = (hi - (s - bb)) + (b - bb);
+  hi = s;
+  lo = err;
+}
+
+// State vector of one node: values (hi) plus low parts for compensated
+// states.  For double / plain float the low parts are never touched and are
+// removed by the compiler.
+template <class M, class R, bool COMP>
+struct NodeState {
+  static constexpr int S = M::num_states;
+  R y[S];
+  R lo[S];
+
+  // y[i] += inc
+  __device__ GOSS_INLINE void add(const int i, const R inc)
+  {
+    if constexpr (COMP) {
+      if (M::linear_terms[i] == 0) {
+        comp_add(y[i], lo[i], inc);
+        return;
+      }
+    }
+    y[i] += inc;
+  }
+};
+
+// Increment of one Rush-Larsen style update, FP64: written as in GRL1.h:80-83
+// so that y0 + increment rounds exactly like the reference.
+__device__ GOSS_INLINE double rl_update(const double y0, const double a, const double b, const double h)
+{
+  return y0 + b / a * (exp(a * h) - 1.0);
+}
+
+// ---------------------------------------------------------------------------
+// One sub-step of each explicit scheme.  `lt` is the FP64 sub-step clock.
+// ---------------------------------------------------------------------------
+
+template <class M, int SOLVER, class R, bool COMP>
+struct Step;
+
+// ExplicitEuler.cpp:76-85
+template <class M, class R, bool COMP>
+struct Step<M, kExplicitEuler, R, COMP> {
+  template <class A>
+  __device__ GOSS_INLINE static void run(NodeState<M, R, COMP>& n, const double lt, const double ldt, const A& acc,
+                                         const R)
+  {
+    constexpr int S = M::num_states;
+    R f[S];
+    M::template eval<R>(n.y, lt, acc, f);
+    const R h = R(ldt);
+#pragma unroll
+    for (int i = 0; i < S; ++i) n.add(i, h * f[i]);
+  }
+};
+
+// RK2.cpp:73-91 (midpoint)
+template <class M, class R, bool COMP>
+struct Step<M, kRK2, R, COMP> {
+  template <class A>
+  __device__ GOSS_INLINE static void run(NodeState<M, R, COMP>& n, const double lt, const double ldt, const A& acc,
+                                         const R)
+  {
+    constexpr int S = M::num_states;
+    R k1[S], tmp[S];
+    M::template eval<R>(n.y, lt, acc, k1);
+    const R hh = R(0.5 * ldt);
+#pragma unroll
+    for (int i = 0; i < S; ++i) tmp[i] = n.y[i] + hh * k1[i];
+    M::template eval<R>(tmp, lt + 0.5 * ldt, acc, k1);
+    const R h = R(ldt);
+#pragma unroll
+    for (int i = 0; i < S; ++i) n.add(i, h * k1[i]);
+  }
+};
+
+// RK4.cpp:79-101
+template <class M, class R, bool COMP>
+struct Step<M, kRK4, R, COMP> {
+  template <class A>
+  __device__ GOSS_INLINE static void run(NodeState<M, R, COMP>& n, const double lt, const double ldt, const A& acc,
+                                         const R)
+  {
+    constexpr int S = M::num_states;
+    R k1[S], k2[S], k3[S], k4[S], tmp[S];
+    const R hh = R(0.5 * ldt);
+    const R h = R(ldt);
+    M::template eval<R>(n.y, lt, acc, k1);
+#pragma unroll
+    for (int i = 0; i < S; ++i) tmp[i] = n.y[i] + hh * k1[i];
+    M::template eval<R>(tmp, lt + 0.5 * ldt, acc, k2);
+#pragma unroll
+    for (int i = 0; i < S; ++i) tmp[i] = n.y[i] + hh * k2[i];
+    M::template eval<R>(tmp, lt + 0.5 * ldt, acc, k3);
+#pragma unroll
+    for (int i = 0; i < S; ++i) tmp[i] = n.y[i] + h * k3[i];
+    M::template eval<R>(tmp, lt + ldt, acc, k4);
+#pragma unroll
+    for (int i = 0; i < S; ++i) n.add(i, h * (k1[i] + R(2.0) * k2[i] + R(2.0) * k3[i] + k4[i]) / R(6.0));
+  }
+};
+
+// Rush-Larsen family: one stage evaluated at `at` and applied to y0 (the
+// y0 of RL1.h:68-77 / GRL1.h:72-84).  APPLY: the result goes to the node
+// state n (whose values are y0); otherwise to `out`.  GEN selects the
+// generalized form (full diagonal, |a| > delta) or the classic one
+// (linear_terms flags, only_linear evaluation).
+template <class M, class R, bool COMP, bool GEN, bool APPLY, class A>
+__device__ GOSS_INLINE void rl_stage(R* out, NodeState<M, R, COMP>& n, const R* at, const double tt, const R h,
+                                     const A& acc, const R delta)
+{
+  constexpr int S = M::num_states;
+  R lin[S], rhs[S];
+  M::template linearized_eval<R>(at, tt, acc, lin, rhs, !GEN);
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const bool use_exp = GEN ? (fabs(lin[i]) > delta) : (M::linear_terms[i] != 0);
+    const R y0 = n.y[i];
+    if constexpr (sizeof(R) == 8) {
+      // Same expression tree as the reference update.
+      const double v = use_exp ? rl_update(y0, lin[i], rhs[i], h) : y0 + rhs[i] * h;
+      if constexpr (APPLY) n.y[i] = v; else out[i] = v;
+    } else {
+      const R inc = use_exp ? rhs[i] / lin[i] * expm1_f32(lin[i] * h) : rhs[i] * h;
+      if constexpr (APPLY) n.add(i, inc); else out[i] = y0 + inc;
+    }
+  }
+}
+
+template <class M, int SOLVER, class R, bool COMP>
+struct RLStep {
+  static constexpr bool GEN = (SOLVER == kGRL1 || SOLVER == kGRL2);
+  static constexpr bool TWO = (SOLVER == kRL2 || SOLVER == kGRL2);
+  template <class A>
+  __device__ GOSS_INLINE static void run(NodeState<M, R, COMP>& n, const double lt, const double ldt, const A& acc,
+                                         const R delta)
+  {
+    constexpr int S = M::num_states;
+    if constexpr (!TWO) {
+      // RL1.cpp:73-77 / GRL1.cpp:73-81
+      rl_stage<M, R, COMP, GEN, true>(nullptr, n, n.y, lt, R(ldt), acc, delta);
+    } else {
+      // RL2.cpp:75-85 / GRL2.cpp:75-85: half step to y2, then a full step
+      // from y evaluated at y2 and at time lt.
+      R y2[S];
+      rl_stage<M, R, COMP, GEN, false>(y2, n, n.y, lt, R(ldt * 0.5), acc, delta);
+      rl_stage<M, R, COMP, GEN, true>(nullptr, n, y2, lt, R(ldt), acc, delta);
+    }
+  }
+};
+
+template <class M, class R, bool COMP>
+struct Step<M, kRL1, R, COMP> : RLStep<M, kRL1, R, COMP> {};
+template <class M, class R, bool COMP>
+struct Step<M, kRL2, R, COMP> : RLStep<M, kRL2, R, COMP> {};
+template <class M, class R, bool COMP>
+struct Step<M, kGRL1, R, COMP> : RLStep<M, kGRL1, R, COMP> {};
+template <class M, class R, bool COMP>
+struct Step<M, kGRL2, R, COMP> : RLStep<M, kGRL2, R, COMP> {};
+
+// Number of sub-steps, GRL1.cpp:65-67 (and every fixed-step solver).
+__device__ GOSS_INLINE unsigned long substeps(const double ldt_0, const double dt)
+{
+  return ldt_0 > 0 ? (unsigned long)ceil(dt / ldt_0 - 1.0E-12) : 1UL;
+}
+
+// Optional per-outer-step output of field states (ODESystemSolver::solve,
+// ODESystemSolver.cpp:81-99): row k of out holds the field states after
+// outer step k, tangled ([N][F]) or not ([F][N]).
+struct FieldOut {
+  double* out;  // nullptr: no output
+  unsigned long row_stride;  // N * F
+  int tangled;
+};
+
+// ---------------------------------------------------------------------------
+// The fused explicit kernel: `nouter` outer steps (t_k, interval_k).
+// ---------------------------------------------------------------------------
+template <class M, int SOLVER, class R, bool COMP>
+__global__ void __launch_bounds__(kBlock)
+explicit_kernel(R* __restrict__ y, R* __restrict__ ylo, const R* __restrict__ fp, const double2* __restrict__ steps,
+                const int nouter, const SharedParams<R, M::num_parameters> sp, const double ldt_0, const R delta,
+                const unsigned N, const unsigned ld, const FieldOut fo)
+{
+  constexpr int S = M::num_states;
+  constexpr int NFP = M::num_field_parameters;
+  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= N) return;
+
+  NodeState<M, R, COMP> st;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    st.y[i] = y[size_t(i) * ld + node];
+    if constexpr (COMP) {
+      const int c = comp_slot<M>(i);
+      st.lo[i] = c >= 0 ? ylo[size_t(c) * ld + node] : R(0);
+    }
+  }
+  R fpv[NFP > 0 ? NFP : 1];
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
+  const DevAcc<M, R> acc{sp, fpv};
+
+#pragma unroll 1
+  for (int k = 0; k < nouter; ++k) {
+    const double t = steps[k].x;
+    const double dt = steps[k].y;
+    const unsigned long nsteps = substeps(ldt_0, dt);
+    const double ldt = dt / nsteps;
+    double lt = t;
+#pragma unroll 1
+    for (unsigned long s = 0; s < nsteps; ++s) {
+      Step<M, SOLVER, R, COMP>::run(st, lt, ldt, acc, delta);
+      lt += ldt;
+    }
+    if (fo.out) {
+      constexpr int F = M::num_field_states;
+#pragma unroll
+      for (int i = 0; i < F; ++i) {
+        const int si = M::field_state_indices[i];
+        double v = double(st.y[si]);
+        if constexpr (COMP) v += double(st.lo[si]);
+        const size_t idx = fo.tangled ? size_t(node) * F + i : size_t(N) * i + node;
+        fo.out[size_t(k) * fo.row_stride + idx] = v;
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    y[size_t(i) * ld + node] = st.y[i];
+    if constexpr (COMP) {
+      const int c = comp_slot<M>(i);
+      if (c >= 0) ylo[size_t(c) * ld + node] = st.lo[i];
+    }
+  }
+}
+
+}  // namespace goss_hip

@@ -1,0 +1,48 @@
+// Registry of the model variants compiled into libgoss_hip.so.
+//
+// Each variant is compiled in two translation units (variant_kernels.hip with
+// GOSS_VARIANT=<name> and GOSS_PREC_F64 or GOSS_PREC_F32); each fills its
+// part of the variant's launcher table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "goss_kernels.h"
+
+namespace goss_hip {
+
+struct ExplicitArgs {
+  void* y;                 // states, SoA [S][ld] of the precision's type
+  void* ylo;               // FP32C: low parts [NCOMP][ld]
+  const void* fp;          // field parameters [NFP][ld]
+  const double2* steps;    // device (t, interval) pairs
+  int nouter;
+  const double* shared;    // host: the P model parameter values
+  double ldt_0;
+  double delta;
+  unsigned N, ld;
+  FieldOut fo;
+  hipStream_t stream;
+};
+
+typedef hipError_t (*ExplicitLaunch)(const ExplicitArgs&);
+
+struct VariantInfo {
+  const char* name;
+  int S, P, NFS, NFP, NCOMP;
+  const int* fsi;
+  const int* fpi;
+  const double* defaults;
+  const double* ic;
+  const unsigned char* linear_terms;
+  const char* const* state_names;
+  const char* const* parameter_names;
+  const char* const* field_state_names;
+  const char* const* field_parameter_names;
+  ExplicitLaunch explicit_launch[kNumSolvers][kNumPrecisions];
+};
+
+// All variants, in registration order.
+VariantInfo* variants(int* count);
+
+}  // namespace goss_hip

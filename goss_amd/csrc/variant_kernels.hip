@@ -1,0 +1,105 @@
+// Kernel instantiations of one model variant in one precision family.
+//
+// Compiled once per (variant, family) with
+//   -DGOSS_VARIANT=<name> and -DGOSS_PREC_F64 (FP64, -ffp-contract=off)
+//                          or -DGOSS_PREC_F32 (FP32 and FP32C).
+// Exports goss_register_<name>_<family>(VariantInfo&), called by the registry.
+#include "goss_registry.h"
+
+#define GOSS_STR2(x) #x
+#define GOSS_STR(x) GOSS_STR2(x)
+#define GOSS_CAT3_(a, b, c) a##b##c
+#define GOSS_CAT3(a, b, c) GOSS_CAT3_(a, b, c)
+
+#include GOSS_STR(generated/GOSS_VARIANT.h)
+
+namespace goss_hip {
+namespace {
+
+namespace NS = goss_models::GOSS_VARIANT;
+
+// The generated namespace wrapped as a type for the kernel templates.
+struct M {
+  static constexpr int num_states = NS::num_states;
+  static constexpr int num_parameters = NS::num_parameters;
+  static constexpr int num_field_states = NS::num_field_states;
+  static constexpr int num_field_parameters = NS::num_field_parameters;
+  static constexpr const int* field_state_indices = NS::field_state_indices;
+  static constexpr const int* field_parameter_indices = NS::field_parameter_indices;
+  static constexpr const unsigned char* linear_terms = NS::linear_terms;
+
+  template <typename R, typename A, typename T>
+  __device__ GOSS_INLINE static void eval(const R* y, const T t, const A& a, R* out)
+  {
+    NS::eval<R>(y, t, a, out);
+  }
+  template <typename R, typename A, typename T>
+  __device__ GOSS_INLINE static void linearized_eval(const R* y, const T t, const A& a, R* lin, R* rhs, bool ol)
+  {
+    NS::linearized_eval<R>(y, t, a, lin, rhs, ol);
+  }
+};
+
+template <int SOLVER, class R, bool COMP>
+hipError_t launch_explicit(const ExplicitArgs& a)
+{
+  SharedParams<R, M::num_parameters> sp;
+  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = R(a.shared[k]);
+  const dim3 grid((a.N + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((explicit_kernel<M, SOLVER, R, COMP>), grid, dim3(kBlock), 0, a.stream,
+                     static_cast<R*>(a.y), static_cast<R*>(a.ylo), static_cast<const R*>(a.fp), a.steps, a.nouter,
+                     sp, a.ldt_0, R(a.delta), a.N, a.ld, a.fo);
+  return hipGetLastError();
+}
+
+template <class R, bool COMP>
+void register_family(VariantInfo& v, int prec)
+{
+  v.explicit_launch[kExplicitEuler][prec] = &launch_explicit<kExplicitEuler, R, COMP>;
+  v.explicit_launch[kRK2][prec] = &launch_explicit<kRK2, R, COMP>;
+  v.explicit_launch[kRK4][prec] = &launch_explicit<kRK4, R, COMP>;
+  v.explicit_launch[kRL1][prec] = &launch_explicit<kRL1, R, COMP>;
+  v.explicit_launch[kRL2][prec] = &launch_explicit<kRL2, R, COMP>;
+  v.explicit_launch[kGRL1][prec] = &launch_explicit<kGRL1, R, COMP>;
+  v.explicit_launch[kGRL2][prec] = &launch_explicit<kGRL2, R, COMP>;
+}
+
+void fill_info(VariantInfo& v)
+{
+  v.name = GOSS_STR(GOSS_VARIANT);
+  v.S = NS::num_states;
+  v.P = NS::num_parameters;
+  v.NFS = NS::num_field_states;
+  v.NFP = NS::num_field_parameters;
+  v.NCOMP = num_comp<M>();
+  v.fsi = NS::field_state_indices;
+  v.fpi = NS::field_parameter_indices;
+  v.defaults = NS::default_parameters;
+  v.ic = NS::initial_conditions;
+  v.linear_terms = NS::linear_terms;
+  v.state_names = NS::state_names;
+  v.parameter_names = NS::parameter_names;
+  v.field_state_names = NS::field_state_names;
+  v.field_parameter_names = NS::field_parameter_names;
+}
+
+}  // namespace
+
+#if defined(GOSS_PREC_F64)
+void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
+{
+  fill_info(v);
+  register_family<double, false>(v, kFP64);
+}
+#elif defined(GOSS_PREC_F32)
+void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
+{
+  fill_info(v);
+  register_family<float, false>(v, kFP32);
+  register_family<float, true>(v, kFP32C);
+}
+#else
+#error "define GOSS_PREC_F64 or GOSS_PREC_F32"
+#endif
+
+}  // namespace goss_hip

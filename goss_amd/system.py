@@ -1,0 +1,170 @@
+"""Low-level Python handle of one device system (``goss_hip_system``)."""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from ._lib import PRECISIONS, SOLVERS, GossHipError, ModelInfo, SolverParams, check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def model_names() -> List[str]:
+    L = lib()
+    return [L.goss_hip_model_name(i).decode() for i in range(L.goss_hip_model_count())]
+
+
+class ModelVariant:
+    """Metadata of a compiled model variant (sizes, names, defaults)."""
+
+    def __init__(self, name: str):
+        info = ModelInfo()
+        check(lib().goss_hip_model_info_get(name.encode(), ctypes.byref(info)))
+        self.name = name
+        self.num_states = info.num_states
+        self.num_parameters = info.num_parameters
+        self.num_field_states = info.num_field_states
+        self.num_field_parameters = info.num_field_parameters
+        self.state_names = [info.state_names[i].decode() for i in range(self.num_states)]
+        self.parameter_names = [info.parameter_names[i].decode() for i in range(self.num_parameters)]
+        self.field_state_names = [info.field_state_names[i].decode() for i in range(self.num_field_states)]
+        self.field_parameter_names = [info.field_parameter_names[i].decode()
+                                      for i in range(self.num_field_parameters)]
+        self.field_state_indices = [info.field_state_indices[i] for i in range(self.num_field_states)]
+        self.field_parameter_indices = [info.field_parameter_indices[i] for i in range(self.num_field_parameters)]
+        self.default_parameters = np.array([info.default_parameters[i] for i in range(self.num_parameters)])
+        self.initial_conditions = np.array([info.initial_conditions[i] for i in range(self.num_states)])
+        self.linear_terms = [int(info.linear_terms[i]) for i in range(self.num_states)]
+
+
+def solver_params(solver: str, **overrides) -> SolverParams:
+    sp = SolverParams()
+    check(lib().goss_hip_solver_params_default(SOLVERS[solver], ctypes.byref(sp)))
+    for k, v in overrides.items():
+        if v is None:
+            continue
+        if not hasattr(sp, k):
+            raise KeyError(f"unknown solver parameter {k!r}")
+        setattr(sp, k, type(getattr(sp, k))(v))
+    return sp
+
+
+class DeviceSystem:
+    """One ODESystemSolver on the GPU (wraps goss_hip_system_*)."""
+
+    def __init__(self, variant: str, solver: str, num_nodes: int, precision: str = "fp64",
+                 params: Optional[Dict[str, float]] = None):
+        self.model = ModelVariant(variant)
+        self.solver = solver
+        self.precision = precision
+        self.num_nodes = int(num_nodes)
+        self._sp = solver_params(solver, **(params or {}))
+        self._h = ctypes.c_void_p()
+        check(lib().goss_hip_system_create(variant.encode(), SOLVERS[solver], ctypes.byref(self._sp),
+                                           PRECISIONS[precision], self.num_nodes, ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().goss_hip_system_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping -----------------------------------------------------------
+    def forward(self, t: float, interval: float) -> None:
+        check(lib().goss_hip_system_forward(self._h, float(t), float(interval)))
+
+    def forward_times(self, t) -> None:
+        t = _f64(t)
+        check(lib().goss_hip_system_forward_times(self._h, _ptr(t), len(t)))
+
+    def solve(self, t, out: np.ndarray, tangled: bool = True) -> None:
+        t = _f64(t)
+        if not (out.flags.c_contiguous and out.dtype == np.float64):
+            raise ValueError("out must be a C-contiguous float64 array")
+        check(lib().goss_hip_system_solve(self._h, _ptr(out), _ptr(t), len(t), int(tangled)))
+
+    def synchronize(self) -> None:
+        check(lib().goss_hip_system_synchronize(self._h))
+
+    def set_steps_per_launch(self, n: int) -> None:
+        check(lib().goss_hip_system_set_steps_per_launch(self._h, int(n)))
+
+    # -- data ----------------------------------------------------------------
+    def states(self) -> np.ndarray:
+        out = np.zeros((self.num_nodes, self.model.num_states))
+        check(lib().goss_hip_system_get_states(self._h, _ptr(out)))
+        return out
+
+    def set_states(self, states) -> None:
+        a = _f64(states).reshape(self.num_nodes, self.model.num_states)
+        check(lib().goss_hip_system_set_states(self._h, _ptr(a)))
+
+    def get_field_states(self, out: np.ndarray, tangled: bool = True) -> None:
+        check(lib().goss_hip_system_get_field_states(self._h, out.ctypes.data, int(tangled)))
+
+    def set_field_states(self, a: np.ndarray, tangled: bool = True) -> None:
+        a = _f64(a)
+        check(lib().goss_hip_system_set_field_states(self._h, a.ctypes.data, int(tangled)))
+
+    def get_field_state_components(self, out: np.ndarray, components: Sequence[int], tangled: bool = True) -> None:
+        c = np.ascontiguousarray(components, dtype=np.uint32)
+        check(lib().goss_hip_system_get_field_state_components(
+            self._h, _ptr(out), len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(tangled)))
+
+    def set_field_state_components(self, a: np.ndarray, components: Sequence[int], tangled: bool = True) -> None:
+        a = _f64(a)
+        c = np.ascontiguousarray(components, dtype=np.uint32)
+        check(lib().goss_hip_system_set_field_state_components(
+            self._h, _ptr(a), len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(tangled)))
+
+    def get_field_parameters(self, out: np.ndarray, tangled: bool = True) -> None:
+        check(lib().goss_hip_system_get_field_parameters(self._h, out.ctypes.data, int(tangled)))
+
+    def set_field_parameters(self, a: np.ndarray, tangled: bool = True) -> None:
+        a = _f64(a)
+        check(lib().goss_hip_system_set_field_parameters(self._h, a.ctypes.data, int(tangled)))
+
+    def get_field_states_device(self, dev_ptr: int, tangled: bool = True) -> None:
+        check(lib().goss_hip_system_get_field_states_device(self._h, ctypes.c_void_p(dev_ptr), int(tangled)))
+
+    def set_field_states_device(self, dev_ptr: int, tangled: bool = True) -> None:
+        check(lib().goss_hip_system_set_field_states_device(self._h, ctypes.c_void_p(dev_ptr), int(tangled)))
+
+    def reset_default(self) -> None:
+        check(lib().goss_hip_system_reset_default(self._h))
+
+    def set_parameter(self, name: str, value: float) -> None:
+        check(lib().goss_hip_system_set_parameter(self._h, name.encode(), float(value)))
+
+    def get_parameter(self, name: str) -> float:
+        v = ctypes.c_double()
+        check(lib().goss_hip_system_get_parameter(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_num_threads(self, n: int) -> None:
+        check(lib().goss_hip_system_set_num_threads(self._h, int(n)))
+
+    def get_num_threads(self) -> int:
+        v = ctypes.c_uint32()
+        check(lib().goss_hip_system_get_num_threads(self._h, ctypes.byref(v)))
+        return v.value
+
+    def stream(self) -> int:
+        v = ctypes.c_void_p()
+        check(lib().goss_hip_system_stream(self._h, ctypes.byref(v)))
+        return v.value or 0

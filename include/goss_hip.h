@@ -1,0 +1,177 @@
+/* goss_hip.h -- C ABI of the MI355X implementation of goss's
+ * ODESystemSolver::forward hot path.
+ *
+ * One goss_hip_system advances one generated cell-model variant over many
+ * nodes on one GPU.  Every entry point replaces a member of the reference's
+ * goss::ODESystemSolver (cpp/goss/ODESystemSolver.h:38-186) or of the solver /
+ * model objects it owns; the citation is given beside each declaration.
+ *
+ * Conventions
+ *  - All functions return 0 on success and a negative GOSS_HIP_E* code on
+ *    failure; goss_hip_last_error() returns the message of the last failure
+ *    on the calling thread.  Where the reference throws std::runtime_error
+ *    the message text is the reference's.
+ *  - Host buffers are caller-owned, host-resident `double*` of the sizes the
+ *    reference uses (num_nodes * num_field_states, ...); as in the reference,
+ *    no sizes are passed.  The *_device variants take device pointers.
+ *  - "tangled" storage: element (node, i) at node * F + i; untangled at
+ *    num_nodes * i + node (ODESystemSolver.h:244).
+ *  - A system owns one HIP stream; calls on one system are not re-entrant.
+ */
+#ifndef GOSS_HIP_H
+#define GOSS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GOSS_HIP_OK 0
+#define GOSS_HIP_EINVAL -1   /* bad argument (unknown name, index, size) */
+#define GOSS_HIP_EDEVICE -2  /* HIP runtime error */
+#define GOSS_HIP_ESOLVER -3  /* numerical failure reported by a solver */
+#define GOSS_HIP_EUNSUPPORTED -4
+
+/* Solvers (python/goss/solvers.py:423-436 names). */
+typedef enum goss_hip_solver {
+  GOSS_HIP_EXPLICIT_EULER = 0, /* ExplicitEuler.cpp:64-87 */
+  GOSS_HIP_RK2 = 1,            /* RK2.cpp:62-92 */
+  GOSS_HIP_RK4 = 2,            /* RK4.cpp:68-102 */
+  GOSS_HIP_RL1 = 3,            /* RL1.cpp:59-79 */
+  GOSS_HIP_RL2 = 4,            /* RL2.cpp:64-87 */
+  GOSS_HIP_GRL1 = 5,           /* GRL1.cpp:61-82 */
+  GOSS_HIP_GRL2 = 6,           /* GRL2.cpp:63-86 */
+  GOSS_HIP_IMPLICIT_EULER = 7, /* ImplicitEuler.cpp:57-152 */
+  GOSS_HIP_ESDIRK23A = 8,      /* ESDIRK23a.cpp:280-316 */
+  GOSS_HIP_RKF32 = 9           /* RKF32.cpp:105-222 */
+} goss_hip_solver;
+
+/* Arithmetic of the per-node work.  FP64 is the reference's arithmetic.
+ * FP32C computes in FP32 and carries the states whose derivative is not
+ * affine in themselves (linear_terms == 0) as FP32 hi + lo pairs. */
+typedef enum goss_hip_precision {
+  GOSS_HIP_FP64 = 0,
+  GOSS_HIP_FP32 = 1,
+  GOSS_HIP_FP32C = 2
+} goss_hip_precision;
+
+/* Public solver tunables: ODESolver::_ldt (ODESolver.h:119-128),
+ * GRL1::delta (GRL1.h:39), ImplicitODESolver (ImplicitODESolver.h:36-41),
+ * ImplicitEuler (ImplicitEuler.h:36-37), ESDIRK23a (ESDIRK23a.h:37-38),
+ * AdaptiveExplicitSolver tolerances (AdaptiveExplicitSolver.cpp:52-53). */
+typedef struct goss_hip_solver_params {
+  double internal_time_step; /* <= 0: one step per forward interval */
+  double delta;
+  double eta_0;
+  double kappa;
+  double relative_tolerance;
+  double max_relative_previous_residual;
+  double min_dt;
+  double atol;
+  double rtol;
+  int max_iterations;
+  int always_recompute_jacobian;
+  int num_refinements_without_always_recomputing_jacobian;
+} goss_hip_solver_params;
+
+/* Sizes and names of a generated model variant
+ * (ParameterizedODE.h:56-156 getters). */
+typedef struct goss_hip_model_info {
+  int num_states;
+  int num_parameters;
+  int num_field_states;
+  int num_field_parameters;
+  const char* const* state_names;
+  const char* const* parameter_names;
+  const char* const* field_state_names;
+  const char* const* field_parameter_names;
+  const int* field_state_indices;   /* get_field_state_indices() */
+  const int* field_parameter_indices;
+  const double* default_parameters; /* generated defaults */
+  const double* initial_conditions; /* ODE::get_ic */
+  const unsigned char* linear_terms;/* ODE::linear_term */
+} goss_hip_model_info;
+
+typedef struct goss_hip_system goss_hip_system;
+
+const char* goss_hip_last_error(void);
+
+/* Number of GPUs visible (0 on a host without one). */
+int goss_hip_device_count(int* count);
+
+/* Model variants compiled into this library.  Replaces the model plugin
+ * factory create_<Name>() (python/goss/codegeneration.py:109-112) and the
+ * metadata getters of ParameterizedODE (ParameterizedODE.h:56-156). */
+int goss_hip_model_count(void);
+const char* goss_hip_model_name(int index);
+int goss_hip_model_info_get(const char* variant, goss_hip_model_info* out);
+
+/* Default tunables of a solver (GRL1.h:39, ImplicitODESolver.h:36-41, ...). */
+int goss_hip_solver_params_default(int solver, goss_hip_solver_params* out);
+/* Whether a solver is adaptive (ODESolver::is_adaptive, ODESolver.h:131-134). */
+int goss_hip_solver_is_adaptive(int solver);
+
+/* ODESystemSolver(num_nodes, solver, ode) (ODESystemSolver.cpp:45-59):
+ * attaches the model and resets states and field parameters to defaults. */
+int goss_hip_system_create(const char* variant, int solver, const goss_hip_solver_params* params, int precision,
+                           uint32_t num_nodes, goss_hip_system** out);
+int goss_hip_system_destroy(goss_hip_system* sys);
+
+/* ODESystemSolver::forward(t, interval) (ODESystemSolver.cpp:67-79). */
+int goss_hip_system_forward(goss_hip_system* sys, double t, double interval);
+/* forward(t[i], t[i+1]-t[i]) for i = 0..n-2 fused into few launches: the
+ * loop of ODESystemSolver::solve (ODESystemSolver.cpp:81-99) without the
+ * per-step copy-out. */
+int goss_hip_system_forward_times(goss_hip_system* sys, const double* t, uint64_t n);
+/* ODESystemSolver::solve(field_states, t, n, tangled) (ODESystemSolver.cpp:81-99):
+ * rows 1..n-1 of field_states[n][...] receive the field states after each
+ * step; row 0 is left to the caller (python/goss/systemsolver.py:130-132). */
+int goss_hip_system_solve(goss_hip_system* sys, double* field_states, const double* t, uint64_t n, int tangled);
+
+/* get/set_field_states (ODESystemSolver.cpp:134-160). */
+int goss_hip_system_get_field_states(goss_hip_system* sys, double* out, int tangled);
+int goss_hip_system_set_field_states(goss_hip_system* sys, const double* in, int tangled);
+/* get/set_field_state_components (ODESystemSolver.cpp:101-132). */
+int goss_hip_system_get_field_state_components(goss_hip_system* sys, double* out, uint32_t num_components,
+                                               const uint32_t* components, int tangled);
+int goss_hip_system_set_field_state_components(goss_hip_system* sys, const double* in, uint32_t num_components,
+                                               const uint32_t* components, int tangled);
+/* get/set_field_parameters (ODESystemSolver.cpp:162-190). */
+int goss_hip_system_get_field_parameters(goss_hip_system* sys, double* out, int tangled);
+int goss_hip_system_set_field_parameters(goss_hip_system* sys, const double* in, int tangled);
+/* Device-pointer variants of the field-state accessors (no PCIe copy). */
+int goss_hip_system_get_field_states_device(goss_hip_system* sys, double* dev_out, int tangled);
+int goss_hip_system_set_field_states_device(goss_hip_system* sys, const double* dev_in, int tangled);
+
+/* states(): all states, node-major [num_nodes][num_states] as in
+ * ODESystemSolver::_states (ODESystemSolver.h:74-96), copied out / in. */
+int goss_hip_system_get_states(goss_hip_system* sys, double* out);
+int goss_hip_system_set_states(goss_hip_system* sys, const double* in);
+
+/* reset_default (ODESystemSolver.cpp:192-216). */
+int goss_hip_system_reset_default(goss_hip_system* sys);
+
+/* ParameterizedODE::set_parameter / get_parameter (ParameterizedODE.h:110-120):
+ * a shared model parameter.  Unknown names fail with the reference's message. */
+int goss_hip_system_set_parameter(goss_hip_system* sys, const char* name, double value);
+int goss_hip_system_get_parameter(goss_hip_system* sys, const char* name, double* value);
+
+/* set_num_threads / get_num_threads (ODESystemSolver.cpp:218-250): recorded
+ * and returned; the device path does not use host threads. */
+int goss_hip_system_set_num_threads(goss_hip_system* sys, uint32_t num_threads);
+int goss_hip_system_get_num_threads(goss_hip_system* sys, uint32_t* num_threads);
+
+int goss_hip_system_num_nodes(goss_hip_system* sys, uint32_t* num_nodes);
+/* Outer steps fused into one kernel launch by forward_times / solve. */
+int goss_hip_system_set_steps_per_launch(goss_hip_system* sys, uint32_t steps);
+/* Wait for all work of the system. */
+int goss_hip_system_synchronize(goss_hip_system* sys);
+/* The system's HIP stream (hipStream_t) for callers that order their own work. */
+int goss_hip_system_stream(goss_hip_system* sys, void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GOSS_HIP_H */

@@ -21,6 +21,7 @@
 #include "fhn.h"
 #include "beeler_reuter_1977.h"
 #include "oscilator.h"
+#include "linear_decay.h"
 
 namespace {
 
@@ -66,7 +67,7 @@ const VariantOps kVariants[] = {
     GOSS_ORACLE_VARIANT(tp06_m_fp6),     GOSS_ORACLE_VARIANT(tp06_epi),
     GOSS_ORACLE_VARIANT(fitzhughnagumo), GOSS_ORACLE_VARIANT(lorentz),
     GOSS_ORACLE_VARIANT(fhn),            GOSS_ORACLE_VARIANT(beeler_reuter_1977),
-    GOSS_ORACLE_VARIANT(oscilator),
+    GOSS_ORACLE_VARIANT(oscilator),      GOSS_ORACLE_VARIANT(linear_decay),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 

@@ -33,6 +33,7 @@
 #include "fhn_plugin.h"
 #include "beeler_reuter_1977_plugin.h"
 #include "oscilator_plugin.h"
+#include "linear_decay_plugin.h"
 
 using namespace goss;
 
@@ -51,6 +52,7 @@ std::shared_ptr<ParameterizedODE> make_model(const std::string& v)
   if (v == "fhn") return std::make_shared<Fhn>();
   if (v == "beeler_reuter_1977") return std::make_shared<Beeler_reuter_1977>();
   if (v == "oscilator") return std::make_shared<Oscilator>();
+  if (v == "linear_decay") return std::make_shared<Linear_decay>();
   throw std::runtime_error("unknown model variant " + v);
 }
 
