@@ -192,7 +192,21 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, default=250)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-checks", action="store_true", help="skip accuracy/parity passes")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run one device-to-device copy of 256 MiB (read 256 MiB + write 256 MiB), the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the ODE system path shards by node ranges with no "
