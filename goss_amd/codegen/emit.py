@@ -26,6 +26,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import sympy as sp
 
 from .analysis import Jacobian, Linearized, census_body, census_state_ops, jacobian, linearize, needed_names
+from .hoist import ConstantTable, hoist_statements
 from .ode_model import ODEModel, TIME
 from .printer import CPrinter, assignments, c_name, fmt_double
 
@@ -60,22 +61,6 @@ class Variant:
         return n[:1].upper() + n[1:]
 
 
-def _bind_states(model: ODEModel, used: set, arr: str, ind: str) -> List[str]:
-    out = []
-    for i, s in enumerate(model.states):
-        if s.name in used:
-            out.append(f"{ind}const real {c_name(s.name)} = {arr}[{i}];")
-    return out
-
-
-def _bind_params(model: ODEModel, used: set, ind: str) -> List[str]:
-    out = []
-    for k, p in enumerate(model.parameters):
-        if p.name in used:
-            out.append(f"{ind}const real {c_name(p.name)} = pa.template get<{k}>();")
-    return out
-
-
 def _used_symbols(exprs) -> set:
     u = set()
     for e in exprs:
@@ -89,68 +74,138 @@ class Emitted:
     census: Dict[str, Dict[str, int]]
     linear_terms: List[int]
     jac_nnz: int
+    num_constants: int
 
 
-def time_only_names(model: ODEModel) -> List[str]:
-    """Intermediates that depend on time (and parameters) but on no state.
+def _closure(stmts: Dict[str, sp.Expr], roots: Sequence[sp.Expr]) -> set:
+    need: set = set()
+    stack = [s.name for r in roots for s in r.free_symbols]
+    while stack:
+        n = stack.pop()
+        if n in need or n not in stmts:
+            continue
+        need.add(n)
+        stack.extend(s.name for s in stmts[n].free_symbols)
+    return need
 
-    They are evaluated in the type of ``time`` (``treal``, double on every
-    target) so that a single-precision kernel sees the same discontinuities
-    in time (stimulus on/off) as the double-precision oracle."""
-    state_set = set(model.state_names)
-    dep_state: Dict[str, bool] = {}
-    dep_time: Dict[str, bool] = {}
-    out = []
-    for n, e in model.body:
-        ds = dt = False
+
+def _time_only(stmts: Sequence[Tuple[str, sp.Expr]], state_names: Sequence[str]) -> set:
+    """Statements that depend on time (and parameters/constants) but on no
+    state.  They are evaluated in the type of ``time`` (``treal``, FP64 on
+    every target) so that single-precision kernels see the same time
+    discontinuities (stimulus on/off) as the FP64 path."""
+    state_set = set(state_names)
+    ds: Dict[str, bool] = {}
+    dt: Dict[str, bool] = {}
+    out = set()
+    for n, e in stmts:
+        a = b = False
         for s in e.free_symbols:
             if s.name in state_set:
-                ds = True
+                a = True
             elif s == TIME:
-                dt = True
-            elif s.name in dep_state:
-                ds = ds or dep_state[s.name]
-                dt = dt or dep_time[s.name]
-        dep_state[n], dep_time[n] = ds, dt
-        if dt and not ds and n not in model.derivatives.values():
-            out.append(n)
+                b = True
+            elif s.name in ds:
+                a = a or ds[s.name]
+                b = b or dt[s.name]
+        ds[n], dt[n] = a, b
+        if b and not a:
+            out.add(n)
     return out
 
 
-class _Printers:
-    """The main printer (type ``real``) and the printer of time-only
-    statements (type ``treal``)."""
+def time_only_names(model: ODEModel) -> List[str]:
+    """Intermediates of the (unhoisted) model that depend on time but on no
+    state."""
+    t = _time_only(model.body, model.state_names)
+    return [n for n, _ in model.body if n in t and n not in model.derivatives.values()]
 
-    def __init__(self, model: ODEModel):
-        self.time_only = set(time_only_names(model))
+
+class _Gen:
+    """Prints the hoisted statement lists of one variant."""
+
+    def __init__(self, var: Variant, table: ConstantTable, stmts: Sequence[Tuple[str, sp.Expr]]):
+        self.var = var
+        self.model = var.model
+        self.table = table
+        self.pidx = {p.name: k for k, p in enumerate(self.model.parameters)}
+        self.cidx = {sym.name: k for k, (sym, _) in enumerate(table.items)}
+        self.time_only = _time_only(stmts, self.model.state_names)
         main_map = {n: f"real({c_name(n)})" for n in self.time_only}
         main_map["time"] = "real(time)"
         self.main = CPrinter(main_map)
-        tmap = {p.name: f"treal({c_name(p.name)})" for p in model.parameters}
+        tmap = {p: f"pa.template getd<{k}>()" for p, k in self.pidx.items()}
+        tmap.update({c: f"pa.template cstd<{k}>()" for c, k in self.cidx.items()})
         self.time = CPrinter(tmap, lit_type="treal")
 
-    def lines(self, model: ODEModel, names: Sequence[str], ind: str) -> List[str]:
-        bd = model.body_dict()
+    def bind(self, used: set, ind: str) -> List[str]:
         out = []
-        for n in names:
-            if n in self.time_only:
-                out += assignments(self.time, [(n, bd[n])], ind, decl="const treal ")
-            else:
-                out += assignments(self.main, [(n, bd[n])], ind)
+        for i, s in enumerate(self.model.states):
+            if s.name in used:
+                out.append(f"{ind}const real {c_name(s.name)} = states[{i}];")
+        for p, k in self.pidx.items():
+            if p in used:
+                out.append(f"{ind}const real {c_name(p)} = pa.template get<{k}>();")
+        for c, k in self.cidx.items():
+            if c in used:
+                out.append(f"{ind}const real {c} = pa.template cst<{k}>();")
         return out
+
+    def lines(self, stmts: Sequence[Tuple[str, sp.Expr]], ind: str) -> List[str]:
+        out = []
+        for n, e in stmts:
+            if n in self.time_only:
+                out += assignments(self.time, [(n, e)], ind, decl="const treal ")
+            else:
+                out += assignments(self.main, [(n, e)], ind)
+        return out
+
+    def used_main(self, stmts, outputs) -> set:
+        """Symbols referenced from `real` context (time-only statements read
+        parameters and constants through getd/cstd)."""
+        u = set()
+        for n, e in stmts:
+            if n not in self.time_only:
+                u |= {s.name for s in e.free_symbols}
+        for e in outputs:
+            u |= {s.name for s in e.free_symbols}
+        return u
 
 
 def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     model = var.model
-    prs = _Printers(model)
-    pr = prs.main
     lin = linearize(model)
+    jac = jacobian(model) if with_jacobian else None
     S, P = model.num_states, model.num_parameters
     ind = "  "
-    rhs_syms = [sp.Symbol(model.derivatives[s], real=True) for s in model.state_names]
-    rhs_names = needed_names(model, rhs_syms)
-    bd = model.body_dict()
-    rhs_exprs = [bd[n] for n in rhs_names]
+    shared = {p.name for p in model.parameters} - set(var.field_parameters)
+
+    # -- hoisting (one constant table for every function of the variant) ----
+    table = ConstantTable(shared)
+    hbody = hoist_statements(model.body, table)
+    hbd = dict(hbody)
+
+    def out_expr(name: str) -> sp.Expr:
+        return hbd[name] if name in hbd else table.alias[name]
+
+    rhs_out = [out_expr(model.derivatives[s]) for s in model.state_names]
+    n_first = lin.n_linear_stmts
+    hlin_first = hoist_statements(lin.lin_stmts[:n_first], table)
+    hlin_second = hoist_statements(lin.lin_stmts[n_first:], table)
+    hlinear = [table.rewrite(e) for e in lin.linear]
+    if jac is not None:
+        hjac_stmts = hoist_statements(jac.stmts, table)
+        hjac = [[table.rewrite(e) for e in row] for row in jac.entries]
+    derivs = set(model.derivatives.values())
+
+    all_stmts = list(hbody) + hlin_first + hlin_second + (hjac_stmts if jac is not None else [])
+    g = _Gen(var, table, all_stmts)
+    pr = g.main
+    body_inter = [(n, e) for n, e in hbody if n not in derivs]
+
+    def select(stmts, roots):
+        need = _closure(dict(stmts), roots)
+        return [(n, e) for n, e in stmts if n in need]
 
     L: List[str] = []
     L.append(f"// Generated by goss_amd.codegen from {model.name}.ode -- do not edit.")
@@ -162,6 +217,7 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("")
     L.append(f"constexpr int num_states = {S};")
     L.append(f"constexpr int num_parameters = {P};")
+    L.append(f"constexpr int num_constants = {len(table.items)};")
     L.append(f"constexpr int num_field_states = {len(var.field_states)};")
     L.append(f"constexpr int num_field_parameters = {len(var.field_parameters)};")
     fsi = ", ".join(str(i) for i in var.field_state_indices) or "-1"
@@ -179,8 +235,26 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("constexpr unsigned char linear_terms[] = {" + ", ".join(str(f) for f in lin.linear_terms) + "};")
     L.append("")
 
+    # -- constants ------------------------------------------------------------
+    cprinter = CPrinter({c: f"c[{k}]" for c, k in g.cidx.items()})
+    cused = _used_symbols([e for _, e in table.items])
+    L.append("// Subexpressions of shared (non-field) parameters only, evaluated once per launch.")
+    L.append("// c[k] may use c[j] for j < k.  Called with real = double on the host.")
+    L.append("template <typename real, typename PA>")
+    L.append("GOSS_HD GOSS_INLINE void compute_constants(const PA& pa, real* __restrict__ c)")
+    L.append("{")
+    L.append(f"{ind}(void)pa; (void)c;")
+    for p, k in g.pidx.items():
+        if p in cused:
+            L.append(f"{ind}const real {c_name(p)} = pa.template getd<{k}>();")
+    for k, (sym, e) in enumerate(table.items):
+        L.append(f"{ind}c[{k}] = {cprinter.p(e)};")
+    L.append("}")
+    L.append("")
+
     # -- eval ---------------------------------------------------------------
-    used = _used_symbols(rhs_exprs + [bd[model.derivatives[s]] for s in model.state_names])
+    eval_stmts = select(body_inter, rhs_out)
+    used = g.used_main(eval_stmts, rhs_out)
     L.append("// Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on")
     L.append("// no state are computed in the type of `time` (treal).")
     L.append("template <typename real, typename PA, typename treal>")
@@ -188,21 +262,17 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("                              real* __restrict__ values)")
     L.append("{")
     L.append(f"{ind}(void)time; (void)pa;")
-    L += _bind_states(model, used, "states", ind)
-    L += _bind_params(model, used, ind)
-    L += prs.lines(model, [n for n in rhs_names if n not in model.derivatives.values()], ind)
-    for i, s in enumerate(model.state_names):
-        L.append(f"{ind}values[{i}] = {pr.p(bd[model.derivatives[s]])};")
+    L += g.bind(used, ind)
+    L += g.lines(eval_stmts, ind)
+    for i in range(S):
+        L.append(f"{ind}values[{i}] = {pr.p(rhs_out[i])};")
     L.append("}")
     L.append("")
 
     # -- linearized_eval ----------------------------------------------------
-    lin_first = lin.lin_stmts[: lin.n_linear_stmts]
-    lin_second = lin.lin_stmts[lin.n_linear_stmts:]
-    used_l = used | _used_symbols([e for _, e in lin.lin_stmts] + lin.linear)
-    # Intermediates referenced by derivative expressions but not by the RHS.
-    extra = needed_names(model, [e for _, e in lin.lin_stmts] + lin.linear)
-    extra = [n for n in extra if n not in rhs_names]
+    lin_roots = [e for _, e in hlin_first + hlin_second] + hlinear
+    lin_inter = select(body_inter, rhs_out + lin_roots)
+    lused = g.used_main(lin_inter + hlin_first + hlin_second, rhs_out + hlinear)
     L.append("// Linearized right-hand side: rhs[i] as in eval and linear[i] = d rhs[i] / d states[i].")
     L.append("// Entries with linear_terms[i] == 1 are written before the only_linear return.")
     L.append("template <typename real, typename PA, typename treal>")
@@ -211,25 +281,27 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("                                         const bool only_linear)")
     L.append("{")
     L.append(f"{ind}(void)time; (void)pa;")
-    used_all = used_l | _used_symbols([bd[n] for n in extra])
-    L += _bind_states(model, used_all, "states", ind)
-    L += _bind_params(model, used_all, ind)
-    all_inter = [n for n, _ in model.body if n in set(rhs_names) | set(extra) and n not in model.derivatives.values()]
-    L += prs.lines(model, all_inter, ind)
-    for i, s in enumerate(model.state_names):
-        L.append(f"{ind}rhs[{i}] = {pr.p(bd[model.derivatives[s]])};")
-    L += assignments(pr, lin_first, ind)
+    L += g.bind(lused, ind)
+    L += g.lines(lin_inter, ind)
+    for i in range(S):
+        L.append(f"{ind}rhs[{i}] = {pr.p(rhs_out[i])};")
+    L += g.lines(hlin_first, ind)
     for i in range(S):
         if lin.linear_terms[i]:
-            L.append(f"{ind}linear[{i}] = {pr.p(lin.linear[i])};")
+            L.append(f"{ind}linear[{i}] = {pr.p(hlinear[i])};")
     L.append(f"{ind}if (only_linear) return;")
-    L += assignments(pr, lin_second, ind)
+    L += g.lines(hlin_second, ind)
     for i in range(S):
         if not lin.linear_terms[i]:
-            L.append(f"{ind}linear[{i}] = {pr.p(lin.linear[i])};")
+            L.append(f"{ind}linear[{i}] = {pr.p(hlinear[i])};")
     L.append("}")
     L.append("")
 
+    # Census of the (unhoisted) model: state-dependent work after CSE, and
+    # the printed operations of the hoisted text.
+    bd = model.body_dict()
+    rhs_names = needed_names(model, [sp.Symbol(model.derivatives[s], real=True) for s in model.state_names])
+    extra = [n for n in needed_names(model, [e for _, e in lin.lin_stmts] + lin.linear) if n not in rhs_names]
     rhs_stmts = [(n, bd[n]) for n, _ in model.body if n in set(rhs_names)]
     lin_stmts_all = [(n, bd[n]) for n, _ in model.body if n in set(rhs_names) | set(extra)] + lin.lin_stmts
     census = {
@@ -237,44 +309,41 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         # whose operands depend on a state (the algorithmic work).
         "eval_state": census_state_ops(rhs_stmts, [], model.state_names),
         "linearized_eval_state": census_state_ops(lin_stmts_all, lin.linear, model.state_names),
-        # Operations as printed (before compiler CSE), all operations.
-        "eval": census_body(rhs_exprs),
-        "linearized_eval": census_body(
-            [bd[n] for n in all_inter] + [bd[model.derivatives[s]] for s in model.state_names]
-            + [e for _, e in lin.lin_stmts] + lin.linear),
+        # Operations of the emitted (hoisted) text, before compiler CSE.
+        "eval": census_body([e for _, e in eval_stmts] + rhs_out),
+        "linearized_eval": census_body([e for _, e in lin_inter + hlin_first + hlin_second] + rhs_out + hlinear),
+        "constants": census_body([e for _, e in table.items]),
     }
 
     jac_nnz = 0
-    if with_jacobian:
-        jac = jacobian(model)
-        jused = _used_symbols([e for _, e in jac.stmts] + [e for row in jac.entries for e in row])
-        jextra = needed_names(model, [e for _, e in jac.stmts] + [e for row in jac.entries for e in row])
-        jinter = [n for n, _ in model.body if n in set(jextra) and n not in model.derivatives.values()]
-        jused |= _used_symbols([bd[n] for n in jinter])
+    if jac is not None:
+        jroots = [e for _, e in hjac_stmts] + [e for row in hjac for e in row]
+        jinter = select(body_inter, jroots)
+        jstmts = select(hjac_stmts, jroots)
+        jused = g.used_main(jinter + jstmts, [e for row in hjac for e in row])
         L.append("// Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].")
         L.append("template <typename real, typename PA, typename treal>")
         L.append("GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,")
         L.append("                                          const PA& pa, real* __restrict__ jac)")
         L.append("{")
         L.append(f"{ind}(void)time; (void)pa;")
-        L += _bind_states(model, jused, "states", ind)
-        L += _bind_params(model, jused, ind)
-        L += prs.lines(model, jinter, ind)
-        L += assignments(pr, jac.stmts, ind)
+        L += g.bind(jused, ind)
+        L += g.lines(jinter, ind)
+        L += g.lines(jstmts, ind)
         for i in range(S):
             for j in range(S):
-                e = jac.entries[i][j]
+                e = hjac[i][j]
                 if e != 0:
                     jac_nnz += 1
                 L.append(f"{ind}jac[{i * S + j}] = {pr.p(e)};")
         L.append("}")
         L.append("")
-        census["jacobian"] = census_body([bd[n] for n in jinter] + [e for _, e in jac.stmts]
-                                         + [e for row in jac.entries for e in row])
+        census["jacobian"] = census_body([e for _, e in jinter + jstmts] + [e for row in hjac for e in row])
 
     L.append("}}  // namespace goss_models::" + var.name)
     L.append("")
-    return Emitted(header="\n".join(L), census=census, linear_terms=lin.linear_terms, jac_nnz=jac_nnz)
+    return Emitted(header="\n".join(L), census=census, linear_terms=lin.linear_terms, jac_nnz=jac_nnz,
+                   num_constants=len(table.items))
 
 
 def emit_parameterized_ode(var: Variant, header_include: str) -> str:
@@ -301,7 +370,15 @@ def emit_parameterized_ode(var: Variant, header_include: str) -> str:
     L.append("")
     L.append(f"class {N} : public ParameterizedODE")
     L.append("{")
-    L.append("  struct Acc { const double* v; template <int k> double get() const { return v[k]; } };")
+    L.append("  struct Acc {")
+    L.append("    const double* v;")
+    L.append("    const double* c;")
+    L.append("    template <int k> double get() const { return v[k]; }")
+    L.append("    template <int k> double getd() const { return v[k]; }")
+    L.append("    template <int k> double cst() const { return c[k]; }")
+    L.append("    template <int k> double cstd() const { return c[k]; }")
+    L.append("  };")
+    L.append(f"  static constexpr int K = {ns}::num_constants > 0 ? {ns}::num_constants : 1;")
     L.append("public:")
     L.append(f"  {N}() : ParameterizedODE({S}, {P}, {len(var.field_states)}, {len(var.field_parameters)}, 0)")
     L.append("  {")
@@ -323,11 +400,21 @@ def emit_parameterized_ode(var: Variant, header_include: str) -> str:
     L.append(f"    for (int i = 0; i < {P}; ++i) _p[i] = ode._p[i];")
     L.append("    _bind();")
     L.append("  }")
+    L.append("  // Parameters can change through _param_to_value at any time, so the")
+    L.append("  // parameter-only constants are recomputed on every evaluation.")
     L.append("  void eval(const double* states, double time, double* values)")
-    L.append(f"  {{ {ns}::eval<double>(states, time, Acc{{_p}}, values); }}")
+    L.append("  {")
+    L.append("    double c[K];")
+    L.append(f"    {ns}::compute_constants<double>(Acc{{_p, nullptr}}, c);")
+    L.append(f"    {ns}::eval<double>(states, time, Acc{{_p, c}}, values);")
+    L.append("  }")
     L.append("  void linearized_eval(const double* states, double time, double* linear, double* rhs,")
     L.append("                       bool only_linear) const")
-    L.append(f"  {{ {ns}::linearized_eval<double>(states, time, Acc{{_p}}, linear, rhs, only_linear); }}")
+    L.append("  {")
+    L.append("    double c[K];")
+    L.append(f"    {ns}::compute_constants<double>(Acc{{_p, nullptr}}, c);")
+    L.append(f"    {ns}::linearized_eval<double>(states, time, Acc{{_p, c}}, linear, rhs, only_linear);")
+    L.append("  }")
     L.append("  void get_ic(goss::DoubleVector* values) const")
     L.append("  {")
     L.append("    values->n = _num_states;")
@@ -383,4 +470,5 @@ def variant_metadata(var: Variant, em: Emitted) -> dict:
         "linear_terms": em.linear_terms,
         "census": em.census,
         "jacobian_nnz": em.jac_nnz,
+        "num_constants": em.num_constants,
     }

@@ -10,8 +10,10 @@ depending on what ``real`` names.
 Printing rules (they fix the floating-point evaluation order):
 
 * ``Add``: terms in sympy's canonical order, left to right;
-* ``Mul``: numeric coefficient (one literal) first, then numerator factors,
-  then ``/`` the product of the denominator factors;
+* ``Mul``: numeric coefficient (one literal) first, then numerator factors;
+  a denominator makes it ``m_div(numerator, denominator)`` (``m_rcp`` when
+  the numerator is 1), IEEE division in FP64 and a hardware reciprocal in
+  the FP32 kernels;
 * ``x**2``, ``x**3``, ``x**4``: ``m_sq``/``m_cube``/``m_sq(m_sq(x))``;
   ``x**0.5``: ``m_sqrt``; other powers: ``m_pow``;
 * ``Piecewise``: nested ``?:`` selects.
@@ -136,16 +138,15 @@ class CPrinter:
             else:
                 ns = "*".join(self._wrap(f, P["mul"]) for f in num)
             if den:
-                if len(den) == 1:
-                    ds = self._wrap(den[0], P["mul"], strict=True)
-                else:
-                    ds = "(" + "*".join(self._wrap(f, P["mul"]) for f in den) + ")"
-                s = f"{ns}/{ds}"
+                ds = "*".join(self._wrap(f, P["mul"]) for f in den)
+                s = f"m_rcp({ds})" if not num else f"m_div({ns}, {ds})"
+                pr = P["call"]
             else:
                 s = ns
+                pr = P["mul"]
             if sign < 0:
                 return f"-{s}", P["neg"]
-            return s, P["mul"]
+            return s, pr
         if isinstance(e, sp.Pow):
             b, x = e.args
             bs = self.p(b)
@@ -158,14 +159,14 @@ class CPrinter:
                 if n == 4:
                     return f"m_sq(m_sq({bs}))", P["call"]
                 if n == -1:
-                    return f"{self.lit(1.0)}/{self._wrap(b, P['mul'], strict=True)}", P["mul"]
+                    return f"m_rcp({bs})", P["call"]
                 if n in (-2, -3, -4):
                     inner = {2: f"m_sq({bs})", 3: f"m_cube({bs})", 4: f"m_sq(m_sq({bs}))"}[-n]
-                    return f"{self.lit(1.0)}/{inner}", P["mul"]
+                    return f"m_rcp({inner})", P["call"]
             if x == sp.Rational(1, 2):
                 return f"m_sqrt({bs})", P["call"]
             if x == sp.Rational(-1, 2):
-                return f"{self.lit(1.0)}/m_sqrt({bs})", P["mul"]
+                return f"m_rcp(m_sqrt({bs}))", P["call"]
             return f"m_pow({bs}, {self.p(x)})", P["call"]
         if isinstance(e, sp.exp):
             return f"m_exp({self.p(e.args[0])})", P["call"]

@@ -7,6 +7,7 @@ namespace goss_models { namespace fhn {
 
 constexpr int num_states = 2;
 constexpr int num_parameters = 11;
+constexpr int num_constants = 7;
 constexpr int num_field_states = 1;
 constexpr int num_field_parameters = 0;
 constexpr int field_state_indices[] = {0};
@@ -19,6 +20,30 @@ constexpr double default_parameters[] = {0.13, 0.013, 0.26, 0.1, 1.0, 50.0, 1.0,
 constexpr double initial_conditions[] = {-85.0, 0.0};
 constexpr unsigned char linear_terms[] = {0, 1};
 
+// Subexpressions of shared (non-field) parameters only, evaluated once per launch.
+// c[k] may use c[j] for j < k.  Called with real = double on the host.
+template <typename real, typename PA>
+GOSS_HD GOSS_INLINE void compute_constants(const PA& pa, real* __restrict__ c)
+{
+  (void)pa; (void)c;
+  const real a = pa.template getd<0>();
+  const real b = pa.template getd<1>();
+  const real c_1 = pa.template getd<2>();
+  const real c_2 = pa.template getd<3>();
+  const real c_3 = pa.template getd<4>();
+  const real stim_duration = pa.template getd<6>();
+  const real stim_start = pa.template getd<8>();
+  const real v_peak = pa.template getd<9>();
+  const real v_rest = pa.template getd<10>();
+  c[0] = v_peak - v_rest;
+  c[1] = v_rest + a*c[0];
+  c[2] = -m_div(c_2, c[0]);
+  c[3] = m_div(c_1, m_sq(c[0]));
+  c[4] = -real(5.0)*stim_duration - real(5.0)*stim_start;
+  c[5] = -b*c_3;
+  c[6] = -m_div(c_1, m_sq(c[0]));
+}
+
 // Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
 // no state are computed in the type of `time` (treal).
 template <typename real, typename PA, typename treal>
@@ -28,20 +53,15 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time,
   (void)time; (void)pa;
   const real v = states[0];
   const real s = states[1];
-  const real a = pa.template get<0>();
   const real b = pa.template get<1>();
-  const real c_1 = pa.template get<2>();
-  const real c_2 = pa.template get<3>();
   const real c_3 = pa.template get<4>();
-  const real stim_amplitude = pa.template get<5>();
-  const real stim_duration = pa.template get<6>();
-  const real stim_start = pa.template get<8>();
   const real v_peak = pa.template get<9>();
   const real v_rest = pa.template get<10>();
-  const real v_amp = v_peak - v_rest;
-  const real v_th = v_rest + a*v_amp;
-  const real I = -c_2*s*(v - v_rest)/v_amp + c_1*(v - v_rest)*(v - v_th)*(v_peak - v)/m_sq(v_amp);
-  const treal i_Stim = treal(stim_amplitude)*(treal(1.0) - treal(1.0)/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_start))))/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_duration) - treal(5.0)*treal(stim_start)));
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real I = c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v);
+  const treal i_Stim = m_div(pa.template getd<5>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
   values[0] = I + real(i_Stim);
   values[1] = b*(v - v_rest - c_3*s);
 }
@@ -56,26 +76,22 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   (void)time; (void)pa;
   const real v = states[0];
   const real s = states[1];
-  const real a = pa.template get<0>();
   const real b = pa.template get<1>();
-  const real c_1 = pa.template get<2>();
-  const real c_2 = pa.template get<3>();
   const real c_3 = pa.template get<4>();
-  const real stim_amplitude = pa.template get<5>();
-  const real stim_duration = pa.template get<6>();
-  const real stim_start = pa.template get<8>();
   const real v_peak = pa.template get<9>();
   const real v_rest = pa.template get<10>();
-  const real v_amp = v_peak - v_rest;
-  const real v_th = v_rest + a*v_amp;
-  const real I = -c_2*s*(v - v_rest)/v_amp + c_1*(v - v_rest)*(v - v_th)*(v_peak - v)/m_sq(v_amp);
-  const treal i_Stim = treal(stim_amplitude)*(treal(1.0) - treal(1.0)/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_start))))/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_duration) - treal(5.0)*treal(stim_start)));
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  const real I = c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v);
+  const treal i_Stim = m_div(pa.template getd<5>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
   rhs[0] = I + real(i_Stim);
   rhs[1] = b*(v - v_rest - c_3*s);
-  const real dds_dt_ds = -b*c_3;
-  linear[1] = dds_dt_ds;
+  linear[1] = c__5;
   if (only_linear) return;
-  const real dI_dv = -c_2*s/v_amp + c_1*(v - v_rest)*(v_peak - v)/m_sq(v_amp) + c_1*(v - v_th)*(v_peak - v)/m_sq(v_amp) - c_1*(v - v_rest)*(v - v_th)/m_sq(v_amp);
+  const real dI_dv = c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) + c__6*(v - c__1)*(v - v_rest);
   linear[0] = dI_dv;
 }
 
@@ -87,22 +103,20 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   (void)time; (void)pa;
   const real v = states[0];
   const real s = states[1];
-  const real a = pa.template get<0>();
   const real b = pa.template get<1>();
-  const real c_1 = pa.template get<2>();
-  const real c_2 = pa.template get<3>();
-  const real c_3 = pa.template get<4>();
   const real v_peak = pa.template get<9>();
   const real v_rest = pa.template get<10>();
-  const real v_amp = v_peak - v_rest;
-  const real v_th = v_rest + a*v_amp;
-  const real dI_dv = -c_2*s/v_amp + c_1*(v - v_rest)*(v_peak - v)/m_sq(v_amp) + c_1*(v - v_th)*(v_peak - v)/m_sq(v_amp) - c_1*(v - v_rest)*(v - v_th)/m_sq(v_amp);
-  const real dI_ds = -c_2*(v - v_rest)/v_amp;
-  const real dds_dt_ds = -b*c_3;
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  const real dI_dv = c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) + c__6*(v - c__1)*(v - v_rest);
+  const real dI_ds = c__2*(v - v_rest);
   jac[0] = dI_dv;
   jac[1] = dI_ds;
   jac[2] = b;
-  jac[3] = dds_dt_ds;
+  jac[3] = c__5;
 }
 
 }}  // namespace goss_models::fhn

@@ -7,6 +7,7 @@ namespace goss_models { namespace fitzhughnagumo {
 
 constexpr int num_states = 2;
 constexpr int num_parameters = 11;
+constexpr int num_constants = 7;
 constexpr int num_field_states = 1;
 constexpr int num_field_parameters = 0;
 constexpr int field_state_indices[] = {0};
@@ -19,6 +20,30 @@ constexpr double default_parameters[] = {40.0, -85.0, 0.13, 0.013, 0.26, 0.1, 1.
 constexpr double initial_conditions[] = {-85.0, 0.0};
 constexpr unsigned char linear_terms[] = {0, 1};
 
+// Subexpressions of shared (non-field) parameters only, evaluated once per launch.
+// c[k] may use c[j] for j < k.  Called with real = double on the host.
+template <typename real, typename PA>
+GOSS_HD GOSS_INLINE void compute_constants(const PA& pa, real* __restrict__ c)
+{
+  (void)pa; (void)c;
+  const real V_peak = pa.template getd<0>();
+  const real V_rest = pa.template getd<1>();
+  const real a = pa.template getd<2>();
+  const real b = pa.template getd<3>();
+  const real c_1 = pa.template getd<4>();
+  const real c_2 = pa.template getd<5>();
+  const real c_3 = pa.template getd<6>();
+  const real stim_duration = pa.template getd<8>();
+  const real stim_start = pa.template getd<10>();
+  c[0] = V_peak - V_rest;
+  c[1] = V_rest + a*c[0];
+  c[2] = -m_div(c_2, c[0]);
+  c[3] = m_div(c_1, m_sq(c[0]));
+  c[4] = -real(5.0)*stim_duration - real(5.0)*stim_start;
+  c[5] = -b*c_3;
+  c[6] = -m_div(c_1, m_sq(c[0]));
+}
+
 // Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
 // no state are computed in the type of `time` (treal).
 template <typename real, typename PA, typename treal>
@@ -30,18 +55,13 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time,
   const real s = states[1];
   const real V_peak = pa.template get<0>();
   const real V_rest = pa.template get<1>();
-  const real a = pa.template get<2>();
   const real b = pa.template get<3>();
-  const real c_1 = pa.template get<4>();
-  const real c_2 = pa.template get<5>();
   const real c_3 = pa.template get<6>();
-  const real stim_amplitude = pa.template get<7>();
-  const real stim_duration = pa.template get<8>();
-  const real stim_start = pa.template get<10>();
-  const real V_amp = V_peak - V_rest;
-  const real V_th = V_rest + V_amp*a;
-  const real I = -c_2*s*(V - V_rest)/V_amp + c_1*(V - V_rest)*(V - V_th)*(V_peak - V)/m_sq(V_amp);
-  const treal i_Stim = treal(stim_amplitude)*(treal(1.0) - treal(1.0)/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_start))))/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_duration) - treal(5.0)*treal(stim_start)));
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
+  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
   values[0] = I + real(i_Stim);
   values[1] = b*(V - V_rest - c_3*s);
 }
@@ -58,24 +78,20 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   const real s = states[1];
   const real V_peak = pa.template get<0>();
   const real V_rest = pa.template get<1>();
-  const real a = pa.template get<2>();
   const real b = pa.template get<3>();
-  const real c_1 = pa.template get<4>();
-  const real c_2 = pa.template get<5>();
   const real c_3 = pa.template get<6>();
-  const real stim_amplitude = pa.template get<7>();
-  const real stim_duration = pa.template get<8>();
-  const real stim_start = pa.template get<10>();
-  const real V_amp = V_peak - V_rest;
-  const real V_th = V_rest + V_amp*a;
-  const real I = -c_2*s*(V - V_rest)/V_amp + c_1*(V - V_rest)*(V - V_th)*(V_peak - V)/m_sq(V_amp);
-  const treal i_Stim = treal(stim_amplitude)*(treal(1.0) - treal(1.0)/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_start))))/(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*treal(stim_duration) - treal(5.0)*treal(stim_start)));
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
+  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
   rhs[0] = I + real(i_Stim);
   rhs[1] = b*(V - V_rest - c_3*s);
-  const real dds_dt_ds = -b*c_3;
-  linear[1] = dds_dt_ds;
+  linear[1] = c__5;
   if (only_linear) return;
-  const real dI_dV = -c_2*s/V_amp + c_1*(V - V_rest)*(V_peak - V)/m_sq(V_amp) + c_1*(V - V_th)*(V_peak - V)/m_sq(V_amp) - c_1*(V - V_rest)*(V - V_th)/m_sq(V_amp);
+  const real dI_dV = c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) + c__6*(V - V_rest)*(V - c__1);
   linear[0] = dI_dV;
 }
 
@@ -89,20 +105,18 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   const real s = states[1];
   const real V_peak = pa.template get<0>();
   const real V_rest = pa.template get<1>();
-  const real a = pa.template get<2>();
   const real b = pa.template get<3>();
-  const real c_1 = pa.template get<4>();
-  const real c_2 = pa.template get<5>();
-  const real c_3 = pa.template get<6>();
-  const real V_amp = V_peak - V_rest;
-  const real V_th = V_rest + V_amp*a;
-  const real dI_dV = -c_2*s/V_amp + c_1*(V - V_rest)*(V_peak - V)/m_sq(V_amp) + c_1*(V - V_th)*(V_peak - V)/m_sq(V_amp) - c_1*(V - V_rest)*(V - V_th)/m_sq(V_amp);
-  const real dI_ds = -c_2*(V - V_rest)/V_amp;
-  const real dds_dt_ds = -b*c_3;
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  const real dI_dV = c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) + c__6*(V - V_rest)*(V - c__1);
+  const real dI_ds = c__2*(V - V_rest);
   jac[0] = dI_dV;
   jac[1] = dI_ds;
   jac[2] = b;
-  jac[3] = dds_dt_ds;
+  jac[3] = c__5;
 }
 
 }}  // namespace goss_models::fitzhughnagumo

@@ -7,6 +7,7 @@ namespace goss_models { namespace lorentz {
 
 constexpr int num_states = 3;
 constexpr int num_parameters = 3;
+constexpr int num_constants = 0;
 constexpr int num_field_states = 1;
 constexpr int num_field_parameters = 0;
 constexpr int field_state_indices[] = {0};
@@ -18,6 +19,14 @@ static const char* const field_parameter_names[] = {nullptr};
 constexpr double default_parameters[] = {2.6666666666666665, 28.0, 10.0};
 constexpr double initial_conditions[] = {1.0, 0.0, 0.0};
 constexpr unsigned char linear_terms[] = {1, 1, 1};
+
+// Subexpressions of shared (non-field) parameters only, evaluated once per launch.
+// c[k] may use c[j] for j < k.  Called with real = double on the host.
+template <typename real, typename PA>
+GOSS_HD GOSS_INLINE void compute_constants(const PA& pa, real* __restrict__ c)
+{
+  (void)pa; (void)c;
+}
 
 // Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
 // no state are computed in the type of `time` (treal).
@@ -54,11 +63,9 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   rhs[0] = sigma*(y - x);
   rhs[1] = -y + x*(rho - z);
   rhs[2] = x*y - beta*z;
-  const real ddx_dt_dx = -sigma;
-  const real ddz_dt_dz = -beta;
-  linear[0] = ddx_dt_dx;
+  linear[0] = -sigma;
   linear[1] = -real(1.0);
-  linear[2] = ddz_dt_dz;
+  linear[2] = -beta;
   if (only_linear) return;
 }
 
@@ -74,11 +81,9 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   const real beta = pa.template get<0>();
   const real rho = pa.template get<1>();
   const real sigma = pa.template get<2>();
-  const real ddx_dt_dx = -sigma;
   const real ddy_dt_dx = rho - z;
   const real ddy_dt_dz = -x;
-  const real ddz_dt_dz = -beta;
-  jac[0] = ddx_dt_dx;
+  jac[0] = -sigma;
   jac[1] = sigma;
   jac[2] = real(0.0);
   jac[3] = ddy_dt_dx;
@@ -86,7 +91,7 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   jac[5] = ddy_dt_dz;
   jac[6] = y;
   jac[7] = x;
-  jac[8] = ddz_dt_dz;
+  jac[8] = -beta;
 }
 
 }}  // namespace goss_models::lorentz

@@ -7,6 +7,7 @@ namespace goss_models { namespace oscilator {
 
 constexpr int num_states = 2;
 constexpr int num_parameters = 2;
+constexpr int num_constants = 0;
 constexpr int num_field_states = 2;
 constexpr int num_field_parameters = 2;
 constexpr int field_state_indices[] = {0, 1};
@@ -18,6 +19,14 @@ static const char* const field_parameter_names[] = {"a", "b"};
 constexpr double default_parameters[] = {1.0, 1.0};
 constexpr double initial_conditions[] = {1.0, 0.0};
 constexpr unsigned char linear_terms[] = {1, 1};
+
+// Subexpressions of shared (non-field) parameters only, evaluated once per launch.
+// c[k] may use c[j] for j < k.  Called with real = double on the host.
+template <typename real, typename PA>
+GOSS_HD GOSS_INLINE void compute_constants(const PA& pa, real* __restrict__ c)
+{
+  (void)pa; (void)c;
+}
 
 // Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
 // no state are computed in the type of `time` (treal).

@@ -11,7 +11,15 @@ namespace goss {
 
 class Tp06_epi : public ParameterizedODE
 {
-  struct Acc { const double* v; template <int k> double get() const { return v[k]; } };
+  struct Acc {
+    const double* v;
+    const double* c;
+    template <int k> double get() const { return v[k]; }
+    template <int k> double getd() const { return v[k]; }
+    template <int k> double cst() const { return c[k]; }
+    template <int k> double cstd() const { return c[k]; }
+  };
+  static constexpr int K = goss_models::tp06_epi::num_constants > 0 ? goss_models::tp06_epi::num_constants : 1;
 public:
   Tp06_epi() : ParameterizedODE(19, 53, 1, 0, 0)
   {
@@ -30,11 +38,21 @@ public:
     for (int i = 0; i < 53; ++i) _p[i] = ode._p[i];
     _bind();
   }
+  // Parameters can change through _param_to_value at any time, so the
+  // parameter-only constants are recomputed on every evaluation.
   void eval(const double* states, double time, double* values)
-  { goss_models::tp06_epi::eval<double>(states, time, Acc{_p}, values); }
+  {
+    double c[K];
+    goss_models::tp06_epi::compute_constants<double>(Acc{_p, nullptr}, c);
+    goss_models::tp06_epi::eval<double>(states, time, Acc{_p, c}, values);
+  }
   void linearized_eval(const double* states, double time, double* linear, double* rhs,
                        bool only_linear) const
-  { goss_models::tp06_epi::linearized_eval<double>(states, time, Acc{_p}, linear, rhs, only_linear); }
+  {
+    double c[K];
+    goss_models::tp06_epi::compute_constants<double>(Acc{_p, nullptr}, c);
+    goss_models::tp06_epi::linearized_eval<double>(states, time, Acc{_p, c}, linear, rhs, only_linear);
+  }
   void get_ic(goss::DoubleVector* values) const
   {
     values->n = _num_states;

@@ -45,10 +45,22 @@ enum Precision : int { kFP64 = 0, kFP32 = 1, kFP32C = 2, kNumPrecisions = 3 };
 
 constexpr int kBlock = 256;
 
-// Shared (non-field) parameters, passed by value as a kernel argument.
-template <class R, int P>
+// Shared (non-field) parameters and the parameter-only constants of the
+// generated code (compute_constants, evaluated on the host in FP64), passed by
+// value in the kernel-argument block.  The FP32 family also carries FP64
+// copies for the time-only expressions, which are evaluated in FP64.
+template <class R, int P, int K>
 struct SharedParams {
   R v[P > 0 ? P : 1];
+  R c[K > 0 ? K : 1];
+};
+
+template <int P, int K>
+struct SharedParams<float, P, K> {
+  float v[P > 0 ? P : 1];
+  float c[K > 0 ? K : 1];
+  double vd[P > 0 ? P : 1];
+  double cd[K > 0 ? K : 1];
 };
 
 template <class M>
@@ -59,10 +71,15 @@ constexpr int field_slot(int k)
   return -1;
 }
 
-// Parameter accessor handed to the generated model code.
+template <class M, class R>
+using SharedParamsOf = SharedParams<R, M::num_parameters, M::num_constants>;
+
+// Parameter accessor handed to the generated model code: field parameters
+// from per-lane registers, shared parameters and constants from the kernel
+// arguments.  getd/cstd serve the FP64 time-only expressions.
 template <class M, class R>
 struct DevAcc {
-  const SharedParams<R, M::num_parameters>& sp;
+  const SharedParamsOf<M, R>& sp;
   const R* fpv;
   template <int k>
   __device__ GOSS_INLINE R get() const
@@ -72,6 +89,27 @@ struct DevAcc {
       return fpv[j];
     else
       return sp.v[k];
+  }
+  template <int k>
+  __device__ GOSS_INLINE double getd() const
+  {
+    constexpr int j = field_slot<M>(k);
+    if constexpr (j >= 0)
+      return double(fpv[j]);
+    else if constexpr (sizeof(R) == 8)
+      return sp.v[k];
+    else
+      return sp.vd[k];
+  }
+  template <int k>
+  __device__ GOSS_INLINE R cst() const { return sp.c[k]; }
+  template <int k>
+  __device__ GOSS_INLINE double cstd() const
+  {
+    if constexpr (sizeof(R) == 8)
+      return sp.c[k];
+    else
+      return sp.cd[k];
   }
 };
 
@@ -99,7 +137,8 @@ __device__ GOSS_INLINE float expm1_f32(const float x)
 {
   const float p = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x * (1.0f / 120.0f +
                   x * (1.0f / 720.0f + x * (1.0f / 5040.0f)))))));
-  return fabsf(x) < 0.25f ? p : __expf(x) - 1.0f;
+  const float q = __expf(x) - 1.0f;
+  return fabsf(x) < 0.25f ? p : q;
 }
 
 // Exact sum hi + lo += inc (Knuth's TwoSum; 6 additions).
@@ -207,7 +246,12 @@ struct Step<M, kRK4, R, COMP> {
     for (int i = 0; i < S; ++i) tmp[i] = n.y[i] + h * k3[i];
     M::template eval<R>(tmp, lt + ldt, acc, k4);
 #pragma unroll
-    for (int i = 0; i < S; ++i) n.add(i, h * (k1[i] + R(2.0) * k2[i] + R(2.0) * k3[i] + k4[i]) / R(6.0));
+    for (int i = 0; i < S; ++i) {
+      if constexpr (sizeof(R) == 8)
+        n.add(i, h * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0);  // RK4.cpp:99-100
+      else
+        n.add(i, h * (k1[i] + R(2.0) * k2[i] + R(2.0) * k3[i] + k4[i]) * R(1.0 / 6.0));
+    }
   }
 };
 
@@ -232,7 +276,10 @@ __device__ GOSS_INLINE void rl_stage(R* out, NodeState<M, R, COMP>& n, const R* 
       const double v = use_exp ? rl_update(y0, lin[i], rhs[i], h) : y0 + rhs[i] * h;
       if constexpr (APPLY) n.y[i] = v; else out[i] = v;
     } else {
-      const R inc = use_exp ? rhs[i] / lin[i] * expm1_f32(lin[i] * h) : rhs[i] * h;
+      // Both branches are cheap; select instead of branching (lanes diverge on |a| > delta).
+      const float e = expm1_f32(lin[i] * h);
+      const float inc_exp = rhs[i] * e * __builtin_amdgcn_rcpf(lin[i]);
+      const float inc = use_exp ? inc_exp : rhs[i] * h;
       if constexpr (APPLY) n.add(i, inc); else out[i] = y0 + inc;
     }
   }
@@ -290,7 +337,7 @@ struct FieldOut {
 template <class M, int SOLVER, class R, bool COMP>
 __global__ void __launch_bounds__(kBlock)
 explicit_kernel(R* __restrict__ y, R* __restrict__ ylo, const R* __restrict__ fp, const double2* __restrict__ steps,
-                const int nouter, const SharedParams<R, M::num_parameters> sp, const double ldt_0, const R delta,
+                const int nouter, const SharedParamsOf<M, R> sp, const double ldt_0, const R delta,
                 const unsigned N, const unsigned ld, const FieldOut fo)
 {
   constexpr int S = M::num_states;

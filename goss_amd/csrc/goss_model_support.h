@@ -4,10 +4,10 @@
 // serves the CPU model class (g++, double, glibc) and the HIP kernels
 // (hipcc, float or double, ROCm device libraries).
 //
-// Device float: exp/log use the hardware exp2/log2 (v_exp_f32, v_log_f32)
-// with a scale, unless GOSS_F32_ACCURATE_MATH is defined, in which case the
-// correctly rounded-ish ocml routines are used.  Device double always uses
-// ocml (exp/log are software sequences on CDNA).
+// Device float: division, reciprocal, exp, log and sqrt use the hardware
+// approximations (v_rcp_f32, v_exp_f32, v_log_f32, v_sqrt_f32) unless
+// GOSS_F32_ACCURATE_MATH is defined.  Double always uses IEEE division and
+// the ocml/glibc routines, the reference's arithmetic.
 #pragma once
 
 #include <cmath>
@@ -26,6 +26,8 @@ template <typename T> GOSS_HD GOSS_INLINE T m_cube(const T x) { return x * x * x
 template <typename T> GOSS_HD GOSS_INLINE T m_sign(const T x) { return T((x > T(0)) - (x < T(0))); }
 template <typename T> GOSS_HD GOSS_INLINE T m_heaviside(const T x) { return x > T(0) ? T(1) : (x < T(0) ? T(0) : T(0.5)); }
 
+GOSS_HD GOSS_INLINE double m_div(const double a, const double b) { return a / b; }
+GOSS_HD GOSS_INLINE double m_rcp(const double b) { return 1.0 / b; }
 GOSS_HD GOSS_INLINE double m_exp(const double x) { return std::exp(x); }
 GOSS_HD GOSS_INLINE double m_log(const double x) { return std::log(x); }
 GOSS_HD GOSS_INLINE double m_sqrt(const double x) { return std::sqrt(x); }
@@ -45,14 +47,21 @@ GOSS_HD GOSS_INLINE double m_sinh(const double x) { return std::sinh(x); }
 GOSS_HD GOSS_INLINE double m_cosh(const double x) { return std::cosh(x); }
 GOSS_HD GOSS_INLINE double m_tanh(const double x) { return std::tanh(x); }
 
-#if defined(__HIPCC__) && !defined(GOSS_F32_ACCURATE_MATH)
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GOSS_F32_ACCURATE_MATH)
+// FP32 device kernels: hardware reciprocal (v_rcp_f32, 1 ulp), exp2/log2
+// (v_exp_f32 / v_log_f32 with a scale) and square root (v_sqrt_f32).
+GOSS_HD GOSS_INLINE float m_div(const float a, const float b) { return a * __builtin_amdgcn_rcpf(b); }
+GOSS_HD GOSS_INLINE float m_rcp(const float b) { return __builtin_amdgcn_rcpf(b); }
 GOSS_HD GOSS_INLINE float m_exp(const float x) { return __expf(x); }
 GOSS_HD GOSS_INLINE float m_log(const float x) { return __logf(x); }
+GOSS_HD GOSS_INLINE float m_sqrt(const float x) { return __builtin_amdgcn_sqrtf(x); }
 #else
+GOSS_HD GOSS_INLINE float m_div(const float a, const float b) { return a / b; }
+GOSS_HD GOSS_INLINE float m_rcp(const float b) { return 1.0f / b; }
 GOSS_HD GOSS_INLINE float m_exp(const float x) { return std::exp(x); }
 GOSS_HD GOSS_INLINE float m_log(const float x) { return std::log(x); }
-#endif
 GOSS_HD GOSS_INLINE float m_sqrt(const float x) { return std::sqrt(x); }
+#endif
 GOSS_HD GOSS_INLINE float m_fabs(const float x) { return std::fabs(x); }
 GOSS_HD GOSS_INLINE float m_floor(const float x) { return std::floor(x); }
 GOSS_HD GOSS_INLINE float m_ceil(const float x) { return std::ceil(x); }

@@ -22,6 +22,7 @@ namespace NS = goss_models::GOSS_VARIANT;
 struct M {
   static constexpr int num_states = NS::num_states;
   static constexpr int num_parameters = NS::num_parameters;
+  static constexpr int num_constants = NS::num_constants;
   static constexpr int num_field_states = NS::num_field_states;
   static constexpr int num_field_parameters = NS::num_field_parameters;
   static constexpr const int* field_state_indices = NS::field_state_indices;
@@ -40,11 +41,31 @@ struct M {
   }
 };
 
+// Host-side accessor for compute_constants (all parameters in FP64).
+struct HostAcc {
+  const double* v;
+  template <int k> double getd() const { return v[k]; }
+};
+
+template <class R>
+SharedParamsOf<M, R> make_shared_params(const double* shared)
+{
+  SharedParamsOf<M, R> sp;
+  double c[M::num_constants > 0 ? M::num_constants : 1];
+  NS::compute_constants<double>(HostAcc{shared}, c);
+  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = R(shared[k]);
+  for (int k = 0; k < M::num_constants; ++k) sp.c[k] = R(c[k]);
+  if constexpr (sizeof(R) == 4) {
+    for (int k = 0; k < M::num_parameters; ++k) sp.vd[k] = shared[k];
+    for (int k = 0; k < M::num_constants; ++k) sp.cd[k] = c[k];
+  }
+  return sp;
+}
+
 template <int SOLVER, class R, bool COMP>
 hipError_t launch_explicit(const ExplicitArgs& a)
 {
-  SharedParams<R, M::num_parameters> sp;
-  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = R(a.shared[k]);
+  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((explicit_kernel<M, SOLVER, R, COMP>), grid, dim3(kBlock), 0, a.stream,
                      static_cast<R*>(a.y), static_cast<R*>(a.ylo), static_cast<const R*>(a.fp), a.steps, a.nouter,

@@ -27,10 +27,18 @@ namespace {
 
 std::string g_err;
 
+// Parameter accessor of the generated model code: v = the node's parameter
+// vector, c = its parameter-only constants (compute_constants).
 struct Acc {
   const double* v;
+  const double* c;
   template <int k> double get() const { return v[k]; }
+  template <int k> double getd() const { return v[k]; }
+  template <int k> double cst() const { return c[k]; }
+  template <int k> double cstd() const { return c[k]; }
 };
+
+constexpr int kMaxConstants = 512;
 
 struct VariantOps {
   const char* name;
@@ -41,9 +49,11 @@ struct VariantOps {
   const double* ic;
   const unsigned char* linear_terms;
   const char* const* pnames;
-  void (*eval)(const double*, double, const double*, double*);
-  void (*linearized)(const double*, double, const double*, double*, double*, bool);
-  void (*jacobian)(const double*, double, const double*, double*);
+  int K;  // number of parameter-only constants
+  void (*constants)(const double* p, double* c);
+  void (*eval)(const double*, double, const double*, const double*, double*);
+  void (*linearized)(const double*, double, const double*, const double*, double*, double*, bool);
+  void (*jacobian)(const double*, double, const double*, const double*, double*);
 };
 
 #define GOSS_ORACLE_VARIANT(V)                                                                         \
@@ -52,14 +62,15 @@ struct VariantOps {
              goss_models::V::field_state_indices, goss_models::V::field_parameter_indices,             \
              goss_models::V::default_parameters, goss_models::V::initial_conditions,                   \
              goss_models::V::linear_terms, goss_models::V::parameter_names,                            \
-             [](const double* y, double t, const double* p, double* out) {                             \
-               goss_models::V::eval<double>(y, t, Acc{p}, out);                                        \
+             goss_models::V::num_constants,                                                            \
+             [](const double* p, double* c) { goss_models::V::compute_constants<double>(Acc{p, nullptr}, c); }, \
+             [](const double* y, double t, const double* p, const double* c, double* out) {            \
+               goss_models::V::eval<double>(y, t, Acc{p, c}, out);                                     \
              },                                                                                        \
-             [](const double* y, double t, const double* p, double* lin, double* rhs, bool ol) {       \
-               goss_models::V::linearized_eval<double>(y, t, Acc{p}, lin, rhs, ol);                    \
-             },                                                                                        \
-             [](const double* y, double t, const double* p, double* jac) {                             \
-               goss_models::V::compute_jacobian<double>(y, t, Acc{p}, jac);                            \
+             [](const double* y, double t, const double* p, const double* c, double* lin, double* rhs,  \
+                bool ol) { goss_models::V::linearized_eval<double>(y, t, Acc{p, c}, lin, rhs, ol); },   \
+             [](const double* y, double t, const double* p, const double* c, double* jac) {            \
+               goss_models::V::compute_jacobian<double>(y, t, Acc{p, c}, jac);                         \
              }}
 
 const VariantOps kVariants[] = {
@@ -113,16 +124,17 @@ void fb_subst(int n, const double* a, const double* b, double* x)
 
 // ODE::compute_jacobian (ODE.cpp:47-67): forward differences with
 // delta_i = sqrt(1e-15 * max(1e-5, |y_i|)); f1/f2 are caller scratch.
-void fd_jacobian(const VariantOps& m, double* y, double t, const double* p, double* jac, double* f1, double* f2)
+void fd_jacobian(const VariantOps& m, double* y, double t, const double* p, const double* c, double* jac, double* f1,
+                 double* f2)
 {
   const int S = m.S;
-  m.eval(y, t, p, f1);
+  m.eval(y, t, p, c, f1);
   for (int i = 0; i < S; ++i) {
     const double ysafe = y[i];
     const double mx = 1e-5 > std::fabs(ysafe) ? 1e-5 : std::fabs(ysafe);
     const double delta = std::sqrt(1e-15 * mx);
     y[i] += delta;
-    m.eval(y, t, p, f2);
+    m.eval(y, t, p, c, f2);
     for (int j = 0; j < S; ++j) jac[j * S + i] = (f2[j] - f1[j]) / delta;
     y[i] = ysafe;
   }
@@ -165,7 +177,7 @@ struct Work {
 
 // One node: solver.forward(y, t, interval) with the node's parameter vector.
 void forward_explicit(int kind, const VariantOps& m, const SolverParams& sp, double* y, double t, double dt,
-                      const double* p, Work& w)
+                      const double* p, const double* c, Work& w)
 {
   const int S = m.S;
   const unsigned long nsteps = substeps(sp.ldt, dt);
@@ -176,7 +188,7 @@ void forward_explicit(int kind, const VariantOps& m, const SolverParams& sp, dou
   switch (kind) {
   case EE:  // ExplicitEuler.cpp:64-87
     for (unsigned long step = 0; step < nsteps; ++step) {
-      m.eval(y, lt, p, f1);
+      m.eval(y, lt, p, c, f1);
       for (int i = 0; i < S; ++i) y[i] += ldt * f1[i];
       lt += ldt;
     }
@@ -185,9 +197,9 @@ void forward_explicit(int kind, const VariantOps& m, const SolverParams& sp, dou
     for (unsigned long step = 0; step < nsteps; ++step) {
       double* k1 = w.k1.data();
       double* tmp = w.tmp.data();
-      m.eval(y, lt, p, k1);
+      m.eval(y, lt, p, c, k1);
       for (int i = 0; i < S; ++i) tmp[i] = y[i] + 0.5 * ldt * k1[i];
-      m.eval(tmp, lt + 0.5 * ldt, p, k1);
+      m.eval(tmp, lt + 0.5 * ldt, p, c, k1);
       for (int i = 0; i < S; ++i) y[i] += ldt * k1[i];
       lt += ldt;
     }
@@ -195,14 +207,14 @@ void forward_explicit(int kind, const VariantOps& m, const SolverParams& sp, dou
   case RK4:  // RK4.cpp:68-102, axpy RK4.h:70-74
     for (unsigned long step = 0; step < nsteps; ++step) {
       double *k1 = w.k1.data(), *k2 = w.k2.data(), *k3 = w.k3.data(), *k4 = w.k4.data(), *tmp = w.tmp.data();
-      m.eval(y, lt, p, k1);
+      m.eval(y, lt, p, c, k1);
       const double h = 0.5 * ldt;
       for (int i = 0; i < S; ++i) tmp[i] = y[i] + h * k1[i];
-      m.eval(tmp, lt + 0.5 * ldt, p, k2);
+      m.eval(tmp, lt + 0.5 * ldt, p, c, k2);
       for (int i = 0; i < S; ++i) tmp[i] = y[i] + h * k2[i];
-      m.eval(tmp, lt + 0.5 * ldt, p, k3);
+      m.eval(tmp, lt + 0.5 * ldt, p, c, k3);
       for (int i = 0; i < S; ++i) tmp[i] = y[i] + ldt * k3[i];
-      m.eval(tmp, lt + ldt, p, k4);
+      m.eval(tmp, lt + ldt, p, c, k4);
       for (int i = 0; i < S; ++i) y[i] += ldt * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0;
       lt += ldt;
     }
@@ -210,7 +222,7 @@ void forward_explicit(int kind, const VariantOps& m, const SolverParams& sp, dou
   case RL1:  // RL1.cpp:59-79, _one_step RL1.h:68-77
   case RL2: {  // RL2.cpp:64-87
     auto rl_step = [&](double* out, const double* at, const double* y0, double tt, double h) {
-      m.linearized(at, tt, p, f1, f2, true);
+      m.linearized(at, tt, p, c, f1, f2, true);
       for (int i = 0; i < S; ++i)
         out[i] = m.linear_terms[i] ? y0[i] + f2[i] / f1[i] * (std::exp(f1[i] * h) - 1.0) : y0[i] + f2[i] * h;
     };
@@ -230,7 +242,7 @@ void forward_explicit(int kind, const VariantOps& m, const SolverParams& sp, dou
   case GRL2: {  // GRL2.cpp:63-86 (delta is the member value; GRL2.cpp:67 reads an uninitialised local)
     const double delta = sp.delta;
     auto grl_step = [&](double* out, const double* at, const double* y0, double tt, double h) {
-      m.linearized(at, tt, p, f1, f2, false);
+      m.linearized(at, tt, p, c, f1, f2, false);
       for (int i = 0; i < S; ++i)
         out[i] = (std::fabs(f1[i]) > delta) ? y0[i] + f2[i] / f1[i] * (std::exp(f1[i] * h) - 1.0)
                                             : y0[i] + f2[i] * h;
@@ -285,7 +297,9 @@ void forward_node(oracle_system* s, unsigned node, double t, double interval, Wo
   const VariantOps& m = *s->m;
   // ParameterizedODE::set_field_parameters (ODESystemSolver.h:219-224).
   for (int i = 0; i < m.NFP; ++i) prm[m.fpi[i]] = s->fparams[size_t(node) * m.NFP + i];
-  forward_explicit(s->kind, m, s->sp, &s->states[size_t(node) * m.S], t, interval, prm, w);
+  double c[kMaxConstants];
+  m.constants(prm, c);
+  forward_explicit(s->kind, m, s->sp, &s->states[size_t(node) * m.S], t, interval, prm, c, w);
 }
 
 // ODESystemSolver::forward (ODESystemSolver.cpp:67-79).
@@ -369,20 +383,31 @@ void oracle_variant_linear_terms(int v, unsigned char* f)
 {
   std::memcpy(f, kVariants[v].linear_terms, kVariants[v].S);
 }
-void oracle_model_eval(int v, const double* y, double t, const double* p, double* out) { kVariants[v].eval(y, t, p, out); }
+void oracle_model_eval(int v, const double* y, double t, const double* p, double* out)
+{
+  double c[kMaxConstants];
+  kVariants[v].constants(p, c);
+  kVariants[v].eval(y, t, p, c, out);
+}
 void oracle_model_linearized_eval(int v, const double* y, double t, const double* p, double* lin, double* rhs,
                                   int only_linear)
 {
-  kVariants[v].linearized(y, t, p, lin, rhs, only_linear != 0);
+  double c[kMaxConstants];
+  kVariants[v].constants(p, c);
+  kVariants[v].linearized(y, t, p, c, lin, rhs, only_linear != 0);
 }
 void oracle_model_analytic_jacobian(int v, const double* y, double t, const double* p, double* jac)
 {
-  kVariants[v].jacobian(y, t, p, jac);
+  double c[kMaxConstants];
+  kVariants[v].constants(p, c);
+  kVariants[v].jacobian(y, t, p, c, jac);
 }
 void oracle_model_fd_jacobian(int v, double* y, double t, const double* p, double* jac)
 {
   std::vector<double> f1(kVariants[v].S), f2(kVariants[v].S);
-  fd_jacobian(kVariants[v], y, t, p, jac, f1.data(), f2.data());
+  double c[kMaxConstants];
+  kVariants[v].constants(p, c);
+  fd_jacobian(kVariants[v], y, t, p, c, jac, f1.data(), f2.data());
 }
 void oracle_lu_factorize(int n, double* mat) { lu_factorize(n, mat); }
 void oracle_forward_backward_subst(int n, const double* mat, const double* b, double* x) { fb_subst(n, mat, b, x); }
@@ -394,6 +419,7 @@ int oracle_system_create(const char* variant, const char* solver, unsigned num_n
   if (v < 0) { g_err = std::string("unknown model variant ") + variant; return -1; }
   const int kind = solver_kind(solver);
   if (kind < 0) { g_err = std::string("unknown solver ") + solver; return -1; }
+  if (kVariants[v].K > kMaxConstants) { g_err = "model has too many constants"; return -1; }
   if (kind >= IMPLICIT_EULER) { g_err = std::string("solver not available in the oracle: ") + solver; return -1; }
   auto* s = new oracle_system;
   s->m = &kVariants[v];
