@@ -41,6 +41,7 @@ EXPORTS = (
     "goss_hip_system_reset_default", "goss_hip_system_set_parameter", "goss_hip_system_get_parameter",
     "goss_hip_system_set_num_threads", "goss_hip_system_get_num_threads", "goss_hip_system_num_nodes",
     "goss_hip_system_set_steps_per_launch", "goss_hip_system_synchronize", "goss_hip_system_stream",
+    "goss_hip_system_get_solver_counters",
 )
 
 
@@ -127,6 +128,7 @@ def lib() -> ctypes.CDLL:
     L.goss_hip_system_set_steps_per_launch.argtypes = [V, ctypes.c_uint32]
     L.goss_hip_system_synchronize.argtypes = [V]
     L.goss_hip_system_stream.argtypes = [V, ctypes.POINTER(ctypes.c_void_p)]
+    L.goss_hip_system_get_solver_counters.argtypes = [V, ctypes.POINTER(ctypes.c_int64)]
     _lib = L
     return L
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -103,6 +104,7 @@ const VariantInfo* find_variant(const char* name)
 }
 
 bool is_explicit(int solver) { return solver >= 0 && solver <= kGRL2; }
+bool is_implicit(int solver) { return solver == kImplicitEuler || solver == kESDIRK23a; }
 
 void default_params(int solver, goss_hip_solver_params* p)
 {
@@ -145,6 +147,8 @@ struct goss_hip_system {
   uint32_t num_threads = 0;
   uint32_t steps_per_launch = 200;
   IndexMap map{};
+  ImplicitBuffers ib{};  // implicit solvers only
+  int* h_error = nullptr;  // pinned copy of ib.error
 
   size_t elem() const { return prec == kFP64 ? sizeof(double) : sizeof(float); }
   bool is_f64() const { return prec == kFP64; }
@@ -190,8 +194,25 @@ struct goss_hip_system {
     else fn(float());
   }
 
+  // Sub-step counts must be finite and bounded (a kernel loops over them).
+  void check_steps(const double2* h, size_t n) const
+  {
+    const double ldt_0 = sp.internal_time_step;
+    for (size_t i = 0; i < n; ++i) {
+      const double t = h[i].x, dt = h[i].y;
+      if (!std::isfinite(t) || !std::isfinite(dt) || dt < 0)
+        throw Error(GOSS_HIP_EINVAL, "time and interval must be finite and the interval non-negative");
+      if (ldt_0 > 0 && !(dt / ldt_0 < 1e9))
+        throw Error(GOSS_HIP_EINVAL, "interval / internal_time_step exceeds 1e9 sub-steps");
+    }
+  }
+
   void launch_steps(const double2* d, size_t n, double* field_out, int tangled)
   {
+    if (is_implicit(solver)) {
+      launch_implicit(d, n, field_out, tangled);
+      return;
+    }
     const ExplicitLaunch launch = v->explicit_launch[solver][prec];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not available for this model");
     for (size_t off = 0; off < n; off += steps_per_launch) {
@@ -212,6 +233,55 @@ struct goss_hip_system {
       a.fo.tangled = tangled;
       a.stream = stream;
       hip_check(launch(a), "kernel launch");
+    }
+  }
+
+  ImplicitParams implicit_params() const
+  {
+    ImplicitParams ip;
+    ip.ldt_0 = sp.internal_time_step;
+    ip.eta_0 = sp.eta_0;
+    ip.kappa = sp.kappa;
+    ip.relative_tolerance = sp.relative_tolerance;
+    ip.max_relative_previous_residual = sp.max_relative_previous_residual;
+    ip.min_dt = sp.min_dt;
+    ip.max_iterations = sp.max_iterations;
+    ip.always_recompute_jacobian = sp.always_recompute_jacobian;
+    ip.num_refinements = sp.num_refinements_without_always_recomputing_jacobian;
+    return ip;
+  }
+
+  // ImplicitEuler / ESDIRK23a with per-node solver state.  A node whose step
+  // size falls below min_dt stops and flags the system; the call then fails
+  // with the reference's message (ImplicitEuler.cpp:132-141).
+  void launch_implicit(const double2* d, size_t n, double* field_out, int tangled)
+  {
+    const ImplicitLaunch launch = v->implicit_launch[solver == kImplicitEuler ? 0 : 1];
+    if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
+    for (size_t off = 0; off < n; off += steps_per_launch) {
+      const int cnt = int(std::min<size_t>(steps_per_launch, n - off));
+      ImplicitArgs a;
+      a.y = static_cast<double*>(y);
+      a.fp = static_cast<const double*>(fp);
+      a.steps = d + off;
+      a.nouter = cnt;
+      a.shared = shared.data();
+      a.ip = implicit_params();
+      a.B = ib;
+      a.N = N;
+      a.ld = ld;
+      a.fo.out = field_out ? field_out + off * size_t(N) * v->NFS : nullptr;
+      a.fo.row_stride = size_t(N) * v->NFS;
+      a.fo.tangled = tangled;
+      a.stream = stream;
+      hip_check(launch(a), "kernel launch");
+    }
+    hip_check(hipMemcpyAsync(h_error, ib.error, sizeof(int), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(flag)");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (*h_error) {
+      std::ostringstream msg;
+      msg << "Newtons solver failed to converge as dt become smaller than 'min_dt' " << sp.min_dt << std::endl;
+      throw Error(GOSS_HIP_ESOLVER, msg.str());
     }
   }
 
@@ -246,6 +316,12 @@ struct goss_hip_system {
     if (h_steps) (void)hipHostFree(h_steps);
     if (d_stage) (void)hipFree(d_stage);
     if (steps_free) (void)hipEventDestroy(steps_free);
+    if (ib.jac) (void)hipFree(ib.jac);
+    if (ib.jfac) (void)hipFree(ib.jfac);
+    if (ib.ds) (void)hipFree(ib.ds);
+    if (ib.is) (void)hipFree(ib.is);
+    if (ib.error) (void)hipFree(ib.error);
+    if (h_error) (void)hipHostFree(h_error);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -348,9 +424,16 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
     const VariantInfo* v = find_variant(variant);
     if (solver < 0 || solver >= kNumSolvers) throw Error(GOSS_HIP_EINVAL, "unknown solver");
     if (precision < 0 || precision >= kNumPrecisions) throw Error(GOSS_HIP_EINVAL, "unknown precision");
-    if (!is_explicit(solver)) throw Error(GOSS_HIP_EUNSUPPORTED, "solver not available on the device path");
-    if (!v->explicit_launch[solver][precision])
-      throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not compiled for this model");
+    if (is_explicit(solver)) {
+      if (!v->explicit_launch[solver][precision])
+        throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not compiled for this model");
+    } else if (is_implicit(solver)) {
+      if (precision != kFP64) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solvers run in FP64 only");
+      if (!v->implicit_launch[solver == kImplicitEuler ? 0 : 1])
+        throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
+    } else {
+      throw Error(GOSS_HIP_EUNSUPPORTED, "solver not available on the device path");
+    }
     if (v->S > kMaxStates || v->NFP > kMaxFieldParams) throw Error(GOSS_HIP_EUNSUPPORTED, "model too large");
     s = new goss_hip_system;
     s->v = v;
@@ -383,6 +466,21 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
     if (s->ylo) hip_check(hipMemsetAsync(s->ylo, 0, size_t(v->NCOMP) * s->ld * sizeof(float), s->stream), "hipMemset");
     hip_check(hipMemsetAsync(s->fp, 0, size_t(std::max(v->NFP, 1)) * s->ld * s->elem(), s->stream), "hipMemset");
     s->reset_default();
+    if (is_implicit(solver)) {
+      const size_t mat = size_t(v->S) * v->S * s->ld * sizeof(double);
+      hip_check(hipMalloc(&s->ib.jac, mat), "hipMalloc(jacobian)");
+      if (solver == kESDIRK23a) hip_check(hipMalloc(&s->ib.jfac, mat), "hipMalloc(factorized jacobian)");
+      hip_check(hipMalloc(&s->ib.ds, size_t(kNumDState) * s->ld * sizeof(double)), "hipMalloc(solver state)");
+      hip_check(hipMalloc(&s->ib.is, size_t(kNumIState) * s->ld * sizeof(int)), "hipMalloc(solver state)");
+      hip_check(hipMalloc(&s->ib.error, sizeof(int)), "hipMalloc(flag)");
+      hip_check(hipHostMalloc(&s->h_error, sizeof(int), hipHostMallocDefault), "hipHostMalloc(flag)");
+      hip_check(hipMemsetAsync(s->ib.jac, 0, mat, s->stream), "hipMemset");
+      if (s->ib.jfac) hip_check(hipMemsetAsync(s->ib.jfac, 0, mat, s->stream), "hipMemset");
+      hip_check(hipMemsetAsync(s->ib.error, 0, sizeof(int), s->stream), "hipMemset");
+      hip_check(layout_init_implicit(s->ib.ds, s->ib.is, s->N, s->ld, s->sp.eta_0, s->sp.always_recompute_jacobian,
+                                     s->stream),
+                "init_implicit_state");
+    }
     hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
   });
   if (rc != GOSS_HIP_OK) {
@@ -407,6 +505,7 @@ int goss_hip_system_forward(goss_hip_system* s, double t, double interval)
     check_sys(s);
     s->use_device();
     const double2 st = make_double2(t, interval);
+    s->check_steps(&st, 1);
     const double2* d = s->upload_steps(&st, 1);
     s->launch_steps(d, 1, nullptr, 1);
   });
@@ -421,6 +520,7 @@ int goss_hip_system_forward_times(goss_hip_system* s, const double* t, uint64_t 
     // interval = t[i+1] - t[i], computed as in ODESystemSolver::solve (ODESystemSolver.cpp:91-97).
     std::vector<double2> st(n - 1);
     for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
+    s->check_steps(st.data(), st.size());
     const double2* d = s->upload_steps(st.data(), st.size());
     s->launch_steps(d, st.size(), nullptr, 1);
   });
@@ -437,6 +537,7 @@ int goss_hip_system_solve(goss_hip_system* s, double* field_states, const double
     const size_t budget_rows = std::max<size_t>(1, (size_t(256) << 20) / std::max<size_t>(row * sizeof(double), 1));
     std::vector<double2> st(n - 1);
     for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
+    s->check_steps(st.data(), st.size());
     const double2* d = s->upload_steps(st.data(), st.size());
     for (size_t off = 0; off < st.size(); off += budget_rows) {
       const size_t cnt = std::min(budget_rows, st.size() - off);
@@ -656,6 +757,23 @@ int goss_hip_system_set_steps_per_launch(goss_hip_system* s, uint32_t steps)
     check_sys(s);
     if (steps == 0) throw Error(GOSS_HIP_EINVAL, "steps per launch must be positive");
     s->steps_per_launch = steps;
+  });
+}
+
+int goss_hip_system_get_solver_counters(goss_hip_system* s, int64_t* out)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    if (!is_implicit(s->solver)) return;
+    std::vector<int> h(size_t(kNumIState) * s->ld);
+    hip_check(hipMemcpyAsync(h.data(), s->ib.is, h.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream),
+              "hipMemcpyAsync(D2H)");
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    const int fields[4] = {kNewtonIts, kJacComp, kRejects, kAccepted};
+    for (int k = 0; k < 4; ++k)
+      for (uint32_t n = 0; n < s->N; ++n) out[k] += h[size_t(fields[k]) * s->ld + n];
   });
 }
 

@@ -1,0 +1,566 @@
+// Device kernels of the implicit solvers: ImplicitEuler (ImplicitEuler.cpp:57-152
+// over ImplicitODESolver::newton_solve, ImplicitODESolver.cpp:114-245) and
+// ESDIRK23a (ESDIRK23a.cpp:101-316).
+//
+// One node per lane.  Each node owns its solver state (Newton eta, Jacobian
+// reuse flags, ESDIRK step-size history) and its two S x S matrices: the
+// finite-difference Jacobian (ODE::compute_jacobian, ODE.cpp:47-67) and its
+// LU factorisation (ODE::lu_factorize, ODE.cpp:69-100).  Matrices live in HBM
+// as structure of arrays, element (i, j) of every node in one row of stride
+// ld, so a wavefront's 64 lanes touch 512 contiguous bytes per element.
+// Vectors live in registers (fully unrolled loops over the states).
+//
+// The arithmetic is FP64 throughout and follows the reference's operation
+// order (compiled with -ffp-contract=off).
+#pragma once
+
+#include "goss_kernels.h"
+
+namespace goss_hip {
+
+constexpr double kGossEps = 3.0e-16;  // constants.h:22
+
+// std::max / std::min semantics (a NaN first argument is returned), which the
+// reference uses; fmax/fmin would drop a NaN instead.
+GOSS_HD GOSS_INLINE double smax(const double a, const double b) { return (a < b) ? b : a; }
+GOSS_HD GOSS_INLINE double smin(const double a, const double b) { return (b < a) ? b : a; }
+
+// Public tunables (ImplicitODESolver.h:36-41, ImplicitEuler.h:36-37, ESDIRK23a.h:37-38).
+struct ImplicitParams {
+  double ldt_0;
+  double eta_0;
+  double kappa;
+  double relative_tolerance;
+  double max_relative_previous_residual;
+  double min_dt;
+  int max_iterations;
+  int always_recompute_jacobian;
+  int num_refinements;
+};
+
+// Per-node solver state, structure of arrays [field][ld].
+enum DState : int { kEta = 0, kEsEta, kLocError, kPrevDt, kNumDState };
+enum IState : int {
+  kRecompute = 0,   // ImplicitODESolver::_recompute_jacobian
+  kNewtonIts,       // _newton_iterations
+  kAlways,          // always_recompute_jacobian (sticky, ImplicitEuler.cpp:81-82)
+  kJustRefined,     // ImplicitEuler::_justrefined
+  kFirstStep,       // ESDIRK23a::first_step
+  kNewJacobian,     // ESDIRK23a::new_jacobian
+  kJacComp,         // counters
+  kRejects,
+  kAccepted,
+  kFailed,          // ImplicitEuler gave up below min_dt
+  kNumIState
+};
+
+struct ImplicitBuffers {
+  double* jac;   // [S*S][ld]
+  double* jfac;  // [S*S][ld] (ESDIRK23a only)
+  double* ds;    // [kNumDState][ld]
+  int* is;       // [kNumIState][ld]
+  int* error;    // one flag per system (set when a node fails)
+};
+
+// View of one node's matrix.
+struct MatView {
+  double* a;
+  unsigned ld;
+  int S;
+  GOSS_HD GOSS_INLINE double& operator()(int i, int j) const { return a[size_t(i * S + j) * ld]; }
+};
+
+template <class M>
+struct Implicit {
+  static constexpr int S = M::num_states;
+
+  // ODE::compute_jacobian (ODE.cpp:47-67): forward differences, column by
+  // column, delta_i = sqrt(1e-15 * max(1e-5, |y_i|)).
+  template <class A>
+  GOSS_HD static void fd_jacobian(const double* y, const double t, const A& acc, const MatView& J)
+  {
+    double f1[S];
+    M::template eval<double>(y, t, acc, f1);
+#pragma unroll 1
+    for (int i = 0; i < S; ++i) {
+      double yp[S];
+      double delta = 0.0;
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const double ysafe = y[j];
+        const double mx = 1e-5 > fabs(ysafe) ? 1e-5 : fabs(ysafe);
+        const double d = sqrt(1e-15 * mx);
+        yp[j] = (j == i) ? ysafe + d : ysafe;
+        delta = (j == i) ? d : delta;
+      }
+      double f2[S];
+      M::template eval<double>(yp, t, acc, f2);
+#pragma unroll
+      for (int j = 0; j < S; ++j) J(j, i) = (f2[j] - f1[j]) / delta;
+    }
+  }
+
+  // J <- I - scale_dt * J, then in-place LU (ODE.cpp:69-100); scale_dt is
+  // dt*alpha (ImplicitODESolver.cpp:92-105) or dt*gamma (ESDIRK23a.cpp:101-115).
+  GOSS_HD static void scale_add_identity(const MatView& src, const MatView& dst, const double scale)
+  {
+#pragma unroll 1
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int j = 0; j < S; ++j) dst(i, j) = src(i, j) * scale;
+#pragma unroll 1
+    for (int i = 0; i < S; ++i) dst(i, i) += 1.0 * 1.0;
+  }
+
+  GOSS_HD static void lu_factorize(const MatView& a)
+  {
+#pragma unroll 1
+    for (int k = 1; k < S; k++) {
+#pragma unroll 1
+      for (int i = 0; i <= k - 1; ++i) {
+        double acc = 0.0;
+        for (int r = 0; r <= i - 1; r++) acc += a(i, r) * a(r, k);
+        a(i, k) -= acc;
+        acc = 0.0;
+        for (int r = 0; r <= i - 1; r++) acc += a(k, r) * a(r, i);
+        a(k, i) = (a(k, i) - acc) / a(i, i);
+      }
+      double acc = 0.0;
+      for (int r = 0; r <= k - 1; r++) acc += a(k, r) * a(r, k);
+      a(k, k) -= acc;
+    }
+  }
+
+  // ODE::forward_backward_subst (ODE.cpp:102-132); b and x may alias.
+  GOSS_HD static void fb_subst(const MatView& a, const double* b, double* x)
+  {
+    x[0] = b[0];
+#pragma unroll
+    for (int i = 1; i < S; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j <= i - 1; ++j) acc = acc + a(i, j) * x[j];
+      x[i] = b[i] - acc;
+    }
+    x[S - 1] = x[S - 1] / a(S - 1, S - 1);
+#pragma unroll
+    for (int i = S - 2; i >= 0; i--) {
+      double acc = 0;
+#pragma unroll
+      for (int j = i + 1; j < S; ++j) acc = acc + a(i, j) * x[j];
+      x[i] = (x[i] - acc) / a(i, i);
+    }
+  }
+
+  GOSS_HD static GOSS_INLINE double norm(const double* v)
+  {
+    double acc = 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) acc += v[i] * v[i];
+    return sqrt(acc);
+  }
+};
+
+// Per-node state, loaded into registers for a launch.
+struct NodeImplicitState {
+  double eta, es_eta, loc_error, prev_dt;
+  int recompute, newton_its, always, justrefined, first_step, new_jacobian, jac_comp, rejects, accepted, failed;
+};
+
+GOSS_HD GOSS_INLINE NodeImplicitState load_state(const ImplicitBuffers& B, unsigned ld, unsigned n)
+{
+  NodeImplicitState s;
+  s.eta = B.ds[size_t(kEta) * ld + n];
+  s.es_eta = B.ds[size_t(kEsEta) * ld + n];
+  s.loc_error = B.ds[size_t(kLocError) * ld + n];
+  s.prev_dt = B.ds[size_t(kPrevDt) * ld + n];
+  s.recompute = B.is[size_t(kRecompute) * ld + n];
+  s.newton_its = B.is[size_t(kNewtonIts) * ld + n];
+  s.always = B.is[size_t(kAlways) * ld + n];
+  s.justrefined = B.is[size_t(kJustRefined) * ld + n];
+  s.first_step = B.is[size_t(kFirstStep) * ld + n];
+  s.new_jacobian = B.is[size_t(kNewJacobian) * ld + n];
+  s.jac_comp = B.is[size_t(kJacComp) * ld + n];
+  s.rejects = B.is[size_t(kRejects) * ld + n];
+  s.accepted = B.is[size_t(kAccepted) * ld + n];
+  s.failed = B.is[size_t(kFailed) * ld + n];
+  return s;
+}
+
+GOSS_HD GOSS_INLINE void store_state(const ImplicitBuffers& B, unsigned ld, unsigned n, const NodeImplicitState& s)
+{
+  B.ds[size_t(kEta) * ld + n] = s.eta;
+  B.ds[size_t(kEsEta) * ld + n] = s.es_eta;
+  B.ds[size_t(kLocError) * ld + n] = s.loc_error;
+  B.ds[size_t(kPrevDt) * ld + n] = s.prev_dt;
+  B.is[size_t(kRecompute) * ld + n] = s.recompute;
+  B.is[size_t(kNewtonIts) * ld + n] = s.newton_its;
+  B.is[size_t(kAlways) * ld + n] = s.always;
+  B.is[size_t(kJustRefined) * ld + n] = s.justrefined;
+  B.is[size_t(kFirstStep) * ld + n] = s.first_step;
+  B.is[size_t(kNewJacobian) * ld + n] = s.new_jacobian;
+  B.is[size_t(kJacComp) * ld + n] = s.jac_comp;
+  B.is[size_t(kRejects) * ld + n] = s.rejects;
+  B.is[size_t(kAccepted) * ld + n] = s.accepted;
+  B.is[size_t(kFailed) * ld + n] = s.failed;
+}
+
+// ---------------------------------------------------------------------------
+// ImplicitEuler
+// ---------------------------------------------------------------------------
+template <class M>
+struct ImplicitEulerNode {
+  static constexpr int S = M::num_states;
+  using I = Implicit<M>;
+
+  // ImplicitODESolver::newton_solve with prev = 0 and alpha = 1
+  // (ImplicitEuler.cpp:85), mass matrix = identity.
+  template <class A>
+  GOSS_HD static bool newton_solve(double* z, const double* y0, const double t, const double dt, const A& acc,
+                                      const ImplicitParams& p, NodeImplicitState& st, const MatView& J)
+  {
+    bool step_ok = true;
+    st.newton_its = 0;
+    double relative_previous_residual = 1.0, relative_residual = 1.0, previous_residual = 1.0,
+           initial_residual = 1.0, residual;
+    const bool always = st.always != 0;
+#pragma unroll 1
+    do {
+      double yz[S], f1[S], b[S], dz[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) yz[i] = y0[i] + z[i];
+      M::template eval<double>(yz, t, acc, f1);
+#pragma unroll
+      for (int i = 0; i < S; ++i) b[i] = -z[i] * 1.0 + dt * (0.0 + 1.0 * f1[i]);
+      residual = I::norm(b);
+      if (relative_residual < p.relative_tolerance) break;
+      if (st.recompute || always) {
+        I::fd_jacobian(yz, t, acc, J);
+        I::scale_add_identity(J, J, -dt * 1.0);
+        I::lu_factorize(J);
+        st.jac_comp += 1;
+        st.recompute = 0;
+      }
+      I::fb_subst(J, b, dz);
+      if (st.newton_its == 0) {
+        initial_residual = residual;
+        st.eta = st.eta > kGossEps ? st.eta : kGossEps;
+        st.eta = pow(st.eta, 0.8);
+      } else {
+        relative_previous_residual = residual / previous_residual;
+        st.recompute = relative_previous_residual >= p.max_relative_previous_residual;
+        if (relative_previous_residual >= 1) {
+          step_ok = false;
+          st.rejects++;
+          st.recompute = 1;
+          break;
+        }
+        const double scaled = smax(pow(relative_previous_residual, double(p.max_iterations - st.newton_its)),
+                                   kGossEps);
+        if (residual > (p.kappa * p.relative_tolerance * (1 - relative_previous_residual) / scaled))
+          st.recompute = 1;
+        st.eta = relative_previous_residual / (1.0 - relative_previous_residual);
+      }
+      if (st.newton_its > p.max_iterations) {
+        st.recompute = 1;
+        st.rejects++;
+        step_ok = false;
+        break;
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) z[i] += dz[i];
+      relative_residual = residual / initial_residual;
+      previous_residual = residual;
+      st.newton_its++;
+    } while (st.eta * relative_residual >= p.kappa * p.relative_tolerance);
+    return step_ok;
+  }
+
+  // ImplicitEuler::forward (ImplicitEuler.cpp:57-152).  Returns false when
+  // the step size falls below min_dt (the reference throws).
+  template <class A>
+  GOSS_HD static bool forward(double* y, double t, const double dt, const A& acc, const ImplicitParams& p,
+                                 NodeImplicitState& st, const MatView& J)
+  {
+    const double t_end = t + dt;
+    const double ldt_0 = p.ldt_0;
+    double ldt = ldt_0 > 0 ? ldt_0 : dt;
+    int num_refinements = 0;
+    const double eps = kGossEps * 1000;
+#pragma unroll 1
+    while (true) {
+      double z1[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) z1[i] = 0.0;
+      if (num_refinements > p.num_refinements) st.always = 1;
+      const bool step_ok = newton_solve(z1, y, t + ldt, ldt, acc, p, st, J);
+      if (step_ok) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) y[i] += z1[i];
+        t += ldt;
+        if (fabs(t - t_end) < eps) break;
+        if (!st.justrefined) {
+          const double tmp = 2.0 * ldt;
+          if (ldt_0 > 0. && tmp >= ldt_0)
+            ldt = ldt_0;
+          else
+            ldt = tmp;
+        } else {
+          st.justrefined = 0;
+        }
+        if ((t + ldt + kGossEps) > t_end) ldt = t_end - t;
+      } else {
+        ldt /= 2.0;
+        if (ldt < p.min_dt) return false;
+        st.recompute = 1;
+        st.justrefined = 1;
+        num_refinements += 1;
+      }
+    }
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// ESDIRK23a
+// ---------------------------------------------------------------------------
+struct EsdirkCoefficients {
+  double gamma, a21, a22, bh1, bh2, b1, b2, b3, c2, c3, c4;
+};
+
+__host__ __device__ inline EsdirkCoefficients esdirk_coefficients()
+{
+  // ESDIRK23a.cpp:27-44
+  EsdirkCoefficients k;
+  const double gamma = 0.43586652150845899942;
+  k.gamma = gamma;
+  k.a21 = gamma;
+  k.a22 = gamma;
+  k.bh1 = (-4 * gamma * gamma + 6 * gamma - 1) / (4 * gamma);  // a31
+  k.bh2 = (-2 * gamma + 1) / (4 * gamma);                       // a32
+  k.b1 = (6 * gamma - 1) / (12 * gamma);                        // a41
+  k.b2 = -1 / ((24 * gamma - 12) * gamma);                      // a42
+  k.b3 = (-6 * gamma * gamma + 6 * gamma - 1) / (6 * gamma - 3);  // a43
+  k.c2 = 2.0 * gamma;
+  k.c3 = 1.0;
+  k.c4 = 1.0;
+  return k;
+}
+
+template <class M>
+struct EsdirkNode {
+  static constexpr int S = M::num_states;
+  using I = Implicit<M>;
+  static constexpr double s_fac = 0.9, tol = 1.0e-5, etamin = 1.0e-10, kappa = 1.0e-1;
+
+  template <class A>
+  GOSS_HD static void ode_jacobian(const double* y, const double t, const A& acc, NodeImplicitState& st,
+                                      const MatView& J)
+  {
+    I::fd_jacobian(y, t, acc, J);
+    st.jac_comp++;
+    st.new_jacobian = 1;
+  }
+
+  GOSS_HD static void factorized(const EsdirkCoefficients& k, const double dt, const MatView& J,
+                                    const MatView& F)
+  {
+    I::scale_add_identity(J, F, -dt * k.a22);
+    I::lu_factorize(F);
+  }
+
+  // compute_stage_val (ESDIRK23a.cpp:201-258)
+  template <class A>
+  GOSS_HD static bool stage(const EsdirkCoefficients& k, double* z, const double* y, const double* prev,
+                               const double t_s, const double dt, const A& acc, const ImplicitParams& p,
+                               NodeImplicitState& st, const MatView& F)
+  {
+    st.newton_its = 0;
+    bool conv_ok = true;
+    double dz[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) dz[i] = 0.0;
+    st.es_eta = pow(smax(st.es_eta, etamin), 0.8);
+    double ndz;
+#pragma unroll 1
+    do {
+      double y_int[S], z_int[S], b[S], prev_dz[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) y_int[i] = y[i] + dt * prev[i] + k.gamma * dt * z[i];
+      M::template eval<double>(y_int, t_s, acc, z_int);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        b[i] = -z[i] + z_int[i];
+        prev_dz[i] = dz[i];
+      }
+      I::fb_subst(F, b, dz);
+      ndz = I::norm(dz);
+      if (st.newton_its >= 1) {
+        const double theta = ndz / I::norm(prev_dz);
+        if (!(theta <= 1)) {
+          st.rejects++;
+          conv_ok = false;
+          break;
+        }
+        st.es_eta = theta / (1.0 - theta);
+      }
+      if (st.newton_its > p.max_iterations) {
+        st.rejects++;
+        conv_ok = false;
+        break;
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) z[i] += dz[i];
+      st.newton_its++;
+    } while (st.es_eta * ndz > kappa * tol);
+    return conv_ok;
+  }
+
+  // do_step (ESDIRK23a.cpp:134-199)
+  template <class A>
+  GOSS_HD static bool do_step(const EsdirkCoefficients& k, double* y, const double t0, const double dt,
+                                 const A& acc, const ImplicitParams& p, NodeImplicitState& st, const MatView& F)
+  {
+    double z1[S], z2[S], z3[S], z4[S], prev[S];
+    M::template eval<double>(y, t0, acc, z1);
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+      z2[i] = z1[i];
+      prev[i] = z1[i] * k.a21;
+    }
+    bool ok = stage(k, z2, y, prev, t0 + k.c2 * dt, dt, acc, p, st, F);
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < S; i++) {
+        z3[i] = z2[i];
+        prev[i] = z1[i] * k.bh1 + z2[i] * k.bh2;
+      }
+      ok = stage(k, z3, y, prev, t0 + k.c3 * dt, dt, acc, p, st, F);
+    }
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < S; i++) {
+        z4[i] = z3[i];
+        prev[i] = z1[i] * k.b1 + z2[i] * k.b2 + z3[i] * k.b3;
+      }
+      ok = stage(k, z4, y, prev, t0 + k.c4 * dt, dt, acc, p, st, F);
+    }
+    if (ok) {
+      double yh[S];
+#pragma unroll
+      for (int i = 0; i < S; i++)
+        yh[i] = dt * ((k.b1 - k.bh1) * z1[i] + (k.b2 - k.bh2) * z2[i] + (k.b3 - k.gamma) * z3[i] +
+                      k.gamma * z4[i]);
+      I::fb_subst(F, yh, yh);
+      st.loc_error = I::norm(yh);
+      if (st.loc_error > tol) {
+        st.rejects++;
+        ok = false;
+      } else {
+#pragma unroll
+        for (int i = 0; i < S; i++) y[i] = y[i] + dt * (k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * z3[i] + k.gamma * z4[i]);
+        st.accepted++;
+      }
+    }
+    return ok;
+  }
+
+  // forward (ESDIRK23a.cpp:280-316) with advance_one_step (117-132) and
+  // reduce_time_step (260-278) inlined.
+  template <class A>
+  GOSS_HD static void forward(double* y, const double t0, const double interval, const A& acc,
+                                 const ImplicitParams& p, NodeImplicitState& st, const MatView& J, const MatView& F)
+  {
+    const EsdirkCoefficients k = esdirk_coefficients();
+    double local_t = 0;
+    double dt = smax(s_fac * st.prev_dt * pow((tol / st.loc_error), 0.25), p.min_dt);
+    dt = smin(dt, interval);
+    if (st.first_step) {
+      st.prev_dt = dt;
+      ode_jacobian(y, t0, acc, st, J);
+    }
+#pragma unroll 1
+    while (local_t < interval) {
+      if (dt * 1.1 >= interval - local_t) dt = interval - local_t;
+      if (st.newton_its >= 2) ode_jacobian(y, t0 + local_t, acc, st, J);
+      if (fabs((dt - st.prev_dt) / st.prev_dt) > 0.2 || st.new_jacobian) factorized(k, dt, J, F);
+      // advance_one_step
+      const double ts = t0 + local_t;
+      bool accepted = do_step(k, y, ts, dt, acc, p, st, F);
+#pragma unroll 1
+      while (!accepted && dt > p.min_dt) {
+        if (!st.new_jacobian) ode_jacobian(y, ts, acc, st, J);
+        // reduce_time_step (ESDIRK23a.cpp:260-278).  For a finite loc_error
+        // at most 313 passes bring e below tol; the cap only ends the loop
+        // for an infinite loc_error, where the reference would not return.
+        double reduction_factor = 1.0;
+        double e = st.loc_error;
+        int passes = 0;
+        do {
+          reduction_factor *= 0.6;
+          e *= 0.1;
+        } while (e > tol && ++passes < 400);
+        dt = smax(reduction_factor * dt, p.min_dt);
+        factorized(k, dt, J, F);
+        accepted = do_step(k, y, ts, dt, acc, p, st, F);
+      }
+      st.new_jacobian = 0;
+      st.prev_dt = dt;
+      local_t += dt;
+      dt = smax(s_fac * st.prev_dt * pow((tol / st.loc_error), 0.25), p.min_dt);
+    }
+    st.first_step = 0;
+  }
+};
+
+// The fused implicit kernel: `nouter` outer steps (t_k, interval_k).
+template <class M, int SOLVER>
+__global__ void __launch_bounds__(kBlock)
+implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const double2* __restrict__ steps,
+                const int nouter, const SharedParamsOf<M, double> sp, const ImplicitParams ip,
+                const ImplicitBuffers B, const unsigned N, const unsigned ld, const FieldOut fo)
+{
+  constexpr int S = M::num_states;
+  constexpr int NFP = M::num_field_parameters;
+  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= N) return;
+  double ys[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) ys[i] = y[size_t(i) * ld + node];
+  double fpv[NFP > 0 ? NFP : 1];
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
+  const DevAcc<M, double> acc{sp, fpv};
+  NodeImplicitState st = load_state(B, ld, node);
+  const MatView J{B.jac + node, ld, S};
+  const MatView F{B.jfac + node, ld, S};
+
+#pragma unroll 1
+  for (int k = 0; k < nouter; ++k) {
+    if (!st.failed) {
+      const double t = steps[k].x;
+      const double dt = steps[k].y;
+      if constexpr (SOLVER == kImplicitEuler) {
+        if (!ImplicitEulerNode<M>::forward(ys, t, dt, acc, ip, st, J)) {
+          st.failed = 1;
+          atomicOr(B.error, 1);
+        }
+      } else {
+        EsdirkNode<M>::forward(ys, t, dt, acc, ip, st, J, F);
+      }
+    }
+    if (fo.out) {
+      constexpr int Fs = M::num_field_states;
+#pragma unroll
+      for (int i = 0; i < Fs; ++i) {
+        const size_t idx = fo.tangled ? size_t(node) * Fs + i : size_t(N) * i + node;
+        fo.out[size_t(k) * fo.row_stride + idx] = ys[M::field_state_indices[i]];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < S; ++i) y[size_t(i) * ld + node] = ys[i];
+  store_state(B, ld, node, st);
+}
+
+}  // namespace goss_hip

@@ -42,4 +42,7 @@ template <class R>
 hipError_t layout_fill_defaults(void* y, void* ylo, void* fp, const IndexMap& m, const DefaultValues& d, unsigned N,
                                 unsigned ld, hipStream_t s);
 
+hipError_t layout_init_implicit(double* ds, int* is, unsigned N, unsigned ld, double eta_0, int always,
+                                hipStream_t s);
+
 }  // namespace goss_hip

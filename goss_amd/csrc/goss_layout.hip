@@ -2,6 +2,7 @@
 // states, tangled/untangled field arrays of ODESystemSolver.h:238-312) and the
 // device structure-of-arrays buffers of each precision.
 #include "goss_layout.h"
+#include "goss_implicit.h"
 
 namespace goss_hip {
 
@@ -108,9 +109,36 @@ __global__ void fill_defaults(R* __restrict__ y, R* __restrict__ ylo, R* __restr
   for (int j = 0; j < m.NFP; ++j) fp[size_t(j) * ld + n] = R(d.fp[j]);
 }
 
+// Per-node implicit solver state at construction: ImplicitODESolver::reset
+// (ImplicitODESolver.cpp:80-90: recompute Jacobian, eta = eta_0) and the
+// ESDIRK23a constructor values (ESDIRK23a.cpp:27-44: eta 1, loc_error 1e9,
+// prev_dt 0.1, first_step, new_jacobian).
+__global__ void init_implicit_state(double* __restrict__ ds, int* __restrict__ is, unsigned N, unsigned ld,
+                                    double eta_0, int always)
+{
+  const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  ds[size_t(kEta) * ld + n] = eta_0;
+  ds[size_t(kEsEta) * ld + n] = 1.0;
+  ds[size_t(kLocError) * ld + n] = 1e9;
+  ds[size_t(kPrevDt) * ld + n] = 0.1;
+  for (int k = 0; k < kNumIState; ++k) is[size_t(k) * ld + n] = 0;
+  is[size_t(kRecompute) * ld + n] = 1;
+  is[size_t(kAlways) * ld + n] = always;
+  is[size_t(kFirstStep) * ld + n] = 1;
+  is[size_t(kNewJacobian) * ld + n] = 1;
+}
+
 inline dim3 grid_for(size_t work) { return dim3(unsigned((work + 255) / 256)); }
 
 }  // namespace
+
+hipError_t layout_init_implicit(double* ds, int* is, unsigned N, unsigned ld, double eta_0, int always, hipStream_t s)
+{
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(init_implicit_state, grid_for(N), dim3(256), 0, s, ds, is, N, ld, eta_0, always);
+  return hipGetLastError();
+}
 
 template <class R>
 hipError_t layout_states(LayoutOp op, void* y, void* ylo, double* buf, const IndexMap& m, unsigned N, unsigned ld,

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "goss_kernels.h"
+#include "goss_implicit.h"
 
 namespace goss_hip {
 
@@ -27,6 +28,21 @@ struct ExplicitArgs {
 
 typedef hipError_t (*ExplicitLaunch)(const ExplicitArgs&);
 
+struct ImplicitArgs {
+  double* y;               // FP64 states [S][ld]
+  const double* fp;        // FP64 field parameters [NFP][ld]
+  const double2* steps;
+  int nouter;
+  const double* shared;
+  ImplicitParams ip;
+  ImplicitBuffers B;
+  unsigned N, ld;
+  FieldOut fo;
+  hipStream_t stream;
+};
+
+typedef hipError_t (*ImplicitLaunch)(const ImplicitArgs&);
+
 struct VariantInfo {
   const char* name;
   int S, P, NFS, NFP, NCOMP;
@@ -40,6 +56,7 @@ struct VariantInfo {
   const char* const* field_state_names;
   const char* const* field_parameter_names;
   ExplicitLaunch explicit_launch[kNumSolvers][kNumPrecisions];
+  ImplicitLaunch implicit_launch[2];  // [0] ImplicitEuler, [1] ESDIRK23a (FP64)
 };
 
 // All variants, in registration order.

@@ -30,12 +30,12 @@ struct M {
   static constexpr const unsigned char* linear_terms = NS::linear_terms;
 
   template <typename R, typename A, typename T>
-  __device__ GOSS_INLINE static void eval(const R* y, const T t, const A& a, R* out)
+  GOSS_HD GOSS_INLINE static void eval(const R* y, const T t, const A& a, R* out)
   {
     NS::eval<R>(y, t, a, out);
   }
   template <typename R, typename A, typename T>
-  __device__ GOSS_INLINE static void linearized_eval(const R* y, const T t, const A& a, R* lin, R* rhs, bool ol)
+  GOSS_HD GOSS_INLINE static void linearized_eval(const R* y, const T t, const A& a, R* lin, R* rhs, bool ol)
   {
     NS::linearized_eval<R>(y, t, a, lin, rhs, ol);
   }
@@ -72,6 +72,18 @@ hipError_t launch_explicit(const ExplicitArgs& a)
                      sp, a.ldt_0, R(a.delta), a.N, a.ld, a.fo);
   return hipGetLastError();
 }
+
+#if defined(GOSS_PREC_F64)
+template <int SOLVER>
+hipError_t launch_implicit(const ImplicitArgs& a)
+{
+  const SharedParamsOf<M, double> sp = make_shared_params<double>(a.shared);
+  const dim3 grid((a.N + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((implicit_kernel<M, SOLVER>), grid, dim3(kBlock), 0, a.stream, a.y, a.fp, a.steps, a.nouter, sp,
+                     a.ip, a.B, a.N, a.ld, a.fo);
+  return hipGetLastError();
+}
+#endif
 
 template <class R, bool COMP>
 void register_family(VariantInfo& v, int prec)
@@ -111,6 +123,8 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
 {
   fill_info(v);
   register_family<double, false>(v, kFP64);
+  v.implicit_launch[0] = &launch_implicit<kImplicitEuler>;
+  v.implicit_launch[1] = &launch_implicit<kESDIRK23a>;
 }
 #elif defined(GOSS_PREC_F32)
 void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)

@@ -164,6 +164,11 @@ class DeviceSystem:
         check(lib().goss_hip_system_get_num_threads(self._h, ctypes.byref(v)))
         return v.value
 
+    def solver_counters(self) -> np.ndarray:
+        out = (ctypes.c_int64 * 4)()
+        check(lib().goss_hip_system_get_solver_counters(self._h, out))
+        return np.array(list(out))
+
     def stream(self) -> int:
         v = ctypes.c_void_p()
         check(lib().goss_hip_system_stream(self._h, ctypes.byref(v)))

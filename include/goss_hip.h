@@ -166,6 +166,12 @@ int goss_hip_system_get_num_threads(goss_hip_system* sys, uint32_t* num_threads)
 int goss_hip_system_num_nodes(goss_hip_system* sys, uint32_t* num_nodes);
 /* Outer steps fused into one kernel launch by forward_times / solve. */
 int goss_hip_system_set_steps_per_launch(goss_hip_system* sys, uint32_t steps);
+/* Sums over nodes of the implicit solvers' per-node counters:
+ * out[0] Newton iterations of the last stage/solve (ImplicitODESolver::_newton_iterations),
+ * out[1] Jacobian evaluations (num_jac_comp, ImplicitODESolver.h:65-68),
+ * out[2] rejected Newton solves / steps, out[3] accepted ESDIRK23a steps.
+ * Zeros for explicit solvers. */
+int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
 /* Wait for all work of the system. */
 int goss_hip_system_synchronize(goss_hip_system* sys);
 /* The system's HIP stream (hipStream_t) for callers that order their own work. */

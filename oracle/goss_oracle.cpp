@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -158,6 +159,49 @@ int solver_kind(const std::string& s)
 struct SolverParams {
   double ldt = -1.0;  // ODESolver::_ldt (ODESolver.h:137)
   double delta = 1e-8;  // GRL1.h:39
+  // ImplicitODESolver.h:36-41
+  double eta_0 = 1.0;
+  double kappa = 0.1;
+  double relative_tolerance = 1.e-12;
+  int max_iterations = 30;
+  double max_relative_previous_residual = 0.01;
+  bool always_recompute_jacobian = false;
+  // ImplicitEuler.h:36-37 / ESDIRK23a.h:37-38
+  int num_refinements_without_always_recomputing_jacobian = 2;
+  double min_dt = 0.0001;
+};
+
+constexpr double GOSS_EPS = 3.0e-16;  // constants.h:22
+
+// Mutable state of one implicit solver instance (ImplicitODESolver.h:83-101,
+// ImplicitEuler.h:66-68, ESDIRK23a.h:94-115).  The reference keeps one such
+// instance per thread and carries it from node to node; the per-node mode
+// keeps one per node.
+struct ImplicitState {
+  std::vector<double> jac, jfac;
+  // ImplicitODESolver
+  double eta = 1.0;
+  bool recompute_jacobian = true;
+  int newton_iterations = 0;
+  long rejects = 0, jac_comp = 0;
+  bool always_recompute_jacobian = false;  // sticky copy of the public parameter
+  // ImplicitEuler
+  bool justrefined = false;
+  // ESDIRK23a (ctor values, ESDIRK23a.cpp:27-44)
+  double es_eta = 1.0;
+  double loc_error = 1e9;
+  double prev_dt = 0.1;
+  bool first_step = true;
+  bool new_jacobian = true;
+  long nfevals = 0, num_rejected = 0;
+  long accepted = 0;
+  void init(int S, const SolverParams& sp)
+  {
+    jac.assign(size_t(S) * S, 0.0);
+    jfac.assign(size_t(S) * S, 0.0);
+    eta = sp.eta_0;  // ImplicitODESolver::reset (ImplicitODESolver.cpp:80-90)
+    always_recompute_jacobian = sp.always_recompute_jacobian;
+  }
 };
 
 // Sub-step count of the fixed-step solvers, e.g. GRL1.cpp:65-67.
@@ -169,9 +213,13 @@ inline unsigned long substeps(double ldt_0, double dt)
 // Per-thread scratch, sized to the model.
 struct Work {
   std::vector<double> f1, f2, k1, k2, k3, k4, tmp, y2;
+  std::vector<double> z1, z2, z3, z4, yh, prev, b, dz, yz, fy, y_int, z_int, prev_dz, ff1, ff2;
+  ImplicitState carried;  // the per-thread solver instance of the reference
   void size(int S)
   {
-    for (auto* v : {&f1, &f2, &k1, &k2, &k3, &k4, &tmp, &y2}) v->assign(S, 0.0);
+    for (auto* v : {&f1, &f2, &k1, &k2, &k3, &k4, &tmp, &y2, &z1, &z2, &z3, &z4, &yh, &prev, &b, &dz, &yz, &fy,
+                    &y_int, &z_int, &prev_dz, &ff1, &ff2})
+      v->assign(S, 0.0);
   }
 };
 
@@ -264,6 +312,327 @@ void forward_explicit(int kind, const VariantOps& m, const SolverParams& sp, dou
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Implicit solvers
+// ---------------------------------------------------------------------------
+
+inline double l2norm(const double* v, int S)
+{
+  // ImplicitODESolver::norm (ImplicitODESolver.cpp:247-256).
+  double acc = 0;
+  for (int i = 0; i < S; ++i) acc += v[i] * v[i];
+  return std::sqrt(acc);
+}
+
+// ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105):
+// FD Jacobian at y, scaled by -dt*alpha, identity added (all states are
+// differential), LU factorized in place.
+void ie_factorized_jacobian(const VariantOps& m, ImplicitState& st, double* y, double t, double dt, double alpha,
+                            const double* p, const double* c, Work& w)
+{
+  const int S = m.S;
+  fd_jacobian(m, y, t, p, c, st.jac.data(), w.ff1.data(), w.ff2.data());
+  const double scale = -dt * alpha;
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < S; ++j) st.jac[i * S + j] *= scale;
+  for (int i = 0; i < S; ++i) st.jac[i * S + i] += 1.0 * 1.0;
+  lu_factorize(S, st.jac.data());
+  st.jac_comp += 1;
+}
+
+// ImplicitODESolver::newton_solve (ImplicitODESolver.cpp:114-245).
+bool newton_solve(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* z, const double* prev,
+                  const double* y0, double t, double dt, double alpha, bool always_recompute, const double* p,
+                  const double* c, Work& w)
+{
+  const int S = m.S;
+  bool step_ok = true;
+  st.newton_iterations = 0;
+  double relative_previous_residual = 1.0, relative_residual = 1.0, previous_residual = 1.0,
+         initial_residual = 1.0, residual;
+  double* yz = w.yz.data();
+  double* f1 = w.fy.data();
+  double* b = w.b.data();
+  double* dz = w.dz.data();
+  do {
+    for (int i = 0; i < S; ++i) yz[i] = y0[i] + z[i];
+    m.eval(yz, t, p, c, f1);
+    for (int i = 0; i < S; ++i) b[i] = -z[i] * 1.0 + dt * (prev[i] + alpha * f1[i]);
+    residual = l2norm(b, S);
+    if (relative_residual < sp.relative_tolerance) break;
+    if (st.recompute_jacobian || always_recompute) {
+      ie_factorized_jacobian(m, st, yz, t, dt, alpha, p, c, w);
+      st.recompute_jacobian = false;
+    }
+    fb_subst(S, st.jac.data(), b, dz);
+    if (st.newton_iterations == 0) {
+      initial_residual = residual;
+      st.eta = st.eta > GOSS_EPS ? st.eta : GOSS_EPS;
+      st.eta = std::pow(st.eta, 0.8);
+    } else {
+      relative_previous_residual = residual / previous_residual;
+      st.recompute_jacobian = relative_previous_residual >= sp.max_relative_previous_residual;
+      if (relative_previous_residual >= 1) {
+        step_ok = false;
+        st.rejects++;
+        st.recompute_jacobian = true;
+        break;
+      }
+      const double scaled = std::max(std::pow(relative_previous_residual, sp.max_iterations - st.newton_iterations),
+                                     GOSS_EPS);
+      if (residual > (sp.kappa * sp.relative_tolerance * (1 - relative_previous_residual) / scaled))
+        st.recompute_jacobian = true;
+      st.eta = relative_previous_residual / (1.0 - relative_previous_residual);
+    }
+    if (st.newton_iterations > sp.max_iterations) {
+      st.recompute_jacobian = true;
+      st.rejects++;
+      step_ok = false;
+      break;
+    }
+    for (int i = 0; i < S; ++i) z[i] += dz[i];
+    relative_residual = residual / initial_residual;
+    previous_residual = residual;
+    st.newton_iterations++;
+  } while (st.eta * relative_residual >= sp.kappa * sp.relative_tolerance);
+  return step_ok;
+}
+
+// ImplicitEuler::forward (ImplicitEuler.cpp:57-152).
+void implicit_euler_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* y, double t,
+                            double dt, const double* p, const double* c, Work& w)
+{
+  const int S = m.S;
+  const double t_end = t + dt;
+  const double ldt_0 = sp.ldt;
+  double ldt = ldt_0 > 0 ? ldt_0 : dt;
+  int num_refinements = 0;
+  const double eps = GOSS_EPS * 1000;
+  double* z1 = w.z1.data();
+  double* prev = w.prev.data();
+  for (int i = 0; i < S; ++i) prev[i] = 0.0;
+  while (true) {
+    for (int i = 0; i < S; ++i) z1[i] = 0.0;
+    if (num_refinements > sp.num_refinements_without_always_recomputing_jacobian)
+      st.always_recompute_jacobian = true;
+    const bool step_ok = newton_solve(m, sp, st, z1, prev, y, t + ldt, ldt, 1.0, st.always_recompute_jacobian, p,
+                                      c, w);
+    if (step_ok) {
+      for (int i = 0; i < S; ++i) y[i] += z1[i];
+      t += ldt;
+      if (std::fabs(t - t_end) < eps) break;
+      if (!st.justrefined) {
+        const double tmp = 2.0 * ldt;
+        if (ldt_0 > 0. && tmp >= ldt_0)
+          ldt = ldt_0;
+        else
+          ldt = tmp;
+      } else {
+        st.justrefined = false;
+      }
+      if ((t + ldt + GOSS_EPS) > t_end) ldt = t_end - t;
+    } else {
+      ldt /= 2.0;
+      if (ldt < sp.min_dt) {
+        std::ostringstream msg;  // ImplicitEuler.cpp:132-141
+        msg << "Newtons solver failed to converge as dt become smaller than 'min_dt' " << sp.min_dt << std::endl;
+        throw std::runtime_error(msg.str());
+      }
+      st.recompute_jacobian = true;
+      st.justrefined = true;
+      num_refinements += 1;
+    }
+  }
+}
+
+// ESDIRK23a coefficients (ESDIRK23a.cpp:27-44).
+struct EsdirkCoef {
+  double gamma, a21, a22, a31, a32, a33, a41, a42, a43, a44, b1, b2, b3, b4, bh1, bh2, bh3, c2, c3, c4;
+  double s_fac = 0.9, tol = 1.0e-5, etamin = 1.0e-10, kappa = 1.0e-1;
+  EsdirkCoef()
+  {
+    gamma = 0.43586652150845899942;
+    a21 = gamma;
+    a22 = gamma;
+    a31 = (-4 * gamma * gamma + 6 * gamma - 1) / (4 * gamma);
+    a32 = (-2 * gamma + 1) / (4 * gamma);
+    a33 = gamma;
+    a41 = (6 * gamma - 1) / (12 * gamma);
+    a42 = -1 / ((24 * gamma - 12) * gamma);
+    a43 = (-6 * gamma * gamma + 6 * gamma - 1) / (6 * gamma - 3);
+    a44 = gamma;
+    b1 = a41;
+    b2 = a42;
+    b3 = a43;
+    b4 = a44;
+    bh1 = a31;
+    bh2 = a32;
+    bh3 = a33;
+    c2 = 2.0 * gamma;
+    c3 = 1.0;
+    c4 = 1.0;
+  }
+};
+
+// ESDIRK23a::compute_ode_jacobian (ESDIRK23a.h:57-62).
+void es_ode_jacobian(const VariantOps& m, ImplicitState& st, double* y, double t, const double* p, const double* c,
+                     Work& w)
+{
+  fd_jacobian(m, y, t, p, c, st.jac.data(), w.ff1.data(), w.ff2.data());
+  st.jac_comp++;
+  st.new_jacobian = true;
+}
+
+// ESDIRK23a::compute_factorized_jacobian(dt) (ESDIRK23a.cpp:101-115).
+void es_factorized(const VariantOps& m, const EsdirkCoef& k, ImplicitState& st, double dt)
+{
+  const int S = m.S;
+  st.jfac = st.jac;
+  const double scale = -dt * k.a22;
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < S; ++j) st.jfac[i * S + j] *= scale;
+  for (int i = 0; i < S; ++i) st.jfac[i * S + i] += 1.0 * 1.0;
+  lu_factorize(S, st.jfac.data());
+}
+
+// ESDIRK23a::compute_stage_val (ESDIRK23a.cpp:201-258).
+bool es_stage(const VariantOps& m, const SolverParams& sp, const EsdirkCoef& k, ImplicitState& st, double* z,
+              const double* y, const double* prev, double t_s, double dt, const double* p, const double* c, Work& w)
+{
+  const int S = m.S;
+  st.newton_iterations = 0;
+  bool conv_ok = true;
+  double* y_int = w.y_int.data();
+  double* z_int = w.z_int.data();
+  double* dz = w.dz.data();
+  double* prev_dz = w.prev_dz.data();
+  double* b = w.b.data();
+  for (int i = 0; i < S; ++i) dz[i] = 0.0;
+  st.es_eta = std::pow(std::max(st.es_eta, k.etamin), 0.8);
+  do {
+    for (int i = 0; i < S; ++i) y_int[i] = y[i] + dt * prev[i] + k.gamma * dt * z[i];
+    m.eval(y_int, t_s, p, c, z_int);
+    st.nfevals++;
+    for (int i = 0; i < S; ++i) b[i] = -z[i] + z_int[i];
+    for (int i = 0; i < S; ++i) prev_dz[i] = dz[i];
+    fb_subst(S, st.jfac.data(), b, dz);
+    if (st.newton_iterations >= 1) {
+      const double theta = l2norm(dz, S) / l2norm(prev_dz, S);
+      if (!(theta <= 1)) {
+        st.num_rejected++;
+        conv_ok = false;
+        break;
+      }
+      st.es_eta = theta / (1.0 - theta);
+    }
+    if (st.newton_iterations > sp.max_iterations) {
+      st.num_rejected++;
+      conv_ok = false;
+      break;
+    }
+    for (int i = 0; i < S; ++i) z[i] += dz[i];
+    st.newton_iterations++;
+  } while (st.es_eta * l2norm(dz, S) > k.kappa * k.tol);
+  return conv_ok;
+}
+
+// ESDIRK23a::do_step (ESDIRK23a.cpp:134-199).
+bool es_do_step(const VariantOps& m, const SolverParams& sp, const EsdirkCoef& k, ImplicitState& st, double* y,
+                double t0, double dt, const double* p, const double* c, Work& w)
+{
+  const int S = m.S;
+  double *z1 = w.z1.data(), *z2 = w.z2.data(), *z3 = w.z3.data(), *z4 = w.z4.data(), *yh = w.yh.data(),
+         *prev = w.prev.data();
+  m.eval(y, t0, p, c, z1);
+  st.nfevals++;
+  for (int i = 0; i < S; i++) {
+    z2[i] = z1[i];
+    prev[i] = z1[i] * k.a21;
+  }
+  bool ok = es_stage(m, sp, k, st, z2, y, prev, t0 + k.c2 * dt, dt, p, c, w);
+  if (ok) {
+    for (int i = 0; i < S; i++) {
+      z3[i] = z2[i];
+      prev[i] = z1[i] * k.bh1 + z2[i] * k.bh2;
+    }
+    ok = es_stage(m, sp, k, st, z3, y, prev, t0 + k.c3 * dt, dt, p, c, w);
+  }
+  if (ok) {
+    for (int i = 0; i < S; i++) {
+      z4[i] = z3[i];
+      prev[i] = z1[i] * k.b1 + z2[i] * k.b2 + z3[i] * k.b3;
+    }
+    ok = es_stage(m, sp, k, st, z4, y, prev, t0 + k.c4 * dt, dt, p, c, w);
+  }
+  if (ok) {
+    for (int i = 0; i < S; i++)
+      yh[i] = dt * ((k.b1 - k.bh1) * z1[i] + (k.b2 - k.bh2) * z2[i] + (k.b3 - k.gamma) * z3[i] + k.gamma * z4[i]);
+    fb_subst(S, st.jfac.data(), yh, yh);
+    st.loc_error = l2norm(yh, S);
+    if (st.loc_error > k.tol) {
+      st.num_rejected++;
+      ok = false;
+    } else {
+      for (int i = 0; i < S; i++) y[i] = y[i] + dt * (k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * z3[i] + k.gamma * z4[i]);
+      st.accepted++;
+    }
+  }
+  return ok;
+}
+
+// ESDIRK23a::reduce_time_step (ESDIRK23a.cpp:260-278).
+void es_reduce(const EsdirkCoef& k, const SolverParams& sp, const ImplicitState& st, double& dt)
+{
+  double reduction_factor = 1.0;
+  double e = st.loc_error;
+  int passes = 0;  // ends the loop for an infinite loc_error (a finite one needs at most 313 passes)
+  do {
+    reduction_factor *= 0.6;
+    e *= 0.1;
+  } while (e > k.tol && ++passes < 400);
+  dt = std::max(reduction_factor * dt, sp.min_dt);
+}
+
+// ESDIRK23a::advance_one_step (ESDIRK23a.cpp:117-132).
+void es_advance(const VariantOps& m, const SolverParams& sp, const EsdirkCoef& k, ImplicitState& st, double* y,
+                double t0, double& dt, const double* p, const double* c, Work& w)
+{
+  bool accepted = es_do_step(m, sp, k, st, y, t0, dt, p, c, w);
+  while (!accepted && dt > sp.min_dt) {
+    if (!st.new_jacobian) es_ode_jacobian(m, st, y, t0, p, c, w);
+    es_reduce(k, sp, st, dt);
+    es_factorized(m, k, st, dt);
+    accepted = es_do_step(m, sp, k, st, y, t0, dt, p, c, w);
+  }
+  st.new_jacobian = false;
+}
+
+// ESDIRK23a::forward (ESDIRK23a.cpp:280-316).  The internal time step of the
+// reference's ODESolver base is shadowed and unused (AdaptiveImplicitSolver.h:139).
+void esdirk_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* y, double t0,
+                    double interval, const double* p, const double* c, Work& w)
+{
+  static const EsdirkCoef k;
+  double local_t = 0;
+  double dt = std::max(k.s_fac * st.prev_dt * std::pow((k.tol / st.loc_error), 0.25), sp.min_dt);
+  dt = std::min(dt, interval);
+  if (st.first_step) {
+    st.prev_dt = dt;
+    es_ode_jacobian(m, st, y, t0, p, c, w);
+  }
+  while (local_t < interval) {
+    if (dt * 1.1 >= interval - local_t) dt = interval - local_t;
+    if (st.newton_iterations >= 2) es_ode_jacobian(m, st, y, t0 + local_t, p, c, w);
+    if (std::fabs((dt - st.prev_dt) / st.prev_dt) > 0.2 || st.new_jacobian) es_factorized(m, k, st, dt);
+    es_advance(m, sp, k, st, y, t0 + local_t, dt, p, c, w);
+    st.prev_dt = dt;
+    local_t += dt;
+    dt = std::max(k.s_fac * st.prev_dt * std::pow((k.tol / st.loc_error), 0.25), sp.min_dt);
+  }
+  st.first_step = false;
+}
+
 }  // namespace
 
 struct oracle_system {
@@ -277,6 +646,7 @@ struct oracle_system {
   std::vector<double> states;  // [N][S]
   std::vector<double> fparams;  // [N][NFP]
   std::vector<Work> work;  // one per thread
+  std::vector<ImplicitState> node_state;  // per-node mode
 };
 
 namespace {
@@ -299,7 +669,16 @@ void forward_node(oracle_system* s, unsigned node, double t, double interval, Wo
   for (int i = 0; i < m.NFP; ++i) prm[m.fpi[i]] = s->fparams[size_t(node) * m.NFP + i];
   double c[kMaxConstants];
   m.constants(prm, c);
-  forward_explicit(s->kind, m, s->sp, &s->states[size_t(node) * m.S], t, interval, prm, c, w);
+  double* y = &s->states[size_t(node) * m.S];
+  if (s->kind == IMPLICIT_EULER || s->kind == ESDIRK23A) {
+    ImplicitState& st = s->per_node_state ? s->node_state[node] : w.carried;
+    if (s->kind == IMPLICIT_EULER)
+      implicit_euler_forward(m, s->sp, st, y, t, interval, prm, c, w);
+    else
+      esdirk_forward(m, s->sp, st, y, t, interval, prm, c, w);
+    return;
+  }
+  forward_explicit(s->kind, m, s->sp, y, t, interval, prm, c, w);
 }
 
 // ODESystemSolver::forward (ODESystemSolver.cpp:67-79).
@@ -338,6 +717,17 @@ int set_param(oracle_system* s, const std::string& name, double v)
 {
   if (name == "internal_time_step") { s->sp.ldt = v; return 0; }
   if (name == "delta") { s->sp.delta = v; return 0; }
+  if (name == "eta_0") { s->sp.eta_0 = v; return 0; }
+  if (name == "kappa") { s->sp.kappa = v; return 0; }
+  if (name == "relative_tolerance") { s->sp.relative_tolerance = v; return 0; }
+  if (name == "max_iterations") { s->sp.max_iterations = int(v); return 0; }
+  if (name == "max_relative_previous_residual") { s->sp.max_relative_previous_residual = v; return 0; }
+  if (name == "always_recompute_jacobian") { s->sp.always_recompute_jacobian = v != 0; return 0; }
+  if (name == "min_dt") { s->sp.min_dt = v; return 0; }
+  if (name == "num_refinements_without_always_recomputing_jacobian") {
+    s->sp.num_refinements_without_always_recomputing_jacobian = int(v);
+    return 0;
+  }
   g_err = "unknown solver parameter " + name;
   return -1;
 }
@@ -420,10 +810,11 @@ int oracle_system_create(const char* variant, const char* solver, unsigned num_n
   const int kind = solver_kind(solver);
   if (kind < 0) { g_err = std::string("unknown solver ") + solver; return -1; }
   if (kVariants[v].K > kMaxConstants) { g_err = "model has too many constants"; return -1; }
-  if (kind >= IMPLICIT_EULER) { g_err = std::string("solver not available in the oracle: ") + solver; return -1; }
+  if (kind == RKF32) { g_err = std::string("solver not available in the oracle: ") + solver; return -1; }
   auto* s = new oracle_system;
   s->m = &kVariants[v];
   s->kind = kind;
+  if (kind == ESDIRK23A) s->sp.min_dt = 0.001;  // ESDIRK23a.h:38
   s->N = num_nodes;
   s->per_node_state = per_node_state;
   s->shared.assign(s->m->defaults, s->m->defaults + s->m->P);
@@ -434,6 +825,11 @@ int oracle_system_create(const char* variant, const char* solver, unsigned num_n
   s->fparams.assign(size_t(num_nodes) * s->m->NFP, 0.0);
   s->work.resize(1);
   s->work[0].size(s->m->S);
+  s->work[0].carried.init(s->m->S, s->sp);
+  if (per_node_state) {
+    s->node_state.resize(num_nodes);
+    for (auto& st : s->node_state) st.init(s->m->S, s->sp);
+  }
   reset_default(s);
   *out = s;
   return 0;
@@ -457,9 +853,14 @@ int oracle_system_set_parameter(oracle_system* s, const char* name, double value
 void oracle_system_set_num_threads(oracle_system* s, unsigned n)
 {
   // ODESystemSolver::set_num_threads (ODESystemSolver.cpp:218-250).
+  // Each thread gets a fresh copy of the solver (ESDIRK23a's copy constructor
+  // restores first_step, loc_error, prev_dt and eta; ESDIRK23a.cpp:46-61).
   s->num_threads = n;
   s->work.resize(n > 0 ? n : 1);
-  for (auto& w : s->work) w.size(s->m->S);
+  for (auto& w : s->work) {
+    w.size(s->m->S);
+    w.carried.init(s->m->S, s->sp);
+  }
 }
 
 int oracle_system_forward(oracle_system* s, double t, double interval)
@@ -562,9 +963,19 @@ void oracle_system_set_field_parameters(oracle_system* s, const double* in, int 
     for (unsigned i = 0; i < F; ++i) s->fparams[size_t(n) * F + i] = in[field_index(tangled, s->N, F, n, i)];
 }
 void oracle_system_reset_default(oracle_system* s) { reset_default(s); }
-void oracle_system_counters(const oracle_system*, long* out4)
+void oracle_system_counters(const oracle_system* s, long* out4)
 {
   for (int i = 0; i < 4; ++i) out4[i] = 0;
+  auto add = [&](const ImplicitState& st) {
+    out4[0] += st.newton_iterations;
+    out4[1] += st.jac_comp;
+    out4[2] += st.rejects + st.num_rejected;
+    out4[3] += st.accepted;
+  };
+  if (s->per_node_state)
+    for (const auto& st : s->node_state) add(st);
+  else
+    for (const auto& w : s->work) add(w.carried);
 }
 
 }  // extern "C"

@@ -1,0 +1,112 @@
+// CPU build of the device solver logic (goss_amd/csrc/goss_implicit.h) for
+// tests: the per-node ImplicitEuler / ESDIRK23a code of the kernels runs on
+// the host with the host math library.  With the same math library as the
+// oracle, any difference from the oracle's per-node mode is a logic
+// difference of the kernel code; what remains on the GPU is the last-bit
+// difference of the device exp/log/pow.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "goss_implicit.h"
+#include "tp06_m.h"
+#include "tp06_epi.h"
+#include "beeler_reuter_1977.h"
+#include "fhn.h"
+
+using namespace goss_hip;
+
+namespace {
+
+template <class NSW>
+struct HostModel;
+
+#define GOSS_HARNESS_MODEL(V)                                                                           \
+  struct M_##V {                                                                                        \
+    static constexpr int num_states = goss_models::V::num_states;                                       \
+    static constexpr int num_parameters = goss_models::V::num_parameters;                               \
+    static constexpr int num_constants = goss_models::V::num_constants;                                 \
+    static constexpr int num_field_states = goss_models::V::num_field_states;                           \
+    static constexpr int num_field_parameters = goss_models::V::num_field_parameters;                   \
+    static constexpr const int* field_state_indices = goss_models::V::field_state_indices;              \
+    static constexpr const int* field_parameter_indices = goss_models::V::field_parameter_indices;      \
+    static constexpr const unsigned char* linear_terms = goss_models::V::linear_terms;                  \
+    template <typename R, typename A, typename T>                                                       \
+    static void eval(const R* y, const T t, const A& a, R* out) { goss_models::V::eval<R>(y, t, a, out); } \
+    struct HostAcc {                                                                                    \
+      const double* v;                                                                                  \
+      template <int k> double getd() const { return v[k]; }                                             \
+    };                                                                                                  \
+    static void constants(const double* p, double* c) { goss_models::V::compute_constants<double>(HostAcc{p}, c); } \
+  };
+GOSS_HARNESS_MODEL(tp06_m)
+GOSS_HARNESS_MODEL(tp06_epi)
+GOSS_HARNESS_MODEL(beeler_reuter_1977)
+GOSS_HARNESS_MODEL(fhn)
+
+template <class M>
+int run(int solver, unsigned N, const double* y0, const double* t, unsigned long n, const double* prm,
+        const ImplicitParams& ip, double* out_states, long* counters)
+{
+  constexpr int S = M::num_states;
+  SharedParamsOf<M, double> sp;
+  double c[M::num_constants > 0 ? M::num_constants : 1];
+  M::constants(prm, c);
+  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = prm[k];
+  for (int k = 0; k < M::num_constants; ++k) sp.c[k] = c[k];
+  const double fpv[1] = {0.0};
+  const DevAcc<M, double> acc{sp, fpv};
+  std::vector<double> jac(size_t(S) * S), jfac(size_t(S) * S);
+  for (int k = 0; k < 4; ++k) counters[k] = 0;
+  int failed_any = 0;
+  for (unsigned node = 0; node < N; ++node) {
+    double ys[S];
+    for (int i = 0; i < S; ++i) ys[i] = y0[size_t(node) * S + i];
+    NodeImplicitState st{};
+    st.eta = ip.eta_0;
+    st.es_eta = 1.0;
+    st.loc_error = 1e9;
+    st.prev_dt = 0.1;
+    st.recompute = 1;
+    st.always = ip.always_recompute_jacobian;
+    st.first_step = 1;
+    st.new_jacobian = 1;
+    const MatView J{jac.data(), 1, S};
+    const MatView F{jfac.data(), 1, S};
+    for (unsigned long k = 0; k + 1 < n; ++k) {
+      if (st.failed) break;
+      const double tk = t[k], dt = t[k + 1] - t[k];
+      if (solver == kImplicitEuler) {
+        if (!ImplicitEulerNode<M>::forward(ys, tk, dt, acc, ip, st, J)) {
+          st.failed = 1;
+          failed_any = 1;
+        }
+      } else {
+        EsdirkNode<M>::forward(ys, tk, dt, acc, ip, st, J, F);
+      }
+    }
+    for (int i = 0; i < S; ++i) out_states[size_t(node) * S + i] = ys[i];
+    counters[0] += st.newton_its;
+    counters[1] += st.jac_comp;
+    counters[2] += st.rejects;
+    counters[3] += st.accepted;
+  }
+  return failed_any;
+}
+
+}  // namespace
+
+extern "C" int kernel_logic_run(const char* variant, int solver, unsigned N, const double* y0, const double* t,
+                                unsigned long n, const double* params, double ldt_0, double eta_0, double kappa,
+                                double relative_tolerance, double max_rel_prev, double min_dt, int max_iterations,
+                                int always, int num_refinements, double* out_states, long* counters)
+{
+  ImplicitParams ip{ldt_0, eta_0, kappa, relative_tolerance, max_rel_prev, min_dt, max_iterations, always,
+                    num_refinements};
+  const std::string v(variant);
+  if (v == "tp06_m") return run<M_tp06_m>(solver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "tp06_epi") return run<M_tp06_epi>(solver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "beeler_reuter_1977") return run<M_beeler_reuter_1977>(solver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "fhn") return run<M_fhn>(solver, N, y0, t, n, params, ip, out_states, counters);
+  return -2;
+}

@@ -162,3 +162,56 @@ def test_field_storage_roundtrip():
     fs = np.linspace(-80, -70, N)[:, None]
     o.set_field_states(fs)
     np.testing.assert_array_equal(o.states()[:, 17], fs[:, 0])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("solver", ["ImplicitEuler", "ESDIRK23a"])
+@pytest.mark.parametrize("variant,dt,ldt", [("tp06_epi", 0.1, -1.0), ("tp06_m", 0.1, -1.0), ("tp06_m", 0.1, 0.05),
+                                            ("beeler_reuter_1977", 0.5, -1.0), ("fhn", 0.5, -1.0)])
+@pytest.mark.parametrize("threads", [0, 3])
+def test_oracle_implicit_matches_reference_build(solver, variant, dt, ldt, threads):
+    # The reference's carried solver state (one instance per thread, static
+    # partition) is reproduced exactly by the oracle's default mode.
+    N = 17
+    rng = np.random.Generator(np.random.PCG64(13))
+    o = oracle.OracleSystem(variant, solver, N, {"internal_time_step": ldt})
+    r = oracle.RefSystem(variant, solver, N, {"internal_time_step": ldt})
+    if threads:
+        o.set_num_threads(threads)
+        r.set_num_threads(threads)
+    y0 = o.states() * (1 + rng.uniform(-0.01, 0.01, o.states().shape))
+    o.set_states(y0)
+    r.set_states(y0)
+    t = np.arange(151) * dt
+    o.forward_times(t)
+    r.forward_times(t)
+    assert np.array_equal(o.states(), r.states(), equal_nan=True)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build oracle/_ref not present")
+def test_oracle_implicit_failure_message_matches_reference():
+    params = {"max_iterations": 0, "min_dt": 0.5}
+    o = oracle.OracleSystem("tp06_m", "ImplicitEuler", 2, params)
+    r = oracle.RefSystem("tp06_m", "ImplicitEuler", 2, params)
+    with pytest.raises(RuntimeError) as a:
+        o.forward(0.0, 0.1)
+    with pytest.raises(RuntimeError) as b:
+        r.forward(0.0, 0.1)
+    assert str(a.value) == str(b.value)
+
+
+def test_esdirk_per_node_state_is_order_independent():
+    # With per-node solver state a node's result does not depend on the
+    # other nodes: permuting the nodes permutes the results exactly.
+    N = 9
+    rng = np.random.Generator(np.random.PCG64(14))
+    a = oracle.OracleSystem("tp06_epi", "ESDIRK23a", N, per_node_state=True)
+    y0 = a.states() * (1 + rng.uniform(-0.01, 0.01, a.states().shape))
+    perm = rng.permutation(N)
+    b = oracle.OracleSystem("tp06_epi", "ESDIRK23a", N, per_node_state=True)
+    a.set_states(y0)
+    b.set_states(y0[perm])
+    t = np.arange(101) * 0.1
+    a.forward_times(t)
+    b.forward_times(t)
+    assert np.array_equal(a.states()[perm], b.states())
