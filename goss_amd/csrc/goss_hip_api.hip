@@ -318,6 +318,7 @@ struct goss_hip_system {
     if (steps_free) (void)hipEventDestroy(steps_free);
     if (ib.jac) (void)hipFree(ib.jac);
     if (ib.jfac) (void)hipFree(ib.jfac);
+    if (ib.work) (void)hipFree(ib.work);
     if (ib.ds) (void)hipFree(ib.ds);
     if (ib.is) (void)hipFree(ib.is);
     if (ib.error) (void)hipFree(ib.error);
@@ -470,12 +471,15 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
       const size_t mat = size_t(v->S) * v->S * s->ld * sizeof(double);
       hip_check(hipMalloc(&s->ib.jac, mat), "hipMalloc(jacobian)");
       if (solver == kESDIRK23a) hip_check(hipMalloc(&s->ib.jfac, mat), "hipMalloc(factorized jacobian)");
+      const size_t work = size_t(implicit_work_vectors(solver)) * v->S * s->ld * sizeof(double);
+      hip_check(hipMalloc(&s->ib.work, work), "hipMalloc(work vectors)");
       hip_check(hipMalloc(&s->ib.ds, size_t(kNumDState) * s->ld * sizeof(double)), "hipMalloc(solver state)");
       hip_check(hipMalloc(&s->ib.is, size_t(kNumIState) * s->ld * sizeof(int)), "hipMalloc(solver state)");
       hip_check(hipMalloc(&s->ib.error, sizeof(int)), "hipMalloc(flag)");
       hip_check(hipHostMalloc(&s->h_error, sizeof(int), hipHostMallocDefault), "hipHostMalloc(flag)");
       hip_check(hipMemsetAsync(s->ib.jac, 0, mat, s->stream), "hipMemset");
       if (s->ib.jfac) hip_check(hipMemsetAsync(s->ib.jfac, 0, mat, s->stream), "hipMemset");
+      hip_check(hipMemsetAsync(s->ib.work, 0, work, s->stream), "hipMemset");
       hip_check(hipMemsetAsync(s->ib.error, 0, sizeof(int), s->stream), "hipMemset");
       hip_check(layout_init_implicit(s->ib.ds, s->ib.is, s->N, s->ld, s->sp.eta_0, s->sp.always_recompute_jacobian,
                                      s->stream),

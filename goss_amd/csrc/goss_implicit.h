@@ -3,12 +3,19 @@
 // ESDIRK23a (ESDIRK23a.cpp:101-316).
 //
 // One node per lane.  Each node owns its solver state (Newton eta, Jacobian
-// reuse flags, ESDIRK step-size history) and its two S x S matrices: the
-// finite-difference Jacobian (ODE::compute_jacobian, ODE.cpp:47-67) and its
-// LU factorisation (ODE::lu_factorize, ODE.cpp:69-100).  Matrices live in HBM
-// as structure of arrays, element (i, j) of every node in one row of stride
-// ld, so a wavefront's 64 lanes touch 512 contiguous bytes per element.
-// Vectors live in registers (fully unrolled loops over the states).
+// reuse flags, ESDIRK step-size history, held in registers during a launch),
+// its two S x S matrices -- the finite-difference Jacobian
+// (ODE::compute_jacobian, ODE.cpp:47-67) and its LU factorisation
+// (ODE::lu_factorize, ODE.cpp:69-100) -- and a few work vectors (stage
+// values, Newton increments).  States, matrices and work vectors live in HBM
+// as structure of arrays: element i of every node in one row of stride ld,
+// so a wavefront's 64 lanes touch 512 contiguous bytes per element.  Only
+// the input and output of one right-hand-side evaluation are held in
+// registers (the FP64 RHS of a cell model needs most of the register file by
+// itself); the substitutions, norms and updates run over the HBM vectors in
+// rolled loops, so no long-lived vector competes with the RHS for registers.
+// The RHS is instantiated at one site per algorithmic role (Newton residual,
+// first ESDIRK stage, Jacobian columns).
 //
 // The arithmetic is FP64 throughout and follows the reference's operation
 // order (compiled with -ffp-contract=off).
@@ -54,12 +61,29 @@ enum IState : int {
   kNumIState
 };
 
+// Work vectors per node (each S rows of the work buffer).
+enum EsdirkWork : int { kEsZ1 = 0, kEsZ2, kEsZ3, kEsZ4, kEsPrev, kEsF1, kEsDZ, kNumEsdirkWork };
+enum EulerWork : int { kIeZ = 0, kIeYZ, kIeB, kIeF1, kNumEulerWork };
+
+__host__ __device__ constexpr int implicit_work_vectors(int solver)
+{
+  return solver == kESDIRK23a ? int(kNumEsdirkWork) : int(kNumEulerWork);
+}
+
 struct ImplicitBuffers {
   double* jac;   // [S*S][ld]
   double* jfac;  // [S*S][ld] (ESDIRK23a only)
+  double* work;  // [implicit_work_vectors(solver) * S][ld]
   double* ds;    // [kNumDState][ld]
   int* is;       // [kNumIState][ld]
   int* error;    // one flag per system (set when a node fails)
+};
+
+// View of one node's vector (element i at a[i * ld]).
+struct VecView {
+  double* a;
+  unsigned ld;
+  GOSS_HD GOSS_INLINE double& operator[](int i) const { return a[size_t(i) * ld]; }
 };
 
 // View of one node's matrix.
@@ -70,38 +94,54 @@ struct MatView {
   GOSS_HD GOSS_INLINE double& operator()(int i, int j) const { return a[size_t(i * S + j) * ld]; }
 };
 
+// Everything one node works on: its states, work vectors and matrices.
+struct NodeViews {
+  VecView y;
+  double* work;
+  unsigned ld;
+  int S;
+  MatView J, F;
+  GOSS_HD GOSS_INLINE VecView w(int k) const { return VecView{work + size_t(k) * S * ld, ld}; }
+};
+
 template <class M>
 struct Implicit {
   static constexpr int S = M::num_states;
 
   // ODE::compute_jacobian (ODE.cpp:47-67): forward differences, column by
-  // column, delta_i = sqrt(1e-15 * max(1e-5, |y_i|)).
+  // column, delta_i = sqrt(1e-15 * max(1e-5, |y_i|)).  Pass i = -1 evaluates
+  // f(y) into f1, pass i >= 0 the perturbed RHS of column i.
   template <class A>
-  GOSS_HD static void fd_jacobian(const double* y, const double t, const A& acc, const MatView& J)
+  GOSS_HD static void fd_jacobian(const VecView& y, const double t, const A& acc, const VecView& f1,
+                                  const MatView& J)
   {
-    double f1[S];
-    M::template eval<double>(y, t, acc, f1);
 #pragma unroll 1
-    for (int i = 0; i < S; ++i) {
-      double yp[S];
+    for (int i = -1; i < S; ++i) {
       double delta = 0.0;
+      if (i >= 0) {
+        const double ysafe = y[i];
+        const double mx = 1e-5 > fabs(ysafe) ? 1e-5 : fabs(ysafe);
+        delta = sqrt(1e-15 * mx);
+      }
+      double yp[S], f[S];
 #pragma unroll
       for (int j = 0; j < S; ++j) {
         const double ysafe = y[j];
-        const double mx = 1e-5 > fabs(ysafe) ? 1e-5 : fabs(ysafe);
-        const double d = sqrt(1e-15 * mx);
-        yp[j] = (j == i) ? ysafe + d : ysafe;
-        delta = (j == i) ? d : delta;
+        yp[j] = (j == i) ? ysafe + delta : ysafe;
       }
-      double f2[S];
-      M::template eval<double>(yp, t, acc, f2);
+      M::template eval<double>(yp, t, acc, f);
+      if (i < 0) {
 #pragma unroll
-      for (int j = 0; j < S; ++j) J(j, i) = (f2[j] - f1[j]) / delta;
+        for (int j = 0; j < S; ++j) f1[j] = f[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < S; ++j) J(j, i) = (f[j] - f1[j]) / delta;
+      }
     }
   }
 
-  // J <- I - scale_dt * J, then in-place LU (ODE.cpp:69-100); scale_dt is
-  // dt*alpha (ImplicitODESolver.cpp:92-105) or dt*gamma (ESDIRK23a.cpp:101-115).
+  // dst <- scale * src + I, then in-place LU (ODE.cpp:69-100); scale is
+  // -dt*alpha (ImplicitODESolver.cpp:92-105) or -dt*gamma (ESDIRK23a.cpp:101-115).
   GOSS_HD static void scale_add_identity(const MatView& src, const MatView& dst, const double scale)
   {
 #pragma unroll 1
@@ -131,22 +171,21 @@ struct Implicit {
     }
   }
 
-  // ODE::forward_backward_subst (ODE.cpp:102-132); b and x may alias.
-  GOSS_HD static void fb_subst(const MatView& a, const double* b, double* x)
+  // ODE::forward_backward_subst (ODE.cpp:102-132), in place on x.
+  GOSS_HD static void fb_subst(const MatView& a, const VecView& x)
   {
-    x[0] = b[0];
-#pragma unroll
+#pragma unroll 1
     for (int i = 1; i < S; ++i) {
       double acc = 0.0;
-#pragma unroll
+#pragma unroll 1
       for (int j = 0; j <= i - 1; ++j) acc = acc + a(i, j) * x[j];
-      x[i] = b[i] - acc;
+      x[i] = x[i] - acc;
     }
     x[S - 1] = x[S - 1] / a(S - 1, S - 1);
-#pragma unroll
+#pragma unroll 1
     for (int i = S - 2; i >= 0; i--) {
       double acc = 0;
-#pragma unroll
+#pragma unroll 1
       for (int j = i + 1; j < S; ++j) acc = acc + a(i, j) * x[j];
       x[i] = (x[i] - acc) / a(i, i);
     }
@@ -159,9 +198,17 @@ struct Implicit {
     for (int i = 0; i < S; ++i) acc += v[i] * v[i];
     return sqrt(acc);
   }
+
+  GOSS_HD static double norm(const VecView& v)
+  {
+    double acc = 0;
+#pragma unroll 1
+    for (int i = 0; i < S; ++i) acc += v[i] * v[i];
+    return sqrt(acc);
+  }
 };
 
-// Per-node state, loaded into registers for a launch.
+// Per-node solver state, loaded into registers for a launch.
 struct NodeImplicitState {
   double eta, es_eta, loc_error, prev_dt;
   int recompute, newton_its, always, justrefined, first_step, new_jacobian, jac_comp, rejects, accepted, failed;
@@ -214,11 +261,12 @@ struct ImplicitEulerNode {
   using I = Implicit<M>;
 
   // ImplicitODESolver::newton_solve with prev = 0 and alpha = 1
-  // (ImplicitEuler.cpp:85), mass matrix = identity.
+  // (ImplicitEuler.cpp:85), mass matrix = identity; z is the work vector kIeZ.
   template <class A>
-  GOSS_HD static bool newton_solve(double* z, const double* y0, const double t, const double dt, const A& acc,
-                                      const ImplicitParams& p, NodeImplicitState& st, const MatView& J)
+  GOSS_HD static bool newton_solve(const NodeViews& v, const double t, const double dt, const A& acc,
+                                   const ImplicitParams& p, NodeImplicitState& st)
   {
+    const VecView z = v.w(kIeZ), yzv = v.w(kIeYZ), bv = v.w(kIeB), f1 = v.w(kIeF1);
     bool step_ok = true;
     st.newton_its = 0;
     double relative_previous_residual = 1.0, relative_residual = 1.0, previous_residual = 1.0,
@@ -226,22 +274,28 @@ struct ImplicitEulerNode {
     const bool always = st.always != 0;
 #pragma unroll 1
     do {
-      double yz[S], f1[S], b[S], dz[S];
+      {
+        double yz[S], f[S], b[S];
 #pragma unroll
-      for (int i = 0; i < S; ++i) yz[i] = y0[i] + z[i];
-      M::template eval<double>(yz, t, acc, f1);
+        for (int i = 0; i < S; ++i) yz[i] = v.y[i] + z[i];
+        M::template eval<double>(yz, t, acc, f);
 #pragma unroll
-      for (int i = 0; i < S; ++i) b[i] = -z[i] * 1.0 + dt * (0.0 + 1.0 * f1[i]);
-      residual = I::norm(b);
+        for (int i = 0; i < S; ++i) {
+          b[i] = -z[i] * 1.0 + dt * (0.0 + 1.0 * f[i]);
+          bv[i] = b[i];
+          yzv[i] = yz[i];
+        }
+        residual = I::norm(b);
+      }
       if (relative_residual < p.relative_tolerance) break;
       if (st.recompute || always) {
-        I::fd_jacobian(yz, t, acc, J);
-        I::scale_add_identity(J, J, -dt * 1.0);
-        I::lu_factorize(J);
+        I::fd_jacobian(yzv, t, acc, f1, v.J);
+        I::scale_add_identity(v.J, v.J, -dt * 1.0);
+        I::lu_factorize(v.J);
         st.jac_comp += 1;
         st.recompute = 0;
       }
-      I::fb_subst(J, b, dz);
+      I::fb_subst(v.J, bv);  // bv <- dz
       if (st.newton_its == 0) {
         initial_residual = residual;
         st.eta = st.eta > kGossEps ? st.eta : kGossEps;
@@ -267,8 +321,8 @@ struct ImplicitEulerNode {
         step_ok = false;
         break;
       }
-#pragma unroll
-      for (int i = 0; i < S; ++i) z[i] += dz[i];
+#pragma unroll 1
+      for (int i = 0; i < S; ++i) z[i] += bv[i];
       relative_residual = residual / initial_residual;
       previous_residual = residual;
       st.newton_its++;
@@ -279,9 +333,10 @@ struct ImplicitEulerNode {
   // ImplicitEuler::forward (ImplicitEuler.cpp:57-152).  Returns false when
   // the step size falls below min_dt (the reference throws).
   template <class A>
-  GOSS_HD static bool forward(double* y, double t, const double dt, const A& acc, const ImplicitParams& p,
-                                 NodeImplicitState& st, const MatView& J)
+  GOSS_HD static bool forward(const NodeViews& v, double t, const double dt, const A& acc,
+                              const ImplicitParams& p, NodeImplicitState& st)
   {
+    const VecView z = v.w(kIeZ);
     const double t_end = t + dt;
     const double ldt_0 = p.ldt_0;
     double ldt = ldt_0 > 0 ? ldt_0 : dt;
@@ -289,14 +344,13 @@ struct ImplicitEulerNode {
     const double eps = kGossEps * 1000;
 #pragma unroll 1
     while (true) {
-      double z1[S];
-#pragma unroll
-      for (int i = 0; i < S; ++i) z1[i] = 0.0;
+#pragma unroll 1
+      for (int i = 0; i < S; ++i) z[i] = 0.0;
       if (num_refinements > p.num_refinements) st.always = 1;
-      const bool step_ok = newton_solve(z1, y, t + ldt, ldt, acc, p, st, J);
+      const bool step_ok = newton_solve(v, t + ldt, ldt, acc, p, st);
       if (step_ok) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) y[i] += z1[i];
+#pragma unroll 1
+        for (int i = 0; i < S; ++i) v.y[i] += z[i];
         t += ldt;
         if (fabs(t - t_end) < eps) break;
         if (!st.justrefined) {
@@ -354,49 +408,44 @@ struct EsdirkNode {
   static constexpr double s_fac = 0.9, tol = 1.0e-5, etamin = 1.0e-10, kappa = 1.0e-1;
 
   template <class A>
-  GOSS_HD static void ode_jacobian(const double* y, const double t, const A& acc, NodeImplicitState& st,
-                                      const MatView& J)
+  GOSS_HD static void ode_jacobian(const NodeViews& v, const double t, const A& acc, NodeImplicitState& st)
   {
-    I::fd_jacobian(y, t, acc, J);
+    I::fd_jacobian(v.y, t, acc, v.w(kEsF1), v.J);
     st.jac_comp++;
     st.new_jacobian = 1;
   }
 
-  GOSS_HD static void factorized(const EsdirkCoefficients& k, const double dt, const MatView& J,
-                                    const MatView& F)
+  GOSS_HD static void factorized(const EsdirkCoefficients& k, const double dt, const NodeViews& v)
   {
-    I::scale_add_identity(J, F, -dt * k.a22);
-    I::lu_factorize(F);
+    I::scale_add_identity(v.J, v.F, -dt * k.a22);
+    I::lu_factorize(v.F);
   }
 
-  // compute_stage_val (ESDIRK23a.cpp:201-258)
+  // compute_stage_val (ESDIRK23a.cpp:201-258).  The Newton increment of the
+  // previous iteration enters only through its norm, kept as prev_ndz.
   template <class A>
-  GOSS_HD static bool stage(const EsdirkCoefficients& k, double* z, const double* y, const double* prev,
-                               const double t_s, const double dt, const A& acc, const ImplicitParams& p,
-                               NodeImplicitState& st, const MatView& F)
+  GOSS_HD static bool stage(const EsdirkCoefficients& k, const VecView& z, const NodeViews& v, const double t_s,
+                            const double dt, const A& acc, const ImplicitParams& p, NodeImplicitState& st)
   {
+    const VecView prev = v.w(kEsPrev), dz = v.w(kEsDZ);
     st.newton_its = 0;
     bool conv_ok = true;
-    double dz[S];
-#pragma unroll
-    for (int i = 0; i < S; ++i) dz[i] = 0.0;
     st.es_eta = pow(smax(st.es_eta, etamin), 0.8);
-    double ndz;
+    double ndz, prev_ndz = 0.0;
 #pragma unroll 1
     do {
-      double y_int[S], z_int[S], b[S], prev_dz[S];
+      {
+        double y_int[S], z_int[S];
 #pragma unroll
-      for (int i = 0; i < S; ++i) y_int[i] = y[i] + dt * prev[i] + k.gamma * dt * z[i];
-      M::template eval<double>(y_int, t_s, acc, z_int);
+        for (int i = 0; i < S; ++i) y_int[i] = v.y[i] + dt * prev[i] + k.gamma * dt * z[i];
+        M::template eval<double>(y_int, t_s, acc, z_int);
 #pragma unroll
-      for (int i = 0; i < S; ++i) {
-        b[i] = -z[i] + z_int[i];
-        prev_dz[i] = dz[i];
+        for (int i = 0; i < S; ++i) dz[i] = -z[i] + z_int[i];
       }
-      I::fb_subst(F, b, dz);
+      I::fb_subst(v.F, dz);
       ndz = I::norm(dz);
       if (st.newton_its >= 1) {
-        const double theta = ndz / I::norm(prev_dz);
+        const double theta = ndz / prev_ndz;
         if (!(theta <= 1)) {
           st.rejects++;
           conv_ok = false;
@@ -409,56 +458,72 @@ struct EsdirkNode {
         conv_ok = false;
         break;
       }
-#pragma unroll
+#pragma unroll 1
       for (int i = 0; i < S; ++i) z[i] += dz[i];
+      prev_ndz = ndz;
       st.newton_its++;
     } while (st.es_eta * ndz > kappa * tol);
     return conv_ok;
   }
 
-  // do_step (ESDIRK23a.cpp:134-199)
+  // do_step (ESDIRK23a.cpp:134-199): the three implicit stages run through
+  // one loop over their coefficients.
   template <class A>
-  GOSS_HD static bool do_step(const EsdirkCoefficients& k, double* y, const double t0, const double dt,
-                                 const A& acc, const ImplicitParams& p, NodeImplicitState& st, const MatView& F)
+  GOSS_HD static bool do_step(const EsdirkCoefficients& k, const double t0, const double dt, const NodeViews& v,
+                              const A& acc, const ImplicitParams& p, NodeImplicitState& st)
   {
-    double z1[S], z2[S], z3[S], z4[S], prev[S];
-    M::template eval<double>(y, t0, acc, z1);
+    const VecView z1 = v.w(kEsZ1), z2 = v.w(kEsZ2), z3 = v.w(kEsZ3), z4 = v.w(kEsZ4), prev = v.w(kEsPrev);
+    {
+      double y0[S], f[S];
 #pragma unroll
-    for (int i = 0; i < S; i++) {
-      z2[i] = z1[i];
-      prev[i] = z1[i] * k.a21;
+      for (int i = 0; i < S; i++) y0[i] = v.y[i];
+      M::template eval<double>(y0, t0, acc, f);
+#pragma unroll
+      for (int i = 0; i < S; i++) z1[i] = f[i];
     }
-    bool ok = stage(k, z2, y, prev, t0 + k.c2 * dt, dt, acc, p, st, F);
-    if (ok) {
-#pragma unroll
-      for (int i = 0; i < S; i++) {
-        z3[i] = z2[i];
-        prev[i] = z1[i] * k.bh1 + z2[i] * k.bh2;
+    bool ok = true;
+#pragma unroll 1
+    for (int s = 1; s <= 3 && ok; ++s) {
+      const VecView zs = v.w(kEsZ1 + s), zp = v.w(kEsZ1 + s - 1);
+      double c1, c2 = 0.0, c3 = 0.0, cs;
+      if (s == 1) {
+        c1 = k.a21;
+        cs = k.c2;
+      } else if (s == 2) {
+        c1 = k.bh1;
+        c2 = k.bh2;
+        cs = k.c3;
+      } else {
+        c1 = k.b1;
+        c2 = k.b2;
+        c3 = k.b3;
+        cs = k.c4;
       }
-      ok = stage(k, z3, y, prev, t0 + k.c3 * dt, dt, acc, p, st, F);
-    }
-    if (ok) {
-#pragma unroll
+#pragma unroll 1
       for (int i = 0; i < S; i++) {
-        z4[i] = z3[i];
-        prev[i] = z1[i] * k.b1 + z2[i] * k.b2 + z3[i] * k.b3;
+        zs[i] = zp[i];
+        double pv = z1[i] * c1;
+        if (s >= 2) pv = pv + z2[i] * c2;
+        if (s >= 3) pv = pv + z3[i] * c3;
+        prev[i] = pv;
       }
-      ok = stage(k, z4, y, prev, t0 + k.c4 * dt, dt, acc, p, st, F);
+      ok = stage(k, zs, v, t0 + cs * dt, dt, acc, p, st);
     }
     if (ok) {
-      double yh[S];
-#pragma unroll
+      const VecView yh = v.w(kEsDZ);
+#pragma unroll 1
       for (int i = 0; i < S; i++)
         yh[i] = dt * ((k.b1 - k.bh1) * z1[i] + (k.b2 - k.bh2) * z2[i] + (k.b3 - k.gamma) * z3[i] +
                       k.gamma * z4[i]);
-      I::fb_subst(F, yh, yh);
+      I::fb_subst(v.F, yh);
       st.loc_error = I::norm(yh);
       if (st.loc_error > tol) {
         st.rejects++;
         ok = false;
       } else {
-#pragma unroll
-        for (int i = 0; i < S; i++) y[i] = y[i] + dt * (k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * z3[i] + k.gamma * z4[i]);
+#pragma unroll 1
+        for (int i = 0; i < S; i++)
+          v.y[i] = v.y[i] + dt * (k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * z3[i] + k.gamma * z4[i]);
         st.accepted++;
       }
     }
@@ -466,10 +531,11 @@ struct EsdirkNode {
   }
 
   // forward (ESDIRK23a.cpp:280-316) with advance_one_step (117-132) and
-  // reduce_time_step (260-278) inlined.
+  // reduce_time_step (260-278) inlined; the first attempt and the retries
+  // after a reduction share one do_step call.
   template <class A>
-  GOSS_HD static void forward(double* y, const double t0, const double interval, const A& acc,
-                                 const ImplicitParams& p, NodeImplicitState& st, const MatView& J, const MatView& F)
+  GOSS_HD static void forward(const NodeViews& v, const double t0, const double interval, const A& acc,
+                              const ImplicitParams& p, NodeImplicitState& st)
   {
     const EsdirkCoefficients k = esdirk_coefficients();
     double local_t = 0;
@@ -477,19 +543,19 @@ struct EsdirkNode {
     dt = smin(dt, interval);
     if (st.first_step) {
       st.prev_dt = dt;
-      ode_jacobian(y, t0, acc, st, J);
+      ode_jacobian(v, t0, acc, st);
     }
 #pragma unroll 1
     while (local_t < interval) {
       if (dt * 1.1 >= interval - local_t) dt = interval - local_t;
-      if (st.newton_its >= 2) ode_jacobian(y, t0 + local_t, acc, st, J);
-      if (fabs((dt - st.prev_dt) / st.prev_dt) > 0.2 || st.new_jacobian) factorized(k, dt, J, F);
-      // advance_one_step
+      if (st.newton_its >= 2) ode_jacobian(v, t0 + local_t, acc, st);
+      if (fabs((dt - st.prev_dt) / st.prev_dt) > 0.2 || st.new_jacobian) factorized(k, dt, v);
       const double ts = t0 + local_t;
-      bool accepted = do_step(k, y, ts, dt, acc, p, st, F);
 #pragma unroll 1
-      while (!accepted && dt > p.min_dt) {
-        if (!st.new_jacobian) ode_jacobian(y, ts, acc, st, J);
+      while (true) {
+        const bool accepted = do_step(k, ts, dt, v, acc, p, st);
+        if (accepted || !(dt > p.min_dt)) break;
+        if (!st.new_jacobian) ode_jacobian(v, ts, acc, st);
         // reduce_time_step (ESDIRK23a.cpp:260-278).  For a finite loc_error
         // at most 313 passes bring e below tol; the cap only ends the loop
         // for an infinite loc_error, where the reference would not return.
@@ -501,8 +567,7 @@ struct EsdirkNode {
           e *= 0.1;
         } while (e > tol && ++passes < 400);
         dt = smax(reduction_factor * dt, p.min_dt);
-        factorized(k, dt, J, F);
-        accepted = do_step(k, y, ts, dt, acc, p, st, F);
+        factorized(k, dt, v);
       }
       st.new_jacobian = 0;
       st.prev_dt = dt;
@@ -512,6 +577,18 @@ struct EsdirkNode {
     st.first_step = 0;
   }
 };
+
+GOSS_HD GOSS_INLINE NodeViews node_views(double* y, const ImplicitBuffers& B, int S, unsigned ld, unsigned node)
+{
+  NodeViews v;
+  v.y = VecView{y + node, ld};
+  v.work = B.work + node;
+  v.ld = ld;
+  v.S = S;
+  v.J = MatView{B.jac + node, ld, S};
+  v.F = MatView{B.jfac ? B.jfac + node : nullptr, ld, S};
+  return v;
+}
 
 // The fused implicit kernel: `nouter` outer steps (t_k, interval_k).
 template <class M, int SOLVER>
@@ -524,16 +601,12 @@ implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const dou
   constexpr int NFP = M::num_field_parameters;
   const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= N) return;
-  double ys[S];
-#pragma unroll
-  for (int i = 0; i < S; ++i) ys[i] = y[size_t(i) * ld + node];
   double fpv[NFP > 0 ? NFP : 1];
 #pragma unroll
   for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
   const DevAcc<M, double> acc{sp, fpv};
   NodeImplicitState st = load_state(B, ld, node);
-  const MatView J{B.jac + node, ld, S};
-  const MatView F{B.jfac + node, ld, S};
+  const NodeViews v = node_views(y, B, S, ld, node);
 
 #pragma unroll 1
   for (int k = 0; k < nouter; ++k) {
@@ -541,12 +614,12 @@ implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const dou
       const double t = steps[k].x;
       const double dt = steps[k].y;
       if constexpr (SOLVER == kImplicitEuler) {
-        if (!ImplicitEulerNode<M>::forward(ys, t, dt, acc, ip, st, J)) {
+        if (!ImplicitEulerNode<M>::forward(v, t, dt, acc, ip, st)) {
           st.failed = 1;
           atomicOr(B.error, 1);
         }
       } else {
-        EsdirkNode<M>::forward(ys, t, dt, acc, ip, st, J, F);
+        EsdirkNode<M>::forward(v, t, dt, acc, ip, st);
       }
     }
     if (fo.out) {
@@ -554,12 +627,10 @@ implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const dou
 #pragma unroll
       for (int i = 0; i < Fs; ++i) {
         const size_t idx = fo.tangled ? size_t(node) * Fs + i : size_t(N) * i + node;
-        fo.out[size_t(k) * fo.row_stride + idx] = ys[M::field_state_indices[i]];
+        fo.out[size_t(k) * fo.row_stride + idx] = v.y[M::field_state_indices[i]];
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < S; ++i) y[size_t(i) * ld + node] = ys[i];
   store_state(B, ld, node, st);
 }
 

@@ -56,7 +56,11 @@ int run(int solver, unsigned N, const double* y0, const double* t, unsigned long
   for (int k = 0; k < M::num_constants; ++k) sp.c[k] = c[k];
   const double fpv[1] = {0.0};
   const DevAcc<M, double> acc{sp, fpv};
-  std::vector<double> jac(size_t(S) * S), jfac(size_t(S) * S);
+  std::vector<double> jac(size_t(S) * S), jfac(size_t(S) * S), work(size_t(implicit_work_vectors(solver)) * S);
+  ImplicitBuffers B{};
+  B.jac = jac.data();
+  B.jfac = jfac.data();
+  B.work = work.data();
   for (int k = 0; k < 4; ++k) counters[k] = 0;
   int failed_any = 0;
   for (unsigned node = 0; node < N; ++node) {
@@ -71,18 +75,17 @@ int run(int solver, unsigned N, const double* y0, const double* t, unsigned long
     st.always = ip.always_recompute_jacobian;
     st.first_step = 1;
     st.new_jacobian = 1;
-    const MatView J{jac.data(), 1, S};
-    const MatView F{jfac.data(), 1, S};
+    const NodeViews v = node_views(ys, B, S, 1, 0);
     for (unsigned long k = 0; k + 1 < n; ++k) {
       if (st.failed) break;
       const double tk = t[k], dt = t[k + 1] - t[k];
       if (solver == kImplicitEuler) {
-        if (!ImplicitEulerNode<M>::forward(ys, tk, dt, acc, ip, st, J)) {
+        if (!ImplicitEulerNode<M>::forward(v, tk, dt, acc, ip, st)) {
           st.failed = 1;
           failed_any = 1;
         }
       } else {
-        EsdirkNode<M>::forward(ys, tk, dt, acc, ip, st, J, F);
+        EsdirkNode<M>::forward(v, tk, dt, acc, ip, st);
       }
     }
     for (int i = 0; i < S; ++i) out_states[size_t(node) * S + i] = ys[i];

@@ -27,14 +27,6 @@ from test_gpu_explicit import assert_parity, dev_system, perturbed
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("solver", ["ImplicitEuler", "ESDIRK23a"])
-@pytest.mark.parametrize("variant,dt,steps,ldt", [
-    ("tp06_epi", 0.1, 300, -1.0),
-    ("tp06_epi", 0.1, 200, 0.05),
-    ("beeler_reuter_1977", 0.5, 100, -1.0),
-    ("fhn", 0.5, 100, -1.0),
-    ("tp06_m", 0.05, 300, -1.0),
-])
 def ulp_sensitivity(variant, solver, params, y0, t):
     """Spread of the oracle itself between y0 and y0 moved by one ulp."""
     N = y0.shape[0]

@@ -1,0 +1,100 @@
+"""Host build of the implicit kernels' per-node logic against the oracle.
+
+tests/harness/kernel_logic_cpu.hip compiles goss_amd/csrc/goss_implicit.h for
+the host (the same source the GPU kernels instantiate) and runs it node by
+node with the host math library.  The oracle's per-node-state mode
+(oracle/goss_oracle.cpp) gives each node its own solver state, which is what
+the device kernels do; with the same libm the two must agree bit for bit, so
+any difference is a logic difference of the kernel code (an operation order,
+a dropped reset, a wrong branch) and not floating-point noise.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HARNESS = os.path.join(HERE, "harness", "libkernel_logic_cpu.so")
+META = os.path.join(ROOT, "goss_amd", "csrc", "generated", "variants.json")
+
+SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8}
+# ImplicitODESolver / ImplicitEuler / ESDIRK23a defaults
+# (ImplicitODESolver.cpp:31-45, ImplicitEuler.h:36-37, ESDIRK23a.cpp:46-60).
+DEFAULTS = dict(eta_0=1.0, kappa=0.1, relative_tolerance=1e-12, max_relative_previous_residual=0.01,
+                max_iterations=30, always=0, num_refinements=2)
+
+
+def _harness():
+    if not os.path.exists(HARNESS):
+        pytest.skip("host build of the kernel logic is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(HARNESS)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.kernel_logic_run.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint, dp, dp, ctypes.c_ulong, dp,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp,
+                                   ctypes.POINTER(ctypes.c_long)]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def run_harness(variant, solver, y0, t, ldt=-1.0, min_dt=None):
+    L = _harness()
+    meta = json.load(open(META))[variant]
+    params = np.ascontiguousarray(meta["default_parameters"], dtype=np.float64)
+    y0 = np.ascontiguousarray(y0, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    N, S = y0.shape
+    out = np.zeros_like(y0)
+    counters = (ctypes.c_long * 4)()
+    if min_dt is None:
+        min_dt = 1e-3 if solver == "ESDIRK23a" else 1e-4
+    d = DEFAULTS
+    rc = L.kernel_logic_run(variant.encode(), SOLVER_IDS[solver], N, _p(y0), _p(t), len(t), _p(params), ldt,
+                            d["eta_0"], d["kappa"], d["relative_tolerance"], d["max_relative_previous_residual"],
+                            min_dt, d["max_iterations"], d["always"], d["num_refinements"], _p(out), counters)
+    assert rc in (0, 1)
+    return out, np.array(list(counters)), rc
+
+
+def run_oracle(variant, solver, y0, t, ldt=-1.0):
+    N = y0.shape[0]
+    params = {"internal_time_step": ldt}
+    o = oracle.OracleSystem(variant, solver, N, solver_params=params, per_node_state=True)
+    o.set_states(y0)
+    for k in range(len(t) - 1):
+        o.forward(t[k], t[k + 1] - t[k])
+    return o.states(), o.counters()
+
+
+CASES = [
+    ("tp06_m", 0.05, 300, -1.0),      # crosses the V < -40 switch of the h/j gates after the stimulus
+    ("tp06_epi", 0.1, 120, -1.0),
+    ("beeler_reuter_1977", 0.1, 100, -1.0),
+    ("fhn", 0.5, 60, -1.0),
+    ("tp06_epi", 0.1, 60, 0.05),
+]
+
+
+@pytest.mark.parametrize("solver", ["ImplicitEuler", "ESDIRK23a"])
+@pytest.mark.parametrize("variant,dt,nsteps,ldt", CASES)
+def test_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, ldt, solver):
+    S = len(json.load(open(META))[variant]["state_names"])
+    o = oracle.OracleSystem(variant, solver, 1, per_node_state=True)
+    base = o.states()[0]
+    rng = np.random.Generator(np.random.PCG64(41))
+    N = 9
+    y0 = base[None, :] * (1.0 + rng.uniform(-0.01, 0.01, (N, S)))
+    t = np.arange(nsteps + 1) * dt
+    want, want_c = run_oracle(variant, solver, y0, t, ldt)
+    got, got_c, _ = run_harness(variant, solver, y0, t, ldt)
+    np.testing.assert_array_equal(got, want)
+    # counters: Newton iterations of the last solve, Jacobians, rejects, accepted
+    np.testing.assert_array_equal(got_c[1:], want_c[1:])
