@@ -1,0 +1,216 @@
+// Device kernel of the adaptive explicit solver RKF32 (RKF32.cpp:105-222) with
+// the step-size controller of AdaptiveExplicitSolver::new_time_step
+// (AdaptiveExplicitSolver.cpp:168-244).
+//
+// One node per lane.  Each node owns the controller state the reference
+// keeps in its solver instance: the initial step guess _ldt and facmax,
+// which carry over from one forward call to the next, and the counters
+// nfevals / ndtsa / ndtsr.  Step sizes and the local time are FP64 in every
+// precision; the states, stage values and the error vector use the
+// precision R.
+//
+// The states and the vectors that live across RHS evaluations (k1, the
+// partial sums of yn and of the error vector e) are HBM rows of stride ld,
+// as in the implicit kernels; registers hold only one evaluation's input and
+// output, which keeps a 19-state FP64 model free of register spills.  The
+// partial sums keep the reference's left-to-right association:
+// yn = y + dt*((b1 k1 + b2 k2) + b3 k3), e = dt*(((d1 k1 + d2 k2) + d3 k3) + d4 k4).
+#pragma once
+
+#include "goss_implicit.h"
+
+namespace goss_hip {
+
+// Sub-steps one forward call may take before the node is flagged as stuck.
+// The reference loops until |_t - t_end| < 1e-14 (AdaptiveExplicitSolver.cpp:206)
+// and does not return when that cannot be met (a step size underflowed by
+// repeated rejections of a non-finite error).
+constexpr long kRkf32MaxSteps = 1000000;
+
+struct AdaptiveParams {
+  double atol, rtol;  // AdaptiveExplicitSolver::set_tol (AdaptiveExplicitSolver.h:71-75)
+};
+
+// View of one node's vector of precision R (element i at a[i * ld]).
+template <class R>
+struct RVec {
+  R* a;
+  unsigned ld;
+  GOSS_HD GOSS_INLINE R& operator[](int i) const { return a[size_t(i) * ld]; }
+};
+
+// Work vectors of RKF32 per node (each S rows of the work buffer).
+enum Rkf32Work : int { kRkK1 = 0, kRkYn, kRkEp, kNumRkf32Work };
+
+template <class M, class R>
+struct Rkf32Node {
+  static constexpr int S = M::num_states;
+
+  // RKF32::forward for one node.  y, k1, yn and ep are HBM views; only the
+  // input and output of one RHS evaluation are held in registers.  Returns
+  // false when the sub-step cap is hit.
+  template <class A>
+  GOSS_HD static bool forward(const RVec<R>& y, const RVec<R>& k1, const RVec<R>& yn, const RVec<R>& ep,
+                              const double t, const double interval, const A& acc, const AdaptiveParams& ap,
+                              double& ldt_state, double& facmax, int& nfevals, int& ndtsa, int& ndtsr)
+  {
+    // Coefficients (RKF32.cpp:34-37).
+    constexpr double a21 = 1.0 / 2.0, a32 = 3.0 / 4.0, b1 = 2.0 / 9.0, b2 = 1.0 / 3.0, b3 = 4.0 / 9.0;
+    constexpr double bh1 = 7.0 / 24.0, bh2 = 1.0 / 4.0, bh3 = 1.0 / 3.0, bh4 = 1.0 / 8.0;
+    constexpr double d1 = b1 - bh1, d2 = b2 - bh2, d3 = b3 - bh3, d4 = -bh4, c2 = 1.0 / 2.0, c3 = 3.0 / 4.0;
+    // AdaptiveExplicitSolver::reset (AdaptiveExplicitSolver.cpp:58-64), _iord = 3 (RKF32.cpp:39).
+    constexpr double facmin = 0.5, facmaxb = 2.0, stabfac = 0.9, iord = 3.0;
+    const R atol = R(ap.atol), rtol = R(ap.rtol);
+    const double t_end = t + interval;
+    double lt = t;  // _t
+    bool reached_tend = false;
+    // `first` is set by reset() and never cleared (RKF32.cpp:126-131).
+    {
+      R in[S], out[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) in[i] = y[i];
+      M::template eval<R>(in, t, acc, out);
+#pragma unroll
+      for (int i = 0; i < S; ++i) k1[i] = out[i];
+    }
+    nfevals += 1;
+    // AdaptiveExplicitSolver::_ldt is never negative, so dtinit (RKF32.cpp:134-138) is not reachable.
+    double dt = ldt_state;
+    long guard = 0;
+#pragma unroll 1
+    while (!reached_tend) {
+      if (++guard > kRkf32MaxSteps) return false;
+      const R rdt = R(dt);
+      // Stage 2; yn and ep collect b1 k1 + b2 k2 and d1 k1 + d2 k2 (the
+      // reference's left-to-right sums, continued after stage 3).
+      {
+        R ki[S], k2[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) ki[i] = y[i] + rdt * R(a21) * k1[i];
+        M::template eval<R>(ki, t + c2 * dt, acc, k2);  // stage times from the call's t (RKF32.cpp:155, 160)
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const R k1i = k1[i];
+          ki[i] = y[i] + rdt * R(a32) * k2[i];
+          yn[i] = R(b1) * k1i + R(b2) * k2[i];
+          ep[i] = R(d1) * k1i + R(d2) * k2[i];
+        }
+        // Stage 3.
+        M::template eval<R>(ki, t + c3 * dt, acc, k2);
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          yn[i] = y[i] + rdt * (yn[i] + R(b3) * k2[i]);
+          ep[i] = ep[i] + R(d3) * k2[i];
+        }
+      }
+      // FSAL evaluation at yn, then new_time_step: RMS of
+      // e / (atol + rtol * max(|y|, |yn|)).
+      R err = 0;
+      {
+        R in[S], k4[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) in[i] = yn[i];
+        M::template eval<R>(in, t + dt, acc, k4);
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const R e = rdt * (ep[i] + R(d4) * k4[i]);
+          const R yi_abs = fabs(y[i]), yni_abs = fabs(in[i]);
+          const R mx = yi_abs > yni_abs ? yi_abs : yni_abs;
+          const R sk = atol + rtol * mx;
+          const R tmp = e / sk;
+          err += tmp * tmp;
+          ep[i] = k4[i];  // kept for the FSAL swap below
+        }
+      }
+      nfevals += 3;
+      double errd = sqrt(double(err) / S);
+      if (__builtin_isnan(errd) || __builtin_isinf(errd)) errd = 2000.0;
+      bool accepted;
+      if (errd <= 1.0) {
+        lt += dt;
+        accepted = true;
+        if (fabs(lt - t_end) < 1e-14) reached_tend = true;
+      } else {
+        accepted = false;
+      }
+      double fac = stabfac * pow(1.0 / errd, 1.0 / (iord + 1));
+      if (facmin > fac) fac = facmin;
+      if (fac > facmax) fac = facmax;
+      facmax = accepted ? facmaxb : 1.0;
+      dt *= fac;
+      if (lt + dt >= t_end)
+        dt = t_end - lt;
+      else if (lt + 1.5 * dt >= t_end)
+        dt = (t_end - lt) / 2.0;
+      else
+        ldt_state = dt;
+      if (accepted) {
+        ndtsa += 1;
+#pragma unroll 1
+        for (int i = 0; i < S; ++i) {
+          y[i] = yn[i];
+          k1[i] = ep[i];
+        }
+      } else {
+        ndtsr += 1;
+      }
+    }
+    return true;
+  }
+};
+
+// The fused RKF32 kernel: `nouter` outer steps (t_k, interval_k).  States
+// stay in their SoA buffer; k1, yn and ep are rows of the work buffer.
+template <class M, class R>
+__global__ void __launch_bounds__(kBlock)
+rkf32_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2* __restrict__ steps, const int nouter,
+             const SharedParamsOf<M, R> sp, const AdaptiveParams ap, const ImplicitBuffers B, const unsigned N,
+             const unsigned ld, const FieldOut fo)
+{
+  constexpr int S = M::num_states;
+  constexpr int NFP = M::num_field_parameters;
+  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= N) return;
+  R fpv[NFP > 0 ? NFP : 1];
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
+  const DevAcc<M, R> acc{sp, fpv};
+  R* const work = reinterpret_cast<R*>(B.work);
+  const RVec<R> yv{y + node, ld};
+  const RVec<R> k1{work + size_t(kRkK1) * S * ld + node, ld};
+  const RVec<R> yn{work + size_t(kRkYn) * S * ld + node, ld};
+  const RVec<R> ep{work + size_t(kRkEp) * S * ld + node, ld};
+  double ldt_state = B.ds[size_t(kRkLdt) * ld + node];
+  double facmax = B.ds[size_t(kFacmax) * ld + node];
+  int nfevals = B.is[size_t(kNfevals) * ld + node];
+  int ndtsa = B.is[size_t(kAccepted) * ld + node];
+  int ndtsr = B.is[size_t(kRejects) * ld + node];
+  int failed = B.is[size_t(kFailed) * ld + node];
+
+#pragma unroll 1
+  for (int k = 0; k < nouter; ++k) {
+    if (!failed) {
+      if (!Rkf32Node<M, R>::forward(yv, k1, yn, ep, steps[k].x, steps[k].y, acc, ap, ldt_state, facmax, nfevals,
+                                    ndtsa, ndtsr)) {
+        failed = 1;
+        atomicOr(B.error, 1);
+      }
+    }
+    if (fo.out) {
+      constexpr int F = M::num_field_states;
+#pragma unroll
+      for (int i = 0; i < F; ++i) {
+        const size_t idx = fo.tangled ? size_t(node) * F + i : size_t(N) * i + node;
+        fo.out[size_t(k) * fo.row_stride + idx] = double(yv[M::field_state_indices[i]]);
+      }
+    }
+  }
+  B.ds[size_t(kRkLdt) * ld + node] = ldt_state;
+  B.ds[size_t(kFacmax) * ld + node] = facmax;
+  B.is[size_t(kNfevals) * ld + node] = nfevals;
+  B.is[size_t(kAccepted) * ld + node] = ndtsa;
+  B.is[size_t(kRejects) * ld + node] = ndtsr;
+  B.is[size_t(kFailed) * ld + node] = failed;
+}
+
+}  // namespace goss_hip

@@ -105,6 +105,8 @@ const VariantInfo* find_variant(const char* name)
 
 bool is_explicit(int solver) { return solver >= 0 && solver <= kGRL2; }
 bool is_implicit(int solver) { return solver == kImplicitEuler || solver == kESDIRK23a; }
+// Solvers with per-node state buffers (ds / is / error flag).
+bool has_node_state(int solver) { return is_implicit(solver) || solver == kRKF32; }
 
 void default_params(int solver, goss_hip_solver_params* p)
 {
@@ -213,6 +215,10 @@ struct goss_hip_system {
       launch_implicit(d, n, field_out, tangled);
       return;
     }
+    if (solver == kRKF32) {
+      launch_rkf32(d, n, field_out, tangled);
+      return;
+    }
     const ExplicitLaunch launch = v->explicit_launch[solver][prec];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not available for this model");
     for (size_t off = 0; off < n; off += steps_per_launch) {
@@ -276,13 +282,46 @@ struct goss_hip_system {
       a.stream = stream;
       hip_check(launch(a), "kernel launch");
     }
-    hip_check(hipMemcpyAsync(h_error, ib.error, sizeof(int), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(flag)");
-    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    if (*h_error) {
+    if (failed()) {
       std::ostringstream msg;
       msg << "Newtons solver failed to converge as dt become smaller than 'min_dt' " << sp.min_dt << std::endl;
       throw Error(GOSS_HIP_ESOLVER, msg.str());
     }
+  }
+
+  // RKF32 with per-node controller state (step-size guess, facmax, counters).
+  void launch_rkf32(const double2* d, size_t n, double* field_out, int tangled)
+  {
+    const AdaptiveLaunch launch = v->rkf32_launch[prec];
+    if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "RKF32 not compiled for this model and precision");
+    for (size_t off = 0; off < n; off += steps_per_launch) {
+      const int cnt = int(std::min<size_t>(steps_per_launch, n - off));
+      AdaptiveArgs a;
+      a.y = y;
+      a.fp = fp;
+      a.steps = d + off;
+      a.nouter = cnt;
+      a.shared = shared.data();
+      a.ap.atol = sp.atol;
+      a.ap.rtol = sp.rtol;
+      a.B = ib;
+      a.N = N;
+      a.ld = ld;
+      a.fo.out = field_out ? field_out + off * size_t(N) * v->NFS : nullptr;
+      a.fo.row_stride = size_t(N) * v->NFS;
+      a.fo.tangled = tangled;
+      a.stream = stream;
+      hip_check(launch(a), "kernel launch");
+    }
+    if (failed()) throw Error(GOSS_HIP_ESOLVER, "RKF32 did not reach the end of the interval");
+  }
+
+  // Reads the system's failure flag (waits for the stream).
+  bool failed()
+  {
+    hip_check(hipMemcpyAsync(h_error, ib.error, sizeof(int), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(flag)");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return *h_error != 0;
   }
 
   void reset_default()
@@ -432,6 +471,8 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
       if (precision != kFP64) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solvers run in FP64 only");
       if (!v->implicit_launch[solver == kImplicitEuler ? 0 : 1])
         throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
+    } else if (solver == kRKF32) {
+      if (!v->rkf32_launch[precision]) throw Error(GOSS_HIP_EUNSUPPORTED, "RKF32 runs in FP64 and FP32 only");
     } else {
       throw Error(GOSS_HIP_EUNSUPPORTED, "solver not available on the device path");
     }
@@ -473,13 +514,20 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
       if (solver == kESDIRK23a) hip_check(hipMalloc(&s->ib.jfac, mat), "hipMalloc(factorized jacobian)");
       const size_t work = size_t(implicit_work_vectors(solver)) * v->S * s->ld * sizeof(double);
       hip_check(hipMalloc(&s->ib.work, work), "hipMalloc(work vectors)");
+      hip_check(hipMemsetAsync(s->ib.jac, 0, mat, s->stream), "hipMemset");
+      if (s->ib.jfac) hip_check(hipMemsetAsync(s->ib.jfac, 0, mat, s->stream), "hipMemset");
+      hip_check(hipMemsetAsync(s->ib.work, 0, work, s->stream), "hipMemset");
+    }
+    if (solver == kRKF32) {
+      const size_t work = size_t(kNumRkf32Work) * v->S * s->ld * sizeof(double);
+      hip_check(hipMalloc(&s->ib.work, work), "hipMalloc(work vectors)");
+      hip_check(hipMemsetAsync(s->ib.work, 0, work, s->stream), "hipMemset");
+    }
+    if (has_node_state(solver)) {
       hip_check(hipMalloc(&s->ib.ds, size_t(kNumDState) * s->ld * sizeof(double)), "hipMalloc(solver state)");
       hip_check(hipMalloc(&s->ib.is, size_t(kNumIState) * s->ld * sizeof(int)), "hipMalloc(solver state)");
       hip_check(hipMalloc(&s->ib.error, sizeof(int)), "hipMalloc(flag)");
       hip_check(hipHostMalloc(&s->h_error, sizeof(int), hipHostMallocDefault), "hipHostMalloc(flag)");
-      hip_check(hipMemsetAsync(s->ib.jac, 0, mat, s->stream), "hipMemset");
-      if (s->ib.jfac) hip_check(hipMemsetAsync(s->ib.jfac, 0, mat, s->stream), "hipMemset");
-      hip_check(hipMemsetAsync(s->ib.work, 0, work, s->stream), "hipMemset");
       hip_check(hipMemsetAsync(s->ib.error, 0, sizeof(int), s->stream), "hipMemset");
       hip_check(layout_init_implicit(s->ib.ds, s->ib.is, s->N, s->ld, s->sp.eta_0, s->sp.always_recompute_jacobian,
                                      s->stream),
@@ -770,13 +818,14 @@ int goss_hip_system_get_solver_counters(goss_hip_system* s, int64_t* out)
     check_sys(s);
     s->use_device();
     for (int k = 0; k < 4; ++k) out[k] = 0;
-    if (!is_implicit(s->solver)) return;
+    if (!has_node_state(s->solver)) return;
     std::vector<int> h(size_t(kNumIState) * s->ld);
     hip_check(hipMemcpyAsync(h.data(), s->ib.is, h.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream),
               "hipMemcpyAsync(D2H)");
     hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
-    const int fields[4] = {kNewtonIts, kJacComp, kRejects, kAccepted};
-    for (int k = 0; k < 4; ++k)
+    const bool rk = s->solver == kRKF32;
+    const int fields[4] = {kNewtonIts, rk ? kNfevals : kJacComp, kRejects, kAccepted};
+    for (int k = rk ? 1 : 0; k < 4; ++k)
       for (uint32_t n = 0; n < s->N; ++n) out[k] += h[size_t(fields[k]) * s->ld + n];
   });
 }

@@ -45,8 +45,17 @@ struct ImplicitParams {
   int num_refinements;
 };
 
-// Per-node solver state, structure of arrays [field][ld].
-enum DState : int { kEta = 0, kEsEta, kLocError, kPrevDt, kNumDState };
+// Per-node solver state, structure of arrays [field][ld] (shared by the
+// implicit solvers and RKF32).
+enum DState : int {
+  kEta = 0,   // ImplicitODESolver::_eta
+  kEsEta,     // ESDIRK23a::eta
+  kLocError,  // ESDIRK23a::loc_error
+  kPrevDt,    // ESDIRK23a::prev_dt
+  kRkLdt,     // AdaptiveExplicitSolver::_ldt (RKF32 initial step guess)
+  kFacmax,    // AdaptiveExplicitSolver::facmax
+  kNumDState
+};
 enum IState : int {
   kRecompute = 0,   // ImplicitODESolver::_recompute_jacobian
   kNewtonIts,       // _newton_iterations
@@ -57,7 +66,8 @@ enum IState : int {
   kJacComp,         // counters
   kRejects,
   kAccepted,
-  kFailed,          // ImplicitEuler gave up below min_dt
+  kFailed,          // ImplicitEuler gave up below min_dt; RKF32 did not reach the interval end
+  kNfevals,         // RKF32::nfevals
   kNumIState
 };
 

@@ -122,6 +122,9 @@ __global__ void init_implicit_state(double* __restrict__ ds, int* __restrict__ i
   ds[size_t(kEsEta) * ld + n] = 1.0;
   ds[size_t(kLocError) * ld + n] = 1e9;
   ds[size_t(kPrevDt) * ld + n] = 0.1;
+  // AdaptiveExplicitSolver constructor and reset (AdaptiveExplicitSolver.cpp:28-31, 58-63).
+  ds[size_t(kRkLdt) * ld + n] = 0.1;
+  ds[size_t(kFacmax) * ld + n] = 2.0;
   for (int k = 0; k < kNumIState; ++k) is[size_t(k) * ld + n] = 0;
   is[size_t(kRecompute) * ld + n] = 1;
   is[size_t(kAlways) * ld + n] = always;

@@ -9,6 +9,7 @@
 
 #include "goss_kernels.h"
 #include "goss_implicit.h"
+#include "goss_adaptive.h"
 
 namespace goss_hip {
 
@@ -43,6 +44,21 @@ struct ImplicitArgs {
 
 typedef hipError_t (*ImplicitLaunch)(const ImplicitArgs&);
 
+struct AdaptiveArgs {
+  void* y;                 // states [S][ld] of the precision's type
+  const void* fp;          // field parameters [NFP][ld]
+  const double2* steps;
+  int nouter;
+  const double* shared;
+  AdaptiveParams ap;
+  ImplicitBuffers B;       // per-node controller state (ds, is, error)
+  unsigned N, ld;
+  FieldOut fo;
+  hipStream_t stream;
+};
+
+typedef hipError_t (*AdaptiveLaunch)(const AdaptiveArgs&);
+
 struct VariantInfo {
   const char* name;
   int S, P, NFS, NFP, NCOMP;
@@ -57,6 +73,7 @@ struct VariantInfo {
   const char* const* field_parameter_names;
   ExplicitLaunch explicit_launch[kNumSolvers][kNumPrecisions];
   ImplicitLaunch implicit_launch[2];  // [0] ImplicitEuler, [1] ESDIRK23a (FP64)
+  AdaptiveLaunch rkf32_launch[kNumPrecisions];  // FP64 and FP32
 };
 
 // All variants, in registration order.

@@ -85,6 +85,16 @@ hipError_t launch_implicit(const ImplicitArgs& a)
 }
 #endif
 
+template <class R>
+hipError_t launch_rkf32(const AdaptiveArgs& a)
+{
+  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
+  const dim3 grid((a.N + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((rkf32_kernel<M, R>), grid, dim3(kBlock), 0, a.stream, static_cast<R*>(a.y),
+                     static_cast<const R*>(a.fp), a.steps, a.nouter, sp, a.ap, a.B, a.N, a.ld, a.fo);
+  return hipGetLastError();
+}
+
 template <class R, bool COMP>
 void register_family(VariantInfo& v, int prec)
 {
@@ -125,6 +135,7 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   register_family<double, false>(v, kFP64);
   v.implicit_launch[0] = &launch_implicit<kImplicitEuler>;
   v.implicit_launch[1] = &launch_implicit<kESDIRK23a>;
+  v.rkf32_launch[kFP64] = &launch_rkf32<double>;
 }
 #elif defined(GOSS_PREC_F32)
 void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
@@ -132,6 +143,7 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
   fill_info(v);
   register_family<float, false>(v, kFP32);
   register_family<float, true>(v, kFP32C);
+  v.rkf32_launch[kFP32] = &launch_rkf32<float>;
 }
 #else
 #error "define GOSS_PREC_F64 or GOSS_PREC_F32"

@@ -50,7 +50,9 @@ typedef enum goss_hip_solver {
 
 /* Arithmetic of the per-node work.  FP64 is the reference's arithmetic.
  * FP32C computes in FP32 and carries the states whose derivative is not
- * affine in themselves (linear_terms == 0) as FP32 hi + lo pairs. */
+ * affine in themselves (linear_terms == 0) as FP32 hi + lo pairs.
+ * ImplicitEuler and ESDIRK23a run in FP64; RKF32 in FP64 or FP32 (its step
+ * sizes and local time stay FP64). */
 typedef enum goss_hip_precision {
   GOSS_HIP_FP64 = 0,
   GOSS_HIP_FP32 = 1,
@@ -166,11 +168,13 @@ int goss_hip_system_get_num_threads(goss_hip_system* sys, uint32_t* num_threads)
 int goss_hip_system_num_nodes(goss_hip_system* sys, uint32_t* num_nodes);
 /* Outer steps fused into one kernel launch by forward_times / solve. */
 int goss_hip_system_set_steps_per_launch(goss_hip_system* sys, uint32_t steps);
-/* Sums over nodes of the implicit solvers' per-node counters:
- * out[0] Newton iterations of the last stage/solve (ImplicitODESolver::_newton_iterations),
- * out[1] Jacobian evaluations (num_jac_comp, ImplicitODESolver.h:65-68),
- * out[2] rejected Newton solves / steps, out[3] accepted ESDIRK23a steps.
- * Zeros for explicit solvers. */
+/* Sums over nodes of the per-node solver counters.
+ * ImplicitEuler / ESDIRK23a: out[0] Newton iterations of the last stage/solve
+ * (ImplicitODESolver::_newton_iterations), out[1] Jacobian evaluations
+ * (num_jac_comp, ImplicitODESolver.h:65-68), out[2] rejected Newton solves /
+ * steps, out[3] accepted ESDIRK23a steps.
+ * RKF32 (RKF32.h:70-72): out[0] 0, out[1] nfevals, out[2] ndtsr, out[3] ndtsa.
+ * Zeros for the fixed-step solvers. */
 int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
 /* Wait for all work of the system. */
 int goss_hip_system_synchronize(goss_hip_system* sys);

@@ -110,6 +110,7 @@ def ref_lib():
         L.ref_system_set_num_threads.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         L.ref_system_destroy.argtypes = [ctypes.c_void_p]
         L.ref_system_reset_default.argtypes = [ctypes.c_void_p]
+        L.ref_system_rkf32_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
         L.ref_model_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
         L.ref_model_eval.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
         L.ref_model_linearized_eval.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp, ctypes.c_int]
@@ -307,6 +308,13 @@ class RefSystem(_SystemBase):
     def _check(self, rc):
         if rc != 0:
             raise RuntimeError(self._L.ref_last_error().decode())
+
+
+    def rkf32_counters(self) -> np.ndarray:
+        """nfevals, ndtsa, ndtsr of the system's RKF32 solver (serial mode)."""
+        out = (ctypes.c_long * 3)()
+        self._check(self._L.ref_system_rkf32_counters(self._h, out))
+        return np.array(list(out))
 
 
 class RefModel:

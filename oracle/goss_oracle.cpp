@@ -169,6 +169,9 @@ struct SolverParams {
   // ImplicitEuler.h:36-37 / ESDIRK23a.h:37-38
   int num_refinements_without_always_recomputing_jacobian = 2;
   double min_dt = 0.0001;
+  // AdaptiveExplicitSolver::reset (AdaptiveExplicitSolver.cpp:46-65), set_tol (AdaptiveExplicitSolver.h:71-75)
+  double atol = 1.0e-5;
+  double rtol = 1.0e-8;
 };
 
 constexpr double GOSS_EPS = 3.0e-16;  // constants.h:22
@@ -195,6 +198,12 @@ struct ImplicitState {
   bool new_jacobian = true;
   long nfevals = 0, num_rejected = 0;
   long accepted = 0;
+  // RKF32 / AdaptiveExplicitSolver (AdaptiveExplicitSolver.cpp:28-31, 46-65;
+  // RKF32.cpp:91-103): the initial step guess _ldt and facmax persist from
+  // one forward call to the next.
+  double rk_ldt = 0.1;
+  double facmax = 2.0;
+  long rk_nfevals = 0, ndtsa = 0, ndtsr = 0;
   void init(int S, const SolverParams& sp)
   {
     jac.assign(size_t(S) * S, 0.0);
@@ -213,12 +222,12 @@ inline unsigned long substeps(double ldt_0, double dt)
 // Per-thread scratch, sized to the model.
 struct Work {
   std::vector<double> f1, f2, k1, k2, k3, k4, tmp, y2;
-  std::vector<double> z1, z2, z3, z4, yh, prev, b, dz, yz, fy, y_int, z_int, prev_dz, ff1, ff2;
+  std::vector<double> z1, z2, z3, z4, yh, prev, b, dz, yz, fy, y_int, z_int, prev_dz, ff1, ff2, ki, yn, e;
   ImplicitState carried;  // the per-thread solver instance of the reference
   void size(int S)
   {
     for (auto* v : {&f1, &f2, &k1, &k2, &k3, &k4, &tmp, &y2, &z1, &z2, &z3, &z4, &yh, &prev, &b, &dz, &yz, &fy,
-                    &y_int, &z_int, &prev_dz, &ff1, &ff2})
+                    &y_int, &z_int, &prev_dz, &ff1, &ff2, &ki, &yn, &e})
       v->assign(S, 0.0);
   }
 };
@@ -633,6 +642,96 @@ void esdirk_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& 
   st.first_step = false;
 }
 
+// ---------------------------------------------------------------------------
+// RKF32 (RKF32.cpp:105-222) with AdaptiveExplicitSolver::new_time_step
+// (AdaptiveExplicitSolver.cpp:168-244)
+// ---------------------------------------------------------------------------
+
+// Sub-steps one forward call may take before it is declared stuck.  The
+// reference loops until |_t - t_end| < 1e-14 (AdaptiveExplicitSolver.cpp:206)
+// and never returns when that test cannot be met (a step size that has
+// underflowed after repeated rejections of a non-finite error).
+constexpr long kRkf32MaxSteps = 1000000;
+
+void rkf32_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* y_out, double t,
+                   double interval, const double* p, const double* c, Work& w)
+{
+  const int S = m.S;
+  // Coefficients (RKF32.cpp:34-37).
+  const double a21 = 1.0 / 2.0, a32 = 3.0 / 4.0, b1 = 2.0 / 9.0, b2 = 1.0 / 3.0, b3 = 4.0 / 9.0;
+  const double bh1 = 7.0 / 24.0, bh2 = 1.0 / 4.0, bh3 = 1.0 / 3.0, bh4 = 1.0 / 8.0;
+  const double d1 = b1 - bh1, d2 = b2 - bh2, d3 = b3 - bh3, d4 = -bh4, c2 = 1.0 / 2.0, c3 = 3.0 / 4.0;
+  // AdaptiveExplicitSolver::reset values (AdaptiveExplicitSolver.cpp:58-64); _iord = 3 (RKF32.cpp:39).
+  const double facmin = 0.5, facmaxb = 2.0, stabfac = 0.9, iord = 3.0;
+  // The reference swaps the y / yn and k1 / k4 buffers on acceptance; the
+  // values are copied here.
+  double* y = y_out;
+  double *k1 = w.k1.data(), *k2 = w.k2.data(), *k3 = w.k3.data(), *k4 = w.k4.data();
+  double *ki = w.ki.data(), *yn = w.yn.data(), *e = w.e.data();
+  const double t_end = t + interval;
+  bool reached_tend = false;
+  double lt = t;  // _t
+  // `first` is set by reset() and never cleared (RKF32.cpp:126-131).
+  m.eval(y, t, p, c, k1);
+  st.rk_nfevals += 1;
+  // _ldt is AdaptiveExplicitSolver::_ldt, never negative (its dtinit branch,
+  // RKF32.cpp:134-138, is not reachable).
+  double dt = st.rk_ldt;
+  long guard = 0;
+  while (!reached_tend) {
+    if (++guard > kRkf32MaxSteps) throw std::runtime_error("RKF32 did not reach the end of the interval");
+    for (int i = 0; i < S; ++i) ki[i] = y[i] + dt * a21 * k1[i];
+    m.eval(ki, t + c2 * dt, p, c, k2);  // stage times from the call's t (RKF32.cpp:155, 160)
+    for (int i = 0; i < S; ++i) ki[i] = y[i] + dt * a32 * k2[i];
+    m.eval(ki, t + c3 * dt, p, c, k3);
+    for (int i = 0; i < S; ++i) yn[i] = y[i] + dt * (b1 * k1[i] + b2 * k2[i] + b3 * k3[i]);
+    m.eval(yn, t + dt, p, c, k4);
+    st.rk_nfevals += 3;
+    for (int i = 0; i < S; ++i) e[i] = dt * (d1 * k1[i] + d2 * k2[i] + d3 * k3[i] + d4 * k4[i]);
+    // new_time_step
+    double err = 0.0;
+    for (int i = 0; i < S; ++i) {
+      const double yi_abs = std::fabs(y[i]), yni_abs = std::fabs(yn[i]);
+      const double mx = yi_abs > yni_abs ? yi_abs : yni_abs;
+      const double sk = sp.atol + sp.rtol * mx;
+      const double tmp = e[i] / sk;
+      err += tmp * tmp;
+    }
+    err = std::sqrt(err / S);
+    if (std::isnan(err) || std::isinf(err)) err = 2000.0;
+    bool accepted;
+    if (err <= 1.0) {
+      lt += dt;
+      st.accepted += 1;
+      accepted = true;
+      if (std::fabs(lt - t_end) < 1e-14) reached_tend = true;
+    } else {
+      st.num_rejected += 1;
+      accepted = false;
+    }
+    double fac = stabfac * std::pow(1.0 / err, 1.0 / (iord + 1));
+    if (facmin > fac) fac = facmin;
+    if (fac > st.facmax) fac = st.facmax;
+    st.facmax = accepted ? facmaxb : 1.0;
+    dt *= fac;
+    if (lt + dt >= t_end)
+      dt = t_end - lt;
+    else if (lt + 1.5 * dt >= t_end)
+      dt = (t_end - lt) / 2.0;
+    else
+      st.rk_ldt = dt;
+    if (accepted) {
+      st.ndtsa += 1;
+      for (int i = 0; i < S; ++i) {
+        y[i] = yn[i];
+        k1[i] = k4[i];
+      }
+    } else {
+      st.ndtsr += 1;
+    }
+  }
+}
+
 }  // namespace
 
 struct oracle_system {
@@ -670,12 +769,14 @@ void forward_node(oracle_system* s, unsigned node, double t, double interval, Wo
   double c[kMaxConstants];
   m.constants(prm, c);
   double* y = &s->states[size_t(node) * m.S];
-  if (s->kind == IMPLICIT_EULER || s->kind == ESDIRK23A) {
+  if (s->kind == IMPLICIT_EULER || s->kind == ESDIRK23A || s->kind == RKF32) {
     ImplicitState& st = s->per_node_state ? s->node_state[node] : w.carried;
     if (s->kind == IMPLICIT_EULER)
       implicit_euler_forward(m, s->sp, st, y, t, interval, prm, c, w);
-    else
+    else if (s->kind == ESDIRK23A)
       esdirk_forward(m, s->sp, st, y, t, interval, prm, c, w);
+    else
+      rkf32_forward(m, s->sp, st, y, t, interval, prm, c, w);
     return;
   }
   forward_explicit(s->kind, m, s->sp, y, t, interval, prm, c, w);
@@ -724,6 +825,8 @@ int set_param(oracle_system* s, const std::string& name, double v)
   if (name == "max_relative_previous_residual") { s->sp.max_relative_previous_residual = v; return 0; }
   if (name == "always_recompute_jacobian") { s->sp.always_recompute_jacobian = v != 0; return 0; }
   if (name == "min_dt") { s->sp.min_dt = v; return 0; }
+  if (name == "atol") { s->sp.atol = v; return 0; }
+  if (name == "rtol") { s->sp.rtol = v; return 0; }
   if (name == "num_refinements_without_always_recomputing_jacobian") {
     s->sp.num_refinements_without_always_recomputing_jacobian = int(v);
     return 0;
@@ -810,7 +913,6 @@ int oracle_system_create(const char* variant, const char* solver, unsigned num_n
   const int kind = solver_kind(solver);
   if (kind < 0) { g_err = std::string("unknown solver ") + solver; return -1; }
   if (kVariants[v].K > kMaxConstants) { g_err = "model has too many constants"; return -1; }
-  if (kind == RKF32) { g_err = std::string("solver not available in the oracle: ") + solver; return -1; }
   auto* s = new oracle_system;
   s->m = &kVariants[v];
   s->kind = kind;
@@ -966,11 +1068,15 @@ void oracle_system_reset_default(oracle_system* s) { reset_default(s); }
 void oracle_system_counters(const oracle_system* s, long* out4)
 {
   for (int i = 0; i < 4; ++i) out4[i] = 0;
+  // Implicit solvers: Newton iterations of the last solve, Jacobians,
+  // rejections, accepted steps.  RKF32: 0, RHS evaluations (nfevals),
+  // rejected steps (ndtsr), accepted steps (ndtsa).
+  const bool rk = s->kind == RKF32;
   auto add = [&](const ImplicitState& st) {
-    out4[0] += st.newton_iterations;
-    out4[1] += st.jac_comp;
-    out4[2] += st.rejects + st.num_rejected;
-    out4[3] += st.accepted;
+    out4[0] += rk ? 0 : st.newton_iterations;
+    out4[1] += rk ? st.rk_nfevals : st.jac_comp;
+    out4[2] += rk ? st.ndtsr : st.rejects + st.num_rejected;
+    out4[3] += rk ? st.ndtsa : st.accepted;
   };
   if (s->per_node_state)
     for (const auto& st : s->node_state) add(st);

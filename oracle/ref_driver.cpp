@@ -134,6 +134,12 @@ int ref_system_create(const char* variant, const char* solver, unsigned num_node
     // its per-node vector (ODESystemSolver.cpp:52-58).
     for (int i = 0; i < n_solver_params; ++i) set_solver_param(*r->solver, solver_param_names[i], solver_param_values[i]);
     r->sys.reset(new ODESystemSolver(num_nodes, r->solver, r->ode));
+    // AdaptiveExplicitSolver::reset (run by the attach above) restores the
+    // default tolerances (AdaptiveExplicitSolver.cpp:52-53): apply them again.
+    for (int i = 0; i < n_solver_params; ++i) {
+      const std::string name(solver_param_names[i]);
+      if (name == "atol" || name == "rtol") set_solver_param(*r->solver, name, solver_param_values[i]);
+    }
     *out = r;
     return 0;
   } catch (std::exception& e) {
@@ -143,6 +149,19 @@ int ref_system_create(const char* variant, const char* solver, unsigned num_node
 }
 
 void ref_system_destroy(void* h) { delete static_cast<RefSystem*>(h); }
+
+// RKF32 counters of the system's own solver (serial mode): nfevals, ndtsa,
+// ndtsr (RKF32.h:70-72).  Returns -1 for other solvers.
+int ref_system_rkf32_counters(void* h, long* out3)
+{
+  auto* r = static_cast<RefSystem*>(h);
+  auto* rk = dynamic_cast<RKF32*>(r->solver.get());
+  if (!rk) return -1;
+  out3[0] = rk->nfevals;
+  out3[1] = rk->ndtsa;
+  out3[2] = rk->ndtsr;
+  return 0;
+}
 
 int ref_system_set_num_threads(void* h, unsigned n) { REF_TRY(static_cast<RefSystem*>(h)->sys->set_num_threads(n)) }
 

@@ -8,11 +8,14 @@
 #include <string>
 #include <vector>
 
+#include "goss_adaptive.h"
 #include "goss_implicit.h"
 #include "tp06_m.h"
 #include "tp06_epi.h"
 #include "beeler_reuter_1977.h"
 #include "fhn.h"
+#include "lorentz.h"
+#include "fitzhughnagumo.h"
 
 using namespace goss_hip;
 
@@ -43,6 +46,46 @@ GOSS_HARNESS_MODEL(tp06_m)
 GOSS_HARNESS_MODEL(tp06_epi)
 GOSS_HARNESS_MODEL(beeler_reuter_1977)
 GOSS_HARNESS_MODEL(fhn)
+GOSS_HARNESS_MODEL(lorentz)
+GOSS_HARNESS_MODEL(fitzhughnagumo)
+
+// RKF32 per node with the device logic (goss_adaptive.h); counters are
+// nfevals, ndtsa, ndtsr summed over nodes.
+template <class M>
+int run_rkf32(unsigned N, const double* y0, const double* t, unsigned long n, const double* prm, double atol,
+              double rtol, double* out_states, long* counters)
+{
+  constexpr int S = M::num_states;
+  SharedParamsOf<M, double> sp;
+  double c[M::num_constants > 0 ? M::num_constants : 1];
+  M::constants(prm, c);
+  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = prm[k];
+  for (int k = 0; k < M::num_constants; ++k) sp.c[k] = c[k];
+  const double fpv[1] = {0.0};
+  const DevAcc<M, double> acc{sp, fpv};
+  const AdaptiveParams ap{atol, rtol};
+  for (int k = 0; k < 4; ++k) counters[k] = 0;
+  int failed_any = 0;
+  for (unsigned node = 0; node < N; ++node) {
+    double ys[S], w[3 * S];
+    for (int i = 0; i < S; ++i) ys[i] = y0[size_t(node) * S + i];
+    const RVec<double> yv{ys, 1}, k1{w, 1}, yn{w + S, 1}, ep{w + 2 * S, 1};
+    double ldt = 0.1, facmax = 2.0;
+    int nfevals = 0, ndtsa = 0, ndtsr = 0;
+    for (unsigned long k = 0; k + 1 < n; ++k) {
+      if (!Rkf32Node<M, double>::forward(yv, k1, yn, ep, t[k], t[k + 1] - t[k], acc, ap, ldt, facmax, nfevals, ndtsa,
+                                         ndtsr)) {
+        failed_any = 1;
+        break;
+      }
+    }
+    for (int i = 0; i < S; ++i) out_states[size_t(node) * S + i] = ys[i];
+    counters[1] += nfevals;
+    counters[2] += ndtsr;
+    counters[3] += ndtsa;
+  }
+  return failed_any;
+}
 
 template <class M>
 int run(int solver, unsigned N, const double* y0, const double* t, unsigned long n, const double* prm,
@@ -107,6 +150,14 @@ extern "C" int kernel_logic_run(const char* variant, int solver, unsigned N, con
   ImplicitParams ip{ldt_0, eta_0, kappa, relative_tolerance, max_rel_prev, min_dt, max_iterations, always,
                     num_refinements};
   const std::string v(variant);
+  if (solver == kRKF32) {
+    // atol / rtol arrive in the eta_0 / kappa slots.
+    if (v == "tp06_m") return run_rkf32<M_tp06_m>(N, y0, t, n, params, eta_0, kappa, out_states, counters);
+    if (v == "lorentz") return run_rkf32<M_lorentz>(N, y0, t, n, params, eta_0, kappa, out_states, counters);
+    if (v == "fitzhughnagumo")
+      return run_rkf32<M_fitzhughnagumo>(N, y0, t, n, params, eta_0, kappa, out_states, counters);
+    return -2;
+  }
   if (v == "tp06_m") return run<M_tp06_m>(solver, N, y0, t, n, params, ip, out_states, counters);
   if (v == "tp06_epi") return run<M_tp06_epi>(solver, N, y0, t, n, params, ip, out_states, counters);
   if (v == "beeler_reuter_1977") return run<M_beeler_reuter_1977>(solver, N, y0, t, n, params, ip, out_states, counters);

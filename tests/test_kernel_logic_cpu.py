@@ -1,7 +1,7 @@
-"""Host build of the implicit kernels' per-node logic against the oracle.
+"""Host build of the implicit and RKF32 kernels' per-node logic against the oracle.
 
-tests/harness/kernel_logic_cpu.hip compiles goss_amd/csrc/goss_implicit.h for
-the host (the same source the GPU kernels instantiate) and runs it node by
+tests/harness/kernel_logic_cpu.hip compiles goss_amd/csrc/goss_implicit.h and
+goss_adaptive.h for the host (the same source the GPU kernels instantiate) and runs it node by
 node with the host math library.  The oracle's per-node-state mode
 (oracle/goss_oracle.cpp) gives each node its own solver state, which is what
 the device kernels do; with the same libm the two must agree bit for bit, so
@@ -98,3 +98,28 @@ def test_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, ldt, solver)
     np.testing.assert_array_equal(got, want)
     # counters: Newton iterations of the last solve, Jacobians, rejects, accepted
     np.testing.assert_array_equal(got_c[1:], want_c[1:])
+
+
+@pytest.mark.parametrize("variant,dt,nsteps,tol", [
+    ("tp06_m", 0.1, 150, (1e-5, 1e-8)),
+    ("lorentz", 0.01, 100, (1e-5, 1e-8)),
+    ("lorentz", 0.05, 60, (1e-7, 1e-9)),
+    ("fitzhughnagumo", 0.5, 100, (1e-5, 1e-8)),
+])
+def test_rkf32_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, tol):
+    L = _harness()
+    o = oracle.OracleSystem(variant, "RKF32", 7, {"atol": tol[0], "rtol": tol[1]}, per_node_state=True)
+    rng = np.random.Generator(np.random.PCG64(42))
+    y0 = o.states() * (1.0 + rng.uniform(-0.01, 0.01, o.states().shape))
+    t = np.arange(nsteps + 1) * dt
+    o.set_states(y0)
+    o.forward_times(t)
+    params = np.ascontiguousarray(json.load(open(META))[variant]["default_parameters"], dtype=np.float64)
+    out = np.zeros_like(y0)
+    counters = (ctypes.c_long * 4)()
+    # solver 9 = RKF32; atol / rtol travel in the eta_0 / kappa slots of the harness call.
+    rc = L.kernel_logic_run(variant.encode(), 9, y0.shape[0], _p(y0), _p(t), len(t), _p(params), -1.0, tol[0],
+                            tol[1], 0.0, 0.0, 0.0, 0, 0, 0, _p(out), counters)
+    assert rc == 0
+    np.testing.assert_array_equal(out, o.states())
+    np.testing.assert_array_equal(np.array(list(counters))[1:], o.counters()[1:])

@@ -215,3 +215,75 @@ def test_esdirk_per_node_state_is_order_independent():
     a.forward_times(t)
     b.forward_times(t)
     assert np.array_equal(a.states()[perm], b.states())
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("variant,dt,nsteps,params", [
+    ("lorentz", 0.01, 300, {}),
+    ("lorentz", 0.05, 80, {"atol": 1e-7, "rtol": 1e-9}),
+    ("fitzhughnagumo", 0.5, 100, {}),
+    ("tp06_m", 0.1, 150, {}),            # through the stimulus at 10 ms
+    ("oscilator", 0.1, 100, {"atol": 1e-8}),
+])
+def test_oracle_rkf32_matches_reference_build(variant, dt, nsteps, params):
+    # Serial mode: one RKF32 instance carries its step-size guess (_ldt) and
+    # facmax from node to node (AdaptiveExplicitSolver.cpp:216-241); the
+    # oracle's default mode does the same.  Threaded reference runs are not
+    # compared: the thread copies leave RKF32::first uninitialised (RKF32.h:95
+    # is not copied by RKF32.cpp:52-61).
+    N = 11
+    rng = np.random.Generator(np.random.PCG64(21))
+    o = oracle.OracleSystem(variant, "RKF32", N, params)
+    r = oracle.RefSystem(variant, "RKF32", N, params)
+    y0 = o.states() * (1 + rng.uniform(-0.01, 0.01, o.states().shape))
+    o.set_states(y0)
+    r.set_states(y0)
+    t = np.arange(nsteps + 1) * dt
+    o.forward_times(t)
+    r.forward_times(t)
+    assert np.array_equal(o.states(), r.states())
+    nfevals, ndtsa, ndtsr = r.rkf32_counters()
+    oc = o.counters()
+    assert (oc[1], oc[3], oc[2]) == (nfevals, ndtsa, ndtsr)
+
+
+def test_rkf32_linear_decay_tolerance_and_order():
+    # y' = a*y + b on one node: the global error of RKF32 at t = 2 stays
+    # within a small multiple of the tolerance, and tightening atol/rtol
+    # by 1e3 cuts the error by at least 10x (a 3(2) pair, error ~ tol^(3/4)).
+    a, b = -1.3, 0.7
+    exact = lambda t, y0: (y0 + b / a) * np.exp(a * t) - b / a
+    errs = []
+    for tol in (1e-6, 1e-9):
+        o = oracle.OracleSystem("linear_decay", "RKF32", 1, {"atol": tol, "rtol": tol})
+        o.set_field_parameters(np.array([[a, b]]))
+        o.set_states(np.array([[1.0]]))
+        o.forward_times(np.linspace(0.0, 2.0, 21))
+        errs.append(abs(o.states()[0, 0] - exact(2.0, 1.0)))
+        assert errs[-1] < 200 * tol
+    assert errs[1] < errs[0] / 10
+
+
+def test_rkf32_per_node_state_is_order_independent():
+    N = 7
+    rng = np.random.Generator(np.random.PCG64(22))
+    a = oracle.OracleSystem("lorentz", "RKF32", N, per_node_state=True)
+    y0 = a.states() * (1 + rng.uniform(-0.05, 0.05, a.states().shape))
+    perm = rng.permutation(N)
+    b = oracle.OracleSystem("lorentz", "RKF32", N, per_node_state=True)
+    a.set_states(y0)
+    b.set_states(y0[perm])
+    t = np.arange(101) * 0.01
+    a.forward_times(t)
+    b.forward_times(t)
+    assert np.array_equal(a.states()[perm], b.states())
+
+
+def test_rkf32_stuck_node_raises():
+    # A NaN state gives a non-finite error norm on every step (taken as 2000,
+    # AdaptiveExplicitSolver.cpp:195-196); the step size underflows and the
+    # reference would loop forever.  The oracle stops after its sub-step cap.
+    o = oracle.OracleSystem("lorentz", "RKF32", 1)
+    o.set_states(np.array([[np.nan, 0.0, 0.0]]))
+    with pytest.raises(RuntimeError, match="RKF32 did not reach the end of the interval"):
+        o.forward(0.0, 0.01)
