@@ -36,6 +36,8 @@ FP_DEFAULTS = np.array([0.153, 0.098, 3.98e-5])
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# VALU issue: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 VALU instruction per SIMD every 4 cycles.
+PEAK_VALU_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 
 class HipEvents:
@@ -259,6 +261,14 @@ def main():
                 "kernel": "goss_hip::explicit_kernel<tp06_m_fp3, GRL1, float, COMP=true>",
                 "flops_per_node_step": flops, "node_steps_per_launch": N * spl,
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+    valu = (traffic or {}).get("valu", {}).get("f32c")
+    if valu and valu.get("insts_per_node_step"):
+        # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU per node-step, scaled to
+        # this launch) over the live launch time, against the issue peak.
+        insts = valu["insts_per_node_step"] * N * spl / 64
+        roofline["valu_issue"] = {"wave_insts_per_launch": insts, "insts_per_node_step": valu["insts_per_node_step"],
+                                  "achieved": insts / avg_launch_s, "peak": PEAK_VALU_WAVE_INSTS,
+                                  "frac": round(insts / avg_launch_s / PEAK_VALU_WAVE_INSTS, 4)}
     avg64 = res["fp64"]["kernel_ms"] * 1e-3 / res["fp64"]["launches"]
     fp64_achieved = flops * N * spl / avg64 / 1e12
 
