@@ -41,7 +41,8 @@ EXPORTS = (
     "goss_hip_system_reset_default", "goss_hip_system_set_parameter", "goss_hip_system_get_parameter",
     "goss_hip_system_set_num_threads", "goss_hip_system_get_num_threads", "goss_hip_system_num_nodes",
     "goss_hip_system_set_steps_per_launch", "goss_hip_system_synchronize", "goss_hip_system_stream",
-    "goss_hip_system_get_solver_counters",
+    "goss_hip_system_get_solver_counters", "goss_hip_adjoint_buffer_sizes", "goss_hip_adjoint_record",
+    "goss_hip_adjoint_backward",
 )
 
 
@@ -129,6 +130,10 @@ def lib() -> ctypes.CDLL:
     L.goss_hip_system_synchronize.argtypes = [V]
     L.goss_hip_system_stream.argtypes = [V, ctypes.POINTER(ctypes.c_void_p)]
     L.goss_hip_system_get_solver_counters.argtypes = [V, ctypes.POINTER(ctypes.c_int64)]
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.goss_hip_adjoint_buffer_sizes.argtypes = [V, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p, u64p]
+    L.goss_hip_adjoint_record.argtypes = [V, _dp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, V, V]
+    L.goss_hip_adjoint_backward.argtypes = [V, _dp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, V, V, V, V, V]
     _lib = L
     return L
 

@@ -1,0 +1,70 @@
+"""Reverse-mode differentiation of a generated statement list.
+
+The discrete adjoint of a GRL1 step (``GRL1.cpp:61-82``, ``GRL1.h:72-84``)
+
+    y'_i = y_i + b_i * h * phi1(a_i h)    if |a_i| > delta,   y_i + b_i h otherwise,
+    b = rhs(y, p),  a_i = d rhs_i / d y_i (the linearized diagonal),
+    phi1(x) = (exp(x) - 1) / x,
+
+needs the vector-Jacobian product of (y, p) -> (rhs, a) with the seeds
+``bbar_i = lam_i * h * phi1(a_i h)`` and ``abar_i = lam_i * b_i * h^2 * phi1'(a_i h)``.
+:func:`reverse_sweep` writes that product as straight-line accumulations over
+the same named intermediates the forward code computes: walking the
+statements in reverse order, each statement ``n = e`` adds ``n_b * de/dv`` to
+the adjoint ``v_b`` of every differentiable symbol ``v`` in ``e``.  Symbols
+that are shared parameters, host-computed constants, time, or intermediates
+that depend on time only carry no adjoint.
+"""
+from __future__ import annotations
+
+from typing import Dict, Iterable, List, Sequence, Set, Tuple
+
+import sympy as sp
+
+from .analysis import _clean
+
+
+def adj_name(name: str) -> str:
+    return f"{name}__b"
+
+
+def reverse_sweep(stmts: Sequence[Tuple[str, sp.Expr]], seeds: Dict[str, str],
+                  independent: Iterable[str]) -> Tuple[List[Tuple[str, str, sp.Expr]], Set[str]]:
+    """Reverse accumulation over ``stmts`` (evaluation order).
+
+    ``seeds`` maps statement names to the names of their seed adjoints (the
+    outputs).  ``independent`` are the leaf symbols to differentiate with
+    respect to (states, field parameters).  Returns the accumulation list
+    ``(target_adjoint, source_adjoint, partial)`` meaning
+    ``target += source * partial``, in execution order, and the set of
+    adjoint names that receive contributions."""
+    independent = set(independent)
+    names = {n for n, _ in stmts}
+    diffable: Set[str] = set(independent)
+    # Statements that depend on an independent symbol, transitively.
+    for n, e in stmts:
+        if any(s.name in diffable for s in e.free_symbols):
+            diffable.add(n)
+    has_adj: Dict[str, bool] = {n: True for n in seeds}
+    out: List[Tuple[str, str, sp.Expr]] = []
+    for n, e in reversed(stmts):
+        if not has_adj.get(n):
+            continue
+        src = seeds.get(n, adj_name(n))
+        me = sp.Symbol(n, real=True)
+        for v in sorted(e.free_symbols, key=lambda s: s.name):
+            if v.name not in diffable:
+                continue
+            if v.name not in names and v.name not in independent:
+                continue
+            d = _clean(sp.diff(e, v))
+            if d == 0:
+                continue
+            # Reuse the statement's own value inside its derivative
+            # (d exp(u)/du = exp(u) is the statement itself).
+            if not e.is_Symbol and not e.is_Number:
+                d = d.subs(e, me)
+            out.append((adj_name(v.name), src, d))
+            has_adj[v.name] = True
+    targets = {t for t, _, _ in out}
+    return out, targets

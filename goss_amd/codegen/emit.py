@@ -25,6 +25,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import sympy as sp
 
+from .adjoint import reverse_sweep
 from .analysis import Jacobian, Linearized, census_body, census_state_ops, jacobian, linearize, needed_names
 from .hoist import ConstantTable, hoist_statements
 from .ode_model import ODEModel, TIME
@@ -297,6 +298,48 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("}")
     L.append("")
 
+    # -- grl1_vjp: discrete adjoint of one GRL1 step --------------------------
+    out_stmts = [(f"rhs__{i}", rhs_out[i]) for i in range(S)] + [(f"lin__{i}", hlinear[i]) for i in range(S)]
+    vjp_fwd = lin_inter + hlin_first + hlin_second + out_stmts
+    seeds = {f"rhs__{i}": f"bbar__{i}" for i in range(S)}
+    seeds.update({f"lin__{i}": f"abar__{i}" for i in range(S)})
+    independent = list(model.state_names) + list(var.field_parameters)
+    acc_lines, targets = reverse_sweep(vjp_fwd, seeds, independent)
+    for _, _, d in acc_lines:
+        if d.has(sp.Derivative):
+            raise ValueError(f"{model.name}: cannot differentiate {d}")
+    vused = g.used_main(vjp_fwd, [d for _, _, d in acc_lines])
+    g_vjp = _Gen(var, table, vjp_fwd)
+    L.append("// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:")
+    L.append("//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,")
+    L.append("// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h.")
+    L.append("// Reverse accumulation over the linearized_eval statements.")
+    L.append("template <typename real, typename PA, typename treal>")
+    L.append("GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,")
+    L.append("                                  const real h__, const real delta__, const real* __restrict__ lam__,")
+    L.append("                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)")
+    L.append("{")
+    L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;")
+    L += g_vjp.bind(vused | {s_.name for s_ in model.states}, ind)
+    L += g_vjp.lines(vjp_fwd, ind)
+    for i in range(S):
+        L.append(f"{ind}const bool ex__{i} = m_fabs(lin__{i}) > delta__;")
+        L.append(f"{ind}const real bbar__{i} = ex__{i} ? lam__[{i}]*h__*m_phi1(lin__{i}*h__) : lam__[{i}]*h__;")
+        L.append(f"{ind}const real abar__{i} = ex__{i} ? lam__[{i}]*rhs__{i}*(h__*h__)*m_dphi1(lin__{i}*h__) : real(0.0);")
+    for t_ in sorted(targets):
+        L.append(f"{ind}real {t_} = real(0.0);")
+    for tgt, src, d in acc_lines:
+        L.append(f"{ind}{tgt} += {src}*({g_vjp.main.p(d)});")
+    for i, st in enumerate(model.state_names):
+        a = f"{st}__b"
+        L.append(f"{ind}lam_prev__[{i}] = lam__[{i}]" + (f" + {a};" if a in targets else ";"))
+    for j, fpn in enumerate(var.field_parameters):
+        a = f"{fpn}__b"
+        if a in targets:
+            L.append(f"{ind}fp_bar__[{j}] += {a};")
+    L.append("}")
+    L.append("")
+
     # Census of the (unhoisted) model: state-dependent work after CSE, and
     # the printed operations of the hoisted text.
     bd = model.body_dict()
@@ -313,6 +356,8 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         "eval": census_body([e for _, e in eval_stmts] + rhs_out),
         "linearized_eval": census_body([e for _, e in lin_inter + hlin_first + hlin_second] + rhs_out + hlinear),
         "constants": census_body([e for _, e in table.items]),
+        # Forward statements plus the reverse accumulations of grl1_vjp.
+        "grl1_vjp": census_body([e for _, e in vjp_fwd] + [d for _, _, d in acc_lines]),
     }
 
     jac_nnz = 0

@@ -95,6 +95,56 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   linear[0] = dI_dV;
 }
 
+// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
+//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
+// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h.
+// Reverse accumulation over the linearized_eval statements.
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real V = states[0];
+  const real s = states[1];
+  const real V_peak = pa.template get<0>();
+  const real V_rest = pa.template get<1>();
+  const real b = pa.template get<3>();
+  const real c_3 = pa.template get<6>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
+  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
+  const real dI_dV = c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) + c__6*(V - V_rest)*(V - c__1);
+  const real rhs__0 = I + real(i_Stim);
+  const real rhs__1 = b*(V - V_rest - c_3*s);
+  const real lin__0 = dI_dV;
+  const real lin__1 = c__5;
+  const bool ex__0 = m_fabs(lin__0) > delta__;
+  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(lin__0*h__) : lam__[0]*h__;
+  const real abar__0 = ex__0 ? lam__[0]*rhs__0*(h__*h__)*m_dphi1(lin__0*h__) : real(0.0);
+  const bool ex__1 = m_fabs(lin__1) > delta__;
+  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(lin__1*h__) : lam__[1]*h__;
+  const real abar__1 = ex__1 ? lam__[1]*rhs__1*(h__*h__)*m_dphi1(lin__1*h__) : real(0.0);
+  real I__b = real(0.0);
+  real V__b = real(0.0);
+  real dI_dV__b = real(0.0);
+  real s__b = real(0.0);
+  dI_dV__b += abar__0*(real(1.0));
+  V__b += bbar__1*(b);
+  s__b += bbar__1*(-b*c_3);
+  I__b += bbar__0*(real(1.0));
+  V__b += dI_dV__b*(c__6*(V - V_rest) + c__6*(V - c__1) - c__3*(V - V_rest) - c__3*(V - c__1) + real(2.0)*c__3*(V_peak - V));
+  s__b += dI_dV__b*(c__2);
+  V__b += I__b*(c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) - c__3*(V - V_rest)*(V - c__1));
+  s__b += I__b*(c__2*(V - V_rest));
+  lam_prev__[0] = lam__[0] + V__b;
+  lam_prev__[1] = lam__[1] + s__b;
+}
+
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,

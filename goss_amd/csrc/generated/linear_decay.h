@@ -57,6 +57,36 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   if (only_linear) return;
 }
 
+// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
+//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
+// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h.
+// Reverse accumulation over the linearized_eval statements.
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real y = states[0];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  const real rhs__0 = b + a*y;
+  const real lin__0 = a;
+  const bool ex__0 = m_fabs(lin__0) > delta__;
+  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(lin__0*h__) : lam__[0]*h__;
+  const real abar__0 = ex__0 ? lam__[0]*rhs__0*(h__*h__)*m_dphi1(lin__0*h__) : real(0.0);
+  real a__b = real(0.0);
+  real b__b = real(0.0);
+  real y__b = real(0.0);
+  a__b += abar__0*(real(1.0));
+  a__b += bbar__0*(y);
+  b__b += bbar__0*(real(1.0));
+  y__b += bbar__0*(a);
+  lam_prev__[0] = lam__[0] + y__b;
+  fp_bar__[0] += a__b;
+  fp_bar__[1] += b__b;
+}
+
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,

@@ -69,6 +69,53 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   if (only_linear) return;
 }
 
+// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
+//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
+// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h.
+// Reverse accumulation over the linearized_eval statements.
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real x = states[0];
+  const real y = states[1];
+  const real z = states[2];
+  const real beta = pa.template get<0>();
+  const real rho = pa.template get<1>();
+  const real sigma = pa.template get<2>();
+  const real rhs__0 = sigma*(y - x);
+  const real rhs__1 = -y + x*(rho - z);
+  const real rhs__2 = x*y - beta*z;
+  const real lin__0 = -sigma;
+  const real lin__1 = -real(1.0);
+  const real lin__2 = -beta;
+  const bool ex__0 = m_fabs(lin__0) > delta__;
+  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(lin__0*h__) : lam__[0]*h__;
+  const real abar__0 = ex__0 ? lam__[0]*rhs__0*(h__*h__)*m_dphi1(lin__0*h__) : real(0.0);
+  const bool ex__1 = m_fabs(lin__1) > delta__;
+  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(lin__1*h__) : lam__[1]*h__;
+  const real abar__1 = ex__1 ? lam__[1]*rhs__1*(h__*h__)*m_dphi1(lin__1*h__) : real(0.0);
+  const bool ex__2 = m_fabs(lin__2) > delta__;
+  const real bbar__2 = ex__2 ? lam__[2]*h__*m_phi1(lin__2*h__) : lam__[2]*h__;
+  const real abar__2 = ex__2 ? lam__[2]*rhs__2*(h__*h__)*m_dphi1(lin__2*h__) : real(0.0);
+  real x__b = real(0.0);
+  real y__b = real(0.0);
+  real z__b = real(0.0);
+  x__b += bbar__2*(y);
+  y__b += bbar__2*(x);
+  z__b += bbar__2*(-beta);
+  x__b += bbar__1*(rho - z);
+  y__b += bbar__1*(-real(1.0));
+  z__b += bbar__1*(-x);
+  x__b += bbar__0*(-sigma);
+  y__b += bbar__0*(sigma);
+  lam_prev__[0] = lam__[0] + x__b;
+  lam_prev__[1] = lam__[1] + y__b;
+  lam_prev__[2] = lam__[2] + z__b;
+}
+
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,

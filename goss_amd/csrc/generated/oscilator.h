@@ -62,6 +62,44 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   if (only_linear) return;
 }
 
+// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
+//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
+// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h.
+// Reverse accumulation over the linearized_eval statements.
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real x = states[0];
+  const real y = states[1];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  const real rhs__0 = -a*y;
+  const real rhs__1 = b*x;
+  const real lin__0 = real(0.0);
+  const real lin__1 = real(0.0);
+  const bool ex__0 = m_fabs(lin__0) > delta__;
+  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(lin__0*h__) : lam__[0]*h__;
+  const real abar__0 = ex__0 ? lam__[0]*rhs__0*(h__*h__)*m_dphi1(lin__0*h__) : real(0.0);
+  const bool ex__1 = m_fabs(lin__1) > delta__;
+  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(lin__1*h__) : lam__[1]*h__;
+  const real abar__1 = ex__1 ? lam__[1]*rhs__1*(h__*h__)*m_dphi1(lin__1*h__) : real(0.0);
+  real a__b = real(0.0);
+  real b__b = real(0.0);
+  real x__b = real(0.0);
+  real y__b = real(0.0);
+  b__b += bbar__1*(x);
+  x__b += bbar__1*(b);
+  a__b += bbar__0*(-y);
+  y__b += bbar__0*(-a);
+  lam_prev__[0] = lam__[0] + x__b;
+  lam_prev__[1] = lam__[1] + y__b;
+  fp_bar__[0] += a__b;
+  fp_bar__[1] += b__b;
+}
+
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,

@@ -578,6 +578,113 @@ int goss_hip_system_forward_times(goss_hip_system* s, const double* t, uint64_t 
   });
 }
 
+namespace {
+
+// Validates an adjoint request and returns the number of checkpoints and samples.
+void adjoint_check(goss_hip_system* s, uint64_t n, uint32_t sample_every, uint32_t checkpoint_every,
+                   uint64_t* n_ckpt, uint64_t* n_samples)
+{
+  if (s->solver != kGRL1) throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint is available for GRL1 systems only");
+  if (s->prec != kFP64 && s->prec != kFP32)
+    throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint runs in FP64 or FP32 (not FP32C)");
+  if (s->sp.internal_time_step > 0)
+    throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint takes one GRL1 step per interval (internal_time_step <= 0)");
+  if (s->v->NFS < 1) throw Error(GOSS_HIP_EINVAL, "the model has no field state to sample");
+  if (n < 2 || n - 1 > uint64_t(INT32_MAX)) throw Error(GOSS_HIP_EINVAL, "need 2 .. 2^31 time points");
+  if (sample_every == 0 || checkpoint_every == 0) throw Error(GOSS_HIP_EINVAL, "intervals must be positive");
+  const uint64_t steps = n - 1;
+  *n_ckpt = (steps + checkpoint_every - 1) / checkpoint_every;
+  *n_samples = steps / sample_every;
+}
+
+std::vector<double2> make_steps(goss_hip_system* s, const double* t, uint64_t n)
+{
+  std::vector<double2> st(n - 1);
+  for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
+  s->check_steps(st.data(), st.size());
+  return st;
+}
+
+}  // namespace
+
+int goss_hip_adjoint_buffer_sizes(goss_hip_system* s, uint64_t n, uint32_t sample_every, uint32_t checkpoint_every,
+                                  uint64_t* checkpoint_bytes, uint64_t* segment_bytes, uint64_t* num_samples)
+{
+  return guarded([&] {
+    check_sys(s);
+    uint64_t nc = 0, ns = 0;
+    adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
+    const uint64_t per = uint64_t(s->v->S) * s->ld * s->elem();
+    *checkpoint_bytes = nc * per;
+    *segment_bytes = uint64_t(checkpoint_every) * per;
+    *num_samples = ns;
+  });
+}
+
+int goss_hip_adjoint_record(goss_hip_system* s, const double* t, uint64_t n, uint32_t sample_every,
+                            uint32_t checkpoint_every, void* checkpoints, double* samples)
+{
+  return guarded([&] {
+    check_sys(s);
+    uint64_t nc = 0, ns = 0;
+    adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
+    s->use_device();
+    const AdjointLaunch launch = s->v->adjoint_record[s->prec];
+    if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "adjoint not compiled for this model");
+    const std::vector<double2> st = make_steps(s, t, n);
+    const double2* d = s->upload_steps(st.data(), st.size());
+    for (size_t off = 0; off < st.size(); off += s->steps_per_launch) {
+      AdjointArgs a{};
+      a.y = s->y;
+      a.fp = s->fp;
+      a.steps = d + off;
+      a.step0 = int(off);
+      a.nouter = int(std::min<size_t>(s->steps_per_launch, st.size() - off));
+      a.shared = s->shared.data();
+      a.delta = s->sp.delta;
+      a.N = s->N;
+      a.ld = s->ld;
+      a.shape = AdjointShape{int(st.size()), int(checkpoint_every), int(sample_every), s->v->fsi[0]};
+      a.ckpt = checkpoints;
+      a.samples = samples;
+      a.stream = s->stream;
+      hip_check(launch(a), "kernel launch");
+    }
+  });
+}
+
+int goss_hip_adjoint_backward(goss_hip_system* s, const double* t, uint64_t n, uint32_t sample_every,
+                              uint32_t checkpoint_every, const void* checkpoints, const double* grad_samples,
+                              void* segment, double* grad_states, double* grad_field_parameters)
+{
+  return guarded([&] {
+    check_sys(s);
+    uint64_t nc = 0, ns = 0;
+    adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
+    s->use_device();
+    const AdjointLaunch launch = s->v->adjoint_backward[s->prec];
+    if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "adjoint not compiled for this model");
+    const std::vector<double2> st = make_steps(s, t, n);
+    const double2* d = s->upload_steps(st.data(), st.size());
+    AdjointArgs a{};
+    a.fp = s->fp;
+    a.steps = d;
+    a.nouter = int(st.size());
+    a.shared = s->shared.data();
+    a.delta = s->sp.delta;
+    a.N = s->N;
+    a.ld = s->ld;
+    a.shape = AdjointShape{int(st.size()), int(checkpoint_every), int(sample_every), s->v->fsi[0]};
+    a.ckpt = const_cast<void*>(checkpoints);
+    a.grad_samples = grad_samples;
+    a.seg = segment;
+    a.grad_y0 = grad_states;
+    a.grad_fp = grad_field_parameters;
+    a.stream = s->stream;
+    hip_check(launch(a), "kernel launch");
+  });
+}
+
 int goss_hip_system_solve(goss_hip_system* s, double* field_states, const double* t, uint64_t n, int tangled)
 {
   return guarded([&] {

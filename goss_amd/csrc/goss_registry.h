@@ -10,6 +10,7 @@
 #include "goss_kernels.h"
 #include "goss_implicit.h"
 #include "goss_adaptive.h"
+#include "goss_adjoint.h"
 
 namespace goss_hip {
 
@@ -59,6 +60,28 @@ struct AdaptiveArgs {
 
 typedef hipError_t (*AdaptiveLaunch)(const AdaptiveArgs&);
 
+// Adjoint of GRL1 stepping: record (forward with checkpoints and samples) and
+// backward (the reverse sweep).  Precision FP64 or FP32.
+struct AdjointArgs {
+  void* y;                 // states [S][ld] (record: advanced in place)
+  const void* fp;          // field parameters [NFP][ld]
+  const double2* steps;    // record: this launch's steps; backward: all steps
+  int step0, nouter;       // record: global index of steps[0], steps in this launch
+  const double* shared;
+  double delta;
+  unsigned N, ld;
+  AdjointShape shape;
+  void* ckpt;              // [n_ckpt][S][ld]
+  double* samples;         // record: [n_samples][N]
+  const double* grad_samples;  // backward: [n_samples][N]
+  void* seg;               // backward: [ckpt_every][S][ld]
+  double* grad_y0;         // backward: [N][S]
+  double* grad_fp;         // backward: [N][NFP]
+  hipStream_t stream;
+};
+
+typedef hipError_t (*AdjointLaunch)(const AdjointArgs&);
+
 struct VariantInfo {
   const char* name;
   int S, P, NFS, NFP, NCOMP;
@@ -74,6 +97,8 @@ struct VariantInfo {
   ExplicitLaunch explicit_launch[kNumSolvers][kNumPrecisions];
   ImplicitLaunch implicit_launch[2];  // [0] ImplicitEuler, [1] ESDIRK23a (FP64)
   AdaptiveLaunch rkf32_launch[kNumPrecisions];  // FP64 and FP32
+  AdjointLaunch adjoint_record[kNumPrecisions];    // FP64 and FP32
+  AdjointLaunch adjoint_backward[kNumPrecisions];
 };
 
 // All variants, in registration order.

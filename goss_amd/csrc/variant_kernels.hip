@@ -39,6 +39,12 @@ struct M {
   {
     NS::linearized_eval<R>(y, t, a, lin, rhs, ol);
   }
+  template <typename R, typename A, typename T>
+  GOSS_HD GOSS_INLINE static void grl1_vjp(const R* y, const T t, const A& a, const R h, const R delta, const R* lam,
+                                           R* lam_prev, R* fp_bar)
+  {
+    NS::grl1_vjp<R>(y, t, a, h, delta, lam, lam_prev, fp_bar);
+  }
 };
 
 // Host-side accessor for compute_constants (all parameters in FP64).
@@ -95,6 +101,28 @@ hipError_t launch_rkf32(const AdaptiveArgs& a)
   return hipGetLastError();
 }
 
+template <class R>
+hipError_t launch_adjoint_record(const AdjointArgs& a)
+{
+  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
+  const dim3 grid((a.N + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((adjoint_record_kernel<M, R>), grid, dim3(kBlock), 0, a.stream, static_cast<R*>(a.y),
+                     static_cast<const R*>(a.fp), a.steps, a.step0, a.nouter, sp, R(a.delta), a.N, a.ld, a.shape,
+                     static_cast<R*>(a.ckpt), a.samples);
+  return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_adjoint_backward(const AdjointArgs& a)
+{
+  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
+  const dim3 grid((a.N + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((adjoint_backward_kernel<M, R>), grid, dim3(kBlock), 0, a.stream,
+                     static_cast<const R*>(a.ckpt), static_cast<const R*>(a.fp), a.steps, sp, R(a.delta), a.N, a.ld,
+                     a.shape, a.grad_samples, static_cast<R*>(a.seg), a.grad_y0, a.grad_fp);
+  return hipGetLastError();
+}
+
 template <class R, bool COMP>
 void register_family(VariantInfo& v, int prec)
 {
@@ -136,6 +164,8 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   v.implicit_launch[0] = &launch_implicit<kImplicitEuler>;
   v.implicit_launch[1] = &launch_implicit<kESDIRK23a>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
+  v.adjoint_record[kFP64] = &launch_adjoint_record<double>;
+  v.adjoint_backward[kFP64] = &launch_adjoint_backward<double>;
 }
 #elif defined(GOSS_PREC_F32)
 void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
@@ -144,6 +174,8 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
   register_family<float, false>(v, kFP32);
   register_family<float, true>(v, kFP32C);
   v.rkf32_launch[kFP32] = &launch_rkf32<float>;
+  v.adjoint_record[kFP32] = &launch_adjoint_record<float>;
+  v.adjoint_backward[kFP32] = &launch_adjoint_backward<float>;
 }
 #else
 #error "define GOSS_PREC_F64 or GOSS_PREC_F32"

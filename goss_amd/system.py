@@ -169,6 +169,27 @@ class DeviceSystem:
         check(lib().goss_hip_system_get_solver_counters(self._h, out))
         return np.array(list(out))
 
+    # -- adjoint of GRL1 stepping (device buffers passed as integer addresses) --
+    def adjoint_buffer_sizes(self, n: int, sample_every: int, checkpoint_every: int):
+        """(checkpoint bytes, segment bytes, number of samples) for n time points."""
+        c, g, m = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().goss_hip_adjoint_buffer_sizes(self._h, int(n), int(sample_every), int(checkpoint_every),
+                                                  ctypes.byref(c), ctypes.byref(g), ctypes.byref(m)))
+        return c.value, g.value, m.value
+
+    def adjoint_record(self, t, sample_every: int, checkpoint_every: int, ckpt_ptr: int, samples_ptr: int) -> None:
+        t = _f64(t)
+        check(lib().goss_hip_adjoint_record(self._h, _ptr(t), len(t), int(sample_every), int(checkpoint_every),
+                                            ctypes.c_void_p(ckpt_ptr), ctypes.c_void_p(samples_ptr)))
+
+    def adjoint_backward(self, t, sample_every: int, checkpoint_every: int, ckpt_ptr: int, grad_samples_ptr: int,
+                         segment_ptr: int, grad_states_ptr: int, grad_fp_ptr: int) -> None:
+        t = _f64(t)
+        check(lib().goss_hip_adjoint_backward(self._h, _ptr(t), len(t), int(sample_every), int(checkpoint_every),
+                                              ctypes.c_void_p(ckpt_ptr), ctypes.c_void_p(grad_samples_ptr),
+                                              ctypes.c_void_p(segment_ptr), ctypes.c_void_p(grad_states_ptr),
+                                              ctypes.c_void_p(grad_fp_ptr)))
+
     def stream(self) -> int:
         v = ctypes.c_void_p()
         check(lib().goss_hip_system_stream(self._h, ctypes.byref(v)))

@@ -176,6 +176,30 @@ int goss_hip_system_set_steps_per_launch(goss_hip_system* sys, uint32_t steps);
  * RKF32 (RKF32.h:70-72): out[0] 0, out[1] nfevals, out[2] ndtsr, out[3] ndtsa.
  * Zeros for the fixed-step solvers. */
 int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
+/* Discrete adjoint of GRL1 stepping.  No counterpart in the reference: it
+ * serves gradient-based calibration of per-node parameters from the Python
+ * side (a torch.autograd.Function over these calls).  Requirements: solver
+ * GRL1, precision FP64 or FP32, internal_time_step <= 0 (one step per
+ * interval).  The times are those of goss_hip_system_forward_times: n points,
+ * n-1 steps.  The first field state is sampled after every `sample_every`
+ * steps.
+ *
+ * goss_hip_adjoint_buffer_sizes: bytes of the caller-allocated device buffers
+ * (checkpoints: one state copy every `checkpoint_every` steps; segment: the
+ * backward sweep's recomputed states) and the number of samples.
+ * goss_hip_adjoint_record: advances the system like forward_times, writing the
+ * checkpoints and samples[n_samples][N] (device, FP64).
+ * goss_hip_adjoint_backward: given d loss / d samples (device, [n_samples][N]),
+ * writes d loss / d initial states (device, [N][num_states]) and d loss / d
+ * field parameters (device, [N][num_field_parameters]) for the run recorded
+ * into `checkpoints` with the same times and field parameters. */
+int goss_hip_adjoint_buffer_sizes(goss_hip_system* sys, uint64_t n, uint32_t sample_every, uint32_t checkpoint_every,
+                                  uint64_t* checkpoint_bytes, uint64_t* segment_bytes, uint64_t* num_samples);
+int goss_hip_adjoint_record(goss_hip_system* sys, const double* t, uint64_t n, uint32_t sample_every,
+                            uint32_t checkpoint_every, void* checkpoints_dev, double* samples_dev);
+int goss_hip_adjoint_backward(goss_hip_system* sys, const double* t, uint64_t n, uint32_t sample_every,
+                              uint32_t checkpoint_every, const void* checkpoints_dev, const double* grad_samples_dev,
+                              void* segment_dev, double* grad_states_dev, double* grad_field_parameters_dev);
 /* Wait for all work of the system. */
 int goss_hip_system_synchronize(goss_hip_system* sys);
 /* The system's HIP stream (hipStream_t) for callers that order their own work. */

@@ -55,6 +55,9 @@ struct VariantOps {
   void (*eval)(const double*, double, const double*, const double*, double*);
   void (*linearized)(const double*, double, const double*, const double*, double*, double*, bool);
   void (*jacobian)(const double*, double, const double*, const double*, double*);
+  // Generated discrete adjoint of one GRL1 step (test infrastructure for the device adjoint).
+  void (*grl1_vjp)(const double*, double, const double*, const double*, double, double, const double*, double*,
+                   double*);
 };
 
 #define GOSS_ORACLE_VARIANT(V)                                                                         \
@@ -72,6 +75,10 @@ struct VariantOps {
                 bool ol) { goss_models::V::linearized_eval<double>(y, t, Acc{p, c}, lin, rhs, ol); },   \
              [](const double* y, double t, const double* p, const double* c, double* jac) {            \
                goss_models::V::compute_jacobian<double>(y, t, Acc{p, c}, jac);                         \
+             },                                                                                        \
+             [](const double* y, double t, const double* p, const double* c, double h, double delta,    \
+                const double* lam, double* lam_prev, double* fp_bar) {                                 \
+               goss_models::V::grl1_vjp<double>(y, t, Acc{p, c}, h, delta, lam, lam_prev, fp_bar);      \
              }}
 
 const VariantOps kVariants[] = {
@@ -901,6 +908,15 @@ void oracle_model_fd_jacobian(int v, double* y, double t, const double* p, doubl
   double c[kMaxConstants];
   kVariants[v].constants(p, c);
   fd_jacobian(kVariants[v], y, t, p, c, jac, f1.data(), f2.data());
+}
+// lam_prev = lam + (d y'/d y)^T lam and fp_bar += (d y'/d p_field)^T lam for one GRL1
+// step y -> y' of size h at time t (generated grl1_vjp).
+void oracle_model_grl1_vjp(int v, const double* y, double t, const double* p, double h, double delta,
+                           const double* lam, double* lam_prev, double* fp_bar)
+{
+  double c[kMaxConstants];
+  kVariants[v].constants(p, c);
+  kVariants[v].grl1_vjp(y, t, p, c, h, delta, lam, lam_prev, fp_bar);
 }
 void oracle_lu_factorize(int n, double* mat) { lu_factorize(n, mat); }
 void oracle_forward_backward_subst(int n, const double* mat, const double* b, double* x) { fb_subst(n, mat, b, x); }

@@ -1,0 +1,180 @@
+"""Discrete adjoint of GRL1 on the device (goss_amd.adjoint, goss_adjoint.h).
+
+References:
+* plain-PyTorch FP64 autograd through the same GRL1 scheme (tests/torch_reference.py,
+  run on the CPU): the FP64 device gradients must agree to 1e-8 relative per entry
+  (floor: 1e-9 of the entry's column maximum);
+* central finite differences of the device FP64 loss;
+* the FP64 device adjoint itself for FP32 (stated bound) and for other checkpoint
+  intervals (bit-identical: the backward sweep recomputes the forward states
+  with the same code).
+"""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_explicit import dev_system
+
+pytestmark = pytest.mark.gpu
+
+VARIANT = "tp06_m_fp6"
+
+
+def _inputs(N, seed, spread=0.01):
+    from goss_amd.system import ModelVariant
+
+    mv = ModelVariant(VARIANT)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y0 = mv.initial_conditions[None, :] * (1.0 + rng.uniform(-spread, spread, (N, mv.num_states)))
+    g0 = np.array([mv.default_parameters[mv.parameter_names.index(p)] for p in mv.field_parameter_names])
+    fp = g0[None, :] * np.exp(rng.normal(0.0, 0.1, (N, g0.size)))
+    return y0, fp
+
+
+def _times(nsteps, dt=0.02, t0=9.0):
+    # t0 = 9 ms: the run crosses the stimulus (10-11 ms) and the upstroke.
+    return t0 + np.arange(nsteps + 1) * dt
+
+
+def _device_grads(prec, y0, fp, t, sample_every, ckpt_every, weights):
+    from goss_amd.adjoint import voltage_trace
+
+    sys_ = dev_system(VARIANT, "GRL1", y0.shape[0], prec)
+    yt = torch.tensor(y0, requires_grad=True)
+    ft = torch.tensor(fp, requires_grad=True)
+    v = voltage_trace(sys_, yt, ft, t, sample_every, ckpt_every)
+    loss = (v * torch.as_tensor(weights, device=v.device)).sum()
+    loss.backward()
+    return float(loss), v.detach().cpu().numpy(), yt.grad.numpy(), ft.grad.numpy()
+
+
+def _rel(a, b):
+    floor = 1e-9 * np.max(np.abs(b), axis=0, keepdims=True)
+    return float((np.abs(a - b) / np.maximum(np.abs(b), np.maximum(floor, 1e-300))).max())
+
+
+def test_adjoint_fp64_matches_torch_autograd():
+    from torch_reference import TorchModel, sampled_trace
+
+    N, nsteps, se, ce = 6, 300, 25, 40
+    y0, fp = _inputs(N, 61)
+    t = _times(nsteps)
+    rng = np.random.Generator(np.random.PCG64(62))
+    w = rng.normal(size=(nsteps // se, N))
+    loss, v, gy, gp = _device_grads("fp64", y0, fp, t, se, ce, w)
+
+    tm = TorchModel(VARIANT)
+    yt = torch.tensor(y0, requires_grad=True)
+    ft = torch.tensor(fp, requires_grad=True)
+    vr = sampled_trace(tm, yt, ft, t, se, 17)
+    (vr * torch.tensor(w)).sum().backward()
+    print(f"V range {v.min():.2f} .. {v.max():.2f} mV; samples rel err {_rel(v, vr.detach().numpy()):.2e}; "
+          f"grad y0 {_rel(gy, yt.grad.numpy()):.2e}; grad g {_rel(gp, ft.grad.numpy()):.2e}")
+    assert v.max() > 0.0  # the run includes the upstroke
+    assert _rel(v, vr.detach().numpy()) <= 1e-10
+    assert _rel(gy, yt.grad.numpy()) <= 1e-8
+    assert _rel(gp, ft.grad.numpy()) <= 1e-8
+
+
+def test_adjoint_fp64_matches_central_differences():
+    from goss_amd.adjoint import voltage_trace
+
+    N, nsteps, se, ce = 2, 200, 20, 50
+    y0, fp = _inputs(N, 63)
+    t = _times(nsteps)
+    rng = np.random.Generator(np.random.PCG64(64))
+    w = rng.normal(size=(nsteps // se, N))
+    _, _, gy, gp = _device_grads("fp64", y0, fp, t, se, ce, w)
+    sys_ = dev_system(VARIANT, "GRL1", N, "fp64")
+
+    def loss(y, f):
+        with torch.no_grad():
+            v = voltage_trace(sys_, torch.tensor(y), torch.tensor(f), t, se, ce)
+        return (v.cpu().numpy() * w).sum(axis=0)  # per node (nodes are independent)
+
+    for j in range(fp.shape[1]):
+        e = 1e-5 * fp[:, j]
+        fpp, fpm = fp.copy(), fp.copy()
+        fpp[:, j] += e
+        fpm[:, j] -= e
+        fd = (loss(y0, fpp) - loss(y0, fpm)) / (2 * e)
+        np.testing.assert_allclose(gp[:, j], fd, rtol=2e-4, atol=1e-9 * np.abs(gp).max())
+    # V, Ca_i, h.  (Fast gates such as m forget their initial value within a few steps:
+    # d loss / d m0 is at the rounding level of the loss and has no usable difference quotient.)
+    for i in (17, 12, 4):
+        e = 1e-4 * np.abs(y0[:, i])
+        yp, ym = y0.copy(), y0.copy()
+        yp[:, i] += e
+        ym[:, i] -= e
+        fd = (loss(yp, fp) - loss(ym, fp)) / (2 * e)
+        np.testing.assert_allclose(gy[:, i], fd, rtol=2e-4, atol=1e-7 * np.abs(gy[:, i]).max())
+
+
+def test_checkpoint_interval_does_not_change_gradients():
+    N, nsteps, se = 64, 240, 30
+    y0, fp = _inputs(N, 65)
+    t = _times(nsteps)
+    w = np.random.Generator(np.random.PCG64(66)).normal(size=(nsteps // se, N))
+    ref = _device_grads("fp64", y0, fp, t, se, 240, w)
+    for ce in (1, 37, 1000):
+        got = _device_grads("fp64", y0, fp, t, se, ce, w)
+        assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3]), ce
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_record_samples_equal_forward(prec):
+    from goss_amd.adjoint import voltage_trace
+
+    N, nsteps, se = 100, 150, 10
+    y0, fp = _inputs(N, 67)
+    t = _times(nsteps)
+    a = dev_system(VARIANT, "GRL1", N, prec)
+    with torch.no_grad():
+        v = voltage_trace(a, torch.tensor(y0), torch.tensor(fp), t, se, 50).cpu().numpy()
+    b = dev_system(VARIANT, "GRL1", N, prec)
+    b.set_states(y0)
+    b.set_field_parameters(fp)
+    rows = np.zeros((len(t), N, 1))
+    b.solve(t, rows)
+    assert np.array_equal(v, rows[se::se, :, 0])
+    assert np.array_equal(a.states(), b.states())
+
+
+def test_adjoint_fp32_close_to_fp64():
+    # Config-5 shape on a subset: 20 ms through the stimulus and upstroke,
+    # MSE of V sampled every 1 ms against a target from conductances x lognormal(0, 0.1).
+    N, nsteps, se, ce = 256, 1000, 50, 200
+    y0, fp = _inputs(N, 68)
+    t = _times(nsteps)
+    rng = np.random.Generator(np.random.PCG64(69))
+    target_fp = fp * np.exp(rng.normal(0.0, 0.1, fp.shape))
+    from goss_amd.adjoint import voltage_trace
+
+    with torch.no_grad():
+        tgt = voltage_trace(dev_system(VARIANT, "GRL1", N, "fp64"), torch.tensor(y0), torch.tensor(target_fp), t,
+                            se, ce).cpu().numpy()
+    grads = {}
+    for prec in ("fp64", "fp32"):
+        sys_ = dev_system(VARIANT, "GRL1", N, prec)
+        yt = torch.tensor(y0, requires_grad=True)
+        ft = torch.tensor(fp, requires_grad=True)
+        v = voltage_trace(sys_, yt, ft, t, se, ce)
+        loss = ((v - torch.as_tensor(tgt, device=v.device)) ** 2).mean()
+        loss.backward()
+        grads[prec] = (yt.grad.numpy(), ft.grad.numpy())
+    for k, name in ((0, "y0"), (1, "conductances")):
+        a, b = grads["fp32"][k], grads["fp64"][k]
+        rel_l2 = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
+        print(f"FP32 vs FP64 adjoint, d loss / d {name}: relative L2 error per column, max {rel_l2.max():.2e}")
+        assert np.all(np.isfinite(a))
+        # States whose gradient is exactly zero in FP64 (none influence V here) are skipped.
+        cols = np.linalg.norm(b, axis=0) > 0
+        assert rel_l2[cols].max() <= 5e-3  # measured 3.9e-4
+
+
+def test_adjoint_requires_grl1():
+    from goss_amd._lib import GossHipError
+
+    s = dev_system(VARIANT, "RK4", 4, "fp64")
+    with pytest.raises(GossHipError, match="GRL1"):
+        s.adjoint_buffer_sizes(11, 5, 5)
