@@ -175,6 +175,66 @@ def oracle_parity(y0, fp, t_all, gpu_final, nodes=64):
     return float((np.abs(a - b) / np.maximum(np.abs(b), floor)).max())
 
 
+def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every=200, seed=2025):
+    """Adjoint workload: TP06 M-cell GRL1 in FP32 with six per-node conductances
+    (g_Kr, g_Ks, g_CaL, g_Na, g_K1, g_to) and perturbed initial states; loss =
+    MSE of V sampled every 1 ms against a target trace from the conductances x
+    lognormal(0, 0.1); gradients with respect to the conductances and the 19
+    initial states, states checkpointed every 200 steps.  Times the forward
+    (record) and the backward (recompute + reverse sweep) kernels with HIP
+    events, after a short warm-up of both."""
+    import torch
+
+    from goss_amd.adjoint import voltage_trace
+    from goss_amd.system import DeviceSystem, ModelVariant
+
+    variant = "tp06_m_fp6"
+    mv = ModelVariant(variant)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y0 = mv.initial_conditions[None, :] * (1.0 + rng.uniform(-0.01, 0.01, (N, mv.num_states)))
+    g0 = np.array([mv.default_parameters[mv.parameter_names.index(p)] for p in mv.field_parameter_names])
+    fp = g0[None, :] * np.exp(rng.normal(0.0, 0.2, (N, g0.size)))
+    target_fp = fp * np.exp(rng.normal(0.0, 0.1, fp.shape))
+    t = np.arange(steps + 1) * dt
+    sys_ = DeviceSystem(variant, "GRL1", N, "fp32")
+    sys_.set_steps_per_launch(250)
+    with torch.no_grad():
+        target = voltage_trace(sys_, torch.tensor(y0), torch.tensor(target_fp), t, sample_every, checkpoint_every)
+    # warm-up of both kernels on a short run
+    yw = torch.tensor(y0, requires_grad=True)
+    fw = torch.tensor(fp, requires_grad=True)
+    vw = voltage_trace(sys_, yw, fw, t[:401], sample_every, checkpoint_every)
+    (vw ** 2).mean().backward()
+    torch.cuda.synchronize()
+
+    ev = HipEvents()
+    stream = sys_.stream()
+    marks = [ev.create() for _ in range(4)]
+    yt = torch.tensor(y0, requires_grad=True)
+    ft = torch.tensor(fp, requires_grad=True)
+    ev.record(marks[0], stream)
+    v = voltage_trace(sys_, yt, ft, t, sample_every, checkpoint_every)
+    ev.record(marks[1], stream)
+    loss = ((v - target) ** 2).mean()
+    torch.cuda.synchronize()
+    ev.record(marks[2], stream)
+    loss.backward()
+    ev.record(marks[3], stream)
+    sys_.synchronize()
+    fwd_ms = ev.elapsed_ms(marks[0], marks[1])
+    bwd_ms = ev.elapsed_ms(marks[2], marks[3])
+    gy, gf = yt.grad.numpy(), ft.grad.numpy()
+    return {
+        "workload": f"TP06 M-cell GRL1 FP32, {N} nodes x {steps} steps (dt {dt} ms), V sampled every "
+                    f"{sample_every} steps, checkpoints every {checkpoint_every} steps, grads wrt 6 conductances "
+                    f"+ 19 initial states",
+        "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adjoint_over_forward": bwd_ms / fwd_ms,
+        "forward_node_steps_per_s": N * steps / (fwd_ms * 1e-3),
+        "loss": float(loss), "grads_finite": bool(np.all(np.isfinite(gy)) and np.all(np.isfinite(gf))),
+        "checkpoint_bytes": sys_.adjoint_buffer_sizes(len(t), sample_every, checkpoint_every)[0],
+    }
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "grl1_tp06_fp32c_1M_traffic.json")
     if os.path.exists(p):
@@ -194,6 +254,7 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, default=250)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-checks", action="store_true", help="skip accuracy/parity passes")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one device-to-device copy of 256 MiB (read 256 MiB + write 256 MiB), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -298,6 +359,11 @@ def main():
         "adjoint_over_forward": None,
         "roofline": roofline,
     }
+    if not args.no_adjoint:
+        adj = adjoint_run()
+        out["adjoint_over_forward"] = adj["adjoint_over_forward"]
+        out["adjoint"] = adj
+        print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, y0, fp, dt)
         print(f"[bench] cpu {out['cpu_baseline']}", file=sys.stderr, flush=True)

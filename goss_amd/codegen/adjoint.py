@@ -28,6 +28,36 @@ def adj_name(name: str) -> str:
     return f"{name}__b"
 
 
+def _is_call(e: sp.Expr) -> bool:
+    return isinstance(e, (sp.exp, sp.log)) or (isinstance(e, sp.Pow) and not e.exp.is_Integer)
+
+
+def lift_calls(stmts: Sequence[Tuple[str, sp.Expr]], prefix: str = "tr__") -> List[Tuple[str, sp.Expr]]:
+    """Give every exp, log and non-integer power its own statement.
+
+    The reverse sweep differentiates statement by statement; with the
+    transcendental calls named, a derivative reuses the call's value
+    (d exp(u)/du is the statement itself, d x^y/dx = y*n/x) instead of
+    evaluating the call again inside every partial.  Identical calls share
+    one statement."""
+    seen: Dict[sp.Expr, sp.Symbol] = {}
+    out: List[Tuple[str, sp.Expr]] = []
+    for n, e in stmts:
+        calls = [c for c in sp.preorder_traversal(e) if isinstance(c, sp.Expr) and _is_call(c)]
+        calls = sorted(set(calls), key=sp.count_ops)  # inner calls first
+        mapping: Dict[sp.Expr, sp.Symbol] = {}
+        for c in calls:
+            lifted = c.xreplace(mapping)
+            sym = seen.get(lifted)
+            if sym is None:
+                sym = sp.Symbol(f"{prefix}{len(seen)}", real=True)
+                seen[lifted] = sym
+                out.append((sym.name, lifted))
+            mapping[c] = sym
+        out.append((n, e.xreplace(mapping)))
+    return out
+
+
 def reverse_sweep(stmts: Sequence[Tuple[str, sp.Expr]], seeds: Dict[str, str],
                   independent: Iterable[str]) -> Tuple[List[Tuple[str, str, sp.Expr]], Set[str]]:
     """Reverse accumulation over ``stmts`` (evaluation order).

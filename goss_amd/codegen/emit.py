@@ -25,7 +25,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import sympy as sp
 
-from .adjoint import reverse_sweep
+from .adjoint import lift_calls, reverse_sweep
 from .analysis import Jacobian, Linearized, census_body, census_state_ops, jacobian, linearize, needed_names
 from .hoist import ConstantTable, hoist_statements
 from .ode_model import ODEModel, TIME
@@ -300,7 +300,7 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
 
     # -- grl1_vjp: discrete adjoint of one GRL1 step --------------------------
     out_stmts = [(f"rhs__{i}", rhs_out[i]) for i in range(S)] + [(f"lin__{i}", hlinear[i]) for i in range(S)]
-    vjp_fwd = lin_inter + hlin_first + hlin_second + out_stmts
+    vjp_fwd = lift_calls(lin_inter + hlin_first + hlin_second + out_stmts)
     seeds = {f"rhs__{i}": f"bbar__{i}" for i in range(S)}
     seeds.update({f"lin__{i}": f"abar__{i}" for i in range(S)})
     independent = list(model.state_names) + list(var.field_parameters)
@@ -337,6 +337,56 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         a = f"{fpn}__b"
         if a in targets:
             L.append(f"{ind}fp_bar__[{j}] += {a};")
+    L.append("}")
+    L.append("")
+
+    # -- grl1_vjp_tape: the same product with every intermediate and adjoint in
+    # a caller-provided tape (tape.get/put/add of slot k); only the states,
+    # parameters and seeds stay in registers.
+    fwd_names = [n for n, _ in vjp_fwd]
+    slot = {n: k for k, n in enumerate(fwd_names)}
+    adj_list = sorted(targets)
+    aslot = {a: len(fwd_names) + k for k, a in enumerate(adj_list)}
+    n_slots = len(fwd_names) + len(adj_list)
+    tape_map = {n: f"tape.get({k})" for n, k in slot.items()}
+    tp = CPrinter(tape_map)
+    L.append(f"constexpr int grl1_vjp_tape_slots = {n_slots};")
+    L.append("// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):")
+    L.append("// for targets whose registers cannot hold the whole reverse sweep.")
+    L.append("template <typename real, typename PA, typename treal, typename TP>")
+    L.append("GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,")
+    L.append("                                       const real h__, const real delta__, const real* __restrict__ lam__,")
+    L.append("                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,")
+    L.append("                                       const TP& tape)")
+    L.append("{")
+    L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;")
+    L += g_vjp.bind(vused | {s_.name for s_ in model.states}, ind)
+    # Time-only statements stay named (in treal) and are also stored; every
+    # other intermediate is computed from tape reads and only stored.
+    fwd_pr = CPrinter({"time": "real(time)", **tape_map})
+    for n, e in vjp_fwd:
+        if n in g_vjp.time_only:
+            L.append(f"{ind}const treal {c_name(n)} = {g_vjp.time.p(e)};")
+            L.append(f"{ind}tape.put({slot[n]}, real({c_name(n)}));")
+        else:
+            L.append(f"{ind}tape.put({slot[n]}, {fwd_pr.p(e)});")
+    for i in range(S):
+        L.append(f"{ind}const bool ex__{i} = m_fabs(tape.get({slot[f'lin__{i}']})) > delta__;")
+        L.append(f"{ind}const real bbar__{i} = ex__{i} ? lam__[{i}]*h__*m_phi1(tape.get({slot[f'lin__{i}']})*h__) : lam__[{i}]*h__;")
+        L.append(f"{ind}const real abar__{i} = ex__{i} ? lam__[{i}]*tape.get({slot[f'rhs__{i}']})*(h__*h__)*"
+                 f"m_dphi1(tape.get({slot[f'lin__{i}']})*h__) : real(0.0);")
+    for a in adj_list:
+        L.append(f"{ind}tape.put({aslot[a]}, real(0.0));")
+    for tgt, src, d in acc_lines:
+        src_txt = f"tape.get({aslot[src]})" if src in aslot else src
+        L.append(f"{ind}tape.add({aslot[tgt]}, {src_txt}*({tp.p(d)}));")
+    for i, st in enumerate(model.state_names):
+        a = f"{st}__b"
+        L.append(f"{ind}lam_prev__[{i}] = lam__[{i}]" + (f" + tape.get({aslot[a]});" if a in aslot else ";"))
+    for j, fpn in enumerate(var.field_parameters):
+        a = f"{fpn}__b"
+        if a in aslot:
+            L.append(f"{ind}fp_bar__[{j}] += tape.get({aslot[a]});")
     L.append("}")
     L.append("")
 

@@ -109,15 +109,19 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real s = states[1];
   const real b = pa.template get<1>();
   const real c_3 = pa.template get<4>();
+  const real stim_start = pa.template get<8>();
   const real v_peak = pa.template get<9>();
   const real v_rest = pa.template get<10>();
   const real c__1 = pa.template cst<1>();
   const real c__2 = pa.template cst<2>();
   const real c__3 = pa.template cst<3>();
+  const real c__4 = pa.template cst<4>();
   const real c__5 = pa.template cst<5>();
   const real c__6 = pa.template cst<6>();
   const real I = c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v);
-  const treal i_Stim = m_div(pa.template getd<5>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
+  const treal tr__0 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__1 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>());
+  const treal i_Stim = m_div(pa.template getd<5>()*(treal(1.0) - m_rcp(treal(1.0) + tr__1)), (treal(1.0) + tr__0));
   const real dI_dv = c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) + c__6*(v - c__1)*(v - v_rest);
   const real rhs__0 = I + real(i_Stim);
   const real rhs__1 = b*(v - v_rest - c_3*s);
@@ -143,6 +147,63 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   v__b += I__b*(c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) - c__3*(v - c__1)*(v - v_rest));
   lam_prev__[0] = lam__[0] + v__b;
   lam_prev__[1] = lam__[1] + s__b;
+}
+
+constexpr int grl1_vjp_tape_slots = 13;
+// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):
+// for targets whose registers cannot hold the whole reverse sweep.
+template <typename real, typename PA, typename treal, typename TP>
+GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                       const real h__, const real delta__, const real* __restrict__ lam__,
+                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                       const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real v = states[0];
+  const real s = states[1];
+  const real b = pa.template get<1>();
+  const real c_3 = pa.template get<4>();
+  const real stim_start = pa.template get<8>();
+  const real v_peak = pa.template get<9>();
+  const real v_rest = pa.template get<10>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__4 = pa.template cst<4>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  tape.put(0, c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v));
+  const treal tr__0 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  tape.put(1, real(tr__0));
+  const treal tr__1 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>());
+  tape.put(2, real(tr__1));
+  const treal i_Stim = m_div(pa.template getd<5>()*(treal(1.0) - m_rcp(treal(1.0) + tr__1)), (treal(1.0) + tr__0));
+  tape.put(3, real(i_Stim));
+  tape.put(4, c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) + c__6*(v - c__1)*(v - v_rest));
+  tape.put(5, tape.get(0) + tape.get(3));
+  tape.put(6, b*(v - v_rest - c_3*s));
+  tape.put(7, tape.get(4));
+  tape.put(8, c__5);
+  const bool ex__0 = m_fabs(tape.get(7)) > delta__;
+  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(tape.get(7)*h__) : lam__[0]*h__;
+  const real abar__0 = ex__0 ? lam__[0]*tape.get(5)*(h__*h__)*m_dphi1(tape.get(7)*h__) : real(0.0);
+  const bool ex__1 = m_fabs(tape.get(8)) > delta__;
+  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(tape.get(8)*h__) : lam__[1]*h__;
+  const real abar__1 = ex__1 ? lam__[1]*tape.get(6)*(h__*h__)*m_dphi1(tape.get(8)*h__) : real(0.0);
+  tape.put(9, real(0.0));
+  tape.put(10, real(0.0));
+  tape.put(11, real(0.0));
+  tape.put(12, real(0.0));
+  tape.add(10, abar__0*(real(1.0)));
+  tape.add(11, bbar__1*(-b*c_3));
+  tape.add(12, bbar__1*(b));
+  tape.add(9, bbar__0*(real(1.0)));
+  tape.add(11, tape.get(10)*(c__2));
+  tape.add(12, tape.get(10)*(c__6*(v - c__1) + c__6*(v - v_rest) - c__3*(v - c__1) - c__3*(v - v_rest) + real(2.0)*c__3*(v_peak - v)));
+  tape.add(11, tape.get(9)*(c__2*(v - v_rest)));
+  tape.add(12, tape.get(9)*(c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) - c__3*(v - c__1)*(v - v_rest)));
+  lam_prev__[0] = lam__[0] + tape.get(12);
+  lam_prev__[1] = lam__[1] + tape.get(11);
 }
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].

@@ -87,6 +87,36 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   fp_bar__[1] += b__b;
 }
 
+constexpr int grl1_vjp_tape_slots = 5;
+// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):
+// for targets whose registers cannot hold the whole reverse sweep.
+template <typename real, typename PA, typename treal, typename TP>
+GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                       const real h__, const real delta__, const real* __restrict__ lam__,
+                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                       const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real y = states[0];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  tape.put(0, b + a*y);
+  tape.put(1, a);
+  const bool ex__0 = m_fabs(tape.get(1)) > delta__;
+  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(tape.get(1)*h__) : lam__[0]*h__;
+  const real abar__0 = ex__0 ? lam__[0]*tape.get(0)*(h__*h__)*m_dphi1(tape.get(1)*h__) : real(0.0);
+  tape.put(2, real(0.0));
+  tape.put(3, real(0.0));
+  tape.put(4, real(0.0));
+  tape.add(2, abar__0*(real(1.0)));
+  tape.add(2, bbar__0*(y));
+  tape.add(3, bbar__0*(real(1.0)));
+  tape.add(4, bbar__0*(a));
+  lam_prev__[0] = lam__[0] + tape.get(4);
+  fp_bar__[0] += tape.get(2);
+  fp_bar__[1] += tape.get(3);
+}
+
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,

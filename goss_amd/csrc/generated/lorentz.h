@@ -116,6 +116,53 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   lam_prev__[2] = lam__[2] + z__b;
 }
 
+constexpr int grl1_vjp_tape_slots = 9;
+// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):
+// for targets whose registers cannot hold the whole reverse sweep.
+template <typename real, typename PA, typename treal, typename TP>
+GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                       const real h__, const real delta__, const real* __restrict__ lam__,
+                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                       const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real x = states[0];
+  const real y = states[1];
+  const real z = states[2];
+  const real beta = pa.template get<0>();
+  const real rho = pa.template get<1>();
+  const real sigma = pa.template get<2>();
+  tape.put(0, sigma*(y - x));
+  tape.put(1, -y + x*(rho - z));
+  tape.put(2, x*y - beta*z);
+  tape.put(3, -sigma);
+  tape.put(4, -real(1.0));
+  tape.put(5, -beta);
+  const bool ex__0 = m_fabs(tape.get(3)) > delta__;
+  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(tape.get(3)*h__) : lam__[0]*h__;
+  const real abar__0 = ex__0 ? lam__[0]*tape.get(0)*(h__*h__)*m_dphi1(tape.get(3)*h__) : real(0.0);
+  const bool ex__1 = m_fabs(tape.get(4)) > delta__;
+  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(tape.get(4)*h__) : lam__[1]*h__;
+  const real abar__1 = ex__1 ? lam__[1]*tape.get(1)*(h__*h__)*m_dphi1(tape.get(4)*h__) : real(0.0);
+  const bool ex__2 = m_fabs(tape.get(5)) > delta__;
+  const real bbar__2 = ex__2 ? lam__[2]*h__*m_phi1(tape.get(5)*h__) : lam__[2]*h__;
+  const real abar__2 = ex__2 ? lam__[2]*tape.get(2)*(h__*h__)*m_dphi1(tape.get(5)*h__) : real(0.0);
+  tape.put(6, real(0.0));
+  tape.put(7, real(0.0));
+  tape.put(8, real(0.0));
+  tape.add(6, bbar__2*(y));
+  tape.add(7, bbar__2*(x));
+  tape.add(8, bbar__2*(-beta));
+  tape.add(6, bbar__1*(rho - z));
+  tape.add(7, bbar__1*(-real(1.0)));
+  tape.add(8, bbar__1*(-x));
+  tape.add(6, bbar__0*(-sigma));
+  tape.add(7, bbar__0*(sigma));
+  lam_prev__[0] = lam__[0] + tape.get(6);
+  lam_prev__[1] = lam__[1] + tape.get(7);
+  lam_prev__[2] = lam__[2] + tape.get(8);
+}
+
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,

@@ -6,15 +6,20 @@
 // of all states every `ckpt_every` steps ([n_ckpt][S][ld]) and a sample of
 // one field state every `sample_every` steps ([n_samples][N], FP64).
 //
-// Backward (adjoint_backward_kernel): one launch walks the checkpoint
-// segments from the last to the first.  Per segment each lane recomputes the
+// Backward: the checkpoint segments are walked from the last to the first,
+// two launches per segment.  adjoint_recompute_kernel recomputes the
 // segment's states from its checkpoint with the same step code (bit-identical
-// to the forward), keeps them in a segment buffer ([ckpt_every][S][ld]), then
-// runs the steps in reverse: at step k (y_k -> y_k+1) the loss gradient of a
+// to the forward) into a segment buffer ([ckpt_every][S][ld]);
+// adjoint_reverse_kernel runs the steps in reverse: at step k (y_k -> y_k+1) the loss gradient of a
 // sample taken after step k is added to lam_k+1, and the generated grl1_vjp
 // gives lam_k = lam_k+1 + (dy_k+1/dy_k)^T lam_k+1 and the field-parameter
-// contribution (dy_k+1/dp)^T lam_k+1.  lam stays in registers across the
-// whole sweep; parameter gradients accumulate in FP64.
+// contribution (dy_k+1/dp)^T lam_k+1.  lam stays in registers within a
+// segment and in HBM between segments; parameter gradients accumulate in
+// FP64.  Splitting recompute and reverse sweep into two kernels keeps lam
+// out of the recompute's register budget.  In FP32 the reverse
+// sweep of one step runs in registers; in FP64 its intermediates and
+// adjoints live in a per-node tape ([grl1_vjp_tape_slots][ld]) because they
+// do not fit in the register file.
 //
 // One node per lane, no inter-lane communication: nodes are independent.
 #pragma once
@@ -22,6 +27,30 @@
 #include "goss_kernels.h"
 
 namespace goss_hip {
+
+// Per-node tape of grl1_vjp_tape: slot k at a[k * ld + lane].  Volatile
+// accesses keep every intermediate in memory instead of registers (each read
+// is a load), which bounds the registers the FP64 reverse sweep needs.
+template <class R>
+struct VTape {
+  R* a;          // tape base (uniform across the wavefront)
+  unsigned ld;
+  unsigned lane; // node index
+  GOSS_HD GOSS_INLINE volatile R* at(const int k) const
+  {
+    R* row = a + size_t(k) * ld;
+#ifdef __HIP_DEVICE_COMPILE__
+    // The row address is uniform; making it opaque per access (an empty asm with side
+    // effects, kept in SGPRs) stops the compiler from keeping hundreds of slot addresses
+    // alive across the sweep.  It emits no instruction.
+    asm volatile("" : "+s"(row));
+#endif
+    return row + lane;
+  }
+  GOSS_HD GOSS_INLINE R get(const int k) const { return *at(k); }
+  GOSS_HD GOSS_INLINE void put(const int k, const R v) const { *at(k) = v; }
+  GOSS_HD GOSS_INLINE void add(const int k, const R v) const { put(k, get(k) + v); }
+};
 
 struct AdjointShape {
   int n;             // outer steps
@@ -68,12 +97,48 @@ adjoint_record_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2
   for (int i = 0; i < S; ++i) y[size_t(i) * ld + node] = st.y[i];
 }
 
+// Backward, part 1 of a segment: recompute its states y_k0 .. y_k1-1 from the
+// checkpoint into the segment buffer.
 template <class M, class R>
 __global__ void __launch_bounds__(kBlock)
-adjoint_backward_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, const double2* __restrict__ steps,
-                        const SharedParamsOf<M, R> sp, const R delta, const unsigned N, const unsigned ld,
-                        const AdjointShape shp, const double* __restrict__ grad_samples, R* __restrict__ seg,
-                        double* __restrict__ grad_y0, double* __restrict__ grad_fp)
+adjoint_recompute_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, const double2* __restrict__ steps,
+                         const SharedParamsOf<M, R> sp, const R delta, const unsigned N, const unsigned ld,
+                         const AdjointShape shp, const int seg_index, R* __restrict__ seg)
+{
+  constexpr int S = M::num_states;
+  constexpr int NFP = M::num_field_parameters;
+  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= N) return;
+  R fpv[NFP > 0 ? NFP : 1];
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
+  const DevAcc<M, R> acc{sp, fpv};
+  const int k0 = seg_index * shp.ckpt_every;
+  const int k1 = min(shp.n, k0 + shp.ckpt_every);
+  NodeState<M, R, false> st;
+  const R* c = ckpt + size_t(seg_index) * S * ld + node;
+#pragma unroll
+  for (int i = 0; i < S; ++i) st.y[i] = c[size_t(i) * ld];
+#pragma unroll 1
+  for (int k = k0; k < k1; ++k) {
+    R* out = seg + size_t(k - k0) * S * ld + node;
+#pragma unroll
+    for (int i = 0; i < S; ++i) out[size_t(i) * ld] = st.y[i];
+    if (k + 1 < k1) Step<M, kGRL1, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
+  }
+}
+
+// Backward, part 2 of a segment: the reverse sweep over its steps.  lam
+// ([S][ld], precision R) and the parameter gradients ([NFP][ld], FP64) carry
+// over between segments in HBM; `first` starts them at zero, `last` writes
+// the results node-major.
+template <class M, class R>
+__global__ void __launch_bounds__(kBlock)
+adjoint_reverse_kernel(const R* __restrict__ fp, const double2* __restrict__ steps, const SharedParamsOf<M, R> sp,
+                       const R delta, const unsigned N, const unsigned ld, const AdjointShape shp, const int seg_index,
+                       const double* __restrict__ grad_samples, const R* __restrict__ seg, R* __restrict__ tape,
+                       R* __restrict__ lam_buf, double* __restrict__ pbar_buf, const int first, const int last,
+                       double* __restrict__ grad_y0, double* __restrict__ grad_fp)
 {
   constexpr int S = M::num_states;
   constexpr int NFP = M::num_field_parameters;
@@ -84,56 +149,55 @@ adjoint_backward_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, co
   for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
   const DevAcc<M, R> acc{sp, fpv};
   R lam[S];
-#pragma unroll
-  for (int i = 0; i < S; ++i) lam[i] = R(0);
   double pbar[NFP > 0 ? NFP : 1];
 #pragma unroll
-  for (int j = 0; j < NFP; ++j) pbar[j] = 0.0;
+  for (int i = 0; i < S; ++i) lam[i] = first ? R(0) : lam_buf[size_t(i) * ld + node];
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) pbar[j] = first ? 0.0 : pbar_buf[size_t(j) * ld + node];
 
-  const int nseg = (shp.n + shp.ckpt_every - 1) / shp.ckpt_every;
+  const int k0 = seg_index * shp.ckpt_every;
+  const int k1 = min(shp.n, k0 + shp.ckpt_every);
 #pragma unroll 1
-  for (int s = nseg - 1; s >= 0; --s) {
-    const int k0 = s * shp.ckpt_every;
-    const int k1 = min(shp.n, k0 + shp.ckpt_every);
-    // Recompute the segment's states y_k0 .. y_k1-1 from the checkpoint.
-    {
-      NodeState<M, R, false> st;
-      const R* c = ckpt + size_t(s) * S * ld + node;
+  for (int k = k1 - 1; k >= k0; --k) {
+    if ((k + 1) % shp.sample_every == 0) {
+      const R gs = R(grad_samples[size_t((k + 1) / shp.sample_every - 1) * N + node]);
 #pragma unroll
-      for (int i = 0; i < S; ++i) st.y[i] = c[size_t(i) * ld];
-#pragma unroll 1
-      for (int k = k0; k < k1; ++k) {
-        R* out = seg + size_t(k - k0) * S * ld + node;
-#pragma unroll
-        for (int i = 0; i < S; ++i) out[size_t(i) * ld] = st.y[i];
-        if (k + 1 < k1) Step<M, kGRL1, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
-      }
+      for (int i = 0; i < S; ++i) lam[i] += (i == shp.field_index) ? gs : R(0);
     }
-    // Reverse sweep over the segment.
-#pragma unroll 1
-    for (int k = k1 - 1; k >= k0; --k) {
-      if ((k + 1) % shp.sample_every == 0) {
-        const R gs = R(grad_samples[size_t((k + 1) / shp.sample_every - 1) * N + node]);
+    R yk[S], lp[S], pb[NFP > 0 ? NFP : 1];
+    const R* in = seg + size_t(k - k0) * S * ld + node;
 #pragma unroll
-        for (int i = 0; i < S; ++i) lam[i] += (i == shp.field_index) ? gs : R(0);
-      }
-      R yk[S], lp[S], pb[NFP > 0 ? NFP : 1];
-      const R* in = seg + size_t(k - k0) * S * ld + node;
+    for (int i = 0; i < S; ++i) yk[i] = in[size_t(i) * ld];
 #pragma unroll
-      for (int i = 0; i < S; ++i) yk[i] = in[size_t(i) * ld];
-#pragma unroll
-      for (int j = 0; j < NFP; ++j) pb[j] = R(0);
+    for (int j = 0; j < NFP; ++j) pb[j] = R(0);
+    if constexpr (sizeof(R) == 8) {
+      // FP64: the reverse sweep's intermediates do not fit in registers; they go to the tape.
+      // The tape base is made opaque per step so that the compiler does not hoist the
+      // ~440 per-slot addresses (VGPR pairs) out of the step loop; the empty asm emits nothing.
+      R* tp = tape;
+#ifdef __HIP_DEVICE_COMPILE__
+      asm volatile("" : "+s"(tp));
+#endif
+      M::template grl1_vjp_tape<R>(yk, steps[k].x, acc, R(steps[k].y), delta, lam, lp, pb, VTape<R>{tp, ld, node});
+    } else {
       M::template grl1_vjp<R>(yk, steps[k].x, acc, R(steps[k].y), delta, lam, lp, pb);
-#pragma unroll
-      for (int i = 0; i < S; ++i) lam[i] = lp[i];
-#pragma unroll
-      for (int j = 0; j < NFP; ++j) pbar[j] += double(pb[j]);
     }
+#pragma unroll
+    for (int i = 0; i < S; ++i) lam[i] = lp[i];
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) pbar[j] += double(pb[j]);
   }
+  if (last) {
 #pragma unroll
-  for (int i = 0; i < S; ++i) grad_y0[size_t(node) * S + i] = double(lam[i]);
+    for (int i = 0; i < S; ++i) grad_y0[size_t(node) * S + i] = double(lam[i]);
 #pragma unroll
-  for (int j = 0; j < NFP; ++j) grad_fp[size_t(node) * NFP + j] = pbar[j];
+    for (int j = 0; j < NFP; ++j) grad_fp[size_t(node) * NFP + j] = pbar[j];
+  } else {
+#pragma unroll
+    for (int i = 0; i < S; ++i) lam_buf[size_t(i) * ld + node] = lam[i];
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) pbar_buf[size_t(j) * ld + node] = pbar[j];
+  }
 }
 
 }  // namespace goss_hip

@@ -616,7 +616,11 @@ int goss_hip_adjoint_buffer_sizes(goss_hip_system* s, uint64_t n, uint32_t sampl
     adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
     const uint64_t per = uint64_t(s->v->S) * s->ld * s->elem();
     *checkpoint_bytes = nc * per;
-    *segment_bytes = uint64_t(checkpoint_every) * per;
+    // The segment buffer holds the recomputed states of one checkpoint segment
+    // and, in FP64, the per-node tape of the reverse sweep behind them.
+    // Behind them: lam between segments ([S][ld]) and the parameter gradients ([NFP][ld], FP64).
+    const uint64_t tape = s->prec == kFP64 ? uint64_t(s->v->vjp_tape_slots) * s->ld * sizeof(double) : 0;
+    *segment_bytes = uint64_t(checkpoint_every) * per + tape + per + uint64_t(std::max(s->v->NFP, 1)) * s->ld * 8;
     *num_samples = ns;
   });
 }
@@ -678,6 +682,12 @@ int goss_hip_adjoint_backward(goss_hip_system* s, const double* t, uint64_t n, u
     a.ckpt = const_cast<void*>(checkpoints);
     a.grad_samples = grad_samples;
     a.seg = segment;
+    const size_t per = size_t(s->v->S) * s->ld * s->elem();
+    const size_t tape = s->prec == kFP64 ? size_t(s->v->vjp_tape_slots) * s->ld * sizeof(double) : 0;
+    char* base = static_cast<char*>(segment);
+    a.tape = base + size_t(checkpoint_every) * per;
+    a.lam = base + size_t(checkpoint_every) * per + tape;
+    a.pbar = reinterpret_cast<double*>(base + size_t(checkpoint_every) * per + tape + per);
     a.grad_y0 = grad_states;
     a.grad_fp = grad_field_parameters;
     a.stream = s->stream;

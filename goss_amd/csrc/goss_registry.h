@@ -75,6 +75,9 @@ struct AdjointArgs {
   double* samples;         // record: [n_samples][N]
   const double* grad_samples;  // backward: [n_samples][N]
   void* seg;               // backward: [ckpt_every][S][ld]
+  void* tape;              // backward, FP64: [vjp_tape_slots][ld]
+  void* lam;               // backward: adjoint states between segments [S][ld]
+  double* pbar;            // backward: parameter gradients between segments [NFP][ld]
   double* grad_y0;         // backward: [N][S]
   double* grad_fp;         // backward: [N][NFP]
   hipStream_t stream;
@@ -85,6 +88,7 @@ typedef hipError_t (*AdjointLaunch)(const AdjointArgs&);
 struct VariantInfo {
   const char* name;
   int S, P, NFS, NFP, NCOMP;
+  int vjp_tape_slots;      // slots of the FP64 adjoint's per-node tape
   const int* fsi;
   const int* fpi;
   const double* defaults;

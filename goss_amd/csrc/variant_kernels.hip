@@ -39,11 +39,18 @@ struct M {
   {
     NS::linearized_eval<R>(y, t, a, lin, rhs, ol);
   }
+  static constexpr int grl1_vjp_tape_slots = NS::grl1_vjp_tape_slots;
   template <typename R, typename A, typename T>
   GOSS_HD GOSS_INLINE static void grl1_vjp(const R* y, const T t, const A& a, const R h, const R delta, const R* lam,
                                            R* lam_prev, R* fp_bar)
   {
     NS::grl1_vjp<R>(y, t, a, h, delta, lam, lam_prev, fp_bar);
+  }
+  template <typename R, typename A, typename T, typename TP>
+  GOSS_HD GOSS_INLINE static void grl1_vjp_tape(const R* y, const T t, const A& a, const R h, const R delta,
+                                                const R* lam, R* lam_prev, R* fp_bar, const TP& tape)
+  {
+    NS::grl1_vjp_tape<R>(y, t, a, h, delta, lam, lam_prev, fp_bar, tape);
   }
 };
 
@@ -117,10 +124,19 @@ hipError_t launch_adjoint_backward(const AdjointArgs& a)
 {
   const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((adjoint_backward_kernel<M, R>), grid, dim3(kBlock), 0, a.stream,
-                     static_cast<const R*>(a.ckpt), static_cast<const R*>(a.fp), a.steps, sp, R(a.delta), a.N, a.ld,
-                     a.shape, a.grad_samples, static_cast<R*>(a.seg), a.grad_y0, a.grad_fp);
-  return hipGetLastError();
+  const int nseg = (a.shape.n + a.shape.ckpt_every - 1) / a.shape.ckpt_every;
+  for (int s = nseg - 1; s >= 0; --s) {
+    hipLaunchKernelGGL((adjoint_recompute_kernel<M, R>), grid, dim3(kBlock), 0, a.stream,
+                       static_cast<const R*>(a.ckpt), static_cast<const R*>(a.fp), a.steps, sp, R(a.delta), a.N, a.ld,
+                       a.shape, s, static_cast<R*>(a.seg));
+    hipLaunchKernelGGL((adjoint_reverse_kernel<M, R>), grid, dim3(kBlock), 0, a.stream, static_cast<const R*>(a.fp),
+                       a.steps, sp, R(a.delta), a.N, a.ld, a.shape, s, a.grad_samples,
+                       static_cast<const R*>(a.seg), static_cast<R*>(a.tape), static_cast<R*>(a.lam),
+                       a.pbar, int(s == nseg - 1), int(s == 0), a.grad_y0, a.grad_fp);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <class R, bool COMP>
@@ -143,6 +159,7 @@ void fill_info(VariantInfo& v)
   v.NFS = NS::num_field_states;
   v.NFP = NS::num_field_parameters;
   v.NCOMP = num_comp<M>();
+  v.vjp_tape_slots = NS::grl1_vjp_tape_slots;
   v.fsi = NS::field_state_indices;
   v.fpi = NS::field_parameter_indices;
   v.defaults = NS::default_parameters;

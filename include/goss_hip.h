@@ -186,7 +186,8 @@ int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
  *
  * goss_hip_adjoint_buffer_sizes: bytes of the caller-allocated device buffers
  * (checkpoints: one state copy every `checkpoint_every` steps; segment: the
- * backward sweep's recomputed states) and the number of samples.
+ * backward sweep's recomputed states and, in FP64, its per-node tape of
+ * intermediates) and the number of samples.
  * goss_hip_adjoint_record: advances the system like forward_times, writing the
  * checkpoints and samples[n_samples][N] (device, FP64).
  * goss_hip_adjoint_backward: given d loss / d samples (device, [n_samples][N]),

@@ -79,6 +79,7 @@ def lib():
         L.oracle_model_fd_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
         L.oracle_model_grl1_vjp.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, ctypes.c_double,
                                             ctypes.c_double, _dp, _dp, _dp]
+        L.oracle_model_grl1_vjp_tape.argtypes = L.oracle_model_grl1_vjp.argtypes
         L.oracle_lu_factorize.argtypes = [ctypes.c_int, _dp]
         L.oracle_forward_backward_subst.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         _lib = L
@@ -173,14 +174,15 @@ class Variant:
         lib().oracle_model_analytic_jacobian(self.index, _ptr(y), float(t), _ptr(p), _ptr(jac))
         return jac.reshape(self.num_states, self.num_states)
 
-    def grl1_vjp(self, y, t, h, lam, params=None, delta=1e-8):
-        """(lam + (dy'/dy)^T lam, (dy'/dp_field)^T lam) for one GRL1 step y -> y' (generated code)."""
+    def grl1_vjp(self, y, t, h, lam, params=None, delta=1e-8, tape=False):
+        """(lam + (dy'/dy)^T lam, (dy'/dp_field)^T lam) for one GRL1 step y -> y' (generated code;
+        tape=True: the variant that keeps intermediates and adjoints in a tape)."""
         p = _f64(self.default_parameters if params is None else params)
         y, lam = _f64(y), _f64(lam)
         lam_prev = np.zeros(self.num_states)
         fp_bar = np.zeros(max(self.num_field_parameters, 1))
-        lib().oracle_model_grl1_vjp(self.index, _ptr(y), float(t), _ptr(p), float(h), float(delta), _ptr(lam),
-                                    _ptr(lam_prev), _ptr(fp_bar))
+        fn = lib().oracle_model_grl1_vjp_tape if tape else lib().oracle_model_grl1_vjp
+        fn(self.index, _ptr(y), float(t), _ptr(p), float(h), float(delta), _ptr(lam), _ptr(lam_prev), _ptr(fp_bar))
         return lam_prev, fp_bar[: self.num_field_parameters]
 
     def grl1_step(self, y, t, h, params=None, delta=1e-8):

@@ -58,6 +58,16 @@ struct VariantOps {
   // Generated discrete adjoint of one GRL1 step (test infrastructure for the device adjoint).
   void (*grl1_vjp)(const double*, double, const double*, const double*, double, double, const double*, double*,
                    double*);
+  void (*grl1_vjp_tape)(const double*, double, const double*, const double*, double, double, const double*, double*,
+                        double*);
+};
+
+// Host tape for grl1_vjp_tape (slot k at v[k]).
+struct HostTape {
+  double* v;
+  double get(int k) const { return v[k]; }
+  void put(int k, double x) const { v[k] = x; }
+  void add(int k, double x) const { v[k] += x; }
 };
 
 #define GOSS_ORACLE_VARIANT(V)                                                                         \
@@ -79,6 +89,12 @@ struct VariantOps {
              [](const double* y, double t, const double* p, const double* c, double h, double delta,    \
                 const double* lam, double* lam_prev, double* fp_bar) {                                 \
                goss_models::V::grl1_vjp<double>(y, t, Acc{p, c}, h, delta, lam, lam_prev, fp_bar);      \
+             },                                                                                        \
+             [](const double* y, double t, const double* p, const double* c, double h, double delta,    \
+                const double* lam, double* lam_prev, double* fp_bar) {                                 \
+               std::vector<double> tape(goss_models::V::grl1_vjp_tape_slots);                          \
+               goss_models::V::grl1_vjp_tape<double>(y, t, Acc{p, c}, h, delta, lam, lam_prev, fp_bar,  \
+                                                     HostTape{tape.data()});                           \
              }}
 
 const VariantOps kVariants[] = {
@@ -917,6 +933,13 @@ void oracle_model_grl1_vjp(int v, const double* y, double t, const double* p, do
   double c[kMaxConstants];
   kVariants[v].constants(p, c);
   kVariants[v].grl1_vjp(y, t, p, c, h, delta, lam, lam_prev, fp_bar);
+}
+void oracle_model_grl1_vjp_tape(int v, const double* y, double t, const double* p, double h, double delta,
+                                const double* lam, double* lam_prev, double* fp_bar)
+{
+  double c[kMaxConstants];
+  kVariants[v].constants(p, c);
+  kVariants[v].grl1_vjp_tape(y, t, p, c, h, delta, lam, lam_prev, fp_bar);
 }
 void oracle_lu_factorize(int n, double* mat) { lu_factorize(n, mat); }
 void oracle_forward_backward_subst(int n, const double* mat, const double* b, double* x) { fb_subst(n, mat, b, x); }

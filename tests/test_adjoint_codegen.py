@@ -86,3 +86,17 @@ def test_phi1_series_and_closed_form_agree():
         p[m.field_parameter_indices[0]] = a
         lp, fb = m.grl1_vjp(np.array([1.3]), 0.0, h, np.array([1.0]), p)
         np.testing.assert_allclose(lp[0], np.exp(a * h), rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["tp06_m_fp6", "beeler_reuter_1977", "oscilator"])
+def test_grl1_vjp_tape_variant_is_identical(name):
+    # grl1_vjp_tape (the FP64 device kernels' form) performs the same operations
+    # in the same order as grl1_vjp, only through memory.
+    m = oracle.Variant(name)
+    rng = np.random.default_rng(9)
+    for t in (5.0, 10.3, 11.7):
+        y = _state(m, rng, 1)[0]
+        lam = rng.normal(size=m.num_states)
+        a = m.grl1_vjp(y, t, 0.02, lam)
+        b = m.grl1_vjp(y, t, 0.02, lam, tape=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
