@@ -328,7 +328,24 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         L.append(f"{ind}const real abar__{i} = ex__{i} ? lam__[{i}]*rhs__{i}*(h__*h__)*m_dphi1(lin__{i}*h__) : real(0.0);")
     for t_ in sorted(targets):
         L.append(f"{ind}real {t_} = real(0.0);")
-    for tgt, src, d in acc_lines:
+    # Common subexpressions of the partials (they repeat factors such as
+    # 1/(1 + e)^2 across the partials of one statement) become temporaries,
+    # each declared just before its first use.
+    cse_rep, cse_red = sp.cse([d for _, _, d in acc_lines], symbols=sp.numbered_symbols("cs__", real=True))
+    cse_def = {sym.name: e for sym, e in cse_rep}
+    cse_order = [sym.name for sym, _ in cse_rep]
+    emitted: set = set()
+
+    def emit_temps(expr):
+        need = [n for n in cse_order if n not in emitted and sp.Symbol(n, real=True) in expr.free_symbols]
+        for n in need:
+            emit_temps(cse_def[n])
+            if n not in emitted:
+                emitted.add(n)
+                L.append(f"{ind}const real {n} = {g_vjp.main.p(cse_def[n])};")
+
+    for (tgt, src, _), d in zip(acc_lines, cse_red):
+        emit_temps(d)
         L.append(f"{ind}{tgt} += {src}*({g_vjp.main.p(d)});")
     for i, st in enumerate(model.state_names):
         a = f"{st}__b"

@@ -142,9 +142,14 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   v__b += bbar__1*(b);
   I__b += bbar__0*(real(1.0));
   s__b += dI_dv__b*(c__2);
-  v__b += dI_dv__b*(c__6*(v - c__1) + c__6*(v - v_rest) - c__3*(v - c__1) - c__3*(v - v_rest) + real(2.0)*c__3*(v_peak - v));
-  s__b += I__b*(c__2*(v - v_rest));
-  v__b += I__b*(c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) - c__3*(v - c__1)*(v - v_rest));
+  const real cs__0 = v - c__1;
+  const real cs__1 = v - v_rest;
+  const real cs__2 = c__3*cs__0;
+  const real cs__3 = v_peak - v;
+  const real cs__4 = c__3*cs__1;
+  v__b += dI_dv__b*(-cs__2 - cs__4 + c__6*cs__0 + c__6*cs__1 + real(2.0)*c__3*cs__3);
+  s__b += I__b*(c__2*cs__1);
+  v__b += I__b*(c__2*s + cs__2*cs__3 + cs__3*cs__4 - cs__1*cs__2);
   lam_prev__[0] = lam__[0] + v__b;
   lam_prev__[1] = lam__[1] + s__b;
 }

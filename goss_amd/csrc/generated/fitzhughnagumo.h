@@ -141,10 +141,15 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   V__b += bbar__1*(b);
   s__b += bbar__1*(-b*c_3);
   I__b += bbar__0*(real(1.0));
-  V__b += dI_dV__b*(c__6*(V - V_rest) + c__6*(V - c__1) - c__3*(V - V_rest) - c__3*(V - c__1) + real(2.0)*c__3*(V_peak - V));
+  const real cs__0 = V - V_rest;
+  const real cs__1 = V - c__1;
+  const real cs__2 = V_peak - V;
+  const real cs__3 = c__3*cs__0;
+  const real cs__4 = c__3*cs__1;
+  V__b += dI_dV__b*(-cs__3 - cs__4 + c__6*cs__0 + c__6*cs__1 + real(2.0)*c__3*cs__2);
   s__b += dI_dV__b*(c__2);
-  V__b += I__b*(c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) - c__3*(V - V_rest)*(V - c__1));
-  s__b += I__b*(c__2*(V - V_rest));
+  V__b += I__b*(c__2*s + cs__2*cs__3 + cs__2*cs__4 - cs__1*cs__3);
+  s__b += I__b*(c__2*cs__0);
   lam_prev__[0] = lam__[0] + V__b;
   lam_prev__[1] = lam__[1] + s__b;
 }

@@ -89,9 +89,10 @@ def test_phi1_series_and_closed_form_agree():
 
 
 @pytest.mark.parametrize("name", ["tp06_m_fp6", "beeler_reuter_1977", "oscilator"])
-def test_grl1_vjp_tape_variant_is_identical(name):
-    # grl1_vjp_tape (the FP64 device kernels' form) performs the same operations
-    # in the same order as grl1_vjp, only through memory.
+def test_grl1_vjp_tape_variant_agrees(name):
+    # grl1_vjp_tape (the FP64 device kernels' form) runs the same reverse sweep
+    # through memory, without the common-subexpression temporaries of grl1_vjp:
+    # equal to rounding.
     m = oracle.Variant(name)
     rng = np.random.default_rng(9)
     for t in (5.0, 10.3, 11.7):
@@ -99,4 +100,5 @@ def test_grl1_vjp_tape_variant_is_identical(name):
         lam = rng.normal(size=m.num_states)
         a = m.grl1_vjp(y, t, 0.02, lam)
         b = m.grl1_vjp(y, t, 0.02, lam, tape=True)
-        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        np.testing.assert_allclose(b[0], a[0], rtol=1e-12, atol=1e-15 * np.abs(a[0]).max())
+        np.testing.assert_allclose(b[1], a[1], rtol=1e-12, atol=1e-15 * max(np.abs(a[1]).max(initial=0), 1e-300))
