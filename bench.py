@@ -255,6 +255,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-checks", action="store_true", help="skip accuracy/parity passes")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the adjoint workload and print its figures as one JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one device-to-device copy of 256 MiB (read 256 MiB + write 256 MiB), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -269,6 +271,9 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return 0
+    if args.adjoint_only:
+        print(json.dumps(adjoint_run()), flush=True)
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

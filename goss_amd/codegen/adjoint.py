@@ -32,29 +32,55 @@ def _is_call(e: sp.Expr) -> bool:
     return isinstance(e, (sp.exp, sp.log)) or (isinstance(e, sp.Pow) and not e.exp.is_Integer)
 
 
+def _neg_int_pow(f: sp.Expr) -> bool:
+    return isinstance(f, sp.Pow) and f.exp.is_Integer and f.exp < 0
+
+
 def lift_calls(stmts: Sequence[Tuple[str, sp.Expr]], prefix: str = "tr__") -> List[Tuple[str, sp.Expr]]:
-    """Give every exp, log and non-integer power its own statement.
+    """Give every exp, log, non-integer power and reciprocal its own statement.
 
     The reverse sweep differentiates statement by statement; with the
     transcendental calls named, a derivative reuses the call's value
     (d exp(u)/du is the statement itself, d x^y/dx = y*n/x) instead of
-    evaluating the call again inside every partial.  Identical calls share
-    one statement."""
+    evaluating the call again inside every partial.  The denominator of a
+    product (all its negative-integer-power factors together, as the C++
+    printer divides once by their product) becomes ``dn = d1*d2*...`` and
+    ``rc = 1/dn``, so the quotient is ``num*rc`` and every partial that
+    needs ``1/dn^2`` reads ``rc*rc``: the reverse sweep takes no reciprocal
+    of its own.  Identical calls and denominators share one statement."""
     seen: Dict[sp.Expr, sp.Symbol] = {}
     out: List[Tuple[str, sp.Expr]] = []
+
+    def name(e: sp.Expr) -> sp.Symbol:
+        sym = seen.get(e)
+        if sym is None:
+            sym = sp.Symbol(f"{prefix}{len(seen)}", real=True)
+            seen[e] = sym
+            out.append((sym.name, e))
+        return sym
+
+    def recip(den: sp.Expr) -> sp.Symbol:
+        base = den if den.is_Symbol else name(den)
+        return name(sp.Pow(base, -1))
+
+    def lift(e):
+        if not isinstance(e, sp.Basic) or e.is_Atom:
+            return e
+        if isinstance(e, sp.Mul):
+            den = [f for f in e.args if _neg_int_pow(f)]
+            num = [lift(f) for f in e.args if not _neg_int_pow(f)]
+            if den:
+                num.append(recip(sp.Mul(*[lift(f.base) ** (-f.exp) for f in den])))
+            return sp.Mul(*num)
+        if _neg_int_pow(e):
+            return recip(lift(e.base) ** (-e.exp))
+        new = e.func(*[lift(a) for a in e.args]) if e.args else e
+        if isinstance(new, sp.Expr) and _is_call(new):
+            return name(new)
+        return new
+
     for n, e in stmts:
-        calls = [c for c in sp.preorder_traversal(e) if isinstance(c, sp.Expr) and _is_call(c)]
-        calls = sorted(set(calls), key=sp.count_ops)  # inner calls first
-        mapping: Dict[sp.Expr, sp.Symbol] = {}
-        for c in calls:
-            lifted = c.xreplace(mapping)
-            sym = seen.get(lifted)
-            if sym is None:
-                sym = sp.Symbol(f"{prefix}{len(seen)}", real=True)
-                seen[lifted] = sym
-                out.append((sym.name, lifted))
-            mapping[c] = sym
-        out.append((n, e.xreplace(mapping)))
+        out.append((n, lift(e)))
     return out
 
 

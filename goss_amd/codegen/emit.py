@@ -322,10 +322,27 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;")
     L += g_vjp.bind(vused | {s_.name for s_ in model.states}, ind)
     L += g_vjp.lines(vjp_fwd, ind)
+    # 1/linear_i from the statements when linear_i = -1/tau (a gate): -tau, no reciprocal.
+    fwd_def = dict(vjp_fwd)
+
+    def inverse_of(e, depth=0):
+        if depth > 8:
+            return None
+        if isinstance(e, sp.Symbol) and e.name in fwd_def:
+            return inverse_of(fwd_def[e.name], depth + 1)
+        if isinstance(e, sp.Mul) and len(e.args) == 2 and e.args[0] == -1:
+            inv = inverse_of(e.args[1], depth + 1)
+            return None if inv is None else -inv
+        if isinstance(e, sp.Pow) and e.exp == -1:
+            return e.base
+        return None
+
+    L.append(f"{ind}const real hinv__ = m_rcp(h__);")
     for i in range(S):
-        L.append(f"{ind}const bool ex__{i} = m_fabs(lin__{i}) > delta__;")
-        L.append(f"{ind}const real bbar__{i} = ex__{i} ? lam__[{i}]*h__*m_phi1(lin__{i}*h__) : lam__[{i}]*h__;")
-        L.append(f"{ind}const real abar__{i} = ex__{i} ? lam__[{i}]*rhs__{i}*(h__*h__)*m_dphi1(lin__{i}*h__) : real(0.0);")
+        inv = inverse_of(sp.Symbol(f"lin__{i}", real=True))
+        inv_txt = g_vjp.main.p(inv) if inv is not None else f"m_rcp(lin__{i})"
+        L.append(f"{ind}real bbar__{i}, abar__{i};")
+        L.append(f"{ind}m_grl1_seeds(lin__{i}, {inv_txt}, rhs__{i}, lam__[{i}], h__, hinv__, delta__, bbar__{i}, abar__{i});")
     for t_ in sorted(targets):
         L.append(f"{ind}real {t_} = real(0.0);")
     # Common subexpressions of the partials (they repeat factors such as
@@ -388,10 +405,10 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         else:
             L.append(f"{ind}tape.put({slot[n]}, {fwd_pr.p(e)});")
     for i in range(S):
-        L.append(f"{ind}const bool ex__{i} = m_fabs(tape.get({slot[f'lin__{i}']})) > delta__;")
-        L.append(f"{ind}const real bbar__{i} = ex__{i} ? lam__[{i}]*h__*m_phi1(tape.get({slot[f'lin__{i}']})*h__) : lam__[{i}]*h__;")
-        L.append(f"{ind}const real abar__{i} = ex__{i} ? lam__[{i}]*tape.get({slot[f'rhs__{i}']})*(h__*h__)*"
-                 f"m_dphi1(tape.get({slot[f'lin__{i}']})*h__) : real(0.0);")
+        lin_txt = f"real(tape.get({slot[f'lin__{i}']}))"
+        L.append(f"{ind}real bbar__{i}, abar__{i};")
+        L.append(f"{ind}m_grl1_seeds({lin_txt}, m_rcp({lin_txt}), real(tape.get({slot[f'rhs__{i}']})), lam__[{i}], "
+                 f"h__, m_rcp(h__), delta__, bbar__{i}, abar__{i});")
     for a in adj_list:
         L.append(f"{ind}tape.put({aslot[a]}, real(0.0));")
     for tgt, src, d in acc_lines:

@@ -119,20 +119,23 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real c__5 = pa.template cst<5>();
   const real c__6 = pa.template cst<6>();
   const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
-  const treal tr__0 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
-  const treal tr__1 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
-  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + tr__1)), (treal(1.0) + tr__0));
+  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__1 = treal(1.0) + tr__0;
+  const treal tr__2 = m_rcp(tr__1);
+  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__4 = treal(1.0) + tr__3;
+  const treal tr__5 = m_rcp(tr__4);
+  const treal i_Stim = pa.template getd<7>()*tr__5*(treal(1.0) - tr__2);
   const real dI_dV = c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) + c__6*(V - V_rest)*(V - c__1);
   const real rhs__0 = I + real(i_Stim);
   const real rhs__1 = b*(V - V_rest - c_3*s);
   const real lin__0 = dI_dV;
   const real lin__1 = c__5;
-  const bool ex__0 = m_fabs(lin__0) > delta__;
-  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(lin__0*h__) : lam__[0]*h__;
-  const real abar__0 = ex__0 ? lam__[0]*rhs__0*(h__*h__)*m_dphi1(lin__0*h__) : real(0.0);
-  const bool ex__1 = m_fabs(lin__1) > delta__;
-  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(lin__1*h__) : lam__[1]*h__;
-  const real abar__1 = ex__1 ? lam__[1]*rhs__1*(h__*h__)*m_dphi1(lin__1*h__) : real(0.0);
+  const real hinv__ = m_rcp(h__);
+  real bbar__0, abar__0;
+  m_grl1_seeds(lin__0, m_rcp(lin__0), rhs__0, lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(lin__1, m_rcp(lin__1), rhs__1, lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   real I__b = real(0.0);
   real V__b = real(0.0);
   real dI_dV__b = real(0.0);
@@ -154,7 +157,7 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   lam_prev__[1] = lam__[1] + s__b;
 }
 
-constexpr int grl1_vjp_tape_slots = 13;
+constexpr int grl1_vjp_tape_slots = 17;
 // grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):
 // for targets whose registers cannot hold the whole reverse sweep.
 template <typename real, typename PA, typename treal, typename TP>
@@ -178,37 +181,43 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   const real c__5 = pa.template cst<5>();
   const real c__6 = pa.template cst<6>();
   tape.put(0, c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V));
-  const treal tr__0 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
   tape.put(1, real(tr__0));
-  const treal tr__1 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__1 = treal(1.0) + tr__0;
   tape.put(2, real(tr__1));
-  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + tr__1)), (treal(1.0) + tr__0));
-  tape.put(3, real(i_Stim));
-  tape.put(4, c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) + c__6*(V - V_rest)*(V - c__1));
-  tape.put(5, tape.get(0) + tape.get(3));
-  tape.put(6, b*(V - V_rest - c_3*s));
-  tape.put(7, tape.get(4));
-  tape.put(8, c__5);
-  const bool ex__0 = m_fabs(tape.get(7)) > delta__;
-  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(tape.get(7)*h__) : lam__[0]*h__;
-  const real abar__0 = ex__0 ? lam__[0]*tape.get(5)*(h__*h__)*m_dphi1(tape.get(7)*h__) : real(0.0);
-  const bool ex__1 = m_fabs(tape.get(8)) > delta__;
-  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(tape.get(8)*h__) : lam__[1]*h__;
-  const real abar__1 = ex__1 ? lam__[1]*tape.get(6)*(h__*h__)*m_dphi1(tape.get(8)*h__) : real(0.0);
-  tape.put(9, real(0.0));
-  tape.put(10, real(0.0));
-  tape.put(11, real(0.0));
-  tape.put(12, real(0.0));
-  tape.add(11, abar__0*(real(1.0)));
-  tape.add(10, bbar__1*(b));
-  tape.add(12, bbar__1*(-b*c_3));
-  tape.add(9, bbar__0*(real(1.0)));
-  tape.add(10, tape.get(11)*(c__6*(V - V_rest) + c__6*(V - c__1) - c__3*(V - V_rest) - c__3*(V - c__1) + real(2.0)*c__3*(V_peak - V)));
-  tape.add(12, tape.get(11)*(c__2));
-  tape.add(10, tape.get(9)*(c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) - c__3*(V - V_rest)*(V - c__1)));
-  tape.add(12, tape.get(9)*(c__2*(V - V_rest)));
-  lam_prev__[0] = lam__[0] + tape.get(10);
-  lam_prev__[1] = lam__[1] + tape.get(12);
+  const treal tr__2 = m_rcp(tr__1);
+  tape.put(3, real(tr__2));
+  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  tape.put(4, real(tr__3));
+  const treal tr__4 = treal(1.0) + tr__3;
+  tape.put(5, real(tr__4));
+  const treal tr__5 = m_rcp(tr__4);
+  tape.put(6, real(tr__5));
+  const treal i_Stim = pa.template getd<7>()*tr__5*(treal(1.0) - tr__2);
+  tape.put(7, real(i_Stim));
+  tape.put(8, c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) + c__6*(V - V_rest)*(V - c__1));
+  tape.put(9, tape.get(0) + tape.get(7));
+  tape.put(10, b*(V - V_rest - c_3*s));
+  tape.put(11, tape.get(8));
+  tape.put(12, c__5);
+  real bbar__0, abar__0;
+  m_grl1_seeds(real(tape.get(11)), m_rcp(real(tape.get(11))), real(tape.get(9)), lam__[0], h__, m_rcp(h__), delta__, bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(tape.get(12)), m_rcp(real(tape.get(12))), real(tape.get(10)), lam__[1], h__, m_rcp(h__), delta__, bbar__1, abar__1);
+  tape.put(13, real(0.0));
+  tape.put(14, real(0.0));
+  tape.put(15, real(0.0));
+  tape.put(16, real(0.0));
+  tape.add(15, abar__0*(real(1.0)));
+  tape.add(14, bbar__1*(b));
+  tape.add(16, bbar__1*(-b*c_3));
+  tape.add(13, bbar__0*(real(1.0)));
+  tape.add(14, tape.get(15)*(c__6*(V - V_rest) + c__6*(V - c__1) - c__3*(V - V_rest) - c__3*(V - c__1) + real(2.0)*c__3*(V_peak - V)));
+  tape.add(16, tape.get(15)*(c__2));
+  tape.add(14, tape.get(13)*(c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) - c__3*(V - V_rest)*(V - c__1)));
+  tape.add(16, tape.get(13)*(c__2*(V - V_rest)));
+  lam_prev__[0] = lam__[0] + tape.get(14);
+  lam_prev__[1] = lam__[1] + tape.get(16);
 }
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].

@@ -72,9 +72,9 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real b = pa.template get<1>();
   const real rhs__0 = b + a*y;
   const real lin__0 = a;
-  const bool ex__0 = m_fabs(lin__0) > delta__;
-  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(lin__0*h__) : lam__[0]*h__;
-  const real abar__0 = ex__0 ? lam__[0]*rhs__0*(h__*h__)*m_dphi1(lin__0*h__) : real(0.0);
+  const real hinv__ = m_rcp(h__);
+  real bbar__0, abar__0;
+  m_grl1_seeds(lin__0, m_rcp(lin__0), rhs__0, lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
   real a__b = real(0.0);
   real b__b = real(0.0);
   real y__b = real(0.0);
@@ -102,9 +102,8 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   const real b = pa.template get<1>();
   tape.put(0, b + a*y);
   tape.put(1, a);
-  const bool ex__0 = m_fabs(tape.get(1)) > delta__;
-  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(tape.get(1)*h__) : lam__[0]*h__;
-  const real abar__0 = ex__0 ? lam__[0]*tape.get(0)*(h__*h__)*m_dphi1(tape.get(1)*h__) : real(0.0);
+  real bbar__0, abar__0;
+  m_grl1_seeds(real(tape.get(1)), m_rcp(real(tape.get(1))), real(tape.get(0)), lam__[0], h__, m_rcp(h__), delta__, bbar__0, abar__0);
   tape.put(2, real(0.0));
   tape.put(3, real(0.0));
   tape.put(4, real(0.0));

@@ -80,12 +80,11 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real rhs__1 = b*x;
   const real lin__0 = real(0.0);
   const real lin__1 = real(0.0);
-  const bool ex__0 = m_fabs(lin__0) > delta__;
-  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(lin__0*h__) : lam__[0]*h__;
-  const real abar__0 = ex__0 ? lam__[0]*rhs__0*(h__*h__)*m_dphi1(lin__0*h__) : real(0.0);
-  const bool ex__1 = m_fabs(lin__1) > delta__;
-  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(lin__1*h__) : lam__[1]*h__;
-  const real abar__1 = ex__1 ? lam__[1]*rhs__1*(h__*h__)*m_dphi1(lin__1*h__) : real(0.0);
+  const real hinv__ = m_rcp(h__);
+  real bbar__0, abar__0;
+  m_grl1_seeds(lin__0, m_rcp(lin__0), rhs__0, lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(lin__1, m_rcp(lin__1), rhs__1, lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   real a__b = real(0.0);
   real b__b = real(0.0);
   real x__b = real(0.0);
@@ -118,12 +117,10 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   tape.put(1, b*x);
   tape.put(2, real(0.0));
   tape.put(3, real(0.0));
-  const bool ex__0 = m_fabs(tape.get(2)) > delta__;
-  const real bbar__0 = ex__0 ? lam__[0]*h__*m_phi1(tape.get(2)*h__) : lam__[0]*h__;
-  const real abar__0 = ex__0 ? lam__[0]*tape.get(0)*(h__*h__)*m_dphi1(tape.get(2)*h__) : real(0.0);
-  const bool ex__1 = m_fabs(tape.get(3)) > delta__;
-  const real bbar__1 = ex__1 ? lam__[1]*h__*m_phi1(tape.get(3)*h__) : lam__[1]*h__;
-  const real abar__1 = ex__1 ? lam__[1]*tape.get(1)*(h__*h__)*m_dphi1(tape.get(3)*h__) : real(0.0);
+  real bbar__0, abar__0;
+  m_grl1_seeds(real(tape.get(2)), m_rcp(real(tape.get(2))), real(tape.get(0)), lam__[0], h__, m_rcp(h__), delta__, bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(1)), lam__[1], h__, m_rcp(h__), delta__, bbar__1, abar__1);
   tape.put(4, real(0.0));
   tape.put(5, real(0.0));
   tape.put(6, real(0.0));

@@ -81,9 +81,9 @@ adjoint_record_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2
   for (int k = 0; k < nouter; ++k) {
     const int gk = step0 + k;
     if (gk % shp.ckpt_every == 0) {
-      R* c = ckpt + size_t(gk / shp.ckpt_every) * S * ld + node;
+      R* c = ckpt + size_t(gk / shp.ckpt_every) * S * ld;
 #pragma unroll
-      for (int i = 0; i < S; ++i) c[size_t(i) * ld] = st.y[i];
+      for (int i = 0; i < S; ++i) c[size_t(i) * ld + node] = st.y[i];
     }
     Step<M, kGRL1, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
     if ((gk + 1) % shp.sample_every == 0) {
@@ -116,14 +116,14 @@ adjoint_recompute_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, c
   const int k0 = seg_index * shp.ckpt_every;
   const int k1 = min(shp.n, k0 + shp.ckpt_every);
   NodeState<M, R, false> st;
-  const R* c = ckpt + size_t(seg_index) * S * ld + node;
+  const R* c = ckpt + size_t(seg_index) * S * ld;
 #pragma unroll
-  for (int i = 0; i < S; ++i) st.y[i] = c[size_t(i) * ld];
+  for (int i = 0; i < S; ++i) st.y[i] = c[size_t(i) * ld + node];
 #pragma unroll 1
   for (int k = k0; k < k1; ++k) {
-    R* out = seg + size_t(k - k0) * S * ld + node;
+    R* out = seg + size_t(k - k0) * S * ld;
 #pragma unroll
-    for (int i = 0; i < S; ++i) out[size_t(i) * ld] = st.y[i];
+    for (int i = 0; i < S; ++i) out[size_t(i) * ld + node] = st.y[i];
     if (k + 1 < k1) Step<M, kGRL1, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
   }
 }
@@ -165,9 +165,9 @@ adjoint_reverse_kernel(const R* __restrict__ fp, const double2* __restrict__ ste
       for (int i = 0; i < S; ++i) lam[i] += (i == shp.field_index) ? gs : R(0);
     }
     R yk[S], lp[S], pb[NFP > 0 ? NFP : 1];
-    const R* in = seg + size_t(k - k0) * S * ld + node;
+    const R* in = seg + size_t(k - k0) * S * ld;
 #pragma unroll
-    for (int i = 0; i < S; ++i) yk[i] = in[size_t(i) * ld];
+    for (int i = 0; i < S; ++i) yk[i] = in[size_t(i) * ld + node];
 #pragma unroll
     for (int j = 0; j < NFP; ++j) pb[j] = R(0);
     if constexpr (sizeof(R) == 8) {

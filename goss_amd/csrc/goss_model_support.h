@@ -99,6 +99,21 @@ GOSS_HD GOSS_INLINE T m_dphi1(const T x)
     return T(1.0 / 2) + x * (T(1.0 / 3) + x * (T(1.0 / 8) + x * (T(1.0 / 30) + x * (T(1.0 / 144) + x * T(1.0 / 840)))));
   return (x * std::exp(x) - std::expm1(x)) / (x * x);
 }
+// Seeds of the GRL1 step adjoint for one state: with x = linear * h and lh = lam * h,
+//   bbar = lh * phi1(x),  abar = lh * h * rhs * phi1'(x)   when |linear| > delta,
+//   bbar = lh,            abar = 0                          otherwise (the step is y + rhs * h).
+// inv_linear = 1/linear and hinv = 1/h are passed in (a gate's 1/linear is -tau, with no
+// reciprocal to take); this generic form evaluates phi1 and phi1' directly.
+template <typename T>
+GOSS_HD GOSS_INLINE void m_grl1_seeds(const T linear, const T inv_linear, const T rhs, const T lam, const T h,
+                                      const T hinv, const T delta, T& bbar, T& abar)
+{
+  (void)inv_linear; (void)hinv;
+  const bool ex = m_fabs(linear) > delta;
+  const T lh = lam * h;
+  bbar = ex ? lh * m_phi1(linear * h) : lh;
+  abar = ex ? lh * rhs * h * m_dphi1(linear * h) : T(0);
+}
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GOSS_F32_ACCURATE_MATH)
 // FP32 device kernels: beyond |x| = 0.1 the closed forms use the hardware exp and reciprocal
 // (exp(x) - 1 then loses at most ~1e-6 relative to cancellation).
@@ -116,5 +131,26 @@ GOSS_HD GOSS_INLINE float m_dphi1<float>(const float x)
   const float e = __expf(x);
   const float q = (x * e - (e - 1.0f)) * __builtin_amdgcn_rcpf(x * x);
   return fabsf(x) < 0.1f ? p : q;
+}
+// One exponential for both, 1/x = inv_linear * hinv: phi1 = (e - 1) / x, phi1' = (e - phi1) / x.
+// Below |x| = 0.05 the series (truncation below 1e-9 relative for phi1, 1e-7 for phi1'); above it
+// the closed forms lose at most ~2.4e-6 relative to the cancellation in e - 1.
+template <>
+GOSS_HD GOSS_INLINE void m_grl1_seeds<float>(const float linear, const float inv_linear, const float rhs,
+                                             const float lam, const float h, const float hinv, const float delta,
+                                             float& bbar, float& abar)
+{
+  const float x = linear * h;
+  const float e = __expf(x);
+  const float r = inv_linear * hinv;
+  const float pc = (e - 1.0f) * r;
+  const float dpc = (e - pc) * r;
+  const float ps = 1.0f + x * (0.5f + x * (1.0f / 6 + x * (1.0f / 24 + x * (1.0f / 120))));
+  const float dps = 0.5f + x * (1.0f / 3 + x * (0.125f + x * (1.0f / 30)));
+  const bool small = fabsf(x) < 0.05f;
+  const bool ex = fabsf(linear) > delta;
+  const float lh = lam * h;
+  bbar = ex ? lh * (small ? ps : pc) : lh;
+  abar = ex ? lh * rhs * h * (small ? dps : dpc) : 0.0f;
 }
 #endif
