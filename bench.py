@@ -235,6 +235,77 @@ def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every
     }
 
 
+MIXED_POPS = [("tp06_m", "GRL1", 300017, 0.02), ("tp06_epi", "GRL1", 211003, 0.02),
+              ("fitzhughnagumo", "RK4", 700001, 0.05), ("lorentz", "RKF32", 9973, 0.02)]
+
+
+def mixed_run(sim_ms=100.0, tune_ms=1.0, seed=2026):
+    """Mixed persistent batch: TP06 M 300,017 + TP06 epi 211,003 (GRL1, dt 0.02 ms) + FitzHugh-Nagumo
+    700,001 (RK4, dt 0.05) + Lorenz 9,973 (RKF32, outer interval 0.02), FP32, initial states
+    x (1 + U(-0.01, 0.01)).  The batch's launch shape is autotuned on [0, tune_ms]; then [tune_ms,
+    tune_ms + sim_ms] is timed (wall clock around the call and a synchronize).  The same populations
+    advanced one after another, each with its own kernel, are timed beside it."""
+    from goss_amd.batch import DeviceBatch
+    from goss_amd.system import DeviceSystem, ModelVariant
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y0 = []
+    for v, _, n, _ in MIXED_POPS:
+        mv = ModelVariant(v)
+        y0.append(mv.initial_conditions[None, :] * (1.0 + rng.uniform(-0.01, 0.01, (n, mv.num_states))))
+    dts = [dt for *_, dt in MIXED_POPS]
+
+    def make():
+        out = []
+        for (v, solver, n, _), y in zip(MIXED_POPS, y0):
+            s = DeviceSystem(v, solver, n, "fp32")
+            s.set_states(y)
+            out.append(s)
+        return out
+
+    systems = make()
+    batch = DeviceBatch(systems)
+    cfg = batch.autotune(0.0, tune_ms, dts)
+    batch.forward(0.0, tune_ms, dts)  # warm-up, also captures the graph shape once
+    batch.synchronize()
+    t0, t1 = tune_ms, tune_ms + sim_ms
+    w = time.perf_counter()
+    batch.forward(t0, t1, dts)
+    batch.synchronize()
+    fused = time.perf_counter() - w
+    node_steps = sum(n * int(round(sim_ms / dt)) for (_, _, n, dt) in MIXED_POPS)
+
+    def times(a, b, dt):  # the batch's times: t_k = a + k dt, ending exactly at b
+        t = a + np.arange(int(round((b - a) / dt)) + 1) * dt
+        t[-1] = b
+        return t
+
+    alone = make()
+    for s, dt in zip(alone, dts):
+        s.set_steps_per_launch(250)
+        s.forward_times(times(0.0, tune_ms, dt))
+        s.synchronize()
+    w = time.perf_counter()
+    for s, dt in zip(alone, dts):
+        s.forward_times(times(t0, t1, dt))
+    for s in alone:
+        s.synchronize()
+    separate = time.perf_counter() - w
+    same = all(np.array_equal(a.states(), b.states()) for a, b in zip(systems, alone))
+    batch.close()
+    for s in systems + alone:
+        s.close()
+    return {
+        "workload": "TP06 M 300,017 + TP06 epi 211,003 (GRL1, dt 0.02 ms) + FitzHugh-Nagumo 700,001 (RK4, dt 0.05) "
+                    f"+ Lorenz 9,973 (RKF32, interval 0.02), FP32, {sim_ms:g} time units",
+        "node_steps": node_steps, "fused_s": fused, "node_steps_per_s": node_steps / fused,
+        "separate_kernels_s": separate, "fused_speedup": separate / fused,
+        "autotuned": {"block_size": cfg.block_size, "nodes_per_lane": cfg.nodes_per_lane, "span": cfg.span,
+                      "dispatch_order": [MIXED_POPS[p][0] for p in cfg.order]},
+        "states_equal_to_separate": bool(same),
+    }
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "grl1_tp06_fp32c_1M_traffic.json")
     if os.path.exists(p):
@@ -257,6 +328,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the adjoint workload and print its figures as one JSON line")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-population batch workload")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="only run the mixed-population batch workload and print its figures as one JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one device-to-device copy of 256 MiB (read 256 MiB + write 256 MiB), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -274,6 +348,9 @@ def main():
         return 0
     if args.adjoint_only:
         print(json.dumps(adjoint_run()), flush=True)
+        return 0
+    if args.mixed_only:
+        print(json.dumps(mixed_run()), flush=True)
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -369,6 +446,9 @@ def main():
         out["adjoint_over_forward"] = adj["adjoint_over_forward"]
         out["adjoint"] = adj
         print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
+    if not args.no_mixed:
+        out["mixed"] = mixed_run()
+        print(f"[bench] mixed {out['mixed']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, y0, fp, dt)
         print(f"[bench] cpu {out['cpu_baseline']}", file=sys.stderr, flush=True)

@@ -42,7 +42,8 @@ EXPORTS = (
     "goss_hip_system_set_num_threads", "goss_hip_system_get_num_threads", "goss_hip_system_num_nodes",
     "goss_hip_system_set_steps_per_launch", "goss_hip_system_synchronize", "goss_hip_system_stream",
     "goss_hip_system_get_solver_counters", "goss_hip_adjoint_buffer_sizes", "goss_hip_adjoint_record",
-    "goss_hip_adjoint_backward",
+    "goss_hip_adjoint_backward", "goss_hip_batch_create", "goss_hip_batch_destroy", "goss_hip_batch_set_config",
+    "goss_hip_batch_get_config", "goss_hip_batch_forward", "goss_hip_batch_autotune",
 )
 
 
@@ -78,6 +79,18 @@ class ModelInfo(ctypes.Structure):
         ("default_parameters", ctypes.POINTER(ctypes.c_double)),
         ("initial_conditions", ctypes.POINTER(ctypes.c_double)),
         ("linear_terms", ctypes.POINTER(ctypes.c_ubyte)),
+    ]
+
+
+BATCH_MAX_POPULATIONS = 16
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_uint32),
+        ("nodes_per_lane", ctypes.c_uint32),
+        ("span", ctypes.c_double),
+        ("order", ctypes.c_uint32 * BATCH_MAX_POPULATIONS),
     ]
 
 
@@ -134,6 +147,13 @@ def lib() -> ctypes.CDLL:
     L.goss_hip_adjoint_buffer_sizes.argtypes = [V, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p, u64p]
     L.goss_hip_adjoint_record.argtypes = [V, _dp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, V, V]
     L.goss_hip_adjoint_backward.argtypes = [V, _dp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, V, V, V, V, V]
+    L.goss_hip_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+    L.goss_hip_batch_destroy.argtypes = [V]
+    L.goss_hip_batch_set_config.argtypes = [V, ctypes.POINTER(BatchConfig)]
+    L.goss_hip_batch_get_config.argtypes = [V, ctypes.POINTER(BatchConfig)]
+    L.goss_hip_batch_forward.argtypes = [V, ctypes.c_double, ctypes.c_double, _dp]
+    L.goss_hip_batch_autotune.argtypes = [V, ctypes.c_double, ctypes.c_double, _dp, ctypes.POINTER(BatchConfig)]
     _lib = L
     return L
 

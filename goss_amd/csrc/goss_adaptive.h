@@ -34,10 +34,24 @@ struct AdaptiveParams {
 // View of one node's vector of precision R (element i at a[i * ld]).
 template <class R>
 struct RVec {
+  static constexpr bool in_registers = false;
   R* a;
   unsigned ld;
   GOSS_HD GOSS_INLINE R& operator[](int i) const { return a[size_t(i) * ld]; }
 };
+
+// A node's vector held in registers, for models small enough that the RKF32
+// vectors (y, k1, yn, ep: 4 S values) take a few registers; every index is a
+// compile-time constant after unrolling.
+template <class R, int S>
+struct RegVec {
+  static constexpr bool in_registers = true;
+  mutable R v[S];
+  GOSS_HD GOSS_INLINE R& operator[](int i) const { return v[i]; }
+};
+
+// Models with at most this many states keep the RKF32 vectors in registers.
+constexpr int kRkf32RegisterStates = 4;
 
 // Work vectors of RKF32 per node (each S rows of the work buffer).
 enum Rkf32Work : int { kRkK1 = 0, kRkYn, kRkEp, kNumRkf32Work };
@@ -46,11 +60,12 @@ template <class M, class R>
 struct Rkf32Node {
   static constexpr int S = M::num_states;
 
-  // RKF32::forward for one node.  y, k1, yn and ep are HBM views; only the
-  // input and output of one RHS evaluation are held in registers.  Returns
-  // false when the sub-step cap is hit.
-  template <class A>
-  GOSS_HD static bool forward(const RVec<R>& y, const RVec<R>& k1, const RVec<R>& yn, const RVec<R>& ep,
+  // RKF32::forward for one node.  y, k1, yn and ep are HBM views (RVec: only
+  // the input and output of one RHS evaluation are held in registers) or, for
+  // small models, register vectors (RegVec).  Returns false when the sub-step
+  // cap is hit.
+  template <class V, class A>
+  GOSS_HD static bool forward(const V& y, const V& k1, const V& yn, const V& ep,
                               const double t, const double interval, const A& acc, const AdaptiveParams& ap,
                               double& ldt_state, double& facmax, int& nfevals, int& ndtsa, int& ndtsr)
   {
@@ -146,10 +161,18 @@ struct Rkf32Node {
         ldt_state = dt;
       if (accepted) {
         ndtsa += 1;
+        if constexpr (V::in_registers) {
+#pragma unroll
+          for (int i = 0; i < S; ++i) {
+            y[i] = yn[i];
+            k1[i] = ep[i];
+          }
+        } else {
 #pragma unroll 1
-        for (int i = 0; i < S; ++i) {
-          y[i] = yn[i];
-          k1[i] = ep[i];
+          for (int i = 0; i < S; ++i) {
+            y[i] = yn[i];
+            k1[i] = ep[i];
+          }
         }
       } else {
         ndtsr += 1;
@@ -159,27 +182,25 @@ struct Rkf32Node {
   }
 };
 
-// The fused RKF32 kernel: `nouter` outer steps (t_k, interval_k).  States
-// stay in their SoA buffer; k1, yn and ep are rows of the work buffer.
+// The fused RKF32 step loop of one node: `nouter` outer steps (t_k,
+// interval_k).  For models of more than kRkf32RegisterStates states the states
+// stay in their SoA buffer and k1, yn and ep are rows of the work buffer;
+// smaller models hold all four vectors in registers.  The controller state is
+// read from and written back to the per-node rows of B.  Shared by rkf32_kernel (one population) and the
+// mixed-population batch kernel.
 template <class M, class R>
-__global__ void __launch_bounds__(kBlock)
-rkf32_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2* __restrict__ steps, const int nouter,
-             const SharedParamsOf<M, R> sp, const AdaptiveParams ap, const ImplicitBuffers B, const unsigned N,
-             const unsigned ld, const FieldOut fo)
+__device__ GOSS_INLINE void rkf32_node(const unsigned node, R* __restrict__ y, const R* __restrict__ fp,
+                                       const double2* __restrict__ steps, const int nouter,
+                                       const SharedParamsOf<M, R>& sp, const AdaptiveParams& ap,
+                                       const ImplicitBuffers& B, const unsigned N, const unsigned ld,
+                                       const FieldOut& fo)
 {
   constexpr int S = M::num_states;
   constexpr int NFP = M::num_field_parameters;
-  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= N) return;
   R fpv[NFP > 0 ? NFP : 1];
 #pragma unroll
   for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
   const DevAcc<M, R> acc{sp, fpv};
-  R* const work = reinterpret_cast<R*>(B.work);
-  const RVec<R> yv{y + node, ld};
-  const RVec<R> k1{work + size_t(kRkK1) * S * ld + node, ld};
-  const RVec<R> yn{work + size_t(kRkYn) * S * ld + node, ld};
-  const RVec<R> ep{work + size_t(kRkEp) * S * ld + node, ld};
   double ldt_state = B.ds[size_t(kRkLdt) * ld + node];
   double facmax = B.ds[size_t(kFacmax) * ld + node];
   int nfevals = B.is[size_t(kNfevals) * ld + node];
@@ -187,23 +208,40 @@ rkf32_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2* __restr
   int ndtsr = B.is[size_t(kRejects) * ld + node];
   int failed = B.is[size_t(kFailed) * ld + node];
 
+  auto run = [&](const auto& yv, const auto& k1, const auto& yn, const auto& ep) {
 #pragma unroll 1
-  for (int k = 0; k < nouter; ++k) {
-    if (!failed) {
-      if (!Rkf32Node<M, R>::forward(yv, k1, yn, ep, steps[k].x, steps[k].y, acc, ap, ldt_state, facmax, nfevals,
-                                    ndtsa, ndtsr)) {
-        failed = 1;
-        atomicOr(B.error, 1);
+    for (int k = 0; k < nouter; ++k) {
+      if (!failed) {
+        if (!Rkf32Node<M, R>::forward(yv, k1, yn, ep, steps[k].x, steps[k].y, acc, ap, ldt_state, facmax, nfevals,
+                                      ndtsa, ndtsr)) {
+          failed = 1;
+          atomicOr(B.error, 1);
+        }
       }
-    }
-    if (fo.out) {
-      constexpr int F = M::num_field_states;
+      if (fo.out) {
+        constexpr int F = M::num_field_states;
 #pragma unroll
-      for (int i = 0; i < F; ++i) {
-        const size_t idx = fo.tangled ? size_t(node) * F + i : size_t(N) * i + node;
-        fo.out[size_t(k) * fo.row_stride + idx] = double(yv[M::field_state_indices[i]]);
+        for (int i = 0; i < F; ++i) {
+          const size_t idx = fo.tangled ? size_t(node) * F + i : size_t(N) * i + node;
+          fo.out[size_t(k) * fo.row_stride + idx] = double(yv[M::field_state_indices[i]]);
+        }
       }
     }
+  };
+  if constexpr (S <= kRkf32RegisterStates) {
+    RegVec<R, S> yv, k1, yn, ep;
+#pragma unroll
+    for (int i = 0; i < S; ++i) yv[i] = y[size_t(i) * ld + node];
+    run(yv, k1, yn, ep);
+#pragma unroll
+    for (int i = 0; i < S; ++i) y[size_t(i) * ld + node] = yv[i];
+  } else {
+    R* const work = reinterpret_cast<R*>(B.work);
+    const RVec<R> yv{y + node, ld};
+    const RVec<R> k1{work + size_t(kRkK1) * S * ld + node, ld};
+    const RVec<R> yn{work + size_t(kRkYn) * S * ld + node, ld};
+    const RVec<R> ep{work + size_t(kRkEp) * S * ld + node, ld};
+    run(yv, k1, yn, ep);
   }
   B.ds[size_t(kRkLdt) * ld + node] = ldt_state;
   B.ds[size_t(kFacmax) * ld + node] = facmax;
@@ -211,6 +249,17 @@ rkf32_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2* __restr
   B.is[size_t(kAccepted) * ld + node] = ndtsa;
   B.is[size_t(kRejects) * ld + node] = ndtsr;
   B.is[size_t(kFailed) * ld + node] = failed;
+}
+
+template <class M, class R>
+__global__ void __launch_bounds__(kBlock)
+rkf32_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2* __restrict__ steps, const int nouter,
+             const SharedParamsOf<M, R> sp, const AdaptiveParams ap, const ImplicitBuffers B, const unsigned N,
+             const unsigned ld, const FieldOut fo)
+{
+  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= N) return;
+  rkf32_node<M, R>(node, y, fp, steps, nouter, sp, ap, B, N, ld, fo);
 }
 
 }  // namespace goss_hip

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <numeric>
 #include <cmath>
 #include <cstring>
 #include <sstream>
@@ -19,6 +21,7 @@
 
 #include "../../include/goss_hip.h"
 #include "generated/variant_list.h"
+#include "goss_batch.h"
 #include "goss_layout.h"
 #include "goss_registry.h"
 
@@ -961,6 +964,440 @@ int goss_hip_system_stream(goss_hip_system* s, void** stream)
   return guarded([&] {
     check_sys(s);
     *stream = s->stream;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Mixed-population batch (goss_batch.h).
+// ---------------------------------------------------------------------------
+struct goss_hip_batch {
+  std::vector<goss_hip_system*> sys;
+  std::vector<int> kind;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ready = nullptr;  // recorded on a system's stream before the batch's work
+  hipEvent_t done = nullptr;   // recorded on the batch stream after it
+  goss_hip_batch_config cfg{};
+  // Device blob: [npop BatchPop][SharedParams of each population], and its pinned staging.
+  char* d_blob = nullptr;
+  char* h_blob = nullptr;
+  size_t blob_cap = 0;
+  hipEvent_t blob_free = nullptr;
+  // Per population: (t, interval) pairs of the current call.
+  std::vector<double2*> d_steps, h_steps;
+  std::vector<size_t> steps_cap;
+  // Captured launches of the last call and the shape they were captured for.
+  hipGraphExec_t exec = nullptr;
+  std::vector<int64_t> exec_key;
+
+  void use_device() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
+
+  void drop_graph()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    exec_key.clear();
+  }
+
+  ~goss_hip_batch()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
+    if (d_blob) (void)hipFree(d_blob);
+    if (h_blob) (void)hipHostFree(h_blob);
+    for (auto p : d_steps) if (p) (void)hipFree(p);
+    for (auto p : h_steps) if (p) (void)hipHostFree(p);
+    if (blob_free) (void)hipEventDestroy(blob_free);
+    if (done) (void)hipEventDestroy(done);
+    if (ready) (void)hipEventDestroy(ready);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void check_batch(const goss_hip_batch* b)
+{
+  if (!b) throw Error(GOSS_HIP_EINVAL, "null batch");
+}
+
+void check_batch_config(const goss_hip_batch_config& c, size_t npop)
+{
+  bool seen[GOSS_HIP_BATCH_MAX_POPULATIONS] = {};
+  for (size_t i = 0; i < npop; ++i) {
+    if (c.order[i] >= npop || seen[c.order[i]])
+      throw Error(GOSS_HIP_EINVAL, "batch dispatch order must be a permutation of the populations");
+    seen[c.order[i]] = true;
+  }
+  if (c.block_size == 0 || c.block_size > uint32_t(kBlock) || c.block_size % 64 != 0)
+    throw Error(GOSS_HIP_EINVAL, "batch block size must be a multiple of 64 up to 256");
+  if (c.nodes_per_lane == 0 || c.nodes_per_lane > 64) throw Error(GOSS_HIP_EINVAL, "nodes per lane must be 1..64");
+  if (!(c.span > 0) || !std::isfinite(c.span)) throw Error(GOSS_HIP_EINVAL, "batch time span must be positive");
+}
+
+// Outer steps of each population for [t0, t1] with its step dt[p].
+std::vector<int64_t> batch_step_counts(const goss_hip_batch* b, double t0, double t1, const double* dt)
+{
+  if (!dt) throw Error(GOSS_HIP_EINVAL, "null step sizes");
+  if (!std::isfinite(t0) || !std::isfinite(t1) || !(t1 > t0)) throw Error(GOSS_HIP_EINVAL, "need t1 > t0");
+  std::vector<int64_t> n(b->sys.size());
+  for (size_t p = 0; p < n.size(); ++p) {
+    if (!(dt[p] > 0) || !std::isfinite(dt[p])) throw Error(GOSS_HIP_EINVAL, "step sizes must be positive");
+    const double q = (t1 - t0) / dt[p];
+    const double r = std::nearbyint(q);
+    if (r < 1 || std::fabs(q - r) > 1e-9 * std::max(1.0, r) || r > 1e9) {
+      std::ostringstream m;
+      m << "population " << p << ": (t1 - t0) / dt = " << q << " is not a whole number of steps";
+      throw Error(GOSS_HIP_EINVAL, m.str());
+    }
+    n[p] = int64_t(r);
+  }
+  return n;
+}
+
+// Number of launches: the span's share of [t0, t1], lowered to a common divisor of the step counts.
+int64_t batch_blocks(const std::vector<int64_t>& n, double t0, double t1, double span)
+{
+  int64_t g = 0;
+  for (int64_t k : n) g = std::gcd(g, k);
+  int64_t target = std::max<int64_t>(1, std::llround((t1 - t0) / span));
+  target = std::min<int64_t>(target, std::min<int64_t>(g, 100000));
+  for (int64_t d = target; d >= 1; --d)
+    if (g % d == 0) return d;
+  return 1;
+}
+
+// Advances every population over [t0, t1]: steps t_k = t0 + k dt[p], intervals t_k+1 - t_k
+// (as ODESystemSolver::solve forms them, ODESystemSolver.cpp:91-97), one fused launch per time block,
+// the launches of the call in one hipGraph.
+void batch_forward(goss_hip_batch* b, double t0, double t1, const double* dt)
+{
+  const size_t np = b->sys.size();
+  const std::vector<int64_t> n = batch_step_counts(b, t0, t1, dt);
+  const int64_t nblocks = batch_blocks(n, t0, t1, b->cfg.span);
+  const unsigned block = b->cfg.block_size;
+
+  // Steps, per population.
+  for (size_t p = 0; p < np; ++p) {
+    goss_hip_system* s = b->sys[p];
+    if (size_t(n[p]) > b->steps_cap[p]) {
+      b->drop_graph();
+      hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+      if (b->d_steps[p]) hip_check(hipFree(b->d_steps[p]), "hipFree");
+      if (b->h_steps[p]) hip_check(hipHostFree(b->h_steps[p]), "hipHostFree");
+      b->d_steps[p] = nullptr;
+      b->h_steps[p] = nullptr;
+      hip_check(hipMalloc(&b->d_steps[p], size_t(n[p]) * sizeof(double2)), "hipMalloc(steps)");
+      hip_check(hipHostMalloc(&b->h_steps[p], size_t(n[p]) * sizeof(double2), hipHostMallocDefault),
+                "hipHostMalloc(steps)");
+      b->steps_cap[p] = size_t(n[p]);
+    }
+  }
+  hip_check(hipEventSynchronize(b->blob_free), "hipEventSynchronize");
+  for (size_t p = 0; p < np; ++p) {
+    double2* h = b->h_steps[p];
+    double tk = t0;
+    for (int64_t k = 0; k < n[p]; ++k) {
+      const double tn = (k + 1 == n[p]) ? t1 : t0 + double(k + 1) * dt[p];
+      h[k] = make_double2(tk, tn - tk);
+      tk = tn;
+    }
+    b->sys[p]->check_steps(h, size_t(n[p]));
+  }
+
+  // Descriptors and parameters.
+  size_t params_bytes = 0;
+  int nk = 0;
+  const BatchKind* kinds = batch_kinds(&nk);
+  std::vector<size_t> off(np);
+  for (size_t p = 0; p < np; ++p) {
+    off[p] = params_bytes;
+    params_bytes += (kinds[b->kind[p]].sp_bytes + 255) & ~size_t(255);
+  }
+  const size_t pops_bytes = (np * sizeof(BatchPop) + 255) & ~size_t(255);
+  const size_t blob = pops_bytes + params_bytes;
+  if (blob > b->blob_cap) {
+    b->drop_graph();
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (b->d_blob) hip_check(hipFree(b->d_blob), "hipFree");
+    if (b->h_blob) hip_check(hipHostFree(b->h_blob), "hipHostFree");
+    b->d_blob = nullptr;
+    b->h_blob = nullptr;
+    hip_check(hipMalloc(&b->d_blob, blob), "hipMalloc(batch)");
+    hip_check(hipHostMalloc(&b->h_blob, blob, hipHostMallocDefault), "hipHostMalloc(batch)");
+    b->blob_cap = blob;
+  }
+  BatchPop* pops = reinterpret_cast<BatchPop*>(b->h_blob);
+  unsigned nchunks = 0;
+  for (size_t i = 0; i < np; ++i) {
+    const size_t p = b->cfg.order[i];  // descriptors in dispatch order (chunk ranges ascending)
+    goss_hip_system* s = b->sys[p];
+    BatchPop& P = pops[i];
+    std::memset(&P, 0, sizeof(P));
+    P.kind = b->kind[p];
+    P.nouter = int(n[p] / nblocks);
+    P.N = s->N;
+    P.ld = s->ld;
+    P.chunk0 = nchunks;
+    nchunks += (s->N + block - 1) / block;
+    P.y = s->y;
+    P.fp = s->fp;
+    P.steps = b->d_steps[p];
+    P.sp_off = off[p];
+    P.ldt_0 = s->sp.internal_time_step;
+    P.delta = s->sp.delta;
+    P.ap.atol = s->sp.atol;
+    P.ap.rtol = s->sp.rtol;
+    P.B = s->ib;
+    kinds[b->kind[p]].make_params(s->shared.data(), b->h_blob + pops_bytes + off[p]);
+  }
+  const unsigned grid = std::max(1u, (nchunks + b->cfg.nodes_per_lane - 1) / b->cfg.nodes_per_lane);
+
+  // Order after the systems' own pending work.
+  for (goss_hip_system* s : b->sys) {
+    hip_check(hipEventRecord(b->ready, s->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(b->stream, b->ready, 0), "hipStreamWaitEvent");
+  }
+  for (size_t p = 0; p < np; ++p)
+    hip_check(hipMemcpyAsync(b->d_steps[p], b->h_steps[p], size_t(n[p]) * sizeof(double2), hipMemcpyHostToDevice,
+                             b->stream),
+              "hipMemcpyAsync(steps)");
+  hip_check(hipMemcpyAsync(b->d_blob, b->h_blob, blob, hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync(batch)");
+  hip_check(hipEventRecord(b->blob_free, b->stream), "hipEventRecord");
+
+  const std::vector<int64_t> key = {nblocks, int64_t(grid), int64_t(block), int64_t(nchunks), int64_t(np),
+                                    int64_t(reinterpret_cast<uintptr_t>(b->d_blob))};
+  if (!b->exec || key != b->exec_key) {
+    b->drop_graph();
+    hipGraph_t graph = nullptr;
+    hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t e = hipSuccess;
+    for (int64_t tb = 0; tb < nblocks && e == hipSuccess; ++tb)
+      e = launch_batch(reinterpret_cast<const BatchPop*>(b->d_blob), int(np), nchunks, int(tb),
+                       b->d_blob + pops_bytes, grid, block, b->stream);
+    const hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+    hip_check(e, "batch kernel launch");
+    hip_check(ec, "hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    hip_check(ei, "hipGraphInstantiate");
+    b->exec_key = key;
+  }
+  hip_check(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+  hip_check(hipEventRecord(b->done, b->stream), "hipEventRecord");
+  for (goss_hip_system* s : b->sys) hip_check(hipStreamWaitEvent(s->stream, b->done, 0), "hipStreamWaitEvent");
+  for (goss_hip_system* s : b->sys)
+    if (s->solver == kRKF32 && s->failed()) throw Error(GOSS_HIP_ESOLVER, "RKF32 did not reach the end of the interval");
+}
+
+// Device copies of every buffer a forward call changes (states and per-node solver state).
+struct BatchSnapshot {
+  std::vector<std::pair<void*, void*>> saved;  // (buffer, copy)
+  std::vector<size_t> bytes;
+  hipStream_t stream = nullptr;
+
+  void add(void* buf, size_t n)
+  {
+    if (!buf || n == 0) return;
+    void* copy = nullptr;
+    hip_check(hipMalloc(&copy, n), "hipMalloc(snapshot)");
+    saved.emplace_back(buf, copy);
+    bytes.push_back(n);
+    hip_check(hipMemcpyAsync(copy, buf, n, hipMemcpyDeviceToDevice, stream), "hipMemcpyAsync(snapshot)");
+  }
+  void restore()
+  {
+    for (size_t i = 0; i < saved.size(); ++i)
+      hip_check(hipMemcpyAsync(saved[i].first, saved[i].second, bytes[i], hipMemcpyDeviceToDevice, stream),
+                "hipMemcpyAsync(restore)");
+  }
+  ~BatchSnapshot()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& s : saved) (void)hipFree(s.second);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int goss_hip_batch_create(goss_hip_system* const* systems, uint32_t n, goss_hip_batch** out)
+{
+  goss_hip_batch* b = nullptr;
+  const int rc = guarded([&] {
+    if (!out) throw Error(GOSS_HIP_EINVAL, "null output");
+    if (!systems || n == 0) throw Error(GOSS_HIP_EINVAL, "a batch needs at least one system");
+    if (n > uint32_t(kMaxBatchPops)) throw Error(GOSS_HIP_EINVAL, "too many populations in one batch");
+    int nk = 0;
+    const BatchKind* kinds = batch_kinds(&nk);
+    b = new goss_hip_batch;
+    for (uint32_t i = 0; i < n; ++i) {
+      goss_hip_system* s = systems[i];
+      check_sys(s);
+      for (uint32_t j = 0; j < i; ++j)
+        if (systems[j] == s) throw Error(GOSS_HIP_EINVAL, "a system appears twice in the batch");
+      if (i > 0 && s->device != systems[0]->device) throw Error(GOSS_HIP_EINVAL, "batch systems on different devices");
+      if (s->prec != kFP32) throw Error(GOSS_HIP_EUNSUPPORTED, "batch populations run in FP32");
+      int k = -1;
+      for (int j = 0; j < nk; ++j)
+        if (std::strcmp(kinds[j].variant, s->v->name) == 0 && kinds[j].solver == s->solver) k = j;
+      if (k < 0) {
+        std::ostringstream m;
+        m << "no fused batch kernel for model " << s->v->name << " with solver " << s->solver;
+        throw Error(GOSS_HIP_EUNSUPPORTED, m.str());
+      }
+      b->sys.push_back(s);
+      b->kind.push_back(k);
+    }
+    b->cfg.block_size = 256;
+    b->cfg.nodes_per_lane = 1;
+    b->cfg.span = 1.0;
+    for (uint32_t i = 0; i < n; ++i) b->cfg.order[i] = i;
+    b->device = systems[0]->device;
+    b->use_device();
+    b->d_steps.assign(n, nullptr);
+    b->h_steps.assign(n, nullptr);
+    b->steps_cap.assign(n, 0);
+    hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&b->blob_free, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(b->blob_free, b->stream), "hipEventRecord");
+  });
+  if (rc != GOSS_HIP_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return rc;
+}
+
+int goss_hip_batch_destroy(goss_hip_batch* b)
+{
+  return guarded([&] {
+    if (b) b->use_device();
+    delete b;
+  });
+}
+
+int goss_hip_batch_set_config(goss_hip_batch* b, const goss_hip_batch_config* c)
+{
+  return guarded([&] {
+    check_batch(b);
+    if (!c) throw Error(GOSS_HIP_EINVAL, "null config");
+    check_batch_config(*c, b->sys.size());
+    b->cfg = *c;
+  });
+}
+
+int goss_hip_batch_get_config(goss_hip_batch* b, goss_hip_batch_config* c)
+{
+  return guarded([&] {
+    check_batch(b);
+    if (!c) throw Error(GOSS_HIP_EINVAL, "null config");
+    *c = b->cfg;
+  });
+}
+
+int goss_hip_batch_forward(goss_hip_batch* b, double t0, double t1, const double* dt)
+{
+  return guarded([&] {
+    check_batch(b);
+    b->use_device();
+    batch_forward(b, t0, t1, dt);
+  });
+}
+
+int goss_hip_batch_autotune(goss_hip_batch* b, double t0, double t1, const double* dt, goss_hip_batch_config* best)
+{
+  return guarded([&] {
+    check_batch(b);
+    b->use_device();
+    const std::vector<int64_t> n = batch_step_counts(b, t0, t1, dt);
+    int64_t g = 0;
+    for (int64_t k : n) g = std::gcd(g, k);
+    const goss_hip_batch_config keep = b->cfg;
+    // Every candidate starts from the same states; the call leaves the states as it found them.
+    for (goss_hip_system* s : b->sys) hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    BatchSnapshot snap;
+    snap.stream = b->stream;
+    for (goss_hip_system* s : b->sys) {
+      snap.add(s->y, size_t(s->v->S) * s->ld * s->elem());
+      if (s->solver == kRKF32) {
+        snap.add(s->ib.work, size_t(kNumRkf32Work) * s->v->S * s->ld * sizeof(double));
+        snap.add(s->ib.ds, size_t(kNumDState) * s->ld * sizeof(double));
+        snap.add(s->ib.is, size_t(kNumIState) * s->ld * sizeof(int));
+        snap.add(s->ib.error, sizeof(int));
+      }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    auto trial = [&](const goss_hip_batch_config& c) {
+      b->cfg = c;
+      snap.restore();
+      batch_forward(b, t0, t1, dt);  // capture and warm up
+      snap.restore();
+      hip_check(hipEventRecord(e0, b->stream), "hipEventRecord");
+      batch_forward(b, t0, t1, dt);
+      hip_check(hipEventRecord(e1, b->stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      return double(ms);
+    };
+    goss_hip_batch_config best_cfg = keep;
+    try {
+      // Dispatch order first (at the current launch shape): the given order, and each
+      // population moved to the front.
+      const size_t np = b->sys.size();
+      double best_ms = trial(best_cfg);
+      for (size_t lead = 1; lead < np; ++lead) {
+        goss_hip_batch_config c = keep;
+        c.order[0] = keep.order[lead];
+        for (size_t i = 0, j = 1; i < np; ++i)
+          if (i != lead) c.order[j++] = keep.order[i];
+        const double ms = trial(c);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_cfg = c;
+        }
+      }
+      // Then the launch shape.
+      const goss_hip_batch_config ordered = best_cfg;
+      for (const uint32_t block : {64u, 128u, 256u}) {
+        for (const uint32_t npl : {1u, 2u, 4u}) {
+          for (const int64_t launches : {int64_t(1), int64_t(2), int64_t(5), int64_t(10), int64_t(25)}) {
+            if (g % launches != 0) continue;
+            goss_hip_batch_config c = ordered;
+            c.block_size = block;
+            c.nodes_per_lane = npl;
+            c.span = (t1 - t0) / double(launches);
+            const double ms = trial(c);
+            if (ms < best_ms) {
+              best_ms = ms;
+              best_cfg = c;
+            }
+          }
+        }
+      }
+    } catch (...) {
+      b->cfg = keep;
+      snap.restore();
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    snap.restore();
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->cfg = best_cfg;
+    b->drop_graph();
+    if (best) *best = best_cfg;
   });
 }
 

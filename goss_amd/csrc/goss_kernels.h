@@ -332,18 +332,19 @@ struct FieldOut {
 };
 
 // ---------------------------------------------------------------------------
-// The fused explicit kernel: `nouter` outer steps (t_k, interval_k).
+// The fused explicit step loop of one node: `nouter` outer steps
+// (t_k, interval_k), states in registers between the load and the store.
+// Shared by explicit_kernel (one population) and the mixed-population batch
+// kernel.
 // ---------------------------------------------------------------------------
 template <class M, int SOLVER, class R, bool COMP>
-__global__ void __launch_bounds__(kBlock)
-explicit_kernel(R* __restrict__ y, R* __restrict__ ylo, const R* __restrict__ fp, const double2* __restrict__ steps,
-                const int nouter, const SharedParamsOf<M, R> sp, const double ldt_0, const R delta,
-                const unsigned N, const unsigned ld, const FieldOut fo)
+__device__ GOSS_INLINE void explicit_node(const unsigned node, R* __restrict__ y, R* __restrict__ ylo,
+                                          const R* __restrict__ fp, const double2* __restrict__ steps,
+                                          const int nouter, const SharedParamsOf<M, R>& sp, const double ldt_0,
+                                          const R delta, const unsigned N, const unsigned ld, const FieldOut& fo)
 {
   constexpr int S = M::num_states;
   constexpr int NFP = M::num_field_parameters;
-  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= N) return;
 
   NodeState<M, R, COMP> st;
 #pragma unroll
@@ -392,6 +393,17 @@ explicit_kernel(R* __restrict__ y, R* __restrict__ ylo, const R* __restrict__ fp
       if (c >= 0) ylo[size_t(c) * ld + node] = st.lo[i];
     }
   }
+}
+
+template <class M, int SOLVER, class R, bool COMP>
+__global__ void __launch_bounds__(kBlock)
+explicit_kernel(R* __restrict__ y, R* __restrict__ ylo, const R* __restrict__ fp, const double2* __restrict__ steps,
+                const int nouter, const SharedParamsOf<M, R> sp, const double ldt_0, const R delta,
+                const unsigned N, const unsigned ld, const FieldOut fo)
+{
+  const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= N) return;
+  explicit_node<M, SOLVER, R, COMP>(node, y, ylo, fp, steps, nouter, sp, ldt_0, delta, N, ld, fo);
 }
 
 }  // namespace goss_hip

@@ -4,6 +4,7 @@
 //   -DGOSS_VARIANT=<name> and -DGOSS_PREC_F64 (FP64, -ffp-contract=off)
 //                          or -DGOSS_PREC_F32 (FP32 and FP32C).
 // Exports goss_register_<name>_<family>(VariantInfo&), called by the registry.
+#include "goss_model_wrap.h"
 #include "goss_registry.h"
 
 #define GOSS_STR2(x) #x
@@ -19,66 +20,12 @@ namespace {
 namespace NS = goss_models::GOSS_VARIANT;
 
 // The generated namespace wrapped as a type for the kernel templates.
-struct M {
-  static constexpr int num_states = NS::num_states;
-  static constexpr int num_parameters = NS::num_parameters;
-  static constexpr int num_constants = NS::num_constants;
-  static constexpr int num_field_states = NS::num_field_states;
-  static constexpr int num_field_parameters = NS::num_field_parameters;
-  static constexpr const int* field_state_indices = NS::field_state_indices;
-  static constexpr const int* field_parameter_indices = NS::field_parameter_indices;
-  static constexpr const unsigned char* linear_terms = NS::linear_terms;
-
-  template <typename R, typename A, typename T>
-  GOSS_HD GOSS_INLINE static void eval(const R* y, const T t, const A& a, R* out)
-  {
-    NS::eval<R>(y, t, a, out);
-  }
-  template <typename R, typename A, typename T>
-  GOSS_HD GOSS_INLINE static void linearized_eval(const R* y, const T t, const A& a, R* lin, R* rhs, bool ol)
-  {
-    NS::linearized_eval<R>(y, t, a, lin, rhs, ol);
-  }
-  static constexpr int grl1_vjp_tape_slots = NS::grl1_vjp_tape_slots;
-  template <typename R, typename A, typename T>
-  GOSS_HD GOSS_INLINE static void grl1_vjp(const R* y, const T t, const A& a, const R h, const R delta, const R* lam,
-                                           R* lam_prev, R* fp_bar)
-  {
-    NS::grl1_vjp<R>(y, t, a, h, delta, lam, lam_prev, fp_bar);
-  }
-  template <typename R, typename A, typename T, typename TP>
-  GOSS_HD GOSS_INLINE static void grl1_vjp_tape(const R* y, const T t, const A& a, const R h, const R delta,
-                                                const R* lam, R* lam_prev, R* fp_bar, const TP& tape)
-  {
-    NS::grl1_vjp_tape<R>(y, t, a, h, delta, lam, lam_prev, fp_bar, tape);
-  }
-};
-
-// Host-side accessor for compute_constants (all parameters in FP64).
-struct HostAcc {
-  const double* v;
-  template <int k> double getd() const { return v[k]; }
-};
-
-template <class R>
-SharedParamsOf<M, R> make_shared_params(const double* shared)
-{
-  SharedParamsOf<M, R> sp;
-  double c[M::num_constants > 0 ? M::num_constants : 1];
-  NS::compute_constants<double>(HostAcc{shared}, c);
-  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = R(shared[k]);
-  for (int k = 0; k < M::num_constants; ++k) sp.c[k] = R(c[k]);
-  if constexpr (sizeof(R) == 4) {
-    for (int k = 0; k < M::num_parameters; ++k) sp.vd[k] = shared[k];
-    for (int k = 0; k < M::num_constants; ++k) sp.cd[k] = c[k];
-  }
-  return sp;
-}
+GOSS_MODEL_TYPE(M, NS);
 
 template <int SOLVER, class R, bool COMP>
 hipError_t launch_explicit(const ExplicitArgs& a)
 {
-  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
+  const SharedParamsOf<M, R> sp = make_shared_params<M, R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((explicit_kernel<M, SOLVER, R, COMP>), grid, dim3(kBlock), 0, a.stream,
                      static_cast<R*>(a.y), static_cast<R*>(a.ylo), static_cast<const R*>(a.fp), a.steps, a.nouter,
@@ -90,7 +37,7 @@ hipError_t launch_explicit(const ExplicitArgs& a)
 template <int SOLVER>
 hipError_t launch_implicit(const ImplicitArgs& a)
 {
-  const SharedParamsOf<M, double> sp = make_shared_params<double>(a.shared);
+  const SharedParamsOf<M, double> sp = make_shared_params<M, double>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((implicit_kernel<M, SOLVER>), grid, dim3(kBlock), 0, a.stream, a.y, a.fp, a.steps, a.nouter, sp,
                      a.ip, a.B, a.N, a.ld, a.fo);
@@ -101,7 +48,7 @@ hipError_t launch_implicit(const ImplicitArgs& a)
 template <class R>
 hipError_t launch_rkf32(const AdaptiveArgs& a)
 {
-  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
+  const SharedParamsOf<M, R> sp = make_shared_params<M, R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((rkf32_kernel<M, R>), grid, dim3(kBlock), 0, a.stream, static_cast<R*>(a.y),
                      static_cast<const R*>(a.fp), a.steps, a.nouter, sp, a.ap, a.B, a.N, a.ld, a.fo);
@@ -111,7 +58,7 @@ hipError_t launch_rkf32(const AdaptiveArgs& a)
 template <class R>
 hipError_t launch_adjoint_record(const AdjointArgs& a)
 {
-  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
+  const SharedParamsOf<M, R> sp = make_shared_params<M, R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((adjoint_record_kernel<M, R>), grid, dim3(kBlock), 0, a.stream, static_cast<R*>(a.y),
                      static_cast<const R*>(a.fp), a.steps, a.step0, a.nouter, sp, R(a.delta), a.N, a.ld, a.shape,
@@ -122,7 +69,7 @@ hipError_t launch_adjoint_record(const AdjointArgs& a)
 template <class R>
 hipError_t launch_adjoint_backward(const AdjointArgs& a)
 {
-  const SharedParamsOf<M, R> sp = make_shared_params<R>(a.shared);
+  const SharedParamsOf<M, R> sp = make_shared_params<M, R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
   const int nseg = (a.shape.n + a.shape.ckpt_every - 1) / a.shape.ckpt_every;
   for (int s = nseg - 1; s >= 0; --s) {

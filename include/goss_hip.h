@@ -201,6 +201,44 @@ int goss_hip_adjoint_record(goss_hip_system* sys, const double* t, uint64_t n, u
 int goss_hip_adjoint_backward(goss_hip_system* sys, const double* t, uint64_t n, uint32_t sample_every,
                               uint32_t checkpoint_every, const void* checkpoints_dev, const double* grad_samples_dev,
                               void* segment_dev, double* grad_states_dev, double* grad_field_parameters_dev);
+/* Mixed-population batch.  The reference advances one population per
+ * ODESystemSolver::forward call (ODESystemSolver.cpp:67-79); a batch advances
+ * several systems -- different cell models and solvers, uneven sizes --
+ * together, with one fused persistent kernel launch per block of time steps
+ * and the launches of one call captured in a hipGraph.  Each node runs its own
+ * system's per-node code, so results equal those of each system's own
+ * goss_hip_system_forward_times over the same times.  Populations must be
+ * FP32 and of a (model, solver) pair compiled into the fused kernel (TP06 M
+ * and epi with GRL1, FitzHugh-Nagumo with RK4, Lorenz with RKF32); others
+ * give GOSS_HIP_EUNSUPPORTED.  The batch references the systems, which must
+ * outlive it.
+ *
+ * goss_hip_batch_forward advances population p from t0 to t1 in outer steps
+ * of dt[p] (t_k = t0 + k dt[p], intervals t_k+1 - t_k; (t1 - t0) / dt[p] must
+ * be a whole number).  The config's span is the simulated time per launch
+ * (lowered to a common divisor of the step counts); block_size is the
+ * threads per block (64..256, a multiple of 64) and nodes_per_lane the
+ * number of block_size-node chunks each block takes per launch.
+ * goss_hip_batch_autotune times candidate configs (dispatch orders, then
+ * launch shapes) on [t0, t1] and keeps the fastest; the systems' states are
+ * restored afterwards. */
+typedef struct goss_hip_batch goss_hip_batch;
+#define GOSS_HIP_BATCH_MAX_POPULATIONS 16
+typedef struct goss_hip_batch_config {
+  uint32_t block_size;
+  uint32_t nodes_per_lane;
+  double span;
+  /* Dispatch order: population order[i] takes the i-th range of chunks (the first
+   * range starts first; a population of long per-node latency belongs there). */
+  uint32_t order[GOSS_HIP_BATCH_MAX_POPULATIONS];
+} goss_hip_batch_config;
+int goss_hip_batch_create(goss_hip_system* const* systems, uint32_t n, goss_hip_batch** out);
+int goss_hip_batch_destroy(goss_hip_batch* batch);
+int goss_hip_batch_set_config(goss_hip_batch* batch, const goss_hip_batch_config* config);
+int goss_hip_batch_get_config(goss_hip_batch* batch, goss_hip_batch_config* config);
+int goss_hip_batch_forward(goss_hip_batch* batch, double t0, double t1, const double* dt);
+int goss_hip_batch_autotune(goss_hip_batch* batch, double t0, double t1, const double* dt,
+                            goss_hip_batch_config* best);
 /* Wait for all work of the system. */
 int goss_hip_system_synchronize(goss_hip_system* sys);
 /* The system's HIP stream (hipStream_t) for callers that order their own work. */
