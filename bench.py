@@ -235,6 +235,46 @@ def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every
     }
 
 
+def implicit_run(N=262144, steps=150, dt=0.1, seed=2027):
+    """Config-3 shape: TP06 epi with the .ode's default parameters (stim_amplitude 0: the cells stay
+    near rest), ImplicitEuler (relative_tolerance 1e-6) and ESDIRK23a (its fixed tol 1e-5), N nodes,
+    dt 0.1 ms, initial states x (1 + U(-0.01, 0.01)), `steps` steps from t = 0.  Each solver runs with
+    the reference's FP64 forward-difference Jacobian and with the FP32 forward-difference Jacobian
+    (Newton in FP64 in both); wall time around the call and a synchronize, after a one-step warm-up on
+    the same system."""
+    from goss_amd.system import DeviceSystem, ModelVariant
+
+    mv = ModelVariant("tp06_epi")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y0 = mv.initial_conditions[None, :] * (1.0 + rng.uniform(-0.01, 0.01, (N, mv.num_states)))
+    t = np.arange(steps + 1) * dt
+    out = {"workload": f"TP06 epi (default parameters, no stimulus), {N} nodes, dt {dt} ms, {steps} steps "
+                       f"(0-{steps * dt:g} ms)"}
+    for solver in ("ImplicitEuler", "ESDIRK23a"):
+        base = {"relative_tolerance": 1e-6} if solver == "ImplicitEuler" else {}
+        finals = {}
+        for jac in ("fd", "fd_fp32"):
+            s = DeviceSystem("tp06_epi", solver, N, "fp64", {**base, "jacobian": jac})
+            s.set_steps_per_launch(50)
+            s.set_states(y0)
+            s.forward_times(t[:2])  # warm-up
+            s.synchronize()
+            s.set_states(y0)
+            s.synchronize()
+            w = time.perf_counter()
+            s.forward_times(t)
+            s.synchronize()
+            el = time.perf_counter() - w
+            finals[jac] = s.states()
+            out[f"{solver}_{jac}"] = {"s": el, "node_steps_per_s": N * steps / el,
+                                      "counters": [int(c) for c in s.solver_counters()]}
+            s.close()
+        a, b = finals["fd_fp32"], finals["fd"]
+        out[f"{solver}_fd_fp32"]["speedup_vs_fd"] = out[f"{solver}_fd"]["s"] / out[f"{solver}_fd_fp32"]["s"]
+        out[f"{solver}_fd_fp32"]["max_abs_dV_vs_fd_mV"] = float(np.abs(a[:, 17] - b[:, 17]).max())
+    return out
+
+
 MIXED_POPS = [("tp06_m", "GRL1", 300017, 0.02), ("tp06_epi", "GRL1", 211003, 0.02),
               ("fitzhughnagumo", "RK4", 700001, 0.05), ("lorentz", "RKF32", 9973, 0.02)]
 
@@ -329,6 +369,9 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the adjoint workload and print its figures as one JSON line")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-population batch workload")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the implicit-solver workload")
+    ap.add_argument("--implicit-only", action="store_true",
+                    help="only run the implicit-solver workload and print its figures as one JSON line")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only run the mixed-population batch workload and print its figures as one JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -351,6 +394,9 @@ def main():
         return 0
     if args.mixed_only:
         print(json.dumps(mixed_run()), flush=True)
+        return 0
+    if args.implicit_only:
+        print(json.dumps(implicit_run()), flush=True)
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -446,6 +492,9 @@ def main():
         out["adjoint_over_forward"] = adj["adjoint_over_forward"]
         out["adjoint"] = adj
         print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
+    if not args.no_implicit:
+        out["implicit"] = implicit_run()
+        print(f"[bench] implicit {out['implicit']}", file=sys.stderr, flush=True)
     if not args.no_mixed:
         out["mixed"] = mixed_run()
         print(f"[bench] mixed {out['mixed']}", file=sys.stderr, flush=True)

@@ -26,6 +26,7 @@ SOLVERS = {
     "RKF32": 9,
 }
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp32c": 2}
+JACOBIANS = {"fd": 0, "fd_fp32": 1}
 
 # Functions whose C signatures this module binds (checked by the tests against
 # the exported symbols).
@@ -61,6 +62,7 @@ class SolverParams(ctypes.Structure):
         ("max_iterations", ctypes.c_int),
         ("always_recompute_jacobian", ctypes.c_int),
         ("num_refinements_without_always_recomputing_jacobian", ctypes.c_int),
+        ("jacobian", ctypes.c_int),
     ]
 
 

@@ -451,9 +451,10 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         jstmts = select(hjac_stmts, jroots)
         jused = g.used_main(jinter + jstmts, [e for row in hjac for e in row])
         L.append("// Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].")
-        L.append("template <typename real, typename PA, typename treal>")
+        L.append("// `jac` is a pointer or any object whose jac[k] = value stores entry k.")
+        L.append("template <typename real, typename PA, typename treal, typename JAC>")
         L.append("GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,")
-        L.append("                                          const PA& pa, real* __restrict__ jac)")
+        L.append("                                          const PA& pa, JAC jac)")
         L.append("{")
         L.append(f"{ind}(void)time; (void)pa;")
         L += g.bind(jused, ind)

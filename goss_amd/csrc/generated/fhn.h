@@ -221,9 +221,10 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
 }
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
-template <typename real, typename PA, typename treal>
+// `jac` is a pointer or any object whose jac[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JAC>
 GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,
-                                          const PA& pa, real* __restrict__ jac)
+                                          const PA& pa, JAC jac)
 {
   (void)time; (void)pa;
   const real v = states[0];

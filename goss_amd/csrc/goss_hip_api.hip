@@ -126,6 +126,7 @@ void default_params(int solver, goss_hip_solver_params* p)
   p->min_dt = solver == kESDIRK23a ? 0.001 : 0.0001;           // ESDIRK23a.h:38, ImplicitEuler.h:37
   p->atol = 1e-5;  // AdaptiveExplicitSolver.cpp:52-53
   p->rtol = 1e-8;
+  p->jacobian = GOSS_HIP_JACOBIAN_FD;
 }
 
 }  // namespace
@@ -265,7 +266,7 @@ struct goss_hip_system {
   // with the reference's message (ImplicitEuler.cpp:132-141).
   void launch_implicit(const double2* d, size_t n, double* field_out, int tangled)
   {
-    const ImplicitLaunch launch = v->implicit_launch[solver == kImplicitEuler ? 0 : 1];
+    const ImplicitLaunch launch = v->implicit_launch[solver == kImplicitEuler ? 0 : 1][sp.jacobian];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
     for (size_t off = 0; off < n; off += steps_per_launch) {
       const int cnt = int(std::min<size_t>(steps_per_launch, n - off));
@@ -472,7 +473,9 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
         throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not compiled for this model");
     } else if (is_implicit(solver)) {
       if (precision != kFP64) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solvers run in FP64 only");
-      if (!v->implicit_launch[solver == kImplicitEuler ? 0 : 1])
+      const int jac = params ? params->jacobian : GOSS_HIP_JACOBIAN_FD;
+      if (jac < 0 || jac >= kNumJacobianModes) throw Error(GOSS_HIP_EINVAL, "unknown Jacobian mode");
+      if (!v->implicit_launch[solver == kImplicitEuler ? 0 : 1][jac])
         throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
     } else if (solver == kRKF32) {
       if (!v->rkf32_launch[precision]) throw Error(GOSS_HIP_EUNSUPPORTED, "RKF32 runs in FP64 and FP32 only");

@@ -18,12 +18,51 @@
 // first ESDIRK stage, Jacobian columns).
 //
 // The arithmetic is FP64 throughout and follows the reference's operation
-// order (compiled with -ffp-contract=off).
+// order (compiled with -ffp-contract=off).  Optionally (JacAcc below) the
+// forward-difference Jacobian differences the model's FP32 right-hand side
+// instead; the Newton iteration, its residual, the LU and the substitutions
+// stay FP64 in every mode.
 #pragma once
+
+#include <type_traits>
 
 #include "goss_kernels.h"
 
 namespace goss_hip {
+
+// How a node's Jacobian is formed.
+enum JacobianMode : int {
+  kJacobianFD = 0,      // ODE::compute_jacobian (ODE.cpp:47-67): FP64 forward differences
+  kJacobianFDFP32 = 1,  // forward differences of the FP32 right-hand side, stored as FP64
+  kNumJacobianModes = 2
+};
+
+// Accessor of the implicit kernels in kJacobianFDFP32 mode: forwards the
+// parameter reads of the FP64 model code to `acc` and carries the FP32
+// accessor for the Jacobian's evaluations.
+template <class M, int JAC>
+struct JacAcc {
+  static constexpr int jacobian_mode = JAC;
+  const DevAcc<M, double>& acc;
+  const DevAcc<M, float>& accf;
+  template <int k>
+  GOSS_HD GOSS_INLINE double get() const { return acc.template get<k>(); }
+  template <int k>
+  GOSS_HD GOSS_INLINE double getd() const { return acc.template getd<k>(); }
+  template <int k>
+  GOSS_HD GOSS_INLINE double cst() const { return acc.template cst<k>(); }
+  template <int k>
+  GOSS_HD GOSS_INLINE double cstd() const { return acc.template cstd<k>(); }
+};
+
+template <class A, class = void>
+struct JacobianModeOf {
+  static constexpr int value = kJacobianFD;
+};
+template <class A>
+struct JacobianModeOf<A, std::void_t<decltype(A::jacobian_mode)>> {
+  static constexpr int value = A::jacobian_mode;
+};
 
 constexpr double kGossEps = 3.0e-16;  // constants.h:22
 
@@ -148,6 +187,52 @@ struct Implicit {
         for (int j = 0; j < S; ++j) J(j, i) = (f[j] - f1[j]) / delta;
       }
     }
+  }
+
+  // Forward differences of the FP32 right-hand side.  Column i perturbs state i
+  // by sqrt(eps_FP32) * max(1e-5, |y_i|) (the reference's floor, with the FP32
+  // epsilon in place of its 1e-15) and divides by the perturbation as it is
+  // represented in FP32.  The base evaluation, also FP32, is kept in f1.
+  template <class A>
+  GOSS_HD static void fd_jacobian_f32(const VecView& y, const double t, const A& acc, const VecView& f1,
+                                      const MatView& J)
+  {
+    constexpr float kSqrtEps = 3.4526698e-4f;  // sqrt(2^-23)
+#pragma unroll 1
+    for (int i = -1; i < S; ++i) {
+      float yp[S], f[S];
+      float dy = 0.0f;
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const float yj = float(y[j]);
+        if (j == i) {
+          const float mx = 1e-5f > fabsf(yj) ? 1e-5f : fabsf(yj);
+          const float yn = yj + kSqrtEps * mx;
+          dy = yn - yj;
+          yp[j] = yn;
+        } else {
+          yp[j] = yj;
+        }
+      }
+      M::template eval<float>(yp, t, acc.accf, f);
+      if (i < 0) {
+#pragma unroll
+        for (int j = 0; j < S; ++j) f1[j] = double(f[j]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < S; ++j) J(j, i) = (double(f[j]) - f1[j]) / double(dy);
+      }
+    }
+  }
+
+  // The Jacobian at y into J, as the accessor's mode says (f1 is overwritten).
+  template <class A>
+  GOSS_HD static void jacobian(const VecView& y, const double t, const A& acc, const VecView& f1, const MatView& J)
+  {
+    if constexpr (JacobianModeOf<A>::value == kJacobianFDFP32)
+      fd_jacobian_f32(y, t, acc, f1, J);
+    else
+      fd_jacobian(y, t, acc, f1, J);
   }
 
   // dst <- scale * src + I, then in-place LU (ODE.cpp:69-100); scale is
@@ -299,7 +384,7 @@ struct ImplicitEulerNode {
       }
       if (relative_residual < p.relative_tolerance) break;
       if (st.recompute || always) {
-        I::fd_jacobian(yzv, t, acc, f1, v.J);
+        I::jacobian(yzv, t, acc, f1, v.J);
         I::scale_add_identity(v.J, v.J, -dt * 1.0);
         I::lu_factorize(v.J);
         st.jac_comp += 1;
@@ -420,7 +505,7 @@ struct EsdirkNode {
   template <class A>
   GOSS_HD static void ode_jacobian(const NodeViews& v, const double t, const A& acc, NodeImplicitState& st)
   {
-    I::fd_jacobian(v.y, t, acc, v.w(kEsF1), v.J);
+    I::jacobian(v.y, t, acc, v.w(kEsF1), v.J);
     st.jac_comp++;
     st.new_jacobian = 1;
   }
@@ -600,21 +685,36 @@ GOSS_HD GOSS_INLINE NodeViews node_views(double* y, const ImplicitBuffers& B, in
   return v;
 }
 
-// The fused implicit kernel: `nouter` outer steps (t_k, interval_k).
-template <class M, int SOLVER>
+// The fused implicit kernel: `nouter` outer steps (t_k, interval_k).  JAC
+// selects the Jacobian (JacobianMode); spf, the FP32 shared parameters, is
+// read only by the FP32 Jacobian evaluations.
+template <class M, int SOLVER, int JAC>
 __global__ void __launch_bounds__(kBlock)
 implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const double2* __restrict__ steps,
-                const int nouter, const SharedParamsOf<M, double> sp, const ImplicitParams ip,
-                const ImplicitBuffers B, const unsigned N, const unsigned ld, const FieldOut fo)
+                const int nouter, const SharedParamsOf<M, double> sp, const SharedParamsOf<M, float> spf,
+                const ImplicitParams ip, const ImplicitBuffers B, const unsigned N, const unsigned ld,
+                const FieldOut fo)
 {
   constexpr int S = M::num_states;
   constexpr int NFP = M::num_field_parameters;
   const unsigned node = blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= N) return;
   double fpv[NFP > 0 ? NFP : 1];
+  float fpvf[NFP > 0 ? NFP : 1];
 #pragma unroll
-  for (int j = 0; j < NFP; ++j) fpv[j] = fp[size_t(j) * ld + node];
-  const DevAcc<M, double> acc{sp, fpv};
+  for (int j = 0; j < NFP; ++j) {
+    fpv[j] = fp[size_t(j) * ld + node];
+    fpvf[j] = float(fpv[j]);
+  }
+  const DevAcc<M, double> acc64{sp, fpv};
+  const DevAcc<M, float> accf{spf, fpvf};
+  const JacAcc<M, JAC> jacc{acc64, accf};
+  using Acc = std::conditional_t<JAC == kJacobianFD, DevAcc<M, double>, JacAcc<M, JAC>>;
+  const Acc& acc = [&]() -> const Acc& {
+    if constexpr (JAC == kJacobianFD) return acc64;
+    else return jacc;
+  }();
+  (void)jacc;
   NodeImplicitState st = load_state(B, ld, node);
   const NodeViews v = node_views(y, B, S, ld, node);
 

@@ -31,6 +31,11 @@
     {                                                                                                                \
       NSN::linearized_eval<R>(y, t, a, lin, rhs, ol);                                                                \
     }                                                                                                                \
+    template <typename R, typename A, typename T, typename J>                                                        \
+    GOSS_HD GOSS_INLINE static void compute_jacobian(const R* y, const T t, const A& a, J jac)                       \
+    {                                                                                                                \
+      NSN::compute_jacobian<R>(y, t, a, jac);                                                                        \
+    }                                                                                                                \
     static constexpr int grl1_vjp_tape_slots = NSN::grl1_vjp_tape_slots;                                             \
     template <typename R, typename A, typename T>                                                                    \
     GOSS_HD GOSS_INLINE static void grl1_vjp(const R* y, const T t, const A& a, const R h, const R delta,            \

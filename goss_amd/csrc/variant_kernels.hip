@@ -34,13 +34,14 @@ hipError_t launch_explicit(const ExplicitArgs& a)
 }
 
 #if defined(GOSS_PREC_F64)
-template <int SOLVER>
+template <int SOLVER, int JAC>
 hipError_t launch_implicit(const ImplicitArgs& a)
 {
   const SharedParamsOf<M, double> sp = make_shared_params<M, double>(a.shared);
+  const SharedParamsOf<M, float> spf = make_shared_params<M, float>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((implicit_kernel<M, SOLVER>), grid, dim3(kBlock), 0, a.stream, a.y, a.fp, a.steps, a.nouter, sp,
-                     a.ip, a.B, a.N, a.ld, a.fo);
+  hipLaunchKernelGGL((implicit_kernel<M, SOLVER, JAC>), grid, dim3(kBlock), 0, a.stream, a.y, a.fp, a.steps,
+                     a.nouter, sp, spf, a.ip, a.B, a.N, a.ld, a.fo);
   return hipGetLastError();
 }
 #endif
@@ -125,8 +126,10 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
 {
   fill_info(v);
   register_family<double, false>(v, kFP64);
-  v.implicit_launch[0] = &launch_implicit<kImplicitEuler>;
-  v.implicit_launch[1] = &launch_implicit<kESDIRK23a>;
+  v.implicit_launch[0][kJacobianFD] = &launch_implicit<kImplicitEuler, kJacobianFD>;
+  v.implicit_launch[0][kJacobianFDFP32] = &launch_implicit<kImplicitEuler, kJacobianFDFP32>;
+  v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
+  v.implicit_launch[1][kJacobianFDFP32] = &launch_implicit<kESDIRK23a, kJacobianFDFP32>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
   v.adjoint_record[kFP64] = &launch_adjoint_record<double>;
   v.adjoint_backward[kFP64] = &launch_adjoint_backward<double>;

@@ -6,7 +6,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from ._lib import PRECISIONS, SOLVERS, GossHipError, ModelInfo, SolverParams, check, lib
+from ._lib import JACOBIANS, PRECISIONS, SOLVERS, GossHipError, ModelInfo, SolverParams, check, lib
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -55,6 +55,8 @@ def solver_params(solver: str, **overrides) -> SolverParams:
             continue
         if not hasattr(sp, k):
             raise KeyError(f"unknown solver parameter {k!r}")
+        if k == "jacobian" and isinstance(v, str):
+            v = JACOBIANS[v]  # "fd" or "fd_fp32"
         setattr(sp, k, type(getattr(sp, k))(v))
     return sp
 

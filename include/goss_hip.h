@@ -76,7 +76,15 @@ typedef struct goss_hip_solver_params {
   int max_iterations;
   int always_recompute_jacobian;
   int num_refinements_without_always_recomputing_jacobian;
+  /* ImplicitEuler / ESDIRK23a: how the Jacobian is formed.  FD is the
+   * reference's (ODE.cpp:47-67); FD_FP32 has no counterpart in the reference:
+   * forward differences of the model's FP32 right-hand side.  The Newton
+   * iteration, its residual, the LU and the substitutions are FP64 in both. */
+  int jacobian;
 } goss_hip_solver_params;
+
+#define GOSS_HIP_JACOBIAN_FD 0      /* forward differences of the FP64 right-hand side */
+#define GOSS_HIP_JACOBIAN_FD_FP32 1 /* forward differences of the FP32 right-hand side */
 
 /* Sizes and names of a generated model variant
  * (ParameterizedODE.h:56-156 getters). */

@@ -117,3 +117,76 @@ def test_implicit_euler_failure_reports_reference_message():
         dev.forward(0.0, 0.1)
     assert str(gpu_err.value) == str(cpu_err.value)
     assert "Newtons solver failed to converge as dt become smaller than 'min_dt' 0.5" in str(gpu_err.value)
+
+
+def _epi_run(kind, solver, params, y0, t, jacobian="fd"):
+    """TP06 epi with the M-cell's 52 pA/pF stimulus (the epi .ode ships with
+    stim_amplitude 0), so that the window holds an upstroke."""
+    if kind == "oracle":
+        s = oracle.OracleSystem("tp06_epi", solver, y0.shape[0], params, per_node_state=True)
+        s.set_num_threads(8)
+    else:
+        s = dev_system("tp06_epi", solver, y0.shape[0], "fp64", {**params, "jacobian": jacobian})
+    s.set_parameter("stim_amplitude", 52.0)
+    s.set_states(y0)
+    s.forward_times(t)
+    return s.states(), (s.counters() if kind == "oracle" else s.solver_counters())
+
+
+def _diff(a, b):
+    floor = 1e-6 * np.max(np.abs(b), axis=0, keepdims=True)
+    return float(np.abs(a[:, 17] - b[:, 17]).max()), float((np.abs(a - b) / np.maximum(np.abs(b), floor)).max())
+
+
+@pytest.mark.parametrize("solver,rtol", [("ImplicitEuler", 1e-6), ("ImplicitEuler", None), ("ESDIRK23a", None)])
+def test_fp32_jacobian_config3_precision(solver, rtol):
+    """Config 3 precision pattern: the Jacobian from forward differences of the
+    FP32 right-hand side, the Newton iteration (residual, LU, substitutions)
+    in FP64.  TP06 epi, dt 0.1 ms, 30 ms through the stimulus and upstroke,
+    against the FP64 oracle with the reference's Jacobian; ImplicitEuler at
+    the config's relative_tolerance 1e-6 and at the default (ESDIRK23a's
+    Newton tolerance is fixed, ESDIRK23a.cpp:34,189).
+
+    A different Jacobian changes the Newton iterates within what the stopping
+    rule allows, and with them the solvers' discrete decisions: where
+    ImplicitEuler's Newton iteration is declared divergent and the step is
+    halved (ImplicitEuler.cpp:108-150).  A different halving pattern is a
+    different, equally valid discretisation, so ImplicitEuler is bounded by
+    its own discretisation error: the FP32-Jacobian result must lie within
+    the distance between the oracle at dt 0.1 and at dt 0.01, in every state
+    (measured: 0.083 mV in V against 0.21 mV, worst state ratio 0.79; a
+    one-ulp change of y0 flips no halving decision here, a different
+    Jacobian flips a few).  ESDIRK23a controls its local error (tol 1e-5): within
+    1e-3 mV in V and 1e-3 relative in every state (floor 1e-6 of the state's
+    maximum).  The figures, and the oracle's own one-ulp spread, are printed."""
+    N = 256
+    y0 = perturbed(oracle.OracleSystem("tp06_epi", solver, N, {}, per_node_state=True).states(), 47)
+    t = np.arange(301) * 0.1
+    params = {} if rtol is None else {"relative_tolerance": rtol}
+    ref, oc = _epi_run("oracle", solver, params, y0, t)
+    got, dc = _epi_run("device", solver, params, y0, t, "fd_fp32")
+    ulp, _ = _epi_run("oracle", solver, params, np.nextafter(y0, np.inf), t)
+    dv, rel = _diff(got, ref)
+    dv_ulp, rel_ulp = _diff(ulp, ref)
+    msg = (f"{solver} rtol {rtol}: FP32 Jacobian vs oracle max |dV| {dv:.3e} mV, max rel {rel:.3e}; oracle one-ulp "
+           f"spread |dV| {dv_ulp:.3e} rel {rel_ulp:.3e}; V {ref[:, 17].min():.1f}..{ref[:, 17].max():.1f} mV; "
+           f"counters oracle {oc} device {dc}")
+    assert ref[:, 17].max() > 0  # the upstroke is in the window
+    if solver == "ImplicitEuler":
+        fine, _ = _epi_run("oracle", solver, params, y0, np.arange(3001) * 0.01)
+        floor = 1e-6 * np.max(np.abs(fine), axis=0, keepdims=True)
+        disc = np.abs(ref - fine).max(axis=0)
+        err = np.abs(got - ref).max(axis=0)
+        print(msg + f"; ImplicitEuler dt 0.1 vs dt 0.01 |dV| {disc[17]:.3e} mV, worst ratio "
+                    f"{(err / np.maximum(disc, floor[0])).max():.3e}")
+        assert np.all(err <= np.maximum(disc, floor[0])), msg
+    else:
+        print(msg)
+        assert dv <= 1e-3 and rel <= 1e-3, msg
+
+
+def test_unknown_jacobian_mode_is_rejected():
+    from goss_amd._lib import GossHipError
+
+    with pytest.raises(GossHipError, match="Jacobian"):
+        dev_system("tp06_epi", "ImplicitEuler", 4, "fp64", {"jacobian": 7})
