@@ -85,12 +85,15 @@ def lift_calls(stmts: Sequence[Tuple[str, sp.Expr]], prefix: str = "tr__") -> Li
 
 
 def reverse_sweep(stmts: Sequence[Tuple[str, sp.Expr]], seeds: Dict[str, str],
-                  independent: Iterable[str]) -> Tuple[List[Tuple[str, str, sp.Expr]], Set[str]]:
+                  independent: Iterable[str],
+                  initial: Iterable[str] = ()) -> Tuple[List[Tuple[str, str, sp.Expr]], Set[str]]:
     """Reverse accumulation over ``stmts`` (evaluation order).
 
     ``seeds`` maps statement names to the names of their seed adjoints (the
-    outputs).  ``independent`` are the leaf symbols to differentiate with
-    respect to (states, field parameters).  Returns the accumulation list
+    outputs).  ``initial`` names statements whose adjoint the caller starts
+    at a nonzero value (and which are swept like any other).
+    ``independent`` are the leaf symbols to differentiate with respect to
+    (states, field parameters).  Returns the accumulation list
     ``(target_adjoint, source_adjoint, partial)`` meaning
     ``target += source * partial``, in execution order, and the set of
     adjoint names that receive contributions."""
@@ -102,6 +105,8 @@ def reverse_sweep(stmts: Sequence[Tuple[str, sp.Expr]], seeds: Dict[str, str],
         if any(s.name in diffable for s in e.free_symbols):
             diffable.add(n)
     has_adj: Dict[str, bool] = {n: True for n in seeds}
+    for n in initial:
+        has_adj[n] = True
     out: List[Tuple[str, str, sp.Expr]] = []
     for n, e in reversed(stmts):
         if not has_adj.get(n):
@@ -124,3 +129,37 @@ def reverse_sweep(stmts: Sequence[Tuple[str, sp.Expr]], seeds: Dict[str, str],
             has_adj[v.name] = True
     targets = {t for t, _, _ in out}
     return out, targets
+
+
+def gate_states(body: Sequence[Tuple[str, sp.Expr]], rhs: Sequence[sp.Expr], state_names: Sequence[str],
+                linear_terms: Sequence[int]) -> Dict[int, sp.Symbol]:
+    """States whose derivative is ``(A - y_i) * r`` with A and r independent of
+    y_i (Hodgkin-Huxley gates): index -> the symbol A.
+
+    For such a state the GRL1 step is y' = A + (y - A) exp(a h) with
+    a = -r the linearized diagonal (GRL1.h:72-84 with b = a (y - A)), so its
+    adjoint seeds follow from that closed form.  ``body`` are the statements
+    ``rhs`` refers to, in evaluation order."""
+    deps: Dict[str, Set[str]] = {}
+    states = set(state_names)
+    for n, e in body:
+        d: Set[str] = set()
+        for sym in e.free_symbols:
+            if sym.name in states:
+                d.add(sym.name)
+            d |= deps.get(sym.name, set())
+        deps[n] = d
+    out: Dict[int, sp.Symbol] = {}
+    for i, (name, e) in enumerate(zip(state_names, rhs)):
+        if not linear_terms[i]:
+            continue
+        y = sp.Symbol(name, real=True)
+        if any(name in deps.get(sym.name, set()) for sym in e.free_symbols if sym != y):
+            continue
+        c1 = sp.diff(e, y)
+        if c1 == 0 or c1.has(y):
+            continue
+        a = sp.simplify(-e.subs(y, 0) / c1)
+        if isinstance(a, sp.Symbol) and a != y:
+            out[i] = a
+    return out

@@ -25,7 +25,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import sympy as sp
 
-from .adjoint import lift_calls, reverse_sweep
+from .adjoint import gate_states, lift_calls, reverse_sweep
 from .analysis import Jacobian, Linearized, census_body, census_state_ops, jacobian, linearize, needed_names
 from .hoist import ConstantTable, hoist_statements
 from .ode_model import ODEModel, TIME
@@ -299,20 +299,33 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("")
 
     # -- grl1_vjp: discrete adjoint of one GRL1 step --------------------------
-    out_stmts = [(f"rhs__{i}", rhs_out[i]) for i in range(S)] + [(f"lin__{i}", hlinear[i]) for i in range(S)]
+    # Gates (rhs_i = (A - y_i) r): y'_i = A + (y_i - A) exp(linear_i h); their seeds go to A, to
+    # linear_i and straight to lam_prev_i, and rhs_i is not needed.  Other states: b/a seeds.
+    gates = gate_states(hbody, rhs_out, model.state_names, lin.linear_terms)
+    out_stmts = [(f"rhs__{i}", rhs_out[i]) for i in range(S) if i not in gates]
+    out_stmts += [(f"lin__{i}", hlinear[i]) for i in range(S)]
     vjp_fwd = lift_calls(lin_inter + hlin_first + hlin_second + out_stmts)
-    seeds = {f"rhs__{i}": f"bbar__{i}" for i in range(S)}
-    seeds.update({f"lin__{i}": f"abar__{i}" for i in range(S)})
+    seeds = {f"rhs__{i}": f"bbar__{i}" for i in range(S) if i not in gates}
+    seeds.update({f"lin__{i}": (f"gc__{i}" if i in gates else f"abar__{i}") for i in range(S)})
     independent = list(model.state_names) + list(var.field_parameters)
-    acc_lines, targets = reverse_sweep(vjp_fwd, seeds, independent)
+    fwd_names_set = {n for n, _ in vjp_fwd}
+    # Initial adjoint contributions d y'_i/d A_i (only where A is differentiable).
+    init_adj: Dict[str, List[str]] = {}
+    for i, a in gates.items():
+        if a.name in fwd_names_set or a.name in independent:
+            init_adj.setdefault(a.name, []).append(f"ga__{i}")
+    acc_lines, targets = reverse_sweep(vjp_fwd, seeds, independent,
+                                       initial=[n for n in init_adj if n in fwd_names_set])
+    targets = set(targets) | {f"{n}__b" for n in init_adj}
     for _, _, d in acc_lines:
         if d.has(sp.Derivative):
             raise ValueError(f"{model.name}: cannot differentiate {d}")
-    vused = g.used_main(vjp_fwd, [d for _, _, d in acc_lines])
+    vused = g.used_main(vjp_fwd, [d for _, _, d in acc_lines] + list(gates.values()))
     g_vjp = _Gen(var, table, vjp_fwd)
     L.append("// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:")
     L.append("//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,")
-    L.append("// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h.")
+    L.append("// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h;")
+    L.append("// for a gate (rhs_i = (A - y_i) r) this is A + (y_i - A) exp(linear_i*h).")
     L.append("// Reverse accumulation over the linearized_eval statements.")
     L.append("template <typename real, typename PA, typename treal>")
     L.append("GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,")
@@ -337,14 +350,22 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
             return e.base
         return None
 
-    L.append(f"{ind}const real hinv__ = m_rcp(h__);")
+    if len(gates) < S:
+        L.append(f"{ind}const real hinv__ = m_rcp(h__);")
     for i in range(S):
+        if i in gates:
+            y_txt = g_vjp.main.p(sp.Symbol(model.state_names[i], real=True))
+            L.append(f"{ind}real ey__{i}, ga__{i}, gc__{i};")
+            L.append(f"{ind}m_grl1_gate_seeds(lin__{i}, {y_txt}, {g_vjp.main.p(gates[i])}, lam__[{i}], h__, delta__, "
+                     f"ey__{i}, ga__{i}, gc__{i});")
+            continue
         inv = inverse_of(sp.Symbol(f"lin__{i}", real=True))
         inv_txt = g_vjp.main.p(inv) if inv is not None else f"m_rcp(lin__{i})"
         L.append(f"{ind}real bbar__{i}, abar__{i};")
         L.append(f"{ind}m_grl1_seeds(lin__{i}, {inv_txt}, rhs__{i}, lam__[{i}], h__, hinv__, delta__, bbar__{i}, abar__{i});")
     for t_ in sorted(targets):
-        L.append(f"{ind}real {t_} = real(0.0);")
+        init = " + ".join(init_adj.get(t_[:-3], [])) or "real(0.0)"
+        L.append(f"{ind}real {t_} = {init};")
     # Common subexpressions of the partials (they repeat factors such as
     # 1/(1 + e)^2 across the partials of one statement) become temporaries,
     # each declared just before its first use.
@@ -366,7 +387,8 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         L.append(f"{ind}{tgt} += {src}*({g_vjp.main.p(d)});")
     for i, st in enumerate(model.state_names):
         a = f"{st}__b"
-        L.append(f"{ind}lam_prev__[{i}] = lam__[{i}]" + (f" + {a};" if a in targets else ";"))
+        direct = f"ey__{i}" if i in gates else f"lam__[{i}]"
+        L.append(f"{ind}lam_prev__[{i}] = {direct}" + (f" + {a};" if a in targets else ";"))
     for j, fpn in enumerate(var.field_parameters):
         a = f"{fpn}__b"
         if a in targets:
@@ -406,17 +428,26 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
             L.append(f"{ind}tape.put({slot[n]}, {fwd_pr.p(e)});")
     for i in range(S):
         lin_txt = f"real(tape.get({slot[f'lin__{i}']}))"
+        if i in gates:
+            a_txt = tp.p(gates[i]) if gates[i].name in slot else g_vjp.main.p(gates[i])
+            y_txt = g_vjp.main.p(sp.Symbol(model.state_names[i], real=True))
+            L.append(f"{ind}real ey__{i}, ga__{i}, gc__{i};")
+            L.append(f"{ind}m_grl1_gate_seeds({lin_txt}, {y_txt}, real({a_txt}), lam__[{i}], h__, delta__, "
+                     f"ey__{i}, ga__{i}, gc__{i});")
+            continue
         L.append(f"{ind}real bbar__{i}, abar__{i};")
         L.append(f"{ind}m_grl1_seeds({lin_txt}, m_rcp({lin_txt}), real(tape.get({slot[f'rhs__{i}']})), lam__[{i}], "
                  f"h__, m_rcp(h__), delta__, bbar__{i}, abar__{i});")
     for a in adj_list:
-        L.append(f"{ind}tape.put({aslot[a]}, real(0.0));")
+        init = " + ".join(init_adj.get(a[:-3], [])) or "real(0.0)"
+        L.append(f"{ind}tape.put({aslot[a]}, {init});")
     for tgt, src, d in acc_lines:
         src_txt = f"tape.get({aslot[src]})" if src in aslot else src
         L.append(f"{ind}tape.add({aslot[tgt]}, {src_txt}*({tp.p(d)}));")
     for i, st in enumerate(model.state_names):
         a = f"{st}__b"
-        L.append(f"{ind}lam_prev__[{i}] = lam__[{i}]" + (f" + tape.get({aslot[a]});" if a in aslot else ";"))
+        direct = f"ey__{i}" if i in gates else f"lam__[{i}]"
+        L.append(f"{ind}lam_prev__[{i}] = {direct}" + (f" + tape.get({aslot[a]});" if a in aslot else ";"))
     for j, fpn in enumerate(var.field_parameters):
         a = f"{fpn}__b"
         if a in aslot:

@@ -97,7 +97,8 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
 
 // Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
 //   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
-// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h.
+// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h;
+// for a gate (rhs_i = (A - y_i) r) this is A + (y_i - A) exp(linear_i*h).
 // Reverse accumulation over the linearized_eval statements.
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,

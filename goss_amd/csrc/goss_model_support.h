@@ -114,6 +114,21 @@ GOSS_HD GOSS_INLINE void m_grl1_seeds(const T linear, const T inv_linear, const 
   bbar = ex ? lh * m_phi1(linear * h) : lh;
   abar = ex ? lh * rhs * h * m_dphi1(linear * h) : T(0);
 }
+// Seeds of the GRL1 step adjoint for a gate, whose step is y' = A + (y - A) exp(x), x = linear * h
+// (y + (linear (y - A)) h when |linear| <= delta):
+//   ey = lam dy'/dy,  ga = lam dy'/dA,  gc = lam dy'/dlinear.
+// exp(x) - 1 comes from expm1, so 1 - exp(x) keeps its relative accuracy for slow gates.
+template <typename T>
+GOSS_HD GOSS_INLINE void m_grl1_gate_seeds(const T linear, const T y, const T a, const T lam, const T h,
+                                           const T delta, T& ey, T& ga, T& gc)
+{
+  const bool ex = m_fabs(linear) > delta;
+  const T x = linear * h;
+  const T em1 = ex ? T(std::expm1(x)) : x;
+  ey = lam * (T(1) + em1);
+  ga = -lam * em1;
+  gc = lam * h * (y - a) * (ex ? T(1) + em1 : T(1));
+}
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GOSS_F32_ACCURATE_MATH)
 // FP32 device kernels: beyond |x| = 0.1 the closed forms use the hardware exp and reciprocal
 // (exp(x) - 1 then loses at most ~1e-6 relative to cancellation).
@@ -131,6 +146,19 @@ GOSS_HD GOSS_INLINE float m_dphi1<float>(const float x)
   const float e = __expf(x);
   const float q = (x * e - (e - 1.0f)) * __builtin_amdgcn_rcpf(x * x);
   return fabsf(x) < 0.1f ? p : q;
+}
+// exp(x) - 1: Taylor polynomial to x^4 below |x| = 1/64, hardware exp beyond.
+template <>
+GOSS_HD GOSS_INLINE void m_grl1_gate_seeds<float>(const float linear, const float y, const float a, const float lam,
+                                                  const float h, const float delta, float& ey, float& ga, float& gc)
+{
+  const bool ex = fabsf(linear) > delta;
+  const float x = linear * h;
+  const float p = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f))));
+  const float em1 = ex ? (fabsf(x) < 0.015625f ? p : __expf(x) - 1.0f) : x;
+  ey = lam * (1.0f + em1);
+  ga = -lam * em1;
+  gc = lam * h * (y - a) * (ex ? 1.0f + em1 : 1.0f);
 }
 // One exponential for both, 1/x = inv_linear * hinv: phi1 = (e - 1) / x, phi1' = (e - phi1) / x.
 // Below |x| = 0.05 the series (truncation below 1e-9 relative for phi1, 1e-7 for phi1'); above it
