@@ -12,8 +12,10 @@
 // so a wavefront's 64 lanes touch 512 contiguous bytes per element.  Only
 // the input and output of one right-hand-side evaluation are held in
 // registers (the FP64 RHS of a cell model needs most of the register file by
-// itself); the substitutions, norms and updates run over the HBM vectors in
-// rolled loops, so no long-lived vector competes with the RHS for registers.
+// itself); the norms and vector updates run over the HBM vectors in unrolled
+// loops (all loads of a vector in flight at once, no value kept beyond the
+// loop), the substitutions hold their vector in registers only while they
+// run, and no long-lived vector competes with the RHS for registers.
 // The RHS is instantiated at one site per algorithmic role (Newton residual,
 // first ESDIRK stage, Jacobian columns).
 //
@@ -243,7 +245,7 @@ struct Implicit {
     for (int i = 0; i < S; ++i)
 #pragma unroll
       for (int j = 0; j < S; ++j) dst(i, j) = src(i, j) * scale;
-#pragma unroll 1
+#pragma unroll
     for (int i = 0; i < S; ++i) dst(i, i) += 1.0 * 1.0;
   }
 
@@ -254,36 +256,63 @@ struct Implicit {
 #pragma unroll 1
       for (int i = 0; i <= k - 1; ++i) {
         double acc = 0.0;
+#pragma unroll 4
         for (int r = 0; r <= i - 1; r++) acc += a(i, r) * a(r, k);
         a(i, k) -= acc;
         acc = 0.0;
+#pragma unroll 4
         for (int r = 0; r <= i - 1; r++) acc += a(k, r) * a(r, i);
         a(k, i) = (a(k, i) - acc) / a(i, i);
       }
       double acc = 0.0;
+#pragma unroll 4
       for (int r = 0; r <= k - 1; r++) acc += a(k, r) * a(r, k);
       a(k, k) -= acc;
     }
   }
 
-  // ODE::forward_backward_subst (ODE.cpp:102-132), in place on x.
-  GOSS_HD static void fb_subst(const MatView& a, const VecView& x)
+  // Row i of a as a pointer that the compiler cannot compute before `after`
+  // is known: the substitution then loads one row at a time (its loads all in
+  // flight together) instead of hoisting every factor into registers at once.
+  GOSS_HD static GOSS_INLINE const double* row_after(const MatView& a, const int i, const double after)
   {
-#pragma unroll 1
+    const double* r = a.a + size_t(i) * S * a.ld;
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" : "+v"(r) : "v"(after));
+#else
+    (void)after;
+#endif
+    return r;
+  }
+
+  // ODE::forward_backward_subst (ODE.cpp:102-132), in place on xv.  The
+  // vector is held in registers while the substitution runs (it lives only
+  // here); each row of factors is read from HBM once its predecessor's
+  // unknown is known, and the arithmetic keeps the reference's order.
+  GOSS_HD static void fb_subst(const MatView& a, const VecView& xv)
+  {
+    double x[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = xv[i];
+#pragma unroll
     for (int i = 1; i < S; ++i) {
+      const double* r = row_after(a, i, x[i - 1]);
       double acc = 0.0;
-#pragma unroll 1
-      for (int j = 0; j <= i - 1; ++j) acc = acc + a(i, j) * x[j];
+#pragma unroll
+      for (int j = 0; j <= i - 1; ++j) acc = acc + r[size_t(j) * a.ld] * x[j];
       x[i] = x[i] - acc;
     }
     x[S - 1] = x[S - 1] / a(S - 1, S - 1);
-#pragma unroll 1
+#pragma unroll
     for (int i = S - 2; i >= 0; i--) {
+      const double* r = row_after(a, i, x[i + 1]);
       double acc = 0;
-#pragma unroll 1
-      for (int j = i + 1; j < S; ++j) acc = acc + a(i, j) * x[j];
-      x[i] = (x[i] - acc) / a(i, i);
+#pragma unroll
+      for (int j = i + 1; j < S; ++j) acc = acc + r[size_t(j) * a.ld] * x[j];
+      x[i] = (x[i] - acc) / r[size_t(i) * a.ld];
     }
+#pragma unroll
+    for (int i = 0; i < S; ++i) xv[i] = x[i];
   }
 
   GOSS_HD static GOSS_INLINE double norm(const double* v)
@@ -297,7 +326,7 @@ struct Implicit {
   GOSS_HD static double norm(const VecView& v)
   {
     double acc = 0;
-#pragma unroll 1
+#pragma unroll
     for (int i = 0; i < S; ++i) acc += v[i] * v[i];
     return sqrt(acc);
   }
@@ -416,7 +445,7 @@ struct ImplicitEulerNode {
         step_ok = false;
         break;
       }
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < S; ++i) z[i] += bv[i];
       relative_residual = residual / initial_residual;
       previous_residual = residual;
@@ -439,12 +468,12 @@ struct ImplicitEulerNode {
     const double eps = kGossEps * 1000;
 #pragma unroll 1
     while (true) {
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < S; ++i) z[i] = 0.0;
       if (num_refinements > p.num_refinements) st.always = 1;
       const bool step_ok = newton_solve(v, t + ldt, ldt, acc, p, st);
       if (step_ok) {
-#pragma unroll 1
+#pragma unroll
         for (int i = 0; i < S; ++i) v.y[i] += z[i];
         t += ldt;
         if (fabs(t - t_end) < eps) break;
@@ -553,7 +582,7 @@ struct EsdirkNode {
         conv_ok = false;
         break;
       }
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < S; ++i) z[i] += dz[i];
       prev_ndz = ndz;
       st.newton_its++;
@@ -594,7 +623,7 @@ struct EsdirkNode {
         c3 = k.b3;
         cs = k.c4;
       }
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < S; i++) {
         zs[i] = zp[i];
         double pv = z1[i] * c1;
@@ -606,7 +635,7 @@ struct EsdirkNode {
     }
     if (ok) {
       const VecView yh = v.w(kEsDZ);
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < S; i++)
         yh[i] = dt * ((k.b1 - k.bh1) * z1[i] + (k.b2 - k.bh2) * z2[i] + (k.b3 - k.gamma) * z3[i] +
                       k.gamma * z4[i]);
@@ -616,7 +645,7 @@ struct EsdirkNode {
         st.rejects++;
         ok = false;
       } else {
-#pragma unroll 1
+#pragma unroll
         for (int i = 0; i < S; i++)
           v.y[i] = v.y[i] + dt * (k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * z3[i] + k.gamma * z4[i]);
         st.accepted++;
