@@ -1,8 +1,9 @@
-"""Gradients of sampled field-state traces through GRL1 time stepping.
+"""Gradients of sampled field-state traces through explicit time stepping.
 
-:class:`GRL1Trace` is a ``torch.autograd.Function``: its forward advances a
-:class:`~goss_amd.system.DeviceSystem` (solver GRL1, FP64 or FP32, one step
-per interval) over the given times and returns the first field state sampled
+:class:`SampledTrace` is a ``torch.autograd.Function``: its forward advances a
+:class:`~goss_amd.system.DeviceSystem` (solver GRL1, RL1, ExplicitEuler or
+RK4, FP64 or FP32, one step per interval) over the given times and returns
+the first field state sampled
 every ``sample_every`` steps, as a ``[n_samples, N]`` FP64 tensor on the GPU.
 Its backward runs the device discrete adjoint (``goss_hip_adjoint_backward``):
 the states are checkpointed every ``checkpoint_every`` steps in the forward
@@ -24,7 +25,7 @@ import torch
 from .system import DeviceSystem
 
 
-class GRL1Trace(torch.autograd.Function):
+class SampledTrace(torch.autograd.Function):
     @staticmethod
     def forward(ctx, y0: torch.Tensor, fp: torch.Tensor, system: DeviceSystem, t: np.ndarray, sample_every: int,
                 checkpoint_every: int):
@@ -70,5 +71,5 @@ class GRL1Trace(torch.autograd.Function):
 
 def voltage_trace(system: DeviceSystem, y0: torch.Tensor, fp: torch.Tensor, t, sample_every: int,
                   checkpoint_every: int = 200) -> torch.Tensor:
-    """Differentiable ``[n_samples, N]`` trace of the first field state (see :class:`GRL1Trace`)."""
-    return GRL1Trace.apply(y0, fp, system, np.asarray(t, dtype=np.float64), int(sample_every), int(checkpoint_every))
+    """Differentiable ``[n_samples, N]`` trace of the first field state (see :class:`SampledTrace`)."""
+    return SampledTrace.apply(y0, fp, system, np.asarray(t, dtype=np.float64), int(sample_every), int(checkpoint_every))

@@ -298,161 +298,209 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("}")
     L.append("")
 
-    # -- grl1_vjp: discrete adjoint of one GRL1 step --------------------------
-    # Gates (rhs_i = (A - y_i) r): y'_i = A + (y_i - A) exp(linear_i h); their seeds go to A, to
-    # linear_i and straight to lam_prev_i, and rhs_i is not needed.  Other states: b/a seeds.
+    # -- step adjoints: grl1_vjp, rl1_vjp, eval_vjp (and their tape forms) ----
+    # Gates (rhs_i = (A - y_i) r): y'_i = A + (y_i - A) exp(linear_i h) under GRL1/RL1; their seeds go
+    # to A, to linear_i and straight to lam_prev_i, and rhs_i is not needed.
     gates = gate_states(hbody, rhs_out, model.state_names, lin.linear_terms)
-    out_stmts = [(f"rhs__{i}", rhs_out[i]) for i in range(S) if i not in gates]
-    out_stmts += [(f"lin__{i}", hlinear[i]) for i in range(S)]
-    vjp_fwd = lift_calls(lin_inter + hlin_first + hlin_second + out_stmts)
-    seeds = {f"rhs__{i}": f"bbar__{i}" for i in range(S) if i not in gates}
-    seeds.update({f"lin__{i}": (f"gc__{i}" if i in gates else f"abar__{i}") for i in range(S)})
     independent = list(model.state_names) + list(var.field_parameters)
-    fwd_names_set = {n for n, _ in vjp_fwd}
-    # Initial adjoint contributions d y'_i/d A_i (only where A is differentiable).
-    init_adj: Dict[str, List[str]] = {}
-    for i, a in gates.items():
-        if a.name in fwd_names_set or a.name in independent:
-            init_adj.setdefault(a.name, []).append(f"ga__{i}")
-    acc_lines, targets = reverse_sweep(vjp_fwd, seeds, independent,
-                                       initial=[n for n in init_adj if n in fwd_names_set])
-    targets = set(targets) | {f"{n}__b" for n in init_adj}
-    for _, _, d in acc_lines:
-        if d.has(sp.Derivative):
-            raise ValueError(f"{model.name}: cannot differentiate {d}")
-    vused = g.used_main(vjp_fwd, [d for _, _, d in acc_lines] + list(gates.values()))
-    g_vjp = _Gen(var, table, vjp_fwd)
-    L.append("// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:")
-    L.append("//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,")
-    L.append("// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h;")
-    L.append("// for a gate (rhs_i = (A - y_i) r) this is A + (y_i - A) exp(linear_i*h).")
-    L.append("// Reverse accumulation over the linearized_eval statements.")
-    L.append("template <typename real, typename PA, typename treal>")
-    L.append("GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,")
-    L.append("                                  const real h__, const real delta__, const real* __restrict__ lam__,")
-    L.append("                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)")
-    L.append("{")
-    L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;")
-    L += g_vjp.bind(vused | {s_.name for s_ in model.states}, ind)
-    L += g_vjp.lines(vjp_fwd, ind)
-    # 1/linear_i from the statements when linear_i = -1/tau (a gate): -tau, no reciprocal.
-    fwd_def = dict(vjp_fwd)
+    n_first = len(hlin_first)
+    vjp_census: List[sp.Expr] = []
+    tape_slots: Dict[str, int] = {}
 
-    def inverse_of(e, depth=0):
-        if depth > 8:
-            return None
-        if isinstance(e, sp.Symbol) and e.name in fwd_def:
-            return inverse_of(fwd_def[e.name], depth + 1)
-        if isinstance(e, sp.Mul) and len(e.args) == 2 and e.args[0] == -1:
-            inv = inverse_of(e.args[1], depth + 1)
-            return None if inv is None else -inv
-        if isinstance(e, sp.Pow) and e.exp == -1:
-            return e.base
-        return None
-
-    if len(gates) < S:
-        L.append(f"{ind}const real hinv__ = m_rcp(h__);")
-    for i in range(S):
-        if i in gates:
-            y_txt = g_vjp.main.p(sp.Symbol(model.state_names[i], real=True))
-            L.append(f"{ind}real ey__{i}, ga__{i}, gc__{i};")
-            L.append(f"{ind}m_grl1_gate_seeds(lin__{i}, {y_txt}, {g_vjp.main.p(gates[i])}, lam__[{i}], h__, delta__, "
-                     f"ey__{i}, ga__{i}, gc__{i});")
-            continue
-        inv = inverse_of(sp.Symbol(f"lin__{i}", real=True))
-        inv_txt = g_vjp.main.p(inv) if inv is not None else f"m_rcp(lin__{i})"
-        L.append(f"{ind}real bbar__{i}, abar__{i};")
-        L.append(f"{ind}m_grl1_seeds(lin__{i}, {inv_txt}, rhs__{i}, lam__[{i}], h__, hinv__, delta__, bbar__{i}, abar__{i});")
-    for t_ in sorted(targets):
-        init = " + ".join(init_adj.get(t_[:-3], [])) or "real(0.0)"
-        L.append(f"{ind}real {t_} = {init};")
-    # Common subexpressions of the partials (they repeat factors such as
-    # 1/(1 + e)^2 across the partials of one statement) become temporaries,
-    # each declared just before its first use.
-    cse_rep, cse_red = sp.cse([d for _, _, d in acc_lines], symbols=sp.numbered_symbols("cs__", real=True))
-    cse_def = {sym.name: e for sym, e in cse_rep}
-    cse_order = [sym.name for sym, _ in cse_rep]
-    emitted: set = set()
-
-    def emit_temps(expr):
-        need = [n for n in cse_order if n not in emitted and sp.Symbol(n, real=True) in expr.free_symbols]
-        for n in need:
-            emit_temps(cse_def[n])
-            if n not in emitted:
-                emitted.add(n)
-                L.append(f"{ind}const real {n} = {g_vjp.main.p(cse_def[n])};")
-
-    for (tgt, src, _), d in zip(acc_lines, cse_red):
-        emit_temps(d)
-        L.append(f"{ind}{tgt} += {src}*({g_vjp.main.p(d)});")
-    for i, st in enumerate(model.state_names):
-        a = f"{st}__b"
-        direct = f"ey__{i}" if i in gates else f"lam__[{i}]"
-        L.append(f"{ind}lam_prev__[{i}] = {direct}" + (f" + {a};" if a in targets else ";"))
-    for j, fpn in enumerate(var.field_parameters):
-        a = f"{fpn}__b"
-        if a in targets:
-            L.append(f"{ind}fp_bar__[{j}] += {a};")
-    L.append("}")
-    L.append("")
-
-    # -- grl1_vjp_tape: the same product with every intermediate and adjoint in
-    # a caller-provided tape (tape.get/put/add of slot k); only the states,
-    # parameters and seeds stay in registers.
-    fwd_names = [n for n, _ in vjp_fwd]
-    slot = {n: k for k, n in enumerate(fwd_names)}
-    adj_list = sorted(targets)
-    aslot = {a: len(fwd_names) + k for k, a in enumerate(adj_list)}
-    n_slots = len(fwd_names) + len(adj_list)
-    tape_map = {n: f"tape.get({k})" for n, k in slot.items()}
-    tp = CPrinter(tape_map)
-    L.append(f"constexpr int grl1_vjp_tape_slots = {n_slots};")
-    L.append("// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):")
-    L.append("// for targets whose registers cannot hold the whole reverse sweep.")
-    L.append("template <typename real, typename PA, typename treal, typename TP>")
-    L.append("GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,")
-    L.append("                                       const real h__, const real delta__, const real* __restrict__ lam__,")
-    L.append("                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,")
-    L.append("                                       const TP& tape)")
-    L.append("{")
-    L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;")
-    L += g_vjp.bind(vused | {s_.name for s_ in model.states}, ind)
-    # Time-only statements stay named (in treal) and are also stored; every
-    # other intermediate is computed from tape reads and only stored.
-    fwd_pr = CPrinter({"time": "real(time)", **tape_map})
-    for n, e in vjp_fwd:
-        if n in g_vjp.time_only:
-            L.append(f"{ind}const treal {c_name(n)} = {g_vjp.time.p(e)};")
-            L.append(f"{ind}tape.put({slot[n]}, real({c_name(n)}));")
+    def emit_vjp(kind: str) -> None:
+        """kind 'grl1': GRL1 step (GRL1.h:72-84); 'rl1': RL1 step (RL1.h:68-77: exponential update for
+        the linear_terms states, Euler for the others, no delta test); 'eval': y_bar += (d rhs/d y)^T seed."""
+        linear_states = [i for i in range(S) if lin.linear_terms[i]]
+        gts = {} if kind == "eval" else gates
+        if kind == "eval":
+            exp_states: List[int] = []
+        elif kind == "rl1":
+            exp_states = linear_states
         else:
-            L.append(f"{ind}tape.put({slot[n]}, {fwd_pr.p(e)});")
-    for i in range(S):
-        lin_txt = f"real(tape.get({slot[f'lin__{i}']}))"
-        if i in gates:
-            a_txt = tp.p(gates[i]) if gates[i].name in slot else g_vjp.main.p(gates[i])
-            y_txt = g_vjp.main.p(sp.Symbol(model.state_names[i], real=True))
-            L.append(f"{ind}real ey__{i}, ga__{i}, gc__{i};")
-            L.append(f"{ind}m_grl1_gate_seeds({lin_txt}, {y_txt}, real({a_txt}), lam__[{i}], h__, delta__, "
-                     f"ey__{i}, ga__{i}, gc__{i});")
-            continue
-        L.append(f"{ind}real bbar__{i}, abar__{i};")
-        L.append(f"{ind}m_grl1_seeds({lin_txt}, m_rcp({lin_txt}), real(tape.get({slot[f'rhs__{i}']})), lam__[{i}], "
-                 f"h__, m_rcp(h__), delta__, bbar__{i}, abar__{i});")
-    for a in adj_list:
-        init = " + ".join(init_adj.get(a[:-3], [])) or "real(0.0)"
-        L.append(f"{ind}tape.put({aslot[a]}, {init});")
-    for tgt, src, d in acc_lines:
-        src_txt = f"tape.get({aslot[src]})" if src in aslot else src
-        L.append(f"{ind}tape.add({aslot[tgt]}, {src_txt}*({tp.p(d)}));")
-    for i, st in enumerate(model.state_names):
-        a = f"{st}__b"
-        direct = f"ey__{i}" if i in gates else f"lam__[{i}]"
-        L.append(f"{ind}lam_prev__[{i}] = {direct}" + (f" + tape.get({aslot[a]});" if a in aslot else ";"))
-    for j, fpn in enumerate(var.field_parameters):
-        a = f"{fpn}__b"
-        if a in aslot:
-            L.append(f"{ind}fp_bar__[{j}] += tape.get({aslot[a]});")
-    L.append("}")
+            exp_states = list(range(S))
+        rhs_needed = [i for i in range(S) if i not in gts]
+        lin_needed = exp_states
+        roots = [rhs_out[i] for i in rhs_needed] + [hlinear[i] for i in lin_needed]
+        lin_part = (hlin_first + hlin_second) if kind == "grl1" else (hlin_first if kind == "rl1" else [])
+        lin_part = select(lin_part, roots)
+        inter = select(body_inter, roots + [e for _, e in lin_part] + list(gts.values()))
+        out_stmts = [(f"rhs__{i}", rhs_out[i]) for i in rhs_needed] + [(f"lin__{i}", hlinear[i]) for i in lin_needed]
+        fwd = lift_calls(inter + lin_part + out_stmts)
+        if kind == "eval":
+            seeds = {f"rhs__{i}": f"seed__[{i}]" for i in range(S)}
+        else:
+            seeds = {f"rhs__{i}": f"bbar__{i}" for i in rhs_needed}
+            seeds.update({f"lin__{i}": (f"gc__{i}" if i in gts else f"abar__{i}") for i in lin_needed})
+        fwd_set = {n for n, _ in fwd}
+        init_adj: Dict[str, List[str]] = {}
+        for i, a_sym in gts.items():
+            if a_sym.name in fwd_set or a_sym.name in independent:
+                init_adj.setdefault(a_sym.name, []).append(f"ga__{i}")
+        acc_lines, targets = reverse_sweep(fwd, seeds, independent, initial=[n for n in init_adj if n in fwd_set])
+        targets = set(targets) | {f"{n}__b" for n in init_adj}
+        for _, _, d in acc_lines:
+            if d.has(sp.Derivative):
+                raise ValueError(f"{model.name}: cannot differentiate {d}")
+        vjp_census.extend([e for _, e in fwd] + [d for _, _, d in acc_lines])
+        vused = g.used_main(fwd, [d for _, _, d in acc_lines] + list(gts.values()))
+        gv = _Gen(var, table, fwd)
+        delta_txt = "real(-1.0)" if kind == "rl1" else "delta__"
+        fwd_def = dict(fwd)
+
+        def inverse_of(e, depth=0):
+            # 1/linear_i from the statements when linear_i = -1/tau: -tau, no reciprocal.
+            if depth > 8:
+                return None
+            if isinstance(e, sp.Symbol) and e.name in fwd_def:
+                return inverse_of(fwd_def[e.name], depth + 1)
+            if isinstance(e, sp.Mul) and len(e.args) == 2 and e.args[0] == -1:
+                inv = inverse_of(e.args[1], depth + 1)
+                return None if inv is None else -inv
+            if isinstance(e, sp.Pow) and e.exp == -1:
+                return e.base
+            return None
+
+        def seed_lines(val, scalar_y) -> List[str]:
+            """Seed computations; val(name) prints a forward value, scalar_y(i) a state."""
+            out: List[str] = []
+            for i in range(S):
+                if i in gts:
+                    out.append(f"{ind}real ey__{i}, ga__{i}, gc__{i};")
+                    out.append(f"{ind}m_grl1_gate_seeds(real({val(f'lin__{i}')}), {scalar_y(i)}, real({val(gts[i].name, gts[i])}), "
+                               f"lam__[{i}], h__, {delta_txt}, ey__{i}, ga__{i}, gc__{i});")
+                elif i in exp_states:
+                    inv = inverse_of(sp.Symbol(f"lin__{i}", real=True)) if val is None_val else None
+                    inv_txt = gv.main.p(inv) if inv is not None else f"m_rcp(real({val(f'lin__{i}')}))"
+                    out.append(f"{ind}real bbar__{i}, abar__{i};")
+                    out.append(f"{ind}m_grl1_seeds(real({val(f'lin__{i}')}), {inv_txt}, real({val(f'rhs__{i}')}), lam__[{i}], "
+                               f"h__, hinv__, {delta_txt}, bbar__{i}, abar__{i});")
+                elif kind != "eval":
+                    out.append(f"{ind}const real bbar__{i} = lam__[{i}]*h__;")
+            return out
+
+        def None_val(name, sym=None):
+            return c_name(name) if sym is None else gv.main.p(sym)
+
+        if kind == "eval":
+            sig = ("GOSS_HD GOSS_INLINE void eval_vjp{T}(const real* __restrict__ states, const treal time, const PA& pa,\n"
+                   "                                  const real* __restrict__ seed__, real* __restrict__ y_bar__,\n"
+                   "                                  real* __restrict__ fp_bar__{TP})")
+        else:
+            sig = ("GOSS_HD GOSS_INLINE void " + kind + "_vjp{T}(const real* __restrict__ states, const treal time, "
+                   "const PA& pa,\n                                  const real h__, const real delta__, "
+                   "const real* __restrict__ lam__,\n                                  real* __restrict__ lam_prev__, "
+                   "real* __restrict__ fp_bar__{TP})")
+
+        def outputs(adj) -> List[str]:
+            out: List[str] = []
+            for i, st in enumerate(model.state_names):
+                a_ = f"{st}__b"
+                if kind == "eval":
+                    if a_ in targets:
+                        out.append(f"{ind}y_bar__[{i}] += {adj(a_)};")
+                else:
+                    direct = f"ey__{i}" if i in gts else f"lam__[{i}]"
+                    out.append(f"{ind}lam_prev__[{i}] = {direct}" + (f" + {adj(a_)};" if a_ in targets else ";"))
+            for j, fpn in enumerate(var.field_parameters):
+                a_ = f"{fpn}__b"
+                if a_ in targets:
+                    out.append(f"{ind}fp_bar__[{j}] += {adj(a_)};")
+            return out
+
+        # Register form.
+        L.append("template <typename real, typename PA, typename treal>")
+        L.append(sig.format(T="", TP=""))
+        L.append("{")
+        L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;" + ("" if kind == "eval" else " (void)delta__;"))
+        L.extend(gv.bind(vused | {s_.name for s_ in model.states}, ind))
+        L.extend(gv.lines(fwd, ind))
+        if any(i in exp_states and i not in gts for i in range(S)):
+            L.append(f"{ind}const real hinv__ = m_rcp(h__);")
+        L.extend(seed_lines(None_val, lambda i: gv.main.p(sp.Symbol(model.state_names[i], real=True))))
+        for t_ in sorted(targets):
+            init = " + ".join(init_adj.get(t_[:-3], [])) or "real(0.0)"
+            L.append(f"{ind}real {t_} = {init};")
+        # Common subexpressions of the partials become temporaries, each declared just before its first use.
+        cse_rep, cse_red = sp.cse([d for _, _, d in acc_lines], symbols=sp.numbered_symbols("cs__", real=True))
+        cse_def = {sym.name: e for sym, e in cse_rep}
+        cse_order = [sym.name for sym, _ in cse_rep]
+        emitted: set = set()
+
+        def emit_temps(expr):
+            need = [n for n in cse_order if n not in emitted and sp.Symbol(n, real=True) in expr.free_symbols]
+            for n in need:
+                emit_temps(cse_def[n])
+                if n not in emitted:
+                    emitted.add(n)
+                    L.append(f"{ind}const real {n} = {gv.main.p(cse_def[n])};")
+
+        for (tgt, src, _), d in zip(acc_lines, cse_red):
+            emit_temps(d)
+            L.append(f"{ind}{tgt} += {src}*({gv.main.p(d)});")
+        L.extend(outputs(lambda a_: a_))
+        L.append("}")
+        L.append("")
+
+        # Tape form: every intermediate and adjoint in a caller-provided tape (tape.get/put/add of
+        # slot k); only the states, parameters and seeds stay in registers.
+        fwd_names = [n for n, _ in fwd]
+        slot = {n: k for k, n in enumerate(fwd_names)}
+        adj_list = sorted(targets)
+        aslot = {a_: len(fwd_names) + k for k, a_ in enumerate(adj_list)}
+        tape_slots[kind] = len(fwd_names) + len(adj_list)
+        tape_map = {n: f"tape.get({k})" for n, k in slot.items()}
+        tp = CPrinter(tape_map)
+        if kind == "eval":
+            # seed__ and y_bar__ may be views (v[i] reads / v[i] += x), e.g. rows of the tape.
+            L.append("template <typename real, typename PA, typename treal, typename TP, typename SV, typename YV>")
+            L.append("GOSS_HD GOSS_INLINE void eval_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,\n"
+                     "                                       const SV& seed__, const YV& y_bar__, real* __restrict__ fp_bar__,\n"
+                     "                                       const TP& tape)")
+        else:
+            L.append("template <typename real, typename PA, typename treal, typename TP>")
+            L.append(sig.format(T="_tape", TP=",\n                                  const TP& tape"))
+        L.append("{")
+        L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;" + ("" if kind == "eval" else " (void)delta__;"))
+        L.extend(gv.bind(vused | {s_.name for s_ in model.states}, ind))
+        fwd_pr = CPrinter({"time": "real(time)", **tape_map})
+        for n, e in fwd:
+            if n in gv.time_only:
+                L.append(f"{ind}const treal {c_name(n)} = {gv.time.p(e)};")
+                L.append(f"{ind}tape.put({slot[n]}, real({c_name(n)}));")
+            else:
+                L.append(f"{ind}tape.put({slot[n]}, {fwd_pr.p(e)});")
+
+        def tape_val(name, sym=None):
+            if name in slot:
+                return f"tape.get({slot[name]})"
+            return gv.main.p(sym) if sym is not None else c_name(name)
+
+        if any(i in exp_states and i not in gts for i in range(S)):
+            L.append(f"{ind}const real hinv__ = m_rcp(h__);")
+        L.extend(seed_lines(tape_val, lambda i: gv.main.p(sp.Symbol(model.state_names[i], real=True))))
+        for a_ in adj_list:
+            init = " + ".join(init_adj.get(a_[:-3], [])) or "real(0.0)"
+            L.append(f"{ind}tape.put({aslot[a_]}, {init});")
+        for tgt, src, d in acc_lines:
+            src_txt = f"tape.get({aslot[src]})" if src in aslot else src
+            L.append(f"{ind}tape.add({aslot[tgt]}, {src_txt}*({tp.p(d)}));")
+        L.extend(outputs(lambda a_: f"tape.get({aslot[a_]})"))
+        L.append("}")
+        L.append("")
+
+    L.append("// Discrete adjoints of one step of size h at `time` (one step per interval):")
+    L.append("//   *_vjp:  lam_prev = (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam;")
+    L.append("//   grl1: y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h (GRL1.h:72-84);")
+    L.append("//   rl1:  the same with the exponential update for the linear_terms states only (RL1.h:68-77);")
+    L.append("//   for a gate (rhs_i = (A - y_i) r) the exponential update is A + (y_i - A) exp(linear_i*h).")
+    L.append("//   eval_vjp: y_bar += (d rhs/d y)^T seed, fp_bar += (d rhs/d p_field)^T seed (the explicit schemes).")
+    L.append("// Reverse accumulation over the generated statements; the *_tape forms keep every intermediate")
+    L.append("// and adjoint in a caller-provided tape (for targets whose registers cannot hold the sweep).")
+    for kind in ("grl1", "rl1", "eval"):
+        emit_vjp(kind)
+    L.append(f"constexpr int grl1_vjp_tape_slots = {tape_slots['grl1']};")
+    L.append(f"constexpr int rl1_vjp_tape_slots = {tape_slots['rl1']};")
+    L.append(f"constexpr int eval_vjp_tape_slots = {tape_slots['eval']};")
     L.append("")
 
     # Census of the (unhoisted) model: state-dependent work after CSE, and
@@ -472,7 +520,7 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         "linearized_eval": census_body([e for _, e in lin_inter + hlin_first + hlin_second] + rhs_out + hlinear),
         "constants": census_body([e for _, e in table.items]),
         # Forward statements plus the reverse accumulations of grl1_vjp.
-        "grl1_vjp": census_body([e for _, e in vjp_fwd] + [d for _, _, d in acc_lines]),
+        "step_vjps": census_body(vjp_census),
     }
 
     jac_nnz = 0

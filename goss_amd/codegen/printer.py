@@ -16,7 +16,8 @@ Printing rules (they fix the floating-point evaluation order):
   the FP32 kernels;
 * ``x**2``, ``x**3``, ``x**4``: ``m_sq``/``m_cube``/``m_sq(m_sq(x))``;
   ``x**0.5``: ``m_sqrt``; other powers: ``m_pow``;
-* ``Piecewise``: nested ``?:`` selects.
+* ``Piecewise``: nested ``GOSS_SELECT(cond, a, b)`` (a ``?:`` select, or both
+  arms evaluated and one selected, as goss_model_support.h defines it).
 """
 from __future__ import annotations
 
@@ -203,9 +204,9 @@ class CPrinter:
             val, cond = pieces[-1]
             if cond is not sp.true and cond != True:  # noqa: E712
                 raise ValueError("Piecewise without an otherwise branch")
-            s = self._wrap(val, P["select"])
+            s = self.p(val)
             for val, cond in reversed(pieces[:-1]):
-                s = f"({self.p(cond)} ? {self._wrap(val, P['select'] + 1)} : {s})"
+                s = f"GOSS_SELECT({self.p(cond)}, {self.p(val)}, {s})"
             return s, P["atom"]
         if isinstance(e, Relational):
             ops = {"<": "<", "<=": "<=", ">": ">", ">=": ">=", "==": "==", "!=": "!="}

@@ -95,17 +95,20 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   linear[0] = dI_dv;
 }
 
-// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
-//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
-// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h;
-// for a gate (rhs_i = (A - y_i) r) this is A + (y_i - A) exp(linear_i*h).
-// Reverse accumulation over the linearized_eval statements.
+// Discrete adjoints of one step of size h at `time` (one step per interval):
+//   *_vjp:  lam_prev = (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam;
+//   grl1: y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h (GRL1.h:72-84);
+//   rl1:  the same with the exponential update for the linear_terms states only (RL1.h:68-77);
+//   for a gate (rhs_i = (A - y_i) r) the exponential update is A + (y_i - A) exp(linear_i*h).
+//   eval_vjp: y_bar += (d rhs/d y)^T seed, fp_bar += (d rhs/d p_field)^T seed (the explicit schemes).
+// Reverse accumulation over the generated statements; the *_tape forms keep every intermediate
+// and adjoint in a caller-provided tape (for targets whose registers cannot hold the sweep).
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
                                   const real h__, const real delta__, const real* __restrict__ lam__,
                                   real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
 {
-  (void)time; (void)pa; (void)fp_bar__;
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
   const real v = states[0];
   const real s = states[1];
   const real b = pa.template get<1>();
@@ -134,9 +137,9 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real lin__1 = c__5;
   const real hinv__ = m_rcp(h__);
   real bbar__0, abar__0;
-  m_grl1_seeds(lin__0, m_rcp(lin__0), rhs__0, lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
+  m_grl1_seeds(real(lin__0), m_rcp(real(lin__0)), real(rhs__0), lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
   real bbar__1, abar__1;
-  m_grl1_seeds(lin__1, m_rcp(lin__1), rhs__1, lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
+  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   real I__b = real(0.0);
   real dI_dv__b = real(0.0);
   real s__b = real(0.0);
@@ -158,16 +161,13 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   lam_prev__[1] = lam__[1] + s__b;
 }
 
-constexpr int grl1_vjp_tape_slots = 17;
-// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):
-// for targets whose registers cannot hold the whole reverse sweep.
 template <typename real, typename PA, typename treal, typename TP>
 GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
-                                       const real h__, const real delta__, const real* __restrict__ lam__,
-                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
-                                       const TP& tape)
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
 {
-  (void)time; (void)pa; (void)fp_bar__;
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
   const real v = states[0];
   const real s = states[1];
   const real b = pa.template get<1>();
@@ -201,10 +201,11 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   tape.put(10, b*(v - v_rest - c_3*s));
   tape.put(11, tape.get(8));
   tape.put(12, c__5);
+  const real hinv__ = m_rcp(h__);
   real bbar__0, abar__0;
-  m_grl1_seeds(real(tape.get(11)), m_rcp(real(tape.get(11))), real(tape.get(9)), lam__[0], h__, m_rcp(h__), delta__, bbar__0, abar__0);
+  m_grl1_seeds(real(tape.get(11)), m_rcp(real(tape.get(11))), real(tape.get(9)), lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
   real bbar__1, abar__1;
-  m_grl1_seeds(real(tape.get(12)), m_rcp(real(tape.get(12))), real(tape.get(10)), lam__[1], h__, m_rcp(h__), delta__, bbar__1, abar__1);
+  m_grl1_seeds(real(tape.get(12)), m_rcp(real(tape.get(12))), real(tape.get(10)), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   tape.put(13, real(0.0));
   tape.put(14, real(0.0));
   tape.put(15, real(0.0));
@@ -220,6 +221,199 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   lam_prev__[0] = lam__[0] + tape.get(16);
   lam_prev__[1] = lam__[1] + tape.get(15);
 }
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void rl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real v = states[0];
+  const real s = states[1];
+  const real b = pa.template get<1>();
+  const real c_3 = pa.template get<4>();
+  const real stim_start = pa.template get<8>();
+  const real v_peak = pa.template get<9>();
+  const real v_rest = pa.template get<10>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__4 = pa.template cst<4>();
+  const real c__5 = pa.template cst<5>();
+  const real I = c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v);
+  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>());
+  const treal tr__1 = treal(1.0) + tr__0;
+  const treal tr__2 = m_rcp(tr__1);
+  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__4 = treal(1.0) + tr__3;
+  const treal tr__5 = m_rcp(tr__4);
+  const treal i_Stim = pa.template getd<5>()*tr__5*(treal(1.0) - tr__2);
+  const real rhs__0 = I + real(i_Stim);
+  const real rhs__1 = b*(v - v_rest - c_3*s);
+  const real lin__1 = c__5;
+  const real hinv__ = m_rcp(h__);
+  const real bbar__0 = lam__[0]*h__;
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
+  real I__b = real(0.0);
+  real s__b = real(0.0);
+  real v__b = real(0.0);
+  s__b += bbar__1*(-b*c_3);
+  v__b += bbar__1*(b);
+  I__b += bbar__0*(real(1.0));
+  const real cs__0 = v - v_rest;
+  s__b += I__b*(c__2*cs__0);
+  const real cs__1 = v - c__1;
+  const real cs__2 = c__3*(v_peak - v);
+  v__b += I__b*(c__2*s + cs__0*cs__2 + cs__1*cs__2 - c__3*cs__0*cs__1);
+  lam_prev__[0] = lam__[0] + v__b;
+  lam_prev__[1] = lam__[1] + s__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP>
+GOSS_HD GOSS_INLINE void rl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real v = states[0];
+  const real s = states[1];
+  const real b = pa.template get<1>();
+  const real c_3 = pa.template get<4>();
+  const real stim_start = pa.template get<8>();
+  const real v_peak = pa.template get<9>();
+  const real v_rest = pa.template get<10>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__4 = pa.template cst<4>();
+  const real c__5 = pa.template cst<5>();
+  tape.put(0, c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v));
+  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>());
+  tape.put(1, real(tr__0));
+  const treal tr__1 = treal(1.0) + tr__0;
+  tape.put(2, real(tr__1));
+  const treal tr__2 = m_rcp(tr__1);
+  tape.put(3, real(tr__2));
+  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  tape.put(4, real(tr__3));
+  const treal tr__4 = treal(1.0) + tr__3;
+  tape.put(5, real(tr__4));
+  const treal tr__5 = m_rcp(tr__4);
+  tape.put(6, real(tr__5));
+  const treal i_Stim = pa.template getd<5>()*tr__5*(treal(1.0) - tr__2);
+  tape.put(7, real(i_Stim));
+  tape.put(8, tape.get(0) + tape.get(7));
+  tape.put(9, b*(v - v_rest - c_3*s));
+  tape.put(10, c__5);
+  const real hinv__ = m_rcp(h__);
+  const real bbar__0 = lam__[0]*h__;
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(tape.get(10)), m_rcp(real(tape.get(10))), real(tape.get(9)), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
+  tape.put(11, real(0.0));
+  tape.put(12, real(0.0));
+  tape.put(13, real(0.0));
+  tape.add(12, bbar__1*(-b*c_3));
+  tape.add(13, bbar__1*(b));
+  tape.add(11, bbar__0*(real(1.0)));
+  tape.add(12, tape.get(11)*(c__2*(v - v_rest)));
+  tape.add(13, tape.get(11)*(c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) - c__3*(v - c__1)*(v - v_rest)));
+  lam_prev__[0] = lam__[0] + tape.get(13);
+  lam_prev__[1] = lam__[1] + tape.get(12);
+}
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real* __restrict__ seed__, real* __restrict__ y_bar__,
+                                  real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real v = states[0];
+  const real s = states[1];
+  const real b = pa.template get<1>();
+  const real c_3 = pa.template get<4>();
+  const real stim_start = pa.template get<8>();
+  const real v_peak = pa.template get<9>();
+  const real v_rest = pa.template get<10>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__4 = pa.template cst<4>();
+  const real I = c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v);
+  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>());
+  const treal tr__1 = treal(1.0) + tr__0;
+  const treal tr__2 = m_rcp(tr__1);
+  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__4 = treal(1.0) + tr__3;
+  const treal tr__5 = m_rcp(tr__4);
+  const treal i_Stim = pa.template getd<5>()*tr__5*(treal(1.0) - tr__2);
+  const real rhs__0 = I + real(i_Stim);
+  const real rhs__1 = b*(v - v_rest - c_3*s);
+  real I__b = real(0.0);
+  real s__b = real(0.0);
+  real v__b = real(0.0);
+  s__b += seed__[1]*(-b*c_3);
+  v__b += seed__[1]*(b);
+  I__b += seed__[0]*(real(1.0));
+  const real cs__0 = v - v_rest;
+  s__b += I__b*(c__2*cs__0);
+  const real cs__1 = v - c__1;
+  const real cs__2 = c__3*(v_peak - v);
+  v__b += I__b*(c__2*s + cs__0*cs__2 + cs__1*cs__2 - c__3*cs__0*cs__1);
+  y_bar__[0] += v__b;
+  y_bar__[1] += s__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP, typename SV, typename YV>
+GOSS_HD GOSS_INLINE void eval_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                       const SV& seed__, const YV& y_bar__, real* __restrict__ fp_bar__,
+                                       const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real v = states[0];
+  const real s = states[1];
+  const real b = pa.template get<1>();
+  const real c_3 = pa.template get<4>();
+  const real stim_start = pa.template get<8>();
+  const real v_peak = pa.template get<9>();
+  const real v_rest = pa.template get<10>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__4 = pa.template cst<4>();
+  tape.put(0, c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v));
+  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<8>());
+  tape.put(1, real(tr__0));
+  const treal tr__1 = treal(1.0) + tr__0;
+  tape.put(2, real(tr__1));
+  const treal tr__2 = m_rcp(tr__1);
+  tape.put(3, real(tr__2));
+  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  tape.put(4, real(tr__3));
+  const treal tr__4 = treal(1.0) + tr__3;
+  tape.put(5, real(tr__4));
+  const treal tr__5 = m_rcp(tr__4);
+  tape.put(6, real(tr__5));
+  const treal i_Stim = pa.template getd<5>()*tr__5*(treal(1.0) - tr__2);
+  tape.put(7, real(i_Stim));
+  tape.put(8, tape.get(0) + tape.get(7));
+  tape.put(9, b*(v - v_rest - c_3*s));
+  tape.put(10, real(0.0));
+  tape.put(11, real(0.0));
+  tape.put(12, real(0.0));
+  tape.add(11, seed__[1]*(-b*c_3));
+  tape.add(12, seed__[1]*(b));
+  tape.add(10, seed__[0]*(real(1.0)));
+  tape.add(11, tape.get(10)*(c__2*(v - v_rest)));
+  tape.add(12, tape.get(10)*(c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) - c__3*(v - c__1)*(v - v_rest)));
+  y_bar__[0] += tape.get(12);
+  y_bar__[1] += tape.get(11);
+}
+
+constexpr int grl1_vjp_tape_slots = 17;
+constexpr int rl1_vjp_tape_slots = 14;
+constexpr int eval_vjp_tape_slots = 13;
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 // `jac` is a pointer or any object whose jac[k] = value stores entry k.

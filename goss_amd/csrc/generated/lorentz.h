@@ -69,17 +69,20 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   if (only_linear) return;
 }
 
-// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
-//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
-// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h;
-// for a gate (rhs_i = (A - y_i) r) this is A + (y_i - A) exp(linear_i*h).
-// Reverse accumulation over the linearized_eval statements.
+// Discrete adjoints of one step of size h at `time` (one step per interval):
+//   *_vjp:  lam_prev = (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam;
+//   grl1: y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h (GRL1.h:72-84);
+//   rl1:  the same with the exponential update for the linear_terms states only (RL1.h:68-77);
+//   for a gate (rhs_i = (A - y_i) r) the exponential update is A + (y_i - A) exp(linear_i*h).
+//   eval_vjp: y_bar += (d rhs/d y)^T seed, fp_bar += (d rhs/d p_field)^T seed (the explicit schemes).
+// Reverse accumulation over the generated statements; the *_tape forms keep every intermediate
+// and adjoint in a caller-provided tape (for targets whose registers cannot hold the sweep).
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
                                   const real h__, const real delta__, const real* __restrict__ lam__,
                                   real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
 {
-  (void)time; (void)pa; (void)fp_bar__;
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
   const real x = states[0];
   const real y = states[1];
   const real z = states[2];
@@ -93,11 +96,11 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real lin__2 = -beta;
   const real hinv__ = m_rcp(h__);
   real ey__0, ga__0, gc__0;
-  m_grl1_gate_seeds(lin__0, x, y, lam__[0], h__, delta__, ey__0, ga__0, gc__0);
+  m_grl1_gate_seeds(real(lin__0), x, real(y), lam__[0], h__, delta__, ey__0, ga__0, gc__0);
   real bbar__1, abar__1;
-  m_grl1_seeds(lin__1, m_rcp(lin__1), rhs__1, lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
+  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   real bbar__2, abar__2;
-  m_grl1_seeds(lin__2, m_rcp(lin__2), rhs__2, lam__[2], h__, hinv__, delta__, bbar__2, abar__2);
+  m_grl1_seeds(real(lin__2), m_rcp(real(lin__2)), real(rhs__2), lam__[2], h__, hinv__, delta__, bbar__2, abar__2);
   real x__b = real(0.0);
   real y__b = ga__0;
   real z__b = real(0.0);
@@ -112,16 +115,13 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   lam_prev__[2] = lam__[2] + z__b;
 }
 
-constexpr int grl1_vjp_tape_slots = 8;
-// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):
-// for targets whose registers cannot hold the whole reverse sweep.
 template <typename real, typename PA, typename treal, typename TP>
 GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
-                                       const real h__, const real delta__, const real* __restrict__ lam__,
-                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
-                                       const TP& tape)
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
 {
-  (void)time; (void)pa; (void)fp_bar__;
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
   const real x = states[0];
   const real y = states[1];
   const real z = states[2];
@@ -133,12 +133,13 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   tape.put(2, -sigma);
   tape.put(3, -real(1.0));
   tape.put(4, -beta);
+  const real hinv__ = m_rcp(h__);
   real ey__0, ga__0, gc__0;
   m_grl1_gate_seeds(real(tape.get(2)), x, real(y), lam__[0], h__, delta__, ey__0, ga__0, gc__0);
   real bbar__1, abar__1;
-  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(0)), lam__[1], h__, m_rcp(h__), delta__, bbar__1, abar__1);
+  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(0)), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   real bbar__2, abar__2;
-  m_grl1_seeds(real(tape.get(4)), m_rcp(real(tape.get(4))), real(tape.get(1)), lam__[2], h__, m_rcp(h__), delta__, bbar__2, abar__2);
+  m_grl1_seeds(real(tape.get(4)), m_rcp(real(tape.get(4))), real(tape.get(1)), lam__[2], h__, hinv__, delta__, bbar__2, abar__2);
   tape.put(5, real(0.0));
   tape.put(6, ga__0);
   tape.put(7, real(0.0));
@@ -152,6 +153,149 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   lam_prev__[1] = lam__[1] + tape.get(6);
   lam_prev__[2] = lam__[2] + tape.get(7);
 }
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void rl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real x = states[0];
+  const real y = states[1];
+  const real z = states[2];
+  const real beta = pa.template get<0>();
+  const real rho = pa.template get<1>();
+  const real sigma = pa.template get<2>();
+  const real rhs__1 = -y + x*(rho - z);
+  const real rhs__2 = x*y - beta*z;
+  const real lin__0 = -sigma;
+  const real lin__1 = -real(1.0);
+  const real lin__2 = -beta;
+  const real hinv__ = m_rcp(h__);
+  real ey__0, ga__0, gc__0;
+  m_grl1_gate_seeds(real(lin__0), x, real(y), lam__[0], h__, real(-1.0), ey__0, ga__0, gc__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
+  real bbar__2, abar__2;
+  m_grl1_seeds(real(lin__2), m_rcp(real(lin__2)), real(rhs__2), lam__[2], h__, hinv__, real(-1.0), bbar__2, abar__2);
+  real x__b = real(0.0);
+  real y__b = ga__0;
+  real z__b = real(0.0);
+  x__b += bbar__2*(y);
+  y__b += bbar__2*(x);
+  z__b += bbar__2*(-beta);
+  x__b += bbar__1*(rho - z);
+  y__b += bbar__1*(-real(1.0));
+  z__b += bbar__1*(-x);
+  lam_prev__[0] = ey__0 + x__b;
+  lam_prev__[1] = lam__[1] + y__b;
+  lam_prev__[2] = lam__[2] + z__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP>
+GOSS_HD GOSS_INLINE void rl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real x = states[0];
+  const real y = states[1];
+  const real z = states[2];
+  const real beta = pa.template get<0>();
+  const real rho = pa.template get<1>();
+  const real sigma = pa.template get<2>();
+  tape.put(0, -y + x*(rho - z));
+  tape.put(1, x*y - beta*z);
+  tape.put(2, -sigma);
+  tape.put(3, -real(1.0));
+  tape.put(4, -beta);
+  const real hinv__ = m_rcp(h__);
+  real ey__0, ga__0, gc__0;
+  m_grl1_gate_seeds(real(tape.get(2)), x, real(y), lam__[0], h__, real(-1.0), ey__0, ga__0, gc__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(0)), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
+  real bbar__2, abar__2;
+  m_grl1_seeds(real(tape.get(4)), m_rcp(real(tape.get(4))), real(tape.get(1)), lam__[2], h__, hinv__, real(-1.0), bbar__2, abar__2);
+  tape.put(5, real(0.0));
+  tape.put(6, ga__0);
+  tape.put(7, real(0.0));
+  tape.add(5, bbar__2*(y));
+  tape.add(6, bbar__2*(x));
+  tape.add(7, bbar__2*(-beta));
+  tape.add(5, bbar__1*(rho - z));
+  tape.add(6, bbar__1*(-real(1.0)));
+  tape.add(7, bbar__1*(-x));
+  lam_prev__[0] = ey__0 + tape.get(5);
+  lam_prev__[1] = lam__[1] + tape.get(6);
+  lam_prev__[2] = lam__[2] + tape.get(7);
+}
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real* __restrict__ seed__, real* __restrict__ y_bar__,
+                                  real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real x = states[0];
+  const real y = states[1];
+  const real z = states[2];
+  const real beta = pa.template get<0>();
+  const real rho = pa.template get<1>();
+  const real sigma = pa.template get<2>();
+  const real rhs__0 = sigma*(y - x);
+  const real rhs__1 = -y + x*(rho - z);
+  const real rhs__2 = x*y - beta*z;
+  real x__b = real(0.0);
+  real y__b = real(0.0);
+  real z__b = real(0.0);
+  x__b += seed__[2]*(y);
+  y__b += seed__[2]*(x);
+  z__b += seed__[2]*(-beta);
+  x__b += seed__[1]*(rho - z);
+  y__b += seed__[1]*(-real(1.0));
+  z__b += seed__[1]*(-x);
+  x__b += seed__[0]*(-sigma);
+  y__b += seed__[0]*(sigma);
+  y_bar__[0] += x__b;
+  y_bar__[1] += y__b;
+  y_bar__[2] += z__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP, typename SV, typename YV>
+GOSS_HD GOSS_INLINE void eval_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                       const SV& seed__, const YV& y_bar__, real* __restrict__ fp_bar__,
+                                       const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real x = states[0];
+  const real y = states[1];
+  const real z = states[2];
+  const real beta = pa.template get<0>();
+  const real rho = pa.template get<1>();
+  const real sigma = pa.template get<2>();
+  tape.put(0, sigma*(y - x));
+  tape.put(1, -y + x*(rho - z));
+  tape.put(2, x*y - beta*z);
+  tape.put(3, real(0.0));
+  tape.put(4, real(0.0));
+  tape.put(5, real(0.0));
+  tape.add(3, seed__[2]*(y));
+  tape.add(4, seed__[2]*(x));
+  tape.add(5, seed__[2]*(-beta));
+  tape.add(3, seed__[1]*(rho - z));
+  tape.add(4, seed__[1]*(-real(1.0)));
+  tape.add(5, seed__[1]*(-x));
+  tape.add(3, seed__[0]*(-sigma));
+  tape.add(4, seed__[0]*(sigma));
+  y_bar__[0] += tape.get(3);
+  y_bar__[1] += tape.get(4);
+  y_bar__[2] += tape.get(5);
+}
+
+constexpr int grl1_vjp_tape_slots = 8;
+constexpr int rl1_vjp_tape_slots = 8;
+constexpr int eval_vjp_tape_slots = 6;
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 // `jac` is a pointer or any object whose jac[k] = value stores entry k.

@@ -62,17 +62,20 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   if (only_linear) return;
 }
 
-// Discrete adjoint of one GRL1 step (GRL1.h:72-84) of size h at `time`:
-//   lam_prev = lam + (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam,
-// where y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h;
-// for a gate (rhs_i = (A - y_i) r) this is A + (y_i - A) exp(linear_i*h).
-// Reverse accumulation over the linearized_eval statements.
+// Discrete adjoints of one step of size h at `time` (one step per interval):
+//   *_vjp:  lam_prev = (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam;
+//   grl1: y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h (GRL1.h:72-84);
+//   rl1:  the same with the exponential update for the linear_terms states only (RL1.h:68-77);
+//   for a gate (rhs_i = (A - y_i) r) the exponential update is A + (y_i - A) exp(linear_i*h).
+//   eval_vjp: y_bar += (d rhs/d y)^T seed, fp_bar += (d rhs/d p_field)^T seed (the explicit schemes).
+// Reverse accumulation over the generated statements; the *_tape forms keep every intermediate
+// and adjoint in a caller-provided tape (for targets whose registers cannot hold the sweep).
 template <typename real, typename PA, typename treal>
 GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
                                   const real h__, const real delta__, const real* __restrict__ lam__,
                                   real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
 {
-  (void)time; (void)pa; (void)fp_bar__;
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
   const real x = states[0];
   const real y = states[1];
   const real a = pa.template get<0>();
@@ -83,9 +86,9 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real lin__1 = real(0.0);
   const real hinv__ = m_rcp(h__);
   real bbar__0, abar__0;
-  m_grl1_seeds(lin__0, m_rcp(lin__0), rhs__0, lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
+  m_grl1_seeds(real(lin__0), m_rcp(real(lin__0)), real(rhs__0), lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
   real bbar__1, abar__1;
-  m_grl1_seeds(lin__1, m_rcp(lin__1), rhs__1, lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
+  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   real a__b = real(0.0);
   real b__b = real(0.0);
   real x__b = real(0.0);
@@ -100,16 +103,13 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   fp_bar__[1] += b__b;
 }
 
-constexpr int grl1_vjp_tape_slots = 8;
-// grl1_vjp with its intermediates and adjoints in `tape` (grl1_vjp_tape_slots slots):
-// for targets whose registers cannot hold the whole reverse sweep.
 template <typename real, typename PA, typename treal, typename TP>
 GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
-                                       const real h__, const real delta__, const real* __restrict__ lam__,
-                                       real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
-                                       const TP& tape)
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
 {
-  (void)time; (void)pa; (void)fp_bar__;
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
   const real x = states[0];
   const real y = states[1];
   const real a = pa.template get<0>();
@@ -118,10 +118,11 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   tape.put(1, b*x);
   tape.put(2, real(0.0));
   tape.put(3, real(0.0));
+  const real hinv__ = m_rcp(h__);
   real bbar__0, abar__0;
-  m_grl1_seeds(real(tape.get(2)), m_rcp(real(tape.get(2))), real(tape.get(0)), lam__[0], h__, m_rcp(h__), delta__, bbar__0, abar__0);
+  m_grl1_seeds(real(tape.get(2)), m_rcp(real(tape.get(2))), real(tape.get(0)), lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
   real bbar__1, abar__1;
-  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(1)), lam__[1], h__, m_rcp(h__), delta__, bbar__1, abar__1);
+  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(1)), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
   tape.put(4, real(0.0));
   tape.put(5, real(0.0));
   tape.put(6, real(0.0));
@@ -135,6 +136,129 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   fp_bar__[0] += tape.get(4);
   fp_bar__[1] += tape.get(5);
 }
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void rl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real x = states[0];
+  const real y = states[1];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  const real rhs__0 = -a*y;
+  const real rhs__1 = b*x;
+  const real lin__0 = real(0.0);
+  const real lin__1 = real(0.0);
+  const real hinv__ = m_rcp(h__);
+  real bbar__0, abar__0;
+  m_grl1_seeds(real(lin__0), m_rcp(real(lin__0)), real(rhs__0), lam__[0], h__, hinv__, real(-1.0), bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
+  real a__b = real(0.0);
+  real b__b = real(0.0);
+  real x__b = real(0.0);
+  real y__b = real(0.0);
+  b__b += bbar__1*(x);
+  x__b += bbar__1*(b);
+  a__b += bbar__0*(-y);
+  y__b += bbar__0*(-a);
+  lam_prev__[0] = lam__[0] + x__b;
+  lam_prev__[1] = lam__[1] + y__b;
+  fp_bar__[0] += a__b;
+  fp_bar__[1] += b__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP>
+GOSS_HD GOSS_INLINE void rl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real x = states[0];
+  const real y = states[1];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  tape.put(0, -a*y);
+  tape.put(1, b*x);
+  tape.put(2, real(0.0));
+  tape.put(3, real(0.0));
+  const real hinv__ = m_rcp(h__);
+  real bbar__0, abar__0;
+  m_grl1_seeds(real(tape.get(2)), m_rcp(real(tape.get(2))), real(tape.get(0)), lam__[0], h__, hinv__, real(-1.0), bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(1)), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
+  tape.put(4, real(0.0));
+  tape.put(5, real(0.0));
+  tape.put(6, real(0.0));
+  tape.put(7, real(0.0));
+  tape.add(5, bbar__1*(x));
+  tape.add(6, bbar__1*(b));
+  tape.add(4, bbar__0*(-y));
+  tape.add(7, bbar__0*(-a));
+  lam_prev__[0] = lam__[0] + tape.get(6);
+  lam_prev__[1] = lam__[1] + tape.get(7);
+  fp_bar__[0] += tape.get(4);
+  fp_bar__[1] += tape.get(5);
+}
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real* __restrict__ seed__, real* __restrict__ y_bar__,
+                                  real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real x = states[0];
+  const real y = states[1];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  const real rhs__0 = -a*y;
+  const real rhs__1 = b*x;
+  real a__b = real(0.0);
+  real b__b = real(0.0);
+  real x__b = real(0.0);
+  real y__b = real(0.0);
+  b__b += seed__[1]*(x);
+  x__b += seed__[1]*(b);
+  a__b += seed__[0]*(-y);
+  y__b += seed__[0]*(-a);
+  y_bar__[0] += x__b;
+  y_bar__[1] += y__b;
+  fp_bar__[0] += a__b;
+  fp_bar__[1] += b__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP, typename SV, typename YV>
+GOSS_HD GOSS_INLINE void eval_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                       const SV& seed__, const YV& y_bar__, real* __restrict__ fp_bar__,
+                                       const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real x = states[0];
+  const real y = states[1];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  tape.put(0, -a*y);
+  tape.put(1, b*x);
+  tape.put(2, real(0.0));
+  tape.put(3, real(0.0));
+  tape.put(4, real(0.0));
+  tape.put(5, real(0.0));
+  tape.add(3, seed__[1]*(x));
+  tape.add(4, seed__[1]*(b));
+  tape.add(2, seed__[0]*(-y));
+  tape.add(5, seed__[0]*(-a));
+  y_bar__[0] += tape.get(4);
+  y_bar__[1] += tape.get(5);
+  fp_bar__[0] += tape.get(2);
+  fp_bar__[1] += tape.get(3);
+}
+
+constexpr int grl1_vjp_tape_slots = 8;
+constexpr int rl1_vjp_tape_slots = 8;
+constexpr int eval_vjp_tape_slots = 6;
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
 // `jac` is a pointer or any object whose jac[k] = value stores entry k.

@@ -1,8 +1,10 @@
-// Discrete adjoint of GRL1 time stepping (GRL1.cpp:61-82, one step per outer
-// interval) for gradients of a loss on sampled field states with respect to
-// the initial states and the per-node field parameters.
+// Discrete adjoint of explicit time stepping -- GRL1 (GRL1.cpp:61-82), RL1
+// (RL1.cpp:59-79), ExplicitEuler (ExplicitEuler.cpp:64-87) and RK4
+// (RK4.cpp:68-102), one step per outer interval -- for gradients of a loss on
+// sampled field states with respect to the initial states and the per-node
+// field parameters.
 //
-// Forward (adjoint_record_kernel): the GRL1 kernel's step, plus a checkpoint
+// Forward (adjoint_record_kernel): the solver kernel's step, plus a checkpoint
 // of all states every `ckpt_every` steps ([n_ckpt][S][ld]) and a sample of
 // one field state every `sample_every` steps ([n_samples][N], FP64).
 //
@@ -11,8 +13,9 @@
 // segment's states from its checkpoint with the same step code (bit-identical
 // to the forward) into a segment buffer ([ckpt_every][S][ld]);
 // adjoint_reverse_kernel runs the steps in reverse: at step k (y_k -> y_k+1) the loss gradient of a
-// sample taken after step k is added to lam_k+1, and the generated grl1_vjp
-// gives lam_k = lam_k+1 + (dy_k+1/dy_k)^T lam_k+1 and the field-parameter
+// sample taken after step k is added to lam_k+1, and the step's adjoint
+// (StepVjp: the generated grl1_vjp / rl1_vjp, or eval_vjp composed over the
+// stages) gives lam_k = (dy_k+1/dy_k)^T lam_k+1 and the field-parameter
 // contribution (dy_k+1/dp)^T lam_k+1.  lam stays in registers within a
 // segment and in HBM between segments; parameter gradients accumulate in
 // FP64.  Splitting recompute and reverse sweep into two kernels keeps lam
@@ -52,6 +55,189 @@ struct VTape {
   GOSS_HD GOSS_INLINE void add(const int k, const R v) const { put(k, get(k) + v); }
 };
 
+// S consecutive tape slots seen as a vector: v[i] is slot off + i.
+template <class R>
+struct TapeVec {
+  VTape<R> t;
+  int off;
+  GOSS_HD GOSS_INLINE volatile R& operator[](const int i) const { return *t.at(off + i); }
+};
+
+// Adjoint of one step y -> y' of size h at time t: lp = (dy'/dy)^T lam and
+// pb += (dy'/dp_field)^T lam.  TAPE: the generated *_tape forms, with the
+// intermediates and adjoints in a per-node tape (FP64).
+template <class M, int SOLVER, class R>
+struct StepVjp;
+
+// Tape slots one step's adjoint needs.
+template <class M, int SOLVER>
+constexpr int step_vjp_tape_slots()
+{
+  if constexpr (SOLVER == kGRL1) return M::grl1_vjp_tape_slots;
+  else if constexpr (SOLVER == kRL1) return M::rl1_vjp_tape_slots;
+  else if constexpr (SOLVER == kRK4) return M::eval_vjp_tape_slots;
+  else return M::eval_vjp_tape_slots;
+}
+
+template <class M, class R>
+struct StepVjp<M, kGRL1, R> {
+  template <class A, class TP>
+  GOSS_HD GOSS_INLINE static void run(const R* y, double t, R h, const A& acc, R delta, const R* lam, R* lp, R* pb,
+                                         const TP* tape, const TP&)
+  {
+    if constexpr (sizeof(R) == 8) M::template grl1_vjp_tape<R>(y, t, acc, h, delta, lam, lp, pb, *tape);
+    else M::template grl1_vjp<R>(y, t, acc, h, delta, lam, lp, pb);
+  }
+};
+
+template <class M, class R>
+struct StepVjp<M, kRL1, R> {
+  template <class A, class TP>
+  GOSS_HD GOSS_INLINE static void run(const R* y, double t, R h, const A& acc, R delta, const R* lam, R* lp, R* pb,
+                                         const TP* tape, const TP&)
+  {
+    if constexpr (sizeof(R) == 8) M::template rl1_vjp_tape<R>(y, t, acc, h, delta, lam, lp, pb, *tape);
+    else M::template rl1_vjp<R>(y, t, acc, h, delta, lam, lp, pb);
+  }
+};
+
+// y' = y + h f(y, t) (ExplicitEuler.cpp:76-85): lp = lam + f_y^T (h lam).
+// In FP64 the seed is a work row (rows 0..S).
+template <class M, class R>
+struct StepVjp<M, kExplicitEuler, R> {
+  template <class A, class TP>
+  GOSS_HD GOSS_INLINE static void run(const R* y, double t, R h, const A& acc, R, const R* lam, R* lp, R* pb,
+                                         const TP* tape, const TP& work)
+  {
+    constexpr int S = M::num_states;
+    if constexpr (sizeof(R) == 8) {
+      const TapeVec<R> seed_v{work, 0};
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] = lam[i];
+        seed_v[i] = h * lam[i];
+      }
+      M::template eval_vjp_tape<R>(y, t, acc, seed_v, lp, pb, *tape);
+    } else {
+      R seed[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] = lam[i];
+        seed[i] = h * lam[i];
+      }
+      M::template eval_vjp<R>(y, t, acc, seed, lp, pb);
+    }
+  }
+};
+
+// RK4 (RK4.cpp:79-101): k1 = f(y, t), k2 = f(y + h/2 k1, t + h/2),
+// k3 = f(y + h/2 k2, t + h/2), k4 = f(y + h k3, t + h),
+// y' = y + h (k1 + 2 k2 + 2 k3 + k4) / 6.  The stage inputs are recomputed
+// into the node's work rows (HBM: they live across the four reverse stages,
+// which need the register file), then the stages are swept in reverse:
+// kb_s = d y'/d k_s lam plus what later stages pass back through their inputs.
+// In FP64 lam, the running result, the seed and the stage's y_bar are work
+// rows too (rows 3S.. 7S), so that only one stage input is held in registers
+// across a stage's sweep.
+template <class M, class R>
+struct StepVjp<M, kRK4, R> {
+  template <class A, class TP>
+  GOSS_HD GOSS_INLINE static void run(const R* y, double t, R h, const A& acc, R, const R* lam, R* lp, R* pb,
+                                         const TP* tape, const TP& work)
+  {
+    constexpr int S = M::num_states;
+    const double hd = double(h);
+    const R hh = R(0.5) * h;
+    {
+      R k[S], yi[S];
+      M::template eval<R>(y, t, acc, k);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        yi[i] = y[i] + hh * k[i];
+        work.put(i, yi[i]);
+      }
+      M::template eval<R>(yi, t + 0.5 * hd, acc, k);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        yi[i] = y[i] + hh * k[i];
+        work.put(S + i, yi[i]);
+      }
+      M::template eval<R>(yi, t + 0.5 * hd, acc, k);
+#pragma unroll
+      for (int i = 0; i < S; ++i) work.put(2 * S + i, y[i] + h * k[i]);
+    }
+    const R w1 = h / R(6), w2 = h / R(3);
+    // Stage inputs, newest first, and the weight of lam in each stage's seed.
+    const int in_row[4] = {2 * S, S, 0, -1};
+    const R wl[4] = {w1, w2, w2, w1};
+    const R wy[4] = {R(0), h, hh, hh};  // d (stage s input) / d k_(s-1) of the stage swept before
+    if constexpr (sizeof(R) == 8) {
+      const TapeVec<R> lam_v{work, 3 * S}, lp_v{work, 4 * S}, seed_v{work, 5 * S}, yb_v{work, 6 * S};
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lam_v[i] = lam[i];
+        lp_v[i] = lam[i];
+        yb_v[i] = R(0);
+      }
+#pragma unroll 1
+      for (int s = 0; s < 4; ++s) {
+        R yi[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          seed_v[i] = wl[s] * lam_v[i] + wy[s] * yb_v[i];
+          yb_v[i] = R(0);
+          yi[i] = s < 3 ? R(work.get(in_row[s] + i)) : y[i];
+        }
+        const double ts = s == 0 ? t + hd : (s < 3 ? t + 0.5 * hd : t);
+        M::template eval_vjp_tape<R>(yi, ts, acc, seed_v, yb_v, pb, *tape);
+#pragma unroll
+        for (int i = 0; i < S; ++i) lp_v[i] = lp_v[i] + yb_v[i];
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) lp[i] = lp_v[i];
+    } else {
+      R seed[S], yb[S], yi[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] = lam[i];
+        seed[i] = w1 * lam[i];  // kb4
+        yb[i] = R(0);
+        yi[i] = work.get(2 * S + i);
+      }
+      M::template eval_vjp<R>(yi, t + hd, acc, seed, yb, pb);  // stage 4: input y + h k3
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] += yb[i];
+        seed[i] = w2 * lam[i] + h * yb[i];  // kb3
+        yb[i] = R(0);
+        yi[i] = work.get(S + i);
+      }
+      M::template eval_vjp<R>(yi, t + 0.5 * hd, acc, seed, yb, pb);  // stage 3: input y + h/2 k2
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] += yb[i];
+        seed[i] = w2 * lam[i] + hh * yb[i];  // kb2
+        yb[i] = R(0);
+        yi[i] = work.get(i);
+      }
+      M::template eval_vjp<R>(yi, t + 0.5 * hd, acc, seed, yb, pb);  // stage 2: input y + h/2 k1
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] += yb[i];
+        seed[i] = w1 * lam[i] + hh * yb[i];  // kb1
+      }
+      M::template eval_vjp<R>(y, t, acc, seed, lp, pb);  // stage 1: input y
+    }
+  }
+};
+
+// Per-node work rows a step's adjoint keeps in HBM (besides the FP64 tape).
+template <class M, int SOLVER>
+constexpr int step_vjp_work_rows()
+{
+  return SOLVER == kRK4 ? 7 * M::num_states : (SOLVER == kExplicitEuler ? M::num_states : 0);
+}
+
 struct AdjointShape {
   int n;             // outer steps
   int ckpt_every;    // steps per checkpoint segment
@@ -59,7 +245,7 @@ struct AdjointShape {
   int field_index;   // state index that is sampled
 };
 
-template <class M, class R>
+template <class M, class R, int SOLVER>
 __global__ void __launch_bounds__(kBlock)
 adjoint_record_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2* __restrict__ steps, const int step0,
                       const int nouter, const SharedParamsOf<M, R> sp, const R delta, const unsigned N,
@@ -85,7 +271,7 @@ adjoint_record_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2
 #pragma unroll
       for (int i = 0; i < S; ++i) c[size_t(i) * ld + node] = st.y[i];
     }
-    Step<M, kGRL1, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
+    Step<M, SOLVER, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
     if ((gk + 1) % shp.sample_every == 0) {
       R v = R(0);
 #pragma unroll
@@ -99,7 +285,7 @@ adjoint_record_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2
 
 // Backward, part 1 of a segment: recompute its states y_k0 .. y_k1-1 from the
 // checkpoint into the segment buffer.
-template <class M, class R>
+template <class M, class R, int SOLVER>
 __global__ void __launch_bounds__(kBlock)
 adjoint_recompute_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, const double2* __restrict__ steps,
                          const SharedParamsOf<M, R> sp, const R delta, const unsigned N, const unsigned ld,
@@ -124,7 +310,7 @@ adjoint_recompute_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, c
     R* out = seg + size_t(k - k0) * S * ld;
 #pragma unroll
     for (int i = 0; i < S; ++i) out[size_t(i) * ld + node] = st.y[i];
-    if (k + 1 < k1) Step<M, kGRL1, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
+    if (k + 1 < k1) Step<M, SOLVER, R, false>::run(st, steps[k].x, steps[k].y, acc, delta);
   }
 }
 
@@ -132,7 +318,7 @@ adjoint_recompute_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, c
 // ([S][ld], precision R) and the parameter gradients ([NFP][ld], FP64) carry
 // over between segments in HBM; `first` starts them at zero, `last` writes
 // the results node-major.
-template <class M, class R>
+template <class M, class R, int SOLVER>
 __global__ void __launch_bounds__(kBlock)
 adjoint_reverse_kernel(const R* __restrict__ fp, const double2* __restrict__ steps, const SharedParamsOf<M, R> sp,
                        const R delta, const unsigned N, const unsigned ld, const AdjointShape shp, const int seg_index,
@@ -170,18 +356,17 @@ adjoint_reverse_kernel(const R* __restrict__ fp, const double2* __restrict__ ste
     for (int i = 0; i < S; ++i) yk[i] = in[size_t(i) * ld + node];
 #pragma unroll
     for (int j = 0; j < NFP; ++j) pb[j] = R(0);
-    if constexpr (sizeof(R) == 8) {
-      // FP64: the reverse sweep's intermediates do not fit in registers; they go to the tape.
-      // The tape base is made opaque per step so that the compiler does not hoist the
-      // ~440 per-slot addresses (VGPR pairs) out of the step loop; the empty asm emits nothing.
-      R* tp = tape;
+    // FP64: the reverse sweep's intermediates do not fit in registers; they go to the tape.
+    // The tape base is made opaque per step so that the compiler does not hoist the
+    // per-slot addresses (VGPR pairs) out of the step loop; the empty asm emits nothing.
+    // The work rows (RK4's stage inputs) follow the FP64 tape.
+    R* tp = tape;
 #ifdef __HIP_DEVICE_COMPILE__
-      asm volatile("" : "+s"(tp));
+    asm volatile("" : "+s"(tp));
 #endif
-      M::template grl1_vjp_tape<R>(yk, steps[k].x, acc, R(steps[k].y), delta, lam, lp, pb, VTape<R>{tp, ld, node});
-    } else {
-      M::template grl1_vjp<R>(yk, steps[k].x, acc, R(steps[k].y), delta, lam, lp, pb);
-    }
+    const VTape<R> vt{tp, ld, node};
+    const VTape<R> work{tp + (sizeof(R) == 8 ? size_t(step_vjp_tape_slots<M, SOLVER>()) : 0) * ld, ld, node};
+    StepVjp<M, SOLVER, R>::run(yk, steps[k].x, R(steps[k].y), acc, delta, lam, lp, pb, &vt, work);
 #pragma unroll
     for (int i = 0; i < S; ++i) lam[i] = lp[i];
 #pragma unroll

@@ -590,11 +590,12 @@ namespace {
 void adjoint_check(goss_hip_system* s, uint64_t n, uint32_t sample_every, uint32_t checkpoint_every,
                    uint64_t* n_ckpt, uint64_t* n_samples)
 {
-  if (s->solver != kGRL1) throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint is available for GRL1 systems only");
+  if (adjoint_index(s->solver) < 0)
+    throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint is available for GRL1, RL1, ExplicitEuler and RK4 systems only");
   if (s->prec != kFP64 && s->prec != kFP32)
     throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint runs in FP64 or FP32 (not FP32C)");
   if (s->sp.internal_time_step > 0)
-    throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint takes one GRL1 step per interval (internal_time_step <= 0)");
+    throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint takes one step per interval (internal_time_step <= 0)");
   if (s->v->NFS < 1) throw Error(GOSS_HIP_EINVAL, "the model has no field state to sample");
   if (n < 2 || n - 1 > uint64_t(INT32_MAX)) throw Error(GOSS_HIP_EINVAL, "need 2 .. 2^31 time points");
   if (sample_every == 0 || checkpoint_every == 0) throw Error(GOSS_HIP_EINVAL, "intervals must be positive");
@@ -622,10 +623,12 @@ int goss_hip_adjoint_buffer_sizes(goss_hip_system* s, uint64_t n, uint32_t sampl
     adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
     const uint64_t per = uint64_t(s->v->S) * s->ld * s->elem();
     *checkpoint_bytes = nc * per;
-    // The segment buffer holds the recomputed states of one checkpoint segment
-    // and, in FP64, the per-node tape of the reverse sweep behind them.
+    // The segment buffer holds the recomputed states of one checkpoint segment,
+    // then the reverse sweep's per-node tape (FP64) and work rows (RK4's stage inputs).
     // Behind them: lam between segments ([S][ld]) and the parameter gradients ([NFP][ld], FP64).
-    const uint64_t tape = s->prec == kFP64 ? uint64_t(s->v->vjp_tape_slots) * s->ld * sizeof(double) : 0;
+    const int ai = adjoint_index(s->solver);
+    const uint64_t tape = (s->prec == kFP64 ? uint64_t(s->v->vjp_tape_slots[ai]) * s->ld * sizeof(double) : 0) +
+                          uint64_t(s->v->vjp_work_rows[ai]) * s->ld * s->elem();
     *segment_bytes = uint64_t(checkpoint_every) * per + tape + per + uint64_t(std::max(s->v->NFP, 1)) * s->ld * 8;
     *num_samples = ns;
   });
@@ -639,7 +642,7 @@ int goss_hip_adjoint_record(goss_hip_system* s, const double* t, uint64_t n, uin
     uint64_t nc = 0, ns = 0;
     adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
     s->use_device();
-    const AdjointLaunch launch = s->v->adjoint_record[s->prec];
+    const AdjointLaunch launch = s->v->adjoint_record[adjoint_index(s->solver)][s->prec];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "adjoint not compiled for this model");
     const std::vector<double2> st = make_steps(s, t, n);
     const double2* d = s->upload_steps(st.data(), st.size());
@@ -672,7 +675,7 @@ int goss_hip_adjoint_backward(goss_hip_system* s, const double* t, uint64_t n, u
     uint64_t nc = 0, ns = 0;
     adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
     s->use_device();
-    const AdjointLaunch launch = s->v->adjoint_backward[s->prec];
+    const AdjointLaunch launch = s->v->adjoint_backward[adjoint_index(s->solver)][s->prec];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "adjoint not compiled for this model");
     const std::vector<double2> st = make_steps(s, t, n);
     const double2* d = s->upload_steps(st.data(), st.size());
@@ -689,7 +692,9 @@ int goss_hip_adjoint_backward(goss_hip_system* s, const double* t, uint64_t n, u
     a.grad_samples = grad_samples;
     a.seg = segment;
     const size_t per = size_t(s->v->S) * s->ld * s->elem();
-    const size_t tape = s->prec == kFP64 ? size_t(s->v->vjp_tape_slots) * s->ld * sizeof(double) : 0;
+    const int ai = adjoint_index(s->solver);
+    const size_t tape = (s->prec == kFP64 ? size_t(s->v->vjp_tape_slots[ai]) * s->ld * sizeof(double) : 0) +
+                        size_t(s->v->vjp_work_rows[ai]) * s->ld * s->elem();
     char* base = static_cast<char*>(segment);
     a.tape = base + size_t(checkpoint_every) * per;
     a.lam = base + size_t(checkpoint_every) * per + tape;

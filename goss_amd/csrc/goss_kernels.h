@@ -272,8 +272,11 @@ __device__ GOSS_INLINE void rl_stage(R* out, NodeState<M, R, COMP>& n, const R* 
     const bool use_exp = GEN ? (fabs(lin[i]) > delta) : (M::linear_terms[i] != 0);
     const R y0 = n.y[i];
     if constexpr (sizeof(R) == 8) {
-      // Same expression tree as the reference update.
-      const double v = use_exp ? rl_update(y0, lin[i], rhs[i], h) : y0 + rhs[i] * h;
+      // Same expression tree as the reference update; both values are computed and
+      // one is selected (no lane-divergent branch, as for the model's piecewise terms).
+      const double v_exp = rl_update(y0, lin[i], rhs[i], h);
+      const double v_lin = y0 + rhs[i] * h;
+      const double v = use_exp ? v_exp : v_lin;
       if constexpr (APPLY) n.y[i] = v; else out[i] = v;
     } else {
       // Both branches are cheap; select instead of branching (lanes diverge on |a| > delta).

@@ -11,6 +11,7 @@
 #pragma once
 
 #include <cmath>
+#include <type_traits>
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -19,6 +20,24 @@
 #else
 #define GOSS_HD
 #define GOSS_INLINE inline
+#endif
+
+// Piecewise expressions print as GOSS_SELECT(cond, a, b).  In the FP64 device
+// kernels both arms are evaluated and one is selected (no divergent branch
+// inside a model evaluation): with an if/else over a lane-dependent condition
+// (V < -40 in the TP06 gates), hipcc (ROCm 7.2, gfx950) has placed a register
+// copy of a live state in the join block ahead of the exec-mask restore, so
+// lanes from one arm kept a stale value.  The FP32 kernels keep the branches
+// (each arm's transcendentals run only where a wavefront needs them).
+template <class A, class B>
+GOSS_HD GOSS_INLINE typename std::common_type<A, B>::type m_select(const bool c, const A a, const B b)
+{
+  return c ? a : b;
+}
+#if defined(__HIP_DEVICE_COMPILE__) && defined(GOSS_PREC_F64)
+#define GOSS_SELECT(c, a, b) m_select((c), (a), (b))
+#else
+#define GOSS_SELECT(c, a, b) ((c) ? (a) : (b))
 #endif
 
 template <typename T> GOSS_HD GOSS_INLINE T m_sq(const T x) { return x * x; }
@@ -82,22 +101,23 @@ GOSS_HD GOSS_INLINE float m_tanh(const float x) { return std::tanh(x); }
 // used by the GRL1 adjoint (a GRL1 increment is rhs * h * phi1(linear * h)).  Near 0 both are
 // evaluated from their Taylor series (phi1 = sum x^k/(k+1)!, phi1' = sum k x^(k-1)/(k+1)!),
 // where the closed forms cancel; the switch points keep either form's relative error below
-// 1e-12 (double) and 2e-6 (float).
+// 1e-12 (double) and 2e-6 (float).  Both forms are evaluated and one selected (no lane-divergent
+// branch; the seeds below do the same).
 template <typename T>
 GOSS_HD GOSS_INLINE T m_phi1(const T x)
 {
   const T small = sizeof(T) == 8 ? T(1e-3) : T(1e-1);
-  if (m_fabs(x) < small)
-    return T(1) + x * (T(1.0 / 2) + x * (T(1.0 / 6) + x * (T(1.0 / 24) + x * (T(1.0 / 120) + x * T(1.0 / 720)))));
-  return std::expm1(x) / x;
+  const T series = T(1) + x * (T(1.0 / 2) + x * (T(1.0 / 6) + x * (T(1.0 / 24) + x * (T(1.0 / 120) + x * T(1.0 / 720)))));
+  const T closed = std::expm1(x) / x;
+  return m_select(m_fabs(x) < small, series, closed);
 }
 template <typename T>
 GOSS_HD GOSS_INLINE T m_dphi1(const T x)
 {
   const T small = sizeof(T) == 8 ? T(1e-3) : T(1e-1);
-  if (m_fabs(x) < small)
-    return T(1.0 / 2) + x * (T(1.0 / 3) + x * (T(1.0 / 8) + x * (T(1.0 / 30) + x * (T(1.0 / 144) + x * T(1.0 / 840)))));
-  return (x * std::exp(x) - std::expm1(x)) / (x * x);
+  const T series = T(1.0 / 2) + x * (T(1.0 / 3) + x * (T(1.0 / 8) + x * (T(1.0 / 30) + x * (T(1.0 / 144) + x * T(1.0 / 840)))));
+  const T closed = (x * std::exp(x) - std::expm1(x)) / (x * x);
+  return m_select(m_fabs(x) < small, series, closed);
 }
 // Seeds of the GRL1 step adjoint for one state: with x = linear * h and lh = lam * h,
 //   bbar = lh * phi1(x),  abar = lh * h * rhs * phi1'(x)   when |linear| > delta,
@@ -111,8 +131,9 @@ GOSS_HD GOSS_INLINE void m_grl1_seeds(const T linear, const T inv_linear, const 
   (void)inv_linear; (void)hinv;
   const bool ex = m_fabs(linear) > delta;
   const T lh = lam * h;
-  bbar = ex ? lh * m_phi1(linear * h) : lh;
-  abar = ex ? lh * rhs * h * m_dphi1(linear * h) : T(0);
+  const T p1 = m_phi1(linear * h), dp1 = m_dphi1(linear * h);
+  bbar = ex ? lh * p1 : lh;
+  abar = ex ? lh * rhs * h * dp1 : T(0);
 }
 // Seeds of the GRL1 step adjoint for a gate, whose step is y' = A + (y - A) exp(x), x = linear * h
 // (y + (linear (y - A)) h when |linear| <= delta):
@@ -124,7 +145,8 @@ GOSS_HD GOSS_INLINE void m_grl1_gate_seeds(const T linear, const T y, const T a,
 {
   const bool ex = m_fabs(linear) > delta;
   const T x = linear * h;
-  const T em1 = ex ? T(std::expm1(x)) : x;
+  const T e1 = T(std::expm1(x));
+  const T em1 = ex ? e1 : x;
   ey = lam * (T(1) + em1);
   ga = -lam * em1;
   gc = lam * h * (y - a) * (ex ? T(1) + em1 : T(1));

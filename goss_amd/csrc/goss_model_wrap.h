@@ -37,6 +37,31 @@
       NSN::compute_jacobian<R>(y, t, a, jac);                                                                        \
     }                                                                                                                \
     static constexpr int grl1_vjp_tape_slots = NSN::grl1_vjp_tape_slots;                                             \
+    static constexpr int rl1_vjp_tape_slots = NSN::rl1_vjp_tape_slots;                                               \
+    static constexpr int eval_vjp_tape_slots = NSN::eval_vjp_tape_slots;                                             \
+    template <typename R, typename A, typename T>                                                                    \
+    GOSS_HD GOSS_INLINE static void rl1_vjp(const R* y, const T t, const A& a, const R h, const R delta,             \
+                                            const R* lam, R* lam_prev, R* fp_bar)                                    \
+    {                                                                                                                \
+      NSN::rl1_vjp<R>(y, t, a, h, delta, lam, lam_prev, fp_bar);                                                     \
+    }                                                                                                                \
+    template <typename R, typename A, typename T, typename TP>                                                       \
+    GOSS_HD GOSS_INLINE static void rl1_vjp_tape(const R* y, const T t, const A& a, const R h, const R delta,        \
+                                                 const R* lam, R* lam_prev, R* fp_bar, const TP& tape)               \
+    {                                                                                                                \
+      NSN::rl1_vjp_tape<R>(y, t, a, h, delta, lam, lam_prev, fp_bar, tape);                                          \
+    }                                                                                                                \
+    template <typename R, typename A, typename T>                                                                    \
+    GOSS_HD GOSS_INLINE static void eval_vjp(const R* y, const T t, const A& a, const R* seed, R* y_bar, R* fp_bar)  \
+    {                                                                                                                \
+      NSN::eval_vjp<R>(y, t, a, seed, y_bar, fp_bar);                                                                \
+    }                                                                                                                \
+    template <typename R, typename A, typename T, typename TP, typename SV, typename YV>                             \
+    GOSS_HD GOSS_INLINE static void eval_vjp_tape(const R* y, const T t, const A& a, const SV& seed, const YV& y_bar, \
+                                                  R* fp_bar, const TP& tape)                                         \
+    {                                                                                                                \
+      NSN::eval_vjp_tape<R>(y, t, a, seed, y_bar, fp_bar, tape);                                                     \
+    }                                                                                                                \
     template <typename R, typename A, typename T>                                                                    \
     GOSS_HD GOSS_INLINE static void grl1_vjp(const R* y, const T t, const A& a, const R h, const R delta,            \
                                              const R* lam, R* lam_prev, R* fp_bar)                                   \

@@ -60,8 +60,9 @@ struct AdaptiveArgs {
 
 typedef hipError_t (*AdaptiveLaunch)(const AdaptiveArgs&);
 
-// Adjoint of GRL1 stepping: record (forward with checkpoints and samples) and
-// backward (the reverse sweep).  Precision FP64 or FP32.
+// Adjoint of explicit stepping (GRL1, RL1, ExplicitEuler, RK4): record (forward
+// with checkpoints and samples) and backward (the reverse sweep).  Precision
+// FP64 or FP32.
 struct AdjointArgs {
   void* y;                 // states [S][ld] (record: advanced in place)
   const void* fp;          // field parameters [NFP][ld]
@@ -75,7 +76,7 @@ struct AdjointArgs {
   double* samples;         // record: [n_samples][N]
   const double* grad_samples;  // backward: [n_samples][N]
   void* seg;               // backward: [ckpt_every][S][ld]
-  void* tape;              // backward, FP64: [vjp_tape_slots][ld]
+  void* tape;              // backward: FP64 tape [vjp_tape_slots][ld], then work rows [vjp_work_rows][ld]
   void* lam;               // backward: adjoint states between segments [S][ld]
   double* pbar;            // backward: parameter gradients between segments [NFP][ld]
   double* grad_y0;         // backward: [N][S]
@@ -85,10 +86,21 @@ struct AdjointArgs {
 
 typedef hipError_t (*AdjointLaunch)(const AdjointArgs&);
 
+// Solvers with a device adjoint, in the order of the adjoint tables.
+constexpr int kAdjointSolvers[] = {kGRL1, kRL1, kExplicitEuler, kRK4};
+constexpr int kNumAdjointSolvers = 4;
+inline int adjoint_index(const int solver)
+{
+  for (int k = 0; k < kNumAdjointSolvers; ++k)
+    if (kAdjointSolvers[k] == solver) return k;
+  return -1;
+}
+
 struct VariantInfo {
   const char* name;
   int S, P, NFS, NFP, NCOMP;
-  int vjp_tape_slots;      // slots of the FP64 adjoint's per-node tape
+  int vjp_tape_slots[kNumAdjointSolvers];  // slots of the FP64 adjoint's per-node tape
+  int vjp_work_rows[kNumAdjointSolvers];   // per-node work rows of the step adjoint (precision R)
   const int* fsi;
   const int* fpi;
   const double* defaults;
@@ -101,8 +113,8 @@ struct VariantInfo {
   ExplicitLaunch explicit_launch[kNumSolvers][kNumPrecisions];
   ImplicitLaunch implicit_launch[2][kNumJacobianModes];  // [0] ImplicitEuler, [1] ESDIRK23a; [JacobianMode]
   AdaptiveLaunch rkf32_launch[kNumPrecisions];  // FP64 and FP32
-  AdjointLaunch adjoint_record[kNumPrecisions];    // FP64 and FP32
-  AdjointLaunch adjoint_backward[kNumPrecisions];
+  AdjointLaunch adjoint_record[kNumAdjointSolvers][kNumPrecisions];  // FP64 and FP32
+  AdjointLaunch adjoint_backward[kNumAdjointSolvers][kNumPrecisions];
 };
 
 // All variants, in registration order.

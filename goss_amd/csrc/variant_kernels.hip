@@ -56,28 +56,28 @@ hipError_t launch_rkf32(const AdaptiveArgs& a)
   return hipGetLastError();
 }
 
-template <class R>
+template <class R, int SOLVER>
 hipError_t launch_adjoint_record(const AdjointArgs& a)
 {
   const SharedParamsOf<M, R> sp = make_shared_params<M, R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((adjoint_record_kernel<M, R>), grid, dim3(kBlock), 0, a.stream, static_cast<R*>(a.y),
+  hipLaunchKernelGGL((adjoint_record_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.stream, static_cast<R*>(a.y),
                      static_cast<const R*>(a.fp), a.steps, a.step0, a.nouter, sp, R(a.delta), a.N, a.ld, a.shape,
                      static_cast<R*>(a.ckpt), a.samples);
   return hipGetLastError();
 }
 
-template <class R>
+template <class R, int SOLVER>
 hipError_t launch_adjoint_backward(const AdjointArgs& a)
 {
   const SharedParamsOf<M, R> sp = make_shared_params<M, R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
   const int nseg = (a.shape.n + a.shape.ckpt_every - 1) / a.shape.ckpt_every;
   for (int s = nseg - 1; s >= 0; --s) {
-    hipLaunchKernelGGL((adjoint_recompute_kernel<M, R>), grid, dim3(kBlock), 0, a.stream,
+    hipLaunchKernelGGL((adjoint_recompute_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.stream,
                        static_cast<const R*>(a.ckpt), static_cast<const R*>(a.fp), a.steps, sp, R(a.delta), a.N, a.ld,
                        a.shape, s, static_cast<R*>(a.seg));
-    hipLaunchKernelGGL((adjoint_reverse_kernel<M, R>), grid, dim3(kBlock), 0, a.stream, static_cast<const R*>(a.fp),
+    hipLaunchKernelGGL((adjoint_reverse_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.stream, static_cast<const R*>(a.fp),
                        a.steps, sp, R(a.delta), a.N, a.ld, a.shape, s, a.grad_samples,
                        static_cast<const R*>(a.seg), static_cast<R*>(a.tape), static_cast<R*>(a.lam),
                        a.pbar, int(s == nseg - 1), int(s == 0), a.grad_y0, a.grad_fp);
@@ -99,6 +99,19 @@ void register_family(VariantInfo& v, int prec)
   v.explicit_launch[kGRL2][prec] = &launch_explicit<kGRL2, R, COMP>;
 }
 
+template <class R>
+void register_adjoint(VariantInfo& v, const int prec)
+{
+  v.adjoint_record[adjoint_index(kGRL1)][prec] = &launch_adjoint_record<R, kGRL1>;
+  v.adjoint_backward[adjoint_index(kGRL1)][prec] = &launch_adjoint_backward<R, kGRL1>;
+  v.adjoint_record[adjoint_index(kRL1)][prec] = &launch_adjoint_record<R, kRL1>;
+  v.adjoint_backward[adjoint_index(kRL1)][prec] = &launch_adjoint_backward<R, kRL1>;
+  v.adjoint_record[adjoint_index(kExplicitEuler)][prec] = &launch_adjoint_record<R, kExplicitEuler>;
+  v.adjoint_backward[adjoint_index(kExplicitEuler)][prec] = &launch_adjoint_backward<R, kExplicitEuler>;
+  v.adjoint_record[adjoint_index(kRK4)][prec] = &launch_adjoint_record<R, kRK4>;
+  v.adjoint_backward[adjoint_index(kRK4)][prec] = &launch_adjoint_backward<R, kRK4>;
+}
+
 void fill_info(VariantInfo& v)
 {
   v.name = GOSS_STR(GOSS_VARIANT);
@@ -107,7 +120,14 @@ void fill_info(VariantInfo& v)
   v.NFS = NS::num_field_states;
   v.NFP = NS::num_field_parameters;
   v.NCOMP = num_comp<M>();
-  v.vjp_tape_slots = NS::grl1_vjp_tape_slots;
+  v.vjp_tape_slots[adjoint_index(kGRL1)] = step_vjp_tape_slots<M, kGRL1>();
+  v.vjp_tape_slots[adjoint_index(kRL1)] = step_vjp_tape_slots<M, kRL1>();
+  v.vjp_tape_slots[adjoint_index(kExplicitEuler)] = step_vjp_tape_slots<M, kExplicitEuler>();
+  v.vjp_tape_slots[adjoint_index(kRK4)] = step_vjp_tape_slots<M, kRK4>();
+  v.vjp_work_rows[adjoint_index(kGRL1)] = step_vjp_work_rows<M, kGRL1>();
+  v.vjp_work_rows[adjoint_index(kRL1)] = step_vjp_work_rows<M, kRL1>();
+  v.vjp_work_rows[adjoint_index(kExplicitEuler)] = step_vjp_work_rows<M, kExplicitEuler>();
+  v.vjp_work_rows[adjoint_index(kRK4)] = step_vjp_work_rows<M, kRK4>();
   v.fsi = NS::field_state_indices;
   v.fpi = NS::field_parameter_indices;
   v.defaults = NS::default_parameters;
@@ -131,8 +151,7 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
   v.implicit_launch[1][kJacobianFDFP32] = &launch_implicit<kESDIRK23a, kJacobianFDFP32>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
-  v.adjoint_record[kFP64] = &launch_adjoint_record<double>;
-  v.adjoint_backward[kFP64] = &launch_adjoint_backward<double>;
+  register_adjoint<double>(v, kFP64);
 }
 #elif defined(GOSS_PREC_F32)
 void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
@@ -141,8 +160,7 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
   register_family<float, false>(v, kFP32);
   register_family<float, true>(v, kFP32C);
   v.rkf32_launch[kFP32] = &launch_rkf32<float>;
-  v.adjoint_record[kFP32] = &launch_adjoint_record<float>;
-  v.adjoint_backward[kFP32] = &launch_adjoint_backward<float>;
+  register_adjoint<float>(v, kFP32);
 }
 #else
 #error "define GOSS_PREC_F64 or GOSS_PREC_F32"

@@ -184,11 +184,11 @@ int goss_hip_system_set_steps_per_launch(goss_hip_system* sys, uint32_t steps);
  * RKF32 (RKF32.h:70-72): out[0] 0, out[1] nfevals, out[2] ndtsr, out[3] ndtsa.
  * Zeros for the fixed-step solvers. */
 int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
-/* Discrete adjoint of GRL1 stepping.  No counterpart in the reference: it
- * serves gradient-based calibration of per-node parameters from the Python
- * side (a torch.autograd.Function over these calls).  Requirements: solver
- * GRL1, precision FP64 or FP32, internal_time_step <= 0 (one step per
- * interval).  The times are those of goss_hip_system_forward_times: n points,
+/* Discrete adjoint of explicit time stepping.  No counterpart in the
+ * reference: it serves gradient-based calibration of per-node parameters from
+ * the Python side (a torch.autograd.Function over these calls).
+ * Requirements: solver GRL1, RL1, ExplicitEuler or RK4, precision FP64 or
+ * FP32, internal_time_step <= 0 (one step per interval).  The times are those of goss_hip_system_forward_times: n points,
  * n-1 steps.  The first field state is sampled after every `sample_every`
  * steps.
  *

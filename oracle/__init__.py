@@ -77,9 +77,8 @@ def lib():
         L.oracle_model_linearized_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int]
         L.oracle_model_analytic_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
         L.oracle_model_fd_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
-        L.oracle_model_grl1_vjp.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, ctypes.c_double,
-                                            ctypes.c_double, _dp, _dp, _dp]
-        L.oracle_model_grl1_vjp_tape.argtypes = L.oracle_model_grl1_vjp.argtypes
+        L.oracle_model_step_vjp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, _dp,
+                                            ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         L.oracle_lu_factorize.argtypes = [ctypes.c_int, _dp]
         L.oracle_forward_backward_subst.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         _lib = L
@@ -174,16 +173,26 @@ class Variant:
         lib().oracle_model_analytic_jacobian(self.index, _ptr(y), float(t), _ptr(p), _ptr(jac))
         return jac.reshape(self.num_states, self.num_states)
 
-    def grl1_vjp(self, y, t, h, lam, params=None, delta=1e-8, tape=False):
-        """(lam + (dy'/dy)^T lam, (dy'/dp_field)^T lam) for one GRL1 step y -> y' (generated code;
-        tape=True: the variant that keeps intermediates and adjoints in a tape)."""
+    def _step_vjp(self, kind, y, t, h, lam, params, delta, tape, out):
         p = _f64(self.default_parameters if params is None else params)
         y, lam = _f64(y), _f64(lam)
-        lam_prev = np.zeros(self.num_states)
         fp_bar = np.zeros(max(self.num_field_parameters, 1))
-        fn = lib().oracle_model_grl1_vjp_tape if tape else lib().oracle_model_grl1_vjp
-        fn(self.index, _ptr(y), float(t), _ptr(p), float(h), float(delta), _ptr(lam), _ptr(lam_prev), _ptr(fp_bar))
-        return lam_prev, fp_bar[: self.num_field_parameters]
+        lib().oracle_model_step_vjp(self.index, kind, int(tape), _ptr(y), float(t), _ptr(p), float(h), float(delta),
+                                    _ptr(lam), _ptr(out), _ptr(fp_bar))
+        return out, fp_bar[: self.num_field_parameters]
+
+    def grl1_vjp(self, y, t, h, lam, params=None, delta=1e-8, tape=False):
+        """((dy'/dy)^T lam, (dy'/dp_field)^T lam) for one GRL1 step y -> y' (generated code;
+        tape=True: the variant that keeps intermediates and adjoints in a tape)."""
+        return self._step_vjp(0, y, t, h, lam, params, delta, tape, np.zeros(self.num_states))
+
+    def rl1_vjp(self, y, t, h, lam, params=None, tape=False):
+        """The same for one RL1 step (RL1.h:68-77)."""
+        return self._step_vjp(1, y, t, h, lam, params, 1e-8, tape, np.zeros(self.num_states))
+
+    def eval_vjp(self, y, t, seed, params=None, tape=False):
+        """((d rhs/dy)^T seed, (d rhs/dp_field)^T seed) (generated code)."""
+        return self._step_vjp(2, y, t, 0.0, seed, params, 0.0, tape, np.zeros(self.num_states))
 
     def grl1_step(self, y, t, h, params=None, delta=1e-8):
         """One GRL1 step (GRL1.h:72-84) in FP64 from linearized_eval."""

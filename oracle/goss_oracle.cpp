@@ -55,11 +55,10 @@ struct VariantOps {
   void (*eval)(const double*, double, const double*, const double*, double*);
   void (*linearized)(const double*, double, const double*, const double*, double*, double*, bool);
   void (*jacobian)(const double*, double, const double*, const double*, double*);
-  // Generated discrete adjoint of one GRL1 step (test infrastructure for the device adjoint).
-  void (*grl1_vjp)(const double*, double, const double*, const double*, double, double, const double*, double*,
-                   double*);
-  void (*grl1_vjp_tape)(const double*, double, const double*, const double*, double, double, const double*, double*,
-                        double*);
+  // Generated step adjoints (test infrastructure for the device adjoint): kind 0 grl1_vjp, 1 rl1_vjp,
+  // 2 eval_vjp (lam is the seed, out accumulates y_bar); tape selects the *_tape form.
+  void (*step_vjp)(int kind, bool tape, const double*, double, const double*, const double*, double, double,
+                   const double*, double*, double*);
 };
 
 // Host tape for grl1_vjp_tape (slot k at v[k]).
@@ -86,15 +85,20 @@ struct HostTape {
              [](const double* y, double t, const double* p, const double* c, double* jac) {            \
                goss_models::V::compute_jacobian<double>(y, t, Acc{p, c}, jac);                         \
              },                                                                                        \
-             [](const double* y, double t, const double* p, const double* c, double h, double delta,    \
-                const double* lam, double* lam_prev, double* fp_bar) {                                 \
-               goss_models::V::grl1_vjp<double>(y, t, Acc{p, c}, h, delta, lam, lam_prev, fp_bar);      \
-             },                                                                                        \
-             [](const double* y, double t, const double* p, const double* c, double h, double delta,    \
-                const double* lam, double* lam_prev, double* fp_bar) {                                 \
-               std::vector<double> tape(goss_models::V::grl1_vjp_tape_slots);                          \
-               goss_models::V::grl1_vjp_tape<double>(y, t, Acc{p, c}, h, delta, lam, lam_prev, fp_bar,  \
-                                                     HostTape{tape.data()});                           \
+             [](int kind, bool tape, const double* y, double t, const double* p, const double* c, double h,  \
+                double delta, const double* lam, double* out, double* fp_bar) {                         \
+               namespace NS = goss_models::V;                                                          \
+               const Acc a{p, c};                                                                      \
+               int n = kind == 0 ? NS::grl1_vjp_tape_slots                                             \
+                                 : (kind == 1 ? NS::rl1_vjp_tape_slots : NS::eval_vjp_tape_slots);     \
+               std::vector<double> tp(size_t(tape ? n : 0));                                           \
+               const HostTape T{tp.data()};                                                            \
+               if (kind == 0 && !tape) NS::grl1_vjp<double>(y, t, a, h, delta, lam, out, fp_bar);      \
+               if (kind == 0 && tape) NS::grl1_vjp_tape<double>(y, t, a, h, delta, lam, out, fp_bar, T); \
+               if (kind == 1 && !tape) NS::rl1_vjp<double>(y, t, a, h, delta, lam, out, fp_bar);       \
+               if (kind == 1 && tape) NS::rl1_vjp_tape<double>(y, t, a, h, delta, lam, out, fp_bar, T); \
+               if (kind == 2 && !tape) NS::eval_vjp<double>(y, t, a, lam, out, fp_bar);                \
+               if (kind == 2 && tape) NS::eval_vjp_tape<double>(y, t, a, lam, out, fp_bar, T);         \
              }}
 
 const VariantOps kVariants[] = {
@@ -925,21 +929,15 @@ void oracle_model_fd_jacobian(int v, double* y, double t, const double* p, doubl
   kVariants[v].constants(p, c);
   fd_jacobian(kVariants[v], y, t, p, c, jac, f1.data(), f2.data());
 }
-// lam_prev = lam + (d y'/d y)^T lam and fp_bar += (d y'/d p_field)^T lam for one GRL1
-// step y -> y' of size h at time t (generated grl1_vjp).
-void oracle_model_grl1_vjp(int v, const double* y, double t, const double* p, double h, double delta,
-                           const double* lam, double* lam_prev, double* fp_bar)
+// Generated step adjoints for one step y -> y' of size h at time t: kind 0 GRL1 and 1 RL1 give
+// out = (d y'/d y)^T lam and fp_bar += (d y'/d p_field)^T lam (grl1_vjp, rl1_vjp); kind 2 gives
+// out += (d rhs/d y)^T lam and fp_bar += (d rhs/d p_field)^T lam (eval_vjp).  tape: the *_tape form.
+void oracle_model_step_vjp(int v, int kind, int tape, const double* y, double t, const double* p, double h,
+                           double delta, const double* lam, double* out, double* fp_bar)
 {
   double c[kMaxConstants];
   kVariants[v].constants(p, c);
-  kVariants[v].grl1_vjp(y, t, p, c, h, delta, lam, lam_prev, fp_bar);
-}
-void oracle_model_grl1_vjp_tape(int v, const double* y, double t, const double* p, double h, double delta,
-                                const double* lam, double* lam_prev, double* fp_bar)
-{
-  double c[kMaxConstants];
-  kVariants[v].constants(p, c);
-  kVariants[v].grl1_vjp_tape(y, t, p, c, h, delta, lam, lam_prev, fp_bar);
+  kVariants[v].step_vjp(kind, tape != 0, y, t, p, c, h, delta, lam, out, fp_bar);
 }
 void oracle_lu_factorize(int n, double* mat) { lu_factorize(n, mat); }
 void oracle_forward_backward_subst(int n, const double* mat, const double* b, double* x) { fb_subst(n, mat, b, x); }

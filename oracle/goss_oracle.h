@@ -41,10 +41,8 @@ void oracle_model_linearized_eval(int v, const double* states, double t, const d
 void oracle_model_analytic_jacobian(int v, const double* states, double t, const double* params, double* jac);
 /* ODE::compute_jacobian (ODE.cpp:47-67): forward differences. */
 void oracle_model_fd_jacobian(int v, double* states, double t, const double* params, double* jac);
-void oracle_model_grl1_vjp(int v, const double* states, double t, const double* params, double h, double delta,
-                           const double* lam, double* lam_prev, double* fp_bar);
-void oracle_model_grl1_vjp_tape(int v, const double* states, double t, const double* params, double h, double delta,
-                                const double* lam, double* lam_prev, double* fp_bar);
+void oracle_model_step_vjp(int v, int kind, int tape, const double* states, double t, const double* params,
+                           double h, double delta, const double* lam, double* out, double* fp_bar);
 /* ODE::lu_factorize (ODE.cpp:69-100) and forward_backward_subst (ODE.cpp:102-132). */
 void oracle_lu_factorize(int n, double* mat);
 void oracle_forward_backward_subst(int n, const double* mat, const double* b, double* x);

@@ -3,13 +3,18 @@
 // the host with the host math library.  With the same math library as the
 // oracle, any difference from the oracle's per-node mode is a logic
 // difference of the kernel code; what remains on the GPU is the last-bit
-// difference of the device exp/log/pow.
+// difference of the device exp/log/pow.  The step adjoints of
+// goss_adjoint.h (StepVjp, the composition the reverse-sweep kernel runs) are
+// built here too, in FP64 (tape form, as on the device) and FP32.
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "goss_adaptive.h"
+#include "goss_adjoint.h"
 #include "goss_implicit.h"
+#include "goss_model_wrap.h"
+#include "tp06_m_fp6.h"
 #include "tp06_m.h"
 #include "tp06_epi.h"
 #include "beeler_reuter_1977.h"
@@ -140,7 +145,55 @@ int run(int solver, unsigned N, const double* y0, const double* t, unsigned long
   return failed_any;
 }
 
+GOSS_MODEL_TYPE(MW_tp06_m_fp6, goss_models::tp06_m_fp6);
+
+// One step adjoint of StepVjp<M, SOLVER, R> at (y, t, h) with seed lam; params are all P parameters.
+template <class M, int SOLVER, class R>
+void step_vjp(const double* y, double t, double h, const double* lam, const double* prm, double delta, double* lp,
+              double* pb)
+{
+  constexpr int S = M::num_states;
+  constexpr int NFP = M::num_field_parameters;
+  const SharedParamsOf<M, R> sp = make_shared_params<M, R>(prm);
+  R fpv[NFP > 0 ? NFP : 1];
+  for (int j = 0; j < NFP; ++j) fpv[j] = R(prm[M::field_parameter_indices[j]]);
+  const DevAcc<M, R> acc{sp, fpv};
+  R yr[S], lr[S], lpr[S], pbr[NFP > 0 ? NFP : 1];
+  for (int i = 0; i < S; ++i) {
+    yr[i] = R(y[i]);
+    lr[i] = R(lam[i]);
+  }
+  for (int j = 0; j < NFP; ++j) pbr[j] = R(0);
+  const int slots = sizeof(R) == 8 ? step_vjp_tape_slots<M, SOLVER>() : 0;
+  std::vector<R> mem(size_t(slots + step_vjp_work_rows<M, SOLVER>()) + 1);
+  const VTape<R> tape{mem.data(), 1u, 0u};
+  const VTape<R> work{mem.data() + slots, 1u, 0u};
+  StepVjp<M, SOLVER, R>::run(yr, t, R(h), acc, R(delta), lr, lpr, pbr, &tape, work);
+  for (int i = 0; i < S; ++i) lp[i] = double(lpr[i]);
+  for (int j = 0; j < NFP; ++j) pb[j] = double(pbr[j]);
+}
+
 }  // namespace
+
+// The reverse-sweep kernel's step adjoint on the host, for tp06_m_fp6:
+// solver GRL1 / RL1 / ExplicitEuler / RK4, fp64 nonzero for double.
+extern "C" int kernel_logic_step_vjp(int solver, int fp64, const double* y, double t, double h, const double* lam,
+                                     const double* prm, double delta, double* lp, double* pb)
+{
+  using M = MW_tp06_m_fp6;
+#define GOSS_HARNESS_VJP(SOLVER)                                                        \
+  if (solver == SOLVER) {                                                               \
+    if (fp64) step_vjp<M, SOLVER, double>(y, t, h, lam, prm, delta, lp, pb);            \
+    else step_vjp<M, SOLVER, float>(y, t, h, lam, prm, delta, lp, pb);                  \
+    return 0;                                                                           \
+  }
+  GOSS_HARNESS_VJP(kGRL1)
+  GOSS_HARNESS_VJP(kRL1)
+  GOSS_HARNESS_VJP(kExplicitEuler)
+  GOSS_HARNESS_VJP(kRK4)
+#undef GOSS_HARNESS_VJP
+  return -2;
+}
 
 extern "C" int kernel_logic_run(const char* variant, int solver, unsigned N, const double* y0, const double* t,
                                 unsigned long n, const double* params, double ldt_0, double eta_0, double kappa,

@@ -102,3 +102,41 @@ def test_grl1_vjp_tape_variant_agrees(name):
         b = m.grl1_vjp(y, t, 0.02, lam, tape=True)
         np.testing.assert_allclose(b[0], a[0], rtol=1e-12, atol=1e-15 * np.abs(a[0]).max())
         np.testing.assert_allclose(b[1], a[1], rtol=1e-12, atol=1e-15 * max(np.abs(a[1]).max(initial=0), 1e-300))
+
+
+@pytest.mark.parametrize("name", MODELS)
+@pytest.mark.parametrize("t", [5.0, 10.3])
+def test_rl1_and_eval_vjp_match_torch_autograd(name, t):
+    # rl1_vjp: one RL1 step; eval_vjp: the right-hand side alone (the explicit schemes' building block).
+    tm = TorchModel(name)
+    m = oracle.Variant(name)
+    rng = np.random.default_rng(11)
+    N = 4
+    y = _state(m, rng, N)
+    fpi = m.field_parameter_indices
+    fp = np.array([m.default_parameters[k] for k in fpi])[None, :].repeat(N, 0) * np.exp(
+        0.1 * rng.normal(size=(N, len(fpi))))
+    h = 0.02
+    lam = rng.normal(size=(N, m.num_states))
+    for kind in ("rl1", "eval"):
+        if kind == "rl1":
+            lin, _ = m.linearized_eval(y[0], t)
+            if np.any(lin[np.array(m.linear_terms, dtype=bool)] == 0):
+                continue  # RL1 divides by a zero linear coefficient here (RL1.h:68-77): the step is NaN
+        yt = torch.tensor(y, requires_grad=True)
+        ft = torch.tensor(fp, requires_grad=True)
+        out = tm.rl1_step(yt, t, h, tm.params(ft)) if kind == "rl1" else tm.rhs(yt, t, tm.params(ft))
+        (out * torch.tensor(lam)).sum().backward()
+        for n in range(N):
+            p = m.default_parameters.copy()
+            for j, k in enumerate(fpi):
+                p[k] = fp[n, j]
+            for tape in (False, True):
+                if kind == "rl1":
+                    gy, gp = m.rl1_vjp(y[n], t, h, lam[n], p, tape=tape)
+                else:
+                    gy, gp = m.eval_vjp(y[n], t, lam[n], p, tape=tape)
+                g = yt.grad[n].numpy()
+                np.testing.assert_allclose(gy, g, rtol=1e-9, atol=1e-12 * np.abs(g).max(), err_msg=f"{kind} {tape}")
+                if fpi:
+                    np.testing.assert_allclose(gp, ft.grad[n].numpy(), rtol=1e-9, atol=1e-14)

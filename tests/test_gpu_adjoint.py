@@ -36,10 +36,10 @@ def _times(nsteps, dt=0.02, t0=9.0):
     return t0 + np.arange(nsteps + 1) * dt
 
 
-def _device_grads(prec, y0, fp, t, sample_every, ckpt_every, weights):
+def _device_grads(prec, y0, fp, t, sample_every, ckpt_every, weights, solver="GRL1"):
     from goss_amd.adjoint import voltage_trace
 
-    sys_ = dev_system(VARIANT, "GRL1", y0.shape[0], prec)
+    sys_ = dev_system(VARIANT, solver, y0.shape[0], prec)
     yt = torch.tensor(y0, requires_grad=True)
     ft = torch.tensor(fp, requires_grad=True)
     v = voltage_trace(sys_, yt, ft, t, sample_every, ckpt_every)
@@ -172,9 +172,55 @@ def test_adjoint_fp32_close_to_fp64():
         assert rel_l2[cols].max() <= 5e-3  # measured 3.9e-4
 
 
-def test_adjoint_requires_grl1():
+def test_adjoint_requires_a_supported_solver():
     from goss_amd._lib import GossHipError
 
-    s = dev_system(VARIANT, "RK4", 4, "fp64")
-    with pytest.raises(GossHipError, match="GRL1"):
+    s = dev_system(VARIANT, "RK2", 4, "fp64")
+    with pytest.raises(GossHipError, match="GRL1, RL1, ExplicitEuler and RK4"):
         s.adjoint_buffer_sizes(11, 5, 5)
+
+
+# The explicit schemes: RL1 at the GRL1 step; RK4 at 2 us and ExplicitEuler at 1 us (TP06's fast gates
+# make 5 us unstable for both, and 2 us for Euler with these perturbed nodes), from 9.8 ms through the
+# stimulus and the upstroke.
+EXPLICIT_CASES = [("RL1", 0.02, 300, 25, 40), ("ExplicitEuler", 0.001, 1200, 100, 128), ("RK4", 0.002, 600, 50, 64)]
+
+
+@pytest.mark.parametrize("solver,dt,nsteps,se,ce", EXPLICIT_CASES)
+def test_explicit_adjoint_fp64_matches_torch_autograd(solver, dt, nsteps, se, ce):
+    from torch_reference import TorchModel, sampled_trace
+
+    N = 5
+    y0, fp = _inputs(N, 71)
+    t = 9.8 + np.arange(nsteps + 1) * dt
+    w = np.random.Generator(np.random.PCG64(72)).normal(size=(nsteps // se, N))
+    loss, v, gy, gp = _device_grads("fp64", y0, fp, t, se, ce, w, solver)
+    tm = TorchModel(VARIANT)
+    yt = torch.tensor(y0, requires_grad=True)
+    ft = torch.tensor(fp, requires_grad=True)
+    vr = sampled_trace(tm, yt, ft, t, se, 17, solver=solver)
+    (vr * torch.tensor(w)).sum().backward()
+    print(f"{solver}: V {v.min():.2f} .. {v.max():.2f} mV; samples rel err {_rel(v, vr.detach().numpy()):.2e}; "
+          f"grad y0 {_rel(gy, yt.grad.numpy()):.2e}; grad g {_rel(gp, ft.grad.numpy()):.2e}")
+    assert np.all(np.isfinite(v))
+    assert _rel(v, vr.detach().numpy()) <= 1e-10
+    assert _rel(gy, yt.grad.numpy()) <= 1e-8
+    assert _rel(gp, ft.grad.numpy()) <= 1e-8
+
+
+@pytest.mark.parametrize("solver,dt,nsteps,se,ce", EXPLICIT_CASES)
+def test_explicit_adjoint_fp32_close_to_fp64_and_checkpoint_invariant(solver, dt, nsteps, se, ce):
+    N = 128
+    y0, fp = _inputs(N, 73)
+    t = 9.8 + np.arange(nsteps + 1) * dt
+    w = np.random.Generator(np.random.PCG64(74)).normal(size=(nsteps // se, N))
+    ref = _device_grads("fp64", y0, fp, t, se, ce, w, solver)
+    other = _device_grads("fp64", y0, fp, t, se, 7, w, solver)
+    assert np.array_equal(ref[2], other[2]) and np.array_equal(ref[3], other[3])
+    got = _device_grads("fp32", y0, fp, t, se, ce, w, solver)
+    for k, name in ((2, "y0"), (3, "conductances")):
+        a, b = got[k], ref[k]
+        rel_l2 = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
+        cols = np.linalg.norm(b, axis=0) > 0
+        print(f"{solver} FP32 vs FP64, d loss / d {name}: max relative L2 per column {rel_l2[cols].max():.2e}")
+        assert np.all(np.isfinite(a)) and rel_l2[cols].max() <= 5e-3

@@ -123,3 +123,53 @@ def test_rkf32_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, tol):
     assert rc == 0
     np.testing.assert_array_equal(out, o.states())
     np.testing.assert_array_equal(np.array(list(counters))[1:], o.counters()[1:])
+
+
+STEP_VJP_SOLVERS = {"GRL1": 5, "RL1": 3, "ExplicitEuler": 0, "RK4": 2}
+
+
+@pytest.mark.parametrize("solver", list(STEP_VJP_SOLVERS))
+@pytest.mark.parametrize("fp64,tol", [(1, 1e-12), (0, 2e-4)])
+def test_step_adjoint_logic_matches_torch_autograd(solver, fp64, tol):
+    """The reverse-sweep kernel's step adjoint (StepVjp in goss_adjoint.h, built for the host) against
+    torch autograd of the same step (tests/torch_reference.py, FP64) for tp06_m_fp6: lp = (dy'/dy)^T lam
+    and pb = (dy'/d field parameters)^T lam, at states on both sides of the V = -40 mV switch of the
+    sodium gates and with the stimulus on."""
+    import torch
+
+    from torch_reference import TorchModel
+
+    L = _harness()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.kernel_logic_step_vjp.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp, dp,
+                                        ctypes.c_double, dp, dp]
+    meta = json.load(open(META))["tp06_m_fp6"]
+    tm = TorchModel("tp06_m_fp6")
+    S = tm.S
+    fpi = tm.field_parameter_indices
+    rng = np.random.Generator(np.random.PCG64(91))
+    ic = np.asarray(meta["initial_conditions"], dtype=np.float64)
+    vi = meta["state_names"].index("V")
+    for V in (-85.0, -41.0, -39.0, 15.0):
+        y = ic * (1.0 + rng.uniform(-0.02, 0.02, S))
+        y[vi] = V
+        prm = np.asarray(meta["default_parameters"], dtype=np.float64) * np.exp(rng.normal(0.0, 0.1, tm.P))
+        lam = rng.normal(size=S)
+        t, h = 10.3, 0.01
+        lp = np.zeros(S)
+        pb = np.zeros(len(fpi))
+        rc = L.kernel_logic_step_vjp(STEP_VJP_SOLVERS[solver], fp64, _p(y), t, h, _p(lam), _p(prm), 1e-8, _p(lp),
+                                     _p(pb))
+        assert rc == 0
+        yt = torch.tensor(y[None], requires_grad=True)
+        pt = torch.tensor(prm[None])
+        ft = torch.tensor(prm[fpi][None], requires_grad=True)
+        cols = [pt[:, k] for k in range(tm.P)]
+        for j, k in enumerate(fpi):
+            cols[k] = ft[:, j]
+        y1 = tm.step(solver, yt, t, h, torch.stack(cols, dim=1))
+        (y1[0] * torch.tensor(lam)).sum().backward()
+        ref_y, ref_p = yt.grad.numpy()[0], ft.grad.numpy()[0]
+        err_y = np.abs(lp - ref_y).max() / np.abs(ref_y).max()
+        err_p = np.abs(pb - ref_p).max() / np.abs(ref_p).max()
+        assert err_y <= tol and err_p <= tol, (solver, fp64, V, err_y, err_p)

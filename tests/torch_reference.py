@@ -84,6 +84,7 @@ class TorchModel:
         lin = linearize(model)
         self.S = model.num_states
         self.P = model.num_parameters
+        self.linear_terms = torch.tensor([bool(x) for x in lin.linear_terms])
         self.field_parameter_indices = var.field_parameter_indices
         names: Dict[str, str] = {}
         for i, s in enumerate(model.state_names):
@@ -123,6 +124,33 @@ class TorchModel:
         inc = torch.where(big, rhs * torch.expm1(safe * h) / safe, rhs * h)
         return y + inc
 
+    def rhs(self, y, t: float, p):
+        return self.linearized(y, t, p)[1]
+
+    def rl1_step(self, y, t: float, h: float, p):
+        """RL1 (RL1.h:68-77): exponential update for the linear_terms states, Euler for the others."""
+        lin, rhs = self.linearized(y, t, p)
+        lt = self.linear_terms[None, :].expand_as(lin)
+        safe = torch.where(lt, lin, torch.ones_like(lin))
+        return y + torch.where(lt, rhs * torch.expm1(safe * h) / safe, rhs * h)
+
+    def ee_step(self, y, t: float, h: float, p):
+        """ExplicitEuler (ExplicitEuler.cpp:76-85)."""
+        return y + h * self.rhs(y, t, p)
+
+    def rk4_step(self, y, t: float, h: float, p):
+        """RK4 (RK4.cpp:79-101)."""
+        k1 = self.rhs(y, t, p)
+        k2 = self.rhs(y + 0.5 * h * k1, t + 0.5 * h, p)
+        k3 = self.rhs(y + 0.5 * h * k2, t + 0.5 * h, p)
+        k4 = self.rhs(y + h * k3, t + h, p)
+        return y + h * (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0
+
+    def step(self, solver: str, y, t: float, h: float, p, delta: float = 1e-8):
+        if solver == "GRL1":
+            return self.grl1_step(y, t, h, p, delta)
+        return {"RL1": self.rl1_step, "ExplicitEuler": self.ee_step, "RK4": self.rk4_step}[solver](y, t, h, p)
+
     def params(self, fp):
         """Per-node parameter matrix [N, P] with the field parameters fp [N, NFP] in their slots."""
         N = fp.shape[0]
@@ -132,13 +160,14 @@ class TorchModel:
         return torch.stack(cols, dim=1)
 
 
-def sampled_trace(model: TorchModel, y0, fp, t, sample_every: int, field_index: int, delta: float = 1e-8):
-    """GRL1 over the times t (one step per interval); returns [n_samples, N] of state field_index."""
+def sampled_trace(model: TorchModel, y0, fp, t, sample_every: int, field_index: int, delta: float = 1e-8,
+                  solver: str = "GRL1"):
+    """`solver` over the times t (one step per interval); returns [n_samples, N] of state field_index."""
     p = model.params(fp)
     y = y0
     out = []
     for k in range(len(t) - 1):
-        y = model.grl1_step(y, float(t[k]), float(t[k + 1] - t[k]), p, delta)
+        y = model.step(solver, y, float(t[k]), float(t[k + 1] - t[k]), p, delta)
         if (k + 1) % sample_every == 0:
             out.append(y[:, field_index])
     return torch.stack(out, dim=0)
