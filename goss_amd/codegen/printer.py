@@ -16,6 +16,8 @@ Printing rules (they fix the floating-point evaluation order):
   the FP32 kernels;
 * ``x**2``, ``x**3``, ``x**4``: ``m_sq``/``m_cube``/``m_sq(m_sq(x))``;
   ``x**0.5``: ``m_sqrt``; other powers: ``m_pow``;
+* ``exp(x)``: ``m_exp_l2(x log2(e), x)``, the first argument with log2(e) folded into the
+  literals (the FP32 kernels' exp2 form; double uses x);
 * ``Piecewise``: nested ``GOSS_SELECT(cond, a, b)`` (a ``?:`` select, or both
   arms evaluated and one selected, as goss_model_support.h defines it).
 """
@@ -71,6 +73,25 @@ def split_mul(e: sp.Mul) -> Tuple[int, List[sp.Expr], List[sp.Expr]]:
         else:
             num.append(f)
     return sign, num, den
+
+
+LOG2E = 1.4426950408889634
+
+
+def log2e_scaled(arg: sp.Expr) -> sp.Expr:
+    """arg * log2(e) with the constant folded into arg's numeric coefficients.
+
+    exp(x) = exp2(x log2 e).  The FP32 kernels evaluate exp2 in hardware; with log2 e folded into the
+    literals of x (term by term over a sum) the scaling costs no multiply of its own.  Double and
+    host code use x itself (``m_exp_l2``)."""
+    if arg.is_Number:
+        return sp.Float(float(arg) * LOG2E)
+    if isinstance(arg, sp.Add):
+        return sp.Add(*[log2e_scaled(t) for t in arg.args], evaluate=False)
+    coeff, rest = arg.as_coeff_Mul()
+    if coeff.is_Number:
+        return sp.Mul(sp.Float(float(coeff) * LOG2E), rest, evaluate=False)
+    return sp.Mul(sp.Float(LOG2E), arg, evaluate=False)
 
 
 class CPrinter:
@@ -170,7 +191,8 @@ class CPrinter:
                 return f"m_rcp(m_sqrt({bs}))", P["call"]
             return f"m_pow({bs}, {self.p(x)})", P["call"]
         if isinstance(e, sp.exp):
-            return f"m_exp({self.p(e.args[0])})", P["call"]
+            arg = e.args[0]
+            return f"m_exp_l2({self.p(log2e_scaled(arg))}, {self.p(arg)})", P["call"]
         if isinstance(e, sp.log):
             if len(e.args) != 1:
                 raise ValueError("log with base")

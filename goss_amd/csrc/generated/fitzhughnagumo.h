@@ -61,7 +61,7 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time,
   const real c__2 = pa.template cst<2>();
   const real c__3 = pa.template cst<3>();
   const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
-  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
+  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>()))), (treal(1.0) + m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time)));
   values[0] = I + real(i_Stim);
   values[1] = b*(V - V_rest - c_3*s);
 }
@@ -86,7 +86,7 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   const real c__5 = pa.template cst<5>();
   const real c__6 = pa.template cst<6>();
   const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
-  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>()))), (treal(1.0) + m_exp(pa.template cstd<4>() + treal(5.0)*time)));
+  const treal i_Stim = m_div(pa.template getd<7>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>()))), (treal(1.0) + m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time)));
   rhs[0] = I + real(i_Stim);
   rhs[1] = b*(V - V_rest - c_3*s);
   linear[1] = c__5;
@@ -123,10 +123,10 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   const real c__5 = pa.template cst<5>();
   const real c__6 = pa.template cst<6>();
   const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
-  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__0 = m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>());
   const treal tr__1 = treal(1.0) + tr__0;
   const treal tr__2 = m_rcp(tr__1);
-  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__3 = m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time);
   const treal tr__4 = treal(1.0) + tr__3;
   const treal tr__5 = m_rcp(tr__4);
   const treal i_Stim = pa.template getd<7>()*tr__5*(treal(1.0) - tr__2);
@@ -182,13 +182,13 @@ GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const tr
   const real c__5 = pa.template cst<5>();
   const real c__6 = pa.template cst<6>();
   tape.put(0, c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V));
-  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__0 = m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>());
   tape.put(1, real(tr__0));
   const treal tr__1 = treal(1.0) + tr__0;
   tape.put(2, real(tr__1));
   const treal tr__2 = m_rcp(tr__1);
   tape.put(3, real(tr__2));
-  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__3 = m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time);
   tape.put(4, real(tr__3));
   const treal tr__4 = treal(1.0) + tr__3;
   tape.put(5, real(tr__4));
@@ -241,10 +241,10 @@ GOSS_HD GOSS_INLINE void rl1_vjp(const real* __restrict__ states, const treal ti
   const real c__4 = pa.template cst<4>();
   const real c__5 = pa.template cst<5>();
   const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
-  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__0 = m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>());
   const treal tr__1 = treal(1.0) + tr__0;
   const treal tr__2 = m_rcp(tr__1);
-  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__3 = m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time);
   const treal tr__4 = treal(1.0) + tr__3;
   const treal tr__5 = m_rcp(tr__4);
   const treal i_Stim = pa.template getd<7>()*tr__5*(treal(1.0) - tr__2);
@@ -290,13 +290,13 @@ GOSS_HD GOSS_INLINE void rl1_vjp_tape(const real* __restrict__ states, const tre
   const real c__4 = pa.template cst<4>();
   const real c__5 = pa.template cst<5>();
   tape.put(0, c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V));
-  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__0 = m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>());
   tape.put(1, real(tr__0));
   const treal tr__1 = treal(1.0) + tr__0;
   tape.put(2, real(tr__1));
   const treal tr__2 = m_rcp(tr__1);
   tape.put(3, real(tr__2));
-  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__3 = m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time);
   tape.put(4, real(tr__3));
   const treal tr__4 = treal(1.0) + tr__3;
   tape.put(5, real(tr__4));
@@ -341,10 +341,10 @@ GOSS_HD GOSS_INLINE void eval_vjp(const real* __restrict__ states, const treal t
   const real c__3 = pa.template cst<3>();
   const real c__4 = pa.template cst<4>();
   const real I = c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V);
-  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__0 = m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>());
   const treal tr__1 = treal(1.0) + tr__0;
   const treal tr__2 = m_rcp(tr__1);
-  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__3 = m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time);
   const treal tr__4 = treal(1.0) + tr__3;
   const treal tr__5 = m_rcp(tr__4);
   const treal i_Stim = pa.template getd<7>()*tr__5*(treal(1.0) - tr__2);
@@ -383,13 +383,13 @@ GOSS_HD GOSS_INLINE void eval_vjp_tape(const real* __restrict__ states, const tr
   const real c__3 = pa.template cst<3>();
   const real c__4 = pa.template cst<4>();
   tape.put(0, c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V));
-  const treal tr__0 = m_exp(treal(5.0)*time - treal(5.0)*pa.template getd<10>());
+  const treal tr__0 = m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<10>(), treal(5.0)*time - treal(5.0)*pa.template getd<10>());
   tape.put(1, real(tr__0));
   const treal tr__1 = treal(1.0) + tr__0;
   tape.put(2, real(tr__1));
   const treal tr__2 = m_rcp(tr__1);
   tape.put(3, real(tr__2));
-  const treal tr__3 = m_exp(pa.template cstd<4>() + treal(5.0)*time);
+  const treal tr__3 = m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time);
   tape.put(4, real(tr__3));
   const treal tr__4 = treal(1.0) + tr__3;
   tape.put(5, real(tr__4));

@@ -138,7 +138,7 @@ __device__ GOSS_INLINE float expm1_f32(const float x)
 {
   const float p = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f))));
   const float q = __expf(x) - 1.0f;
-  return fabsf(x) < 0.015625f ? p : q;
+  return m_select(fabsf(x) < 0.015625f, p, q);
 }
 
 // Exact sum hi + lo += inc (Knuth's TwoSum; 6 additions).
@@ -282,7 +282,8 @@ __device__ GOSS_INLINE void rl_stage(R* out, NodeState<M, R, COMP>& n, const R* 
       // Both branches are cheap; select instead of branching (lanes diverge on |a| > delta).
       const float e = expm1_f32(lin[i] * h);
       const float inc_exp = rhs[i] * e * __builtin_amdgcn_rcpf(lin[i]);
-      const float inc = use_exp ? inc_exp : rhs[i] * h;
+      const float inc_lin = rhs[i] * h;
+      const float inc = m_select(use_exp, inc_exp, inc_lin);
       if constexpr (APPLY) n.add(i, inc); else out[i] = y0 + inc;
     }
   }

@@ -22,19 +22,19 @@
 #define GOSS_INLINE inline
 #endif
 
-// Piecewise expressions print as GOSS_SELECT(cond, a, b).  In the FP64 device
-// kernels both arms are evaluated and one is selected (no divergent branch
-// inside a model evaluation): with an if/else over a lane-dependent condition
-// (V < -40 in the TP06 gates), hipcc (ROCm 7.2, gfx950) has placed a register
-// copy of a live state in the join block ahead of the exec-mask restore, so
-// lanes from one arm kept a stale value.  The FP32 kernels keep the branches
-// (each arm's transcendentals run only where a wavefront needs them).
+// Piecewise expressions print as GOSS_SELECT(cond, a, b).  In device code both
+// arms are evaluated and one is selected, so a model evaluation has no
+// lane-divergent branch.  With an if/else over a lane-dependent condition
+// (V < -40 in the TP06 gates), hipcc (ROCm 7.2, gfx950) has placed register
+// copies in the join block ahead of the exec-mask restore, so lanes from the
+// other arm kept stale values: a state in an FP64 kernel, a per-node address
+// in an FP32 one.  Host code keeps the ?: form.
 template <class A, class B>
 GOSS_HD GOSS_INLINE typename std::common_type<A, B>::type m_select(const bool c, const A a, const B b)
 {
   return c ? a : b;
 }
-#if defined(__HIP_DEVICE_COMPILE__) && defined(GOSS_PREC_F64)
+#if defined(__HIP_DEVICE_COMPILE__)
 #define GOSS_SELECT(c, a, b) m_select((c), (a), (b))
 #else
 #define GOSS_SELECT(c, a, b) ((c) ? (a) : (b))
@@ -48,6 +48,8 @@ template <typename T> GOSS_HD GOSS_INLINE T m_heaviside(const T x) { return x > 
 GOSS_HD GOSS_INLINE double m_div(const double a, const double b) { return a / b; }
 GOSS_HD GOSS_INLINE double m_rcp(const double b) { return 1.0 / b; }
 GOSS_HD GOSS_INLINE double m_exp(const double x) { return std::exp(x); }
+// m_exp_l2(x * log2(e), x): the FP32 device kernels use the first argument (see below); double uses x.
+GOSS_HD GOSS_INLINE double m_exp_l2(const double x2, const double x) { (void)x2; return std::exp(x); }
 GOSS_HD GOSS_INLINE double m_log(const double x) { return std::log(x); }
 GOSS_HD GOSS_INLINE double m_sqrt(const double x) { return std::sqrt(x); }
 GOSS_HD GOSS_INLINE double m_fabs(const double x) { return std::fabs(x); }
@@ -74,7 +76,11 @@ GOSS_HD GOSS_INLINE float m_rcp(const float b) { return __builtin_amdgcn_rcpf(b)
 GOSS_HD GOSS_INLINE float m_exp(const float x) { return __expf(x); }
 GOSS_HD GOSS_INLINE float m_log(const float x) { return __logf(x); }
 GOSS_HD GOSS_INLINE float m_sqrt(const float x) { return __builtin_amdgcn_sqrtf(x); }
+// exp(x) given x2 = x * log2(e) with the constant folded into x's coefficients by the code
+// generator: one v_exp_f32, no scaling multiply.
+GOSS_HD GOSS_INLINE float m_exp_l2(const float x2, const float x) { (void)x; return __builtin_amdgcn_exp2f(x2); }
 #else
+GOSS_HD GOSS_INLINE float m_exp_l2(const float x2, const float x) { (void)x2; return std::exp(x); }
 GOSS_HD GOSS_INLINE float m_div(const float a, const float b) { return a / b; }
 GOSS_HD GOSS_INLINE float m_rcp(const float b) { return 1.0f / b; }
 GOSS_HD GOSS_INLINE float m_exp(const float x) { return std::exp(x); }
@@ -131,9 +137,10 @@ GOSS_HD GOSS_INLINE void m_grl1_seeds(const T linear, const T inv_linear, const 
   (void)inv_linear; (void)hinv;
   const bool ex = m_fabs(linear) > delta;
   const T lh = lam * h;
-  const T p1 = m_phi1(linear * h), dp1 = m_dphi1(linear * h);
-  bbar = ex ? lh * p1 : lh;
-  abar = ex ? lh * rhs * h * dp1 : T(0);
+  const T b_ex = lh * m_phi1(linear * h);
+  const T a_ex = lh * rhs * h * m_dphi1(linear * h);
+  bbar = m_select(ex, b_ex, lh);
+  abar = m_select(ex, a_ex, T(0));
 }
 // Seeds of the GRL1 step adjoint for a gate, whose step is y' = A + (y - A) exp(x), x = linear * h
 // (y + (linear (y - A)) h when |linear| <= delta):
@@ -146,10 +153,10 @@ GOSS_HD GOSS_INLINE void m_grl1_gate_seeds(const T linear, const T y, const T a,
   const bool ex = m_fabs(linear) > delta;
   const T x = linear * h;
   const T e1 = T(std::expm1(x));
-  const T em1 = ex ? e1 : x;
+  const T em1 = m_select(ex, e1, x);
   ey = lam * (T(1) + em1);
   ga = -lam * em1;
-  gc = lam * h * (y - a) * (ex ? T(1) + em1 : T(1));
+  gc = lam * h * (y - a) * m_select(ex, T(1) + em1, T(1));
 }
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GOSS_F32_ACCURATE_MATH)
 // FP32 device kernels: beyond |x| = 0.1 the closed forms use the hardware exp and reciprocal
@@ -177,10 +184,11 @@ GOSS_HD GOSS_INLINE void m_grl1_gate_seeds<float>(const float linear, const floa
   const bool ex = fabsf(linear) > delta;
   const float x = linear * h;
   const float p = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f))));
-  const float em1 = ex ? (fabsf(x) < 0.015625f ? p : __expf(x) - 1.0f) : x;
+  const float q = __expf(x) - 1.0f;
+  const float em1 = m_select(ex, m_select(fabsf(x) < 0.015625f, p, q), x);
   ey = lam * (1.0f + em1);
   ga = -lam * em1;
-  gc = lam * h * (y - a) * (ex ? 1.0f + em1 : 1.0f);
+  gc = lam * h * (y - a) * m_select(ex, 1.0f + em1, 1.0f);
 }
 // One exponential for both, 1/x = inv_linear * hinv: phi1 = (e - 1) / x, phi1' = (e - phi1) / x.
 // Below |x| = 0.05 the series (truncation below 1e-9 relative for phi1, 1e-7 for phi1'); above it
@@ -200,7 +208,9 @@ GOSS_HD GOSS_INLINE void m_grl1_seeds<float>(const float linear, const float inv
   const bool small = fabsf(x) < 0.05f;
   const bool ex = fabsf(linear) > delta;
   const float lh = lam * h;
-  bbar = ex ? lh * (small ? ps : pc) : lh;
-  abar = ex ? lh * rhs * h * (small ? dps : dpc) : 0.0f;
+  const float b_ex = lh * m_select(small, ps, pc);
+  const float a_ex = lh * rhs * h * m_select(small, dps, dpc);
+  bbar = m_select(ex, b_ex, lh);
+  abar = m_select(ex, a_ex, 0.0f);
 }
 #endif

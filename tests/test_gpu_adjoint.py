@@ -136,8 +136,16 @@ def test_record_samples_equal_forward(prec):
     b.set_field_parameters(fp)
     rows = np.zeros((len(t), N, 1))
     b.solve(t, rows)
-    assert np.array_equal(v, rows[se::se, :, 0])
-    assert np.array_equal(a.states(), b.states())
+    if prec == "fp64":
+        # FP64 is compiled without contraction: the record kernel's step is the forward kernel's, bit for bit.
+        assert np.array_equal(v, rows[se::se, :, 0])
+        assert np.array_equal(a.states(), b.states())
+    else:
+        # FP32 allows FMA contraction, which the compiler may form differently in the two kernels
+        # (measured: 2 of 100 nodes differ, by 1.2e-4 mV; FP32 vs FP64 is 3.8e-4 mV here).
+        assert np.abs(v - rows[se::se, :, 0]).max() <= 2e-3
+        sa, sb = a.states(), b.states()
+        assert (np.abs(sa - sb) <= 1e-4 * np.abs(sb) + 1e-12).all()
 
 
 def test_adjoint_fp32_close_to_fp64():
