@@ -175,18 +175,25 @@ def oracle_parity(y0, fp, t_all, gpu_final, nodes=64):
     return float((np.abs(a - b) / np.maximum(np.abs(b), floor)).max())
 
 
-def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every=200, seed=2025):
+def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every=None, seed=2025):
     """Adjoint workload: TP06 M-cell GRL1 in FP32 with six per-node conductances
     (g_Kr, g_Ks, g_CaL, g_Na, g_K1, g_to) and perturbed initial states; loss =
     MSE of V sampled every 1 ms against a target trace from the conductances x
     lognormal(0, 0.1); gradients with respect to the conductances and the 19
-    initial states, states checkpointed every 200 steps.  Times the forward
-    (record) and the backward (recompute + reverse sweep) kernels with HIP
-    events, after a short warm-up of both."""
+    initial states.  States are checkpointed every step when they fit in half of
+    the free device memory (the backward is then the reverse sweep alone),
+    otherwise every 200 steps (recompute + reverse sweep per segment).  Times
+    the forward (record), the backward and, for reference, the plain forward
+    (forward_times, no checkpoints) with HIP events, after a short warm-up."""
     import torch
 
     from goss_amd.adjoint import voltage_trace
     from goss_amd.system import DeviceSystem, ModelVariant
+
+    if checkpoint_every is None:
+        free, _ = torch.cuda.mem_get_info()
+        every_step = steps * N * 19 * 4
+        checkpoint_every = 1 if every_step <= free // 2 else 200
 
     variant = "tp06_m_fp6"
     mv = ModelVariant(variant)
@@ -199,7 +206,7 @@ def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every
     sys_ = DeviceSystem(variant, "GRL1", N, "fp32")
     sys_.set_steps_per_launch(250)
     with torch.no_grad():
-        target = voltage_trace(sys_, torch.tensor(y0), torch.tensor(target_fp), t, sample_every, checkpoint_every)
+        target = voltage_trace(sys_, torch.tensor(y0), torch.tensor(target_fp), t, sample_every, 200)
     # warm-up of both kernels on a short run
     yw = torch.tensor(y0, requires_grad=True)
     fw = torch.tensor(fp, requires_grad=True)
@@ -224,13 +231,28 @@ def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every
     fwd_ms = ev.elapsed_ms(marks[0], marks[1])
     bwd_ms = ev.elapsed_ms(marks[2], marks[3])
     gy, gf = yt.grad.numpy(), ft.grad.numpy()
+    loss_value = float(loss)
+    # the plain forward over the same steps (no checkpoints, no samples)
+    del v, loss
+    sys_.set_states(y0)
+    sys_.set_field_parameters(fp)
+    torch.cuda.synchronize()
+    plain = [ev.create() for _ in range(2)]
+    ev.record(plain[0], stream)
+    sys_.forward_times(t)
+    ev.record(plain[1], stream)
+    sys_.synchronize()
+    plain_ms = ev.elapsed_ms(plain[0], plain[1])
     return {
         "workload": f"TP06 M-cell GRL1 FP32, {N} nodes x {steps} steps (dt {dt} ms), V sampled every "
                     f"{sample_every} steps, checkpoints every {checkpoint_every} steps, grads wrt 6 conductances "
                     f"+ 19 initial states",
-        "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adjoint_over_forward": bwd_ms / fwd_ms,
+        "checkpoint_every": checkpoint_every,
+        "forward_ms": fwd_ms, "backward_ms": bwd_ms, "plain_forward_ms": plain_ms,
+        "adjoint_over_forward": bwd_ms / min(fwd_ms, plain_ms),
+        "backward_over_record": bwd_ms / fwd_ms, "backward_over_plain_forward": bwd_ms / plain_ms,
         "forward_node_steps_per_s": N * steps / (fwd_ms * 1e-3),
-        "loss": float(loss), "grads_finite": bool(np.all(np.isfinite(gy)) and np.all(np.isfinite(gf))),
+        "loss": loss_value, "grads_finite": bool(np.all(np.isfinite(gy)) and np.all(np.isfinite(gf))),
         "checkpoint_bytes": sys_.adjoint_buffer_sizes(len(t), sample_every, checkpoint_every)[0],
     }
 

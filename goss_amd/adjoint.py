@@ -7,7 +7,8 @@ the first field state sampled
 every ``sample_every`` steps, as a ``[n_samples, N]`` FP64 tensor on the GPU.
 Its backward runs the device discrete adjoint (``goss_hip_adjoint_backward``):
 the states are checkpointed every ``checkpoint_every`` steps in the forward
-and recomputed segment by segment in the backward sweep, and the gradients
+and recomputed segment by segment in the backward sweep (``checkpoint_every=1``
+stores every state and skips the recomputation), and the gradients
 with respect to the initial states ``[N, S]`` and the per-node field
 parameters ``[N, NFP]`` come back as FP64 tensors.
 

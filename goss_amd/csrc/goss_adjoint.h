@@ -238,6 +238,9 @@ constexpr int step_vjp_work_rows()
   return SOLVER == kRK4 ? 7 * M::num_states : (SOLVER == kExplicitEuler ? M::num_states : 0);
 }
 
+// Steps per reverse-sweep launch when every state is checkpointed (checkpoint interval 1).
+constexpr int kReverseChunk = 250;
+
 struct AdjointShape {
   int n;             // outer steps
   int ckpt_every;    // steps per checkpoint segment

@@ -72,6 +72,25 @@ hipError_t launch_adjoint_backward(const AdjointArgs& a)
 {
   const SharedParamsOf<M, R> sp = make_shared_params<M, R>(a.shared);
   const dim3 grid((a.N + kBlock - 1) / kBlock);
+  if (a.shape.ckpt_every == 1) {
+    // The record stored the state before every step: nothing to recompute.  The reverse
+    // sweep reads the checkpoints in place, kReverseChunk steps per launch (the chunk plays
+    // the role of a segment: k0 = chunk * kReverseChunk, states at ckpt + k * S * ld).
+    AdjointShape shp = a.shape;
+    shp.ckpt_every = kReverseChunk;
+    const int nch = (shp.n + kReverseChunk - 1) / kReverseChunk;
+    const size_t per = size_t(M::num_states) * a.ld;
+    for (int c = nch - 1; c >= 0; --c) {
+      const R* states = static_cast<const R*>(a.ckpt) + size_t(c) * kReverseChunk * per;
+      hipLaunchKernelGGL((adjoint_reverse_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.stream,
+                         static_cast<const R*>(a.fp), a.steps, sp, R(a.delta), a.N, a.ld, shp, c, a.grad_samples,
+                         states, static_cast<R*>(a.tape), static_cast<R*>(a.lam), a.pbar, int(c == nch - 1),
+                         int(c == 0), a.grad_y0, a.grad_fp);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   const int nseg = (a.shape.n + a.shape.ckpt_every - 1) / a.shape.ckpt_every;
   for (int s = nseg - 1; s >= 0; --s) {
     hipLaunchKernelGGL((adjoint_recompute_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.stream,

@@ -201,7 +201,10 @@ int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
  * goss_hip_adjoint_backward: given d loss / d samples (device, [n_samples][N]),
  * writes d loss / d initial states (device, [N][num_states]) and d loss / d
  * field parameters (device, [N][num_field_parameters]) for the run recorded
- * into `checkpoints` with the same times and field parameters. */
+ * into `checkpoints` with the same times and field parameters.
+ * checkpoint_every = 1 stores every state ((n-1) x num_states x N values of
+ * the precision's size) and the backward then runs the reverse sweep alone,
+ * reading them in place: no recomputation, at the cost of the memory. */
 int goss_hip_adjoint_buffer_sizes(goss_hip_system* sys, uint64_t n, uint32_t sample_every, uint32_t checkpoint_every,
                                   uint64_t* checkpoint_bytes, uint64_t* segment_bytes, uint64_t* num_samples);
 int goss_hip_adjoint_record(goss_hip_system* sys, const double* t, uint64_t n, uint32_t sample_every,

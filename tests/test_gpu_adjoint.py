@@ -223,9 +223,12 @@ def test_explicit_adjoint_fp32_close_to_fp64_and_checkpoint_invariant(solver, dt
     t = 9.8 + np.arange(nsteps + 1) * dt
     w = np.random.Generator(np.random.PCG64(74)).normal(size=(nsteps // se, N))
     ref = _device_grads("fp64", y0, fp, t, se, ce, w, solver)
-    other = _device_grads("fp64", y0, fp, t, se, 7, w, solver)
-    assert np.array_equal(ref[2], other[2]) and np.array_equal(ref[3], other[3])
+    for every in (7, 1):  # 1: every state stored, the reverse sweep alone
+        other = _device_grads("fp64", y0, fp, t, se, every, w, solver)
+        assert np.array_equal(ref[2], other[2]) and np.array_equal(ref[3], other[3]), every
     got = _device_grads("fp32", y0, fp, t, se, ce, w, solver)
+    got1 = _device_grads("fp32", y0, fp, t, se, 1, w, solver)
+    assert np.array_equal(got[2], got1[2]) and np.array_equal(got[3], got1[3])
     for k, name in ((2, "y0"), (3, "conductances")):
         a, b = got[k], ref[k]
         rel_l2 = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
