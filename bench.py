@@ -391,6 +391,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the adjoint workload and print its figures as one JSON line")
     ap.add_argument("--adjoint-steps", type=int, default=20000, help="steps of the adjoint workload")
+    ap.add_argument("--adjoint-checkpoint-every", type=int, default=None,
+                    help="checkpoint interval of the adjoint workload (default: 1 when it fits, else 200)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-population batch workload")
     ap.add_argument("--no-implicit", action="store_true", help="skip the implicit-solver workload")
     ap.add_argument("--implicit-only", action="store_true",
@@ -413,7 +415,8 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
         return 0
     if args.adjoint_only:
-        print(json.dumps(adjoint_run(steps=args.adjoint_steps)), flush=True)
+        print(json.dumps(adjoint_run(steps=args.adjoint_steps, checkpoint_every=args.adjoint_checkpoint_every)),
+              flush=True)
         return 0
     if args.mixed_only:
         print(json.dumps(mixed_run()), flush=True)
