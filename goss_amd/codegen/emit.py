@@ -457,8 +457,12 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
                      "                                       const SV& seed__, const YV& y_bar__, real* __restrict__ fp_bar__,\n"
                      "                                       const TP& tape)")
         else:
-            L.append("template <typename real, typename PA, typename treal, typename TP>")
-            L.append(sig.format(T="_tape", TP=",\n                                  const TP& tape"))
+            # lam__ and lam_prev__ may be views as well (rows of the tape).
+            L.append("template <typename real, typename PA, typename treal, typename TP, typename LV, typename OV>")
+            L.append("GOSS_HD GOSS_INLINE void " + kind + "_vjp_tape(const real* __restrict__ states, const treal time, "
+                     "const PA& pa,\n                                  const real h__, const real delta__, const LV& lam__,\n"
+                     "                                  const OV& lam_prev__, real* __restrict__ fp_bar__,\n"
+                     "                                  const TP& tape)")
         L.append("{")
         L.append(f"{ind}(void)time; (void)pa; (void)fp_bar__;" + ("" if kind == "eval" else " (void)delta__;"))
         L.extend(gv.bind(vused | {s_.name for s_ in model.states}, ind))

@@ -115,10 +115,10 @@ GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal t
   lam_prev__[2] = lam__[2] + z__b;
 }
 
-template <typename real, typename PA, typename treal, typename TP>
+template <typename real, typename PA, typename treal, typename TP, typename LV, typename OV>
 GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
-                                  const real h__, const real delta__, const real* __restrict__ lam__,
-                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const real h__, const real delta__, const LV& lam__,
+                                  const OV& lam_prev__, real* __restrict__ fp_bar__,
                                   const TP& tape)
 {
   (void)time; (void)pa; (void)fp_bar__; (void)delta__;
@@ -192,10 +192,10 @@ GOSS_HD GOSS_INLINE void rl1_vjp(const real* __restrict__ states, const treal ti
   lam_prev__[2] = lam__[2] + z__b;
 }
 
-template <typename real, typename PA, typename treal, typename TP>
+template <typename real, typename PA, typename treal, typename TP, typename LV, typename OV>
 GOSS_HD GOSS_INLINE void rl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
-                                  const real h__, const real delta__, const real* __restrict__ lam__,
-                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__,
+                                  const real h__, const real delta__, const LV& lam__,
+                                  const OV& lam_prev__, real* __restrict__ fp_bar__,
                                   const TP& tape)
 {
   (void)time; (void)pa; (void)fp_bar__; (void)delta__;

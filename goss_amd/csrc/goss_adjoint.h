@@ -73,10 +73,9 @@ struct StepVjp;
 template <class M, int SOLVER>
 constexpr int step_vjp_tape_slots()
 {
-  if constexpr (SOLVER == kGRL1) return M::grl1_vjp_tape_slots;
-  else if constexpr (SOLVER == kRL1) return M::rl1_vjp_tape_slots;
-  else if constexpr (SOLVER == kRK4) return M::eval_vjp_tape_slots;
-  else return M::eval_vjp_tape_slots;
+  if constexpr (SOLVER == kGRL1 || SOLVER == kGRL2) return M::grl1_vjp_tape_slots;
+  else if constexpr (SOLVER == kRL1 || SOLVER == kRL2) return M::rl1_vjp_tape_slots;
+  else return M::eval_vjp_tape_slots;  // ExplicitEuler, RK2, RK4
 }
 
 template <class M, class R>
@@ -231,11 +230,155 @@ struct StepVjp<M, kRK4, R> {
   }
 };
 
+// RK2, the midpoint rule (RK2.cpp:73-91): k1 = f(y, t), ym = y + h/2 k1,
+// y' = y + h f(ym, t + h/2).  Reverse: yb = f_y(ym)^T (h lam); lp = lam + yb +
+// f_y(y)^T (h/2 yb).  In FP64 ym, the seed and yb are work rows (0..3S).
+template <class M, class R>
+struct StepVjp<M, kRK2, R> {
+  template <class A, class TP>
+  GOSS_HD GOSS_INLINE static void run(const R* y, double t, R h, const A& acc, R, const R* lam, R* lp, R* pb,
+                                         const TP* tape, const TP& work)
+  {
+    constexpr int S = M::num_states;
+    const double hd = double(h);
+    const R hh = R(0.5) * h;
+    if constexpr (sizeof(R) == 8) {
+      {
+        R k[S];
+        M::template eval<R>(y, t, acc, k);
+#pragma unroll
+        for (int i = 0; i < S; ++i) work.put(i, y[i] + hh * k[i]);
+      }
+      const TapeVec<R> seed_v{work, S}, yb_v{work, 2 * S};
+      {
+        R ym[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          lp[i] = lam[i];
+          seed_v[i] = h * lam[i];
+          yb_v[i] = R(0);
+          ym[i] = work.get(i);
+        }
+        M::template eval_vjp_tape<R>(ym, t + 0.5 * hd, acc, seed_v, yb_v, pb, *tape);
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] += yb_v[i];
+        seed_v[i] = hh * yb_v[i];
+      }
+      M::template eval_vjp_tape<R>(y, t, acc, seed_v, lp, pb, *tape);
+    } else {
+      R ym[S], seed[S], yb[S];
+      {
+        R k[S];
+        M::template eval<R>(y, t, acc, k);
+#pragma unroll
+        for (int i = 0; i < S; ++i) ym[i] = y[i] + hh * k[i];
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] = lam[i];
+        seed[i] = h * lam[i];
+        yb[i] = R(0);
+      }
+      M::template eval_vjp<R>(ym, t + 0.5 * hd, acc, seed, yb, pb);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        lp[i] += yb[i];
+        seed[i] = hh * yb[i];
+      }
+      M::template eval_vjp<R>(y, t, acc, seed, lp, pb);
+    }
+  }
+};
+
+// RL2 / GRL2 (RL2.cpp:75-85, GRL2.cpp:63-85): y2 = y + inc(y; h/2), then
+// y' = y + inc(y2; h), both stages evaluated at time t.  With V(x, k, mu) the
+// one-stage VJP mu + (d inc(x; k)/dx)^T mu (rl1_vjp / grl1_vjp):
+//   lam2 = V(y2, h, lam) - lam,   lp = lam + V(y, h/2, lam2),
+// and the parameter gradients of both calls add up.  In FP64 y2, lam2 and copies
+// of lam and y are work rows (0..4S).
+template <class M, class R, bool GEN>
+struct TwoStageRLVjp {
+  template <class A, class TP, class MV, class OV>
+  GOSS_HD GOSS_INLINE static void vjp(const R* x, double t, R k, const A& acc, R delta, const MV mu, const OV out,
+                                     R* pb, const TP* tape)
+  {
+    if constexpr (GEN) {
+      if constexpr (sizeof(R) == 8) M::template grl1_vjp_tape<R>(x, t, acc, k, delta, mu, out, pb, *tape);
+      else M::template grl1_vjp<R>(x, t, acc, k, delta, mu, out, pb);
+    } else {
+      if constexpr (sizeof(R) == 8) M::template rl1_vjp_tape<R>(x, t, acc, k, delta, mu, out, pb, *tape);
+      else M::template rl1_vjp<R>(x, t, acc, k, delta, mu, out, pb);
+    }
+  }
+  template <class A, class TP>
+  GOSS_HD GOSS_INLINE static void run(const R* y, double t, R h, const A& acc, R delta, const R* lam, R* lp, R* pb,
+                                      const TP* tape, const TP& work)
+  {
+    constexpr int S = M::num_states;
+    const R hh = R(0.5) * h;
+    if constexpr (sizeof(R) == 8) {
+      // Work rows: y2 (0..S), lam2 (S..2S), lam (2S..3S), y (3S..4S).  lam and y are copied
+      // out first, so that their registers are free while y2 is recomputed (a full
+      // linearized_eval) and while the first VJP runs; y2 is recomputed into rows too.
+      const TapeVec<R> lam2_v{work, S}, lam_v{work, 2 * S}, y_v{work, 3 * S};
+#pragma unroll
+      for (int i = 0; i < S; ++i) lam_v[i] = lam[i];
+      {
+        R y2[S];
+        rl_point<M, R, GEN>(y2, y, y, t, hh, acc, delta);
+#pragma unroll
+        for (int i = 0; i < S; ++i) work.put(i, y2[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) y_v[i] = y[i];
+      {
+        R y2[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) y2[i] = work.get(i);
+        vjp(y2, t, h, acc, delta, lam_v, lam2_v, pb, tape);
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) lam2_v[i] = lam2_v[i] - lam_v[i];
+      {
+        R yr[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) yr[i] = y_v[i];
+        vjp(yr, t, hh, acc, delta, lam2_v, lp, pb, tape);
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) lp[i] += lam_v[i];
+    } else {
+      R lam2[S];
+      {
+        R y2[S], g[S];
+        rl_point<M, R, GEN>(y2, y, y, t, hh, acc, delta);
+        vjp(y2, t, h, acc, delta, lam, g, pb, tape);
+#pragma unroll
+        for (int i = 0; i < S; ++i) lam2[i] = g[i] - lam[i];
+      }
+      vjp(y, t, hh, acc, delta, lam2, lp, pb, tape);
+#pragma unroll
+      for (int i = 0; i < S; ++i) lp[i] += lam[i];
+    }
+  }
+};
+
+template <class M, class R>
+struct StepVjp<M, kRL2, R> : TwoStageRLVjp<M, R, false> {};
+template <class M, class R>
+struct StepVjp<M, kGRL2, R> : TwoStageRLVjp<M, R, true> {};
+
 // Per-node work rows a step's adjoint keeps in HBM (besides the FP64 tape).
 template <class M, int SOLVER>
 constexpr int step_vjp_work_rows()
 {
-  return SOLVER == kRK4 ? 7 * M::num_states : (SOLVER == kExplicitEuler ? M::num_states : 0);
+  constexpr int S = M::num_states;
+  return SOLVER == kRK4 ? 7 * S
+       : SOLVER == kRK2 ? 3 * S
+       : (SOLVER == kRL2 || SOLVER == kGRL2) ? 4 * S
+       : SOLVER == kExplicitEuler ? S : 0;
 }
 
 // Steps per reverse-sweep launch when every state is checkpointed (checkpoint interval 1).

@@ -591,7 +591,9 @@ void adjoint_check(goss_hip_system* s, uint64_t n, uint32_t sample_every, uint32
                    uint64_t* n_ckpt, uint64_t* n_samples)
 {
   if (adjoint_index(s->solver) < 0)
-    throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint is available for GRL1, RL1, ExplicitEuler and RK4 systems only");
+    throw Error(GOSS_HIP_EUNSUPPORTED,
+                "the adjoint is available for the explicit fixed-step solvers only (ExplicitEuler, RK2, RK4, RL1, RL2, "
+                "GRL1, GRL2)");
   if (s->prec != kFP64 && s->prec != kFP32)
     throw Error(GOSS_HIP_EUNSUPPORTED, "the adjoint runs in FP64 or FP32 (not FP32C)");
   if (s->sp.internal_time_step > 0)

@@ -134,10 +134,10 @@ constexpr int num_comp()
 // Taylor polynomial to x^4 for |x| < 1/64 (truncation below 2.5e-9
 // relative), hardware exp2 otherwise (cancellation error below 7e-6
 // relative at |x| = 1/64, shrinking as |x| grows).
-__device__ GOSS_INLINE float expm1_f32(const float x)
+GOSS_HD GOSS_INLINE float expm1_f32(const float x)
 {
   const float p = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f))));
-  const float q = __expf(x) - 1.0f;
+  const float q = m_exp(x) - 1.0f;  // __expf on the device
   return m_select(fabsf(x) < 0.015625f, p, q);
 }
 
@@ -176,9 +176,28 @@ struct NodeState {
 
 // Increment of one Rush-Larsen style update, FP64: written as in GRL1.h:80-83
 // so that y0 + increment rounds exactly like the reference.
-__device__ GOSS_INLINE double rl_update(const double y0, const double a, const double b, const double h)
+GOSS_HD GOSS_INLINE double rl_update(const double y0, const double a, const double b, const double h)
 {
-  return y0 + b / a * (exp(a * h) - 1.0);
+  return y0 + b / a * (m_exp(a * h) - 1.0);
+}
+
+// Increment of one Rush-Larsen style update, FP32: rhs * expm1(lin h) / lin, or rhs * h when the
+// state is not updated exponentially; both are computed and one is selected.
+GOSS_HD GOSS_INLINE float rl_increment_f32(const float lin, const float rhs, const float h, const bool use_exp)
+{
+  const float e = expm1_f32(lin * h);
+  const float inc_exp = rhs * e * m_rcp(lin);  // v_rcp_f32 on the device
+  const float inc_lin = rhs * h;
+  return m_select(use_exp, inc_exp, inc_lin);
+}
+
+// Whether state i takes the exponential update: |a| > delta (generalized) or the model's
+// linear_terms flag (classic).
+template <class M, bool GEN, class R>
+GOSS_HD GOSS_INLINE bool rl_use_exp(const int i, const R lin, const R delta)
+{
+  if constexpr (GEN) return m_fabs(lin) > delta;
+  else return M::linear_terms[i] != 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -269,22 +288,40 @@ __device__ GOSS_INLINE void rl_stage(R* out, NodeState<M, R, COMP>& n, const R* 
   M::template linearized_eval<R>(at, tt, acc, lin, rhs, !GEN);
 #pragma unroll
   for (int i = 0; i < S; ++i) {
-    const bool use_exp = GEN ? (fabs(lin[i]) > delta) : (M::linear_terms[i] != 0);
+    const bool use_exp = rl_use_exp<M, GEN>(i, lin[i], delta);
     const R y0 = n.y[i];
     if constexpr (sizeof(R) == 8) {
       // Same expression tree as the reference update; both values are computed and
       // one is selected (no lane-divergent branch, as for the model's piecewise terms).
       const double v_exp = rl_update(y0, lin[i], rhs[i], h);
       const double v_lin = y0 + rhs[i] * h;
-      const double v = use_exp ? v_exp : v_lin;
+      const double v = m_select(use_exp, v_exp, v_lin);
       if constexpr (APPLY) n.y[i] = v; else out[i] = v;
     } else {
-      // Both branches are cheap; select instead of branching (lanes diverge on |a| > delta).
-      const float e = expm1_f32(lin[i] * h);
-      const float inc_exp = rhs[i] * e * __builtin_amdgcn_rcpf(lin[i]);
-      const float inc_lin = rhs[i] * h;
-      const float inc = m_select(use_exp, inc_exp, inc_lin);
+      const float inc = rl_increment_f32(lin[i], rhs[i], h, use_exp);
       if constexpr (APPLY) n.add(i, inc); else out[i] = y0 + inc;
+    }
+  }
+}
+
+// rl_stage without a node state (APPLY = false): out = the stage evaluated at `at` applied to y0.
+// The step adjoints recompute RL2 / GRL2 half steps with it.
+template <class M, class R, bool GEN, class A>
+GOSS_HD GOSS_INLINE void rl_point(R* out, const R* y0, const R* at, const double tt, const R h, const A& acc,
+                                  const R delta)
+{
+  constexpr int S = M::num_states;
+  R lin[S], rhs[S];
+  M::template linearized_eval<R>(at, tt, acc, lin, rhs, !GEN);
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const bool use_exp = rl_use_exp<M, GEN>(i, lin[i], delta);
+    if constexpr (sizeof(R) == 8) {
+      const double v_exp = rl_update(y0[i], lin[i], rhs[i], h);
+      const double v_lin = y0[i] + rhs[i] * h;
+      out[i] = m_select(use_exp, v_exp, v_lin);
+    } else {
+      out[i] = y0[i] + rl_increment_f32(lin[i], rhs[i], h, use_exp);
     }
   }
 }

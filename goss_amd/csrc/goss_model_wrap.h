@@ -45,9 +45,9 @@
     {                                                                                                                \
       NSN::rl1_vjp<R>(y, t, a, h, delta, lam, lam_prev, fp_bar);                                                     \
     }                                                                                                                \
-    template <typename R, typename A, typename T, typename TP>                                                       \
+    template <typename R, typename A, typename T, typename TP, typename LV, typename OV>                             \
     GOSS_HD GOSS_INLINE static void rl1_vjp_tape(const R* y, const T t, const A& a, const R h, const R delta,        \
-                                                 const R* lam, R* lam_prev, R* fp_bar, const TP& tape)               \
+                                                 const LV& lam, const OV& lam_prev, R* fp_bar, const TP& tape)       \
     {                                                                                                                \
       NSN::rl1_vjp_tape<R>(y, t, a, h, delta, lam, lam_prev, fp_bar, tape);                                          \
     }                                                                                                                \
@@ -68,9 +68,9 @@
     {                                                                                                                \
       NSN::grl1_vjp<R>(y, t, a, h, delta, lam, lam_prev, fp_bar);                                                    \
     }                                                                                                                \
-    template <typename R, typename A, typename T, typename TP>                                                       \
+    template <typename R, typename A, typename T, typename TP, typename LV, typename OV>                             \
     GOSS_HD GOSS_INLINE static void grl1_vjp_tape(const R* y, const T t, const A& a, const R h, const R delta,       \
-                                                  const R* lam, R* lam_prev, R* fp_bar, const TP& tape)              \
+                                                  const LV& lam, const OV& lam_prev, R* fp_bar, const TP& tape)      \
     {                                                                                                                \
       NSN::grl1_vjp_tape<R>(y, t, a, h, delta, lam, lam_prev, fp_bar, tape);                                         \
     }                                                                                                                \

@@ -60,7 +60,7 @@ struct AdaptiveArgs {
 
 typedef hipError_t (*AdaptiveLaunch)(const AdaptiveArgs&);
 
-// Adjoint of explicit stepping (GRL1, RL1, ExplicitEuler, RK4): record (forward
+// Adjoint of explicit stepping (ExplicitEuler, RK2, RK4, RL1, RL2, GRL1, GRL2): record (forward
 // with checkpoints and samples) and backward (the reverse sweep).  Precision
 // FP64 or FP32.
 struct AdjointArgs {
@@ -86,9 +86,9 @@ struct AdjointArgs {
 
 typedef hipError_t (*AdjointLaunch)(const AdjointArgs&);
 
-// Solvers with a device adjoint, in the order of the adjoint tables.
-constexpr int kAdjointSolvers[] = {kGRL1, kRL1, kExplicitEuler, kRK4};
-constexpr int kNumAdjointSolvers = 4;
+// Solvers with a device adjoint (every explicit fixed-step solver), in the order of the adjoint tables.
+constexpr int kAdjointSolvers[] = {kGRL1, kRL1, kExplicitEuler, kRK4, kRK2, kRL2, kGRL2};
+constexpr int kNumAdjointSolvers = 7;
 inline int adjoint_index(const int solver)
 {
   for (int k = 0; k < kNumAdjointSolvers; ++k)

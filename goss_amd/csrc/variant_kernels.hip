@@ -129,6 +129,12 @@ void register_adjoint(VariantInfo& v, const int prec)
   v.adjoint_backward[adjoint_index(kExplicitEuler)][prec] = &launch_adjoint_backward<R, kExplicitEuler>;
   v.adjoint_record[adjoint_index(kRK4)][prec] = &launch_adjoint_record<R, kRK4>;
   v.adjoint_backward[adjoint_index(kRK4)][prec] = &launch_adjoint_backward<R, kRK4>;
+  v.adjoint_record[adjoint_index(kRK2)][prec] = &launch_adjoint_record<R, kRK2>;
+  v.adjoint_backward[adjoint_index(kRK2)][prec] = &launch_adjoint_backward<R, kRK2>;
+  v.adjoint_record[adjoint_index(kRL2)][prec] = &launch_adjoint_record<R, kRL2>;
+  v.adjoint_backward[adjoint_index(kRL2)][prec] = &launch_adjoint_backward<R, kRL2>;
+  v.adjoint_record[adjoint_index(kGRL2)][prec] = &launch_adjoint_record<R, kGRL2>;
+  v.adjoint_backward[adjoint_index(kGRL2)][prec] = &launch_adjoint_backward<R, kGRL2>;
 }
 
 void fill_info(VariantInfo& v)
@@ -139,14 +145,17 @@ void fill_info(VariantInfo& v)
   v.NFS = NS::num_field_states;
   v.NFP = NS::num_field_parameters;
   v.NCOMP = num_comp<M>();
-  v.vjp_tape_slots[adjoint_index(kGRL1)] = step_vjp_tape_slots<M, kGRL1>();
-  v.vjp_tape_slots[adjoint_index(kRL1)] = step_vjp_tape_slots<M, kRL1>();
-  v.vjp_tape_slots[adjoint_index(kExplicitEuler)] = step_vjp_tape_slots<M, kExplicitEuler>();
-  v.vjp_tape_slots[adjoint_index(kRK4)] = step_vjp_tape_slots<M, kRK4>();
-  v.vjp_work_rows[adjoint_index(kGRL1)] = step_vjp_work_rows<M, kGRL1>();
-  v.vjp_work_rows[adjoint_index(kRL1)] = step_vjp_work_rows<M, kRL1>();
-  v.vjp_work_rows[adjoint_index(kExplicitEuler)] = step_vjp_work_rows<M, kExplicitEuler>();
-  v.vjp_work_rows[adjoint_index(kRK4)] = step_vjp_work_rows<M, kRK4>();
+#define GOSS_ADJOINT_ROWS(SOLVER)                                                   \
+  v.vjp_tape_slots[adjoint_index(SOLVER)] = step_vjp_tape_slots<M, SOLVER>();     \
+  v.vjp_work_rows[adjoint_index(SOLVER)] = step_vjp_work_rows<M, SOLVER>();
+  GOSS_ADJOINT_ROWS(kGRL1)
+  GOSS_ADJOINT_ROWS(kRL1)
+  GOSS_ADJOINT_ROWS(kExplicitEuler)
+  GOSS_ADJOINT_ROWS(kRK4)
+  GOSS_ADJOINT_ROWS(kRK2)
+  GOSS_ADJOINT_ROWS(kRL2)
+  GOSS_ADJOINT_ROWS(kGRL2)
+#undef GOSS_ADJOINT_ROWS
   v.fsi = NS::field_state_indices;
   v.fpi = NS::field_parameter_indices;
   v.defaults = NS::default_parameters;

@@ -187,7 +187,8 @@ int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
 /* Discrete adjoint of explicit time stepping.  No counterpart in the
  * reference: it serves gradient-based calibration of per-node parameters from
  * the Python side (a torch.autograd.Function over these calls).
- * Requirements: solver GRL1, RL1, ExplicitEuler or RK4, precision FP64 or
+ * Requirements: an explicit fixed-step solver (ExplicitEuler, RK2, RK4, RL1,
+ * RL2, GRL1, GRL2), precision FP64 or
  * FP32, internal_time_step <= 0 (one step per interval).  The times are those of goss_hip_system_forward_times: n points,
  * n-1 steps.  The first field state is sampled after every `sample_every`
  * steps.

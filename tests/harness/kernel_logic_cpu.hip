@@ -176,7 +176,7 @@ void step_vjp(const double* y, double t, double h, const double* lam, const doub
 }  // namespace
 
 // The reverse-sweep kernel's step adjoint on the host, for tp06_m_fp6:
-// solver GRL1 / RL1 / ExplicitEuler / RK4, fp64 nonzero for double.
+// solver GRL1 / RL1 / ExplicitEuler / RK4 / RK2 / RL2 / GRL2, fp64 nonzero for double.
 extern "C" int kernel_logic_step_vjp(int solver, int fp64, const double* y, double t, double h, const double* lam,
                                      const double* prm, double delta, double* lp, double* pb)
 {
@@ -191,6 +191,9 @@ extern "C" int kernel_logic_step_vjp(int solver, int fp64, const double* y, doub
   GOSS_HARNESS_VJP(kRL1)
   GOSS_HARNESS_VJP(kExplicitEuler)
   GOSS_HARNESS_VJP(kRK4)
+  GOSS_HARNESS_VJP(kRK2)
+  GOSS_HARNESS_VJP(kRL2)
+  GOSS_HARNESS_VJP(kGRL2)
 #undef GOSS_HARNESS_VJP
   return -2;
 }

@@ -183,15 +183,16 @@ def test_adjoint_fp32_close_to_fp64():
 def test_adjoint_requires_a_supported_solver():
     from goss_amd._lib import GossHipError
 
-    s = dev_system(VARIANT, "RK2", 4, "fp64")
-    with pytest.raises(GossHipError, match="GRL1, RL1, ExplicitEuler and RK4"):
+    s = dev_system(VARIANT, "ImplicitEuler", 4, "fp64")
+    with pytest.raises(GossHipError, match="explicit fixed-step solvers only"):
         s.adjoint_buffer_sizes(11, 5, 5)
 
 
 # The explicit schemes: RL1 at the GRL1 step; RK4 at 2 us and ExplicitEuler at 1 us (TP06's fast gates
 # make 5 us unstable for both, and 2 us for Euler with these perturbed nodes), from 9.8 ms through the
 # stimulus and the upstroke.
-EXPLICIT_CASES = [("RL1", 0.02, 300, 25, 40), ("ExplicitEuler", 0.001, 1200, 100, 128), ("RK4", 0.002, 600, 50, 64)]
+EXPLICIT_CASES = [("RL1", 0.02, 300, 25, 40), ("ExplicitEuler", 0.001, 1200, 100, 128), ("RK4", 0.002, 600, 50, 64),
+                  ("RK2", 0.001, 1200, 100, 128), ("RL2", 0.02, 300, 25, 40), ("GRL2", 0.02, 300, 25, 40)]
 
 
 @pytest.mark.parametrize("solver,dt,nsteps,se,ce", EXPLICIT_CASES)

@@ -125,7 +125,7 @@ def test_rkf32_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, tol):
     np.testing.assert_array_equal(np.array(list(counters))[1:], o.counters()[1:])
 
 
-STEP_VJP_SOLVERS = {"GRL1": 5, "RL1": 3, "ExplicitEuler": 0, "RK4": 2}
+STEP_VJP_SOLVERS = {"GRL1": 5, "RL1": 3, "ExplicitEuler": 0, "RK4": 2, "RK2": 1, "RL2": 4, "GRL2": 6}
 
 
 @pytest.mark.parametrize("solver", list(STEP_VJP_SOLVERS))

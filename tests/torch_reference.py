@@ -117,26 +117,48 @@ class TorchModel:
         tt = torch.tensor(float(t), dtype=torch.float64)
         return self._f(y, tt, p)
 
-    def grl1_step(self, y, t: float, h: float, p, delta: float = 1e-8):
-        lin, rhs = self.linearized(y, t, p)
+    def grl1_increment(self, at, t: float, h: float, p, delta: float = 1e-8):
+        """GRL1 increment evaluated at `at` (GRL1.h:72-84): b/a (exp(a h) - 1) where |a| > delta, else b h."""
+        lin, rhs = self.linearized(at, t, p)
         big = lin.abs() > delta
         safe = torch.where(big, lin, torch.ones_like(lin))
-        inc = torch.where(big, rhs * torch.expm1(safe * h) / safe, rhs * h)
-        return y + inc
+        return torch.where(big, rhs * torch.expm1(safe * h) / safe, rhs * h)
+
+    def grl1_step(self, y, t: float, h: float, p, delta: float = 1e-8):
+        return y + self.grl1_increment(y, t, h, p, delta)
+
+    def grl2_step(self, y, t: float, h: float, p, delta: float = 1e-8):
+        """GRL2 (GRL2.cpp:63-85): half step to y2, then the increment at y2 over h applied to y, both at t."""
+        y2 = y + self.grl1_increment(y, t, 0.5 * h, p, delta)
+        return y + self.grl1_increment(y2, t, h, p, delta)
 
     def rhs(self, y, t: float, p):
         return self.linearized(y, t, p)[1]
 
-    def rl1_step(self, y, t: float, h: float, p):
-        """RL1 (RL1.h:68-77): exponential update for the linear_terms states, Euler for the others."""
-        lin, rhs = self.linearized(y, t, p)
+    def rl1_increment(self, at, t: float, h: float, p):
+        """RL1 increment evaluated at `at` (RL1.h:68-77): exponential for the linear_terms states, Euler for
+        the others."""
+        lin, rhs = self.linearized(at, t, p)
         lt = self.linear_terms[None, :].expand_as(lin)
         safe = torch.where(lt, lin, torch.ones_like(lin))
-        return y + torch.where(lt, rhs * torch.expm1(safe * h) / safe, rhs * h)
+        return torch.where(lt, rhs * torch.expm1(safe * h) / safe, rhs * h)
+
+    def rl1_step(self, y, t: float, h: float, p):
+        return y + self.rl1_increment(y, t, h, p)
+
+    def rl2_step(self, y, t: float, h: float, p):
+        """RL2 (RL2.cpp:75-85): as GRL2 with the RL1 increment."""
+        y2 = y + self.rl1_increment(y, t, 0.5 * h, p)
+        return y + self.rl1_increment(y2, t, h, p)
 
     def ee_step(self, y, t: float, h: float, p):
         """ExplicitEuler (ExplicitEuler.cpp:76-85)."""
         return y + h * self.rhs(y, t, p)
+
+    def rk2_step(self, y, t: float, h: float, p):
+        """RK2, the midpoint rule (RK2.cpp:73-91)."""
+        k1 = self.rhs(y, t, p)
+        return y + h * self.rhs(y + 0.5 * h * k1, t + 0.5 * h, p)
 
     def rk4_step(self, y, t: float, h: float, p):
         """RK4 (RK4.cpp:79-101)."""
@@ -149,7 +171,10 @@ class TorchModel:
     def step(self, solver: str, y, t: float, h: float, p, delta: float = 1e-8):
         if solver == "GRL1":
             return self.grl1_step(y, t, h, p, delta)
-        return {"RL1": self.rl1_step, "ExplicitEuler": self.ee_step, "RK4": self.rk4_step}[solver](y, t, h, p)
+        if solver == "GRL2":
+            return self.grl2_step(y, t, h, p, delta)
+        return {"RL1": self.rl1_step, "RL2": self.rl2_step, "ExplicitEuler": self.ee_step, "RK2": self.rk2_step,
+                "RK4": self.rk4_step}[solver](y, t, h, p)
 
     def params(self, fp):
         """Per-node parameter matrix [N, P] with the field parameters fp [N, NFP] in their slots."""
