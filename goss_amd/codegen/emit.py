@@ -298,10 +298,36 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("}")
     L.append("")
 
-    # -- step adjoints: grl1_vjp, rl1_vjp, eval_vjp (and their tape forms) ----
-    # Gates (rhs_i = (A - y_i) r): y'_i = A + (y_i - A) exp(linear_i h) under GRL1/RL1; their seeds go
-    # to A, to linear_i and straight to lam_prev_i, and rhs_i is not needed.
+    # Gates (rhs_i = (A - y_i) r): y'_i = A + (y_i - A) exp(linear_i h) under GRL1/RL1.
     gates = gate_states(hbody, rhs_out, model.state_names, lin.linear_terms)
+
+    # -- linearized_eval_gates ------------------------------------------------
+    gate_roots = [gates[i] for i in sorted(gates)]
+    lg_inter = select(body_inter, rhs_out + lin_roots + gate_roots)
+    lgused = g.used_main(lg_inter + hlin_first + hlin_second, rhs_out + hlinear + gate_roots)
+    L.append("// linearized_eval (all entries) plus, for the gate states (gate_states[i] = 1: rhs[i] = (A - y_i) r),")
+    L.append("// gate_target[i] = A.  A GRL stage of a gate is then y0_i + (y_i - A) expm1(linear[i] h).")
+    L.append("constexpr unsigned char gate_states[] = {" + ", ".join("1" if i in gates else "0" for i in range(S)) + "};")
+    L.append("template <typename real, typename PA, typename treal>")
+    L.append("GOSS_HD GOSS_INLINE void linearized_eval_gates(const real* __restrict__ states, const treal time, const PA& pa,")
+    L.append("                                               real* __restrict__ linear, real* __restrict__ rhs,")
+    L.append("                                               real* __restrict__ gate_target)")
+    L.append("{")
+    L.append(f"{ind}(void)time; (void)pa; (void)gate_target;")
+    L += g.bind(lgused, ind)
+    L += g.lines(lg_inter, ind)
+    for i in range(S):
+        L.append(f"{ind}rhs[{i}] = {pr.p(rhs_out[i])};")
+    L += g.lines(hlin_first + hlin_second, ind)
+    for i in range(S):
+        L.append(f"{ind}linear[{i}] = {pr.p(hlinear[i])};")
+    for i in sorted(gates):
+        L.append(f"{ind}gate_target[{i}] = {pr.p(gates[i])};")
+    L.append("}")
+    L.append("")
+
+    # -- step adjoints: grl1_vjp, rl1_vjp, eval_vjp (and their tape forms) ----
+    # The gates' seeds go to A, to linear_i and straight to lam_prev_i, and rhs_i is not needed.
     independent = list(model.state_names) + list(var.field_parameters)
     n_first = len(hlin_first)
     vjp_census: List[sp.Expr] = []

@@ -95,6 +95,35 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   linear[0] = dI_dv;
 }
 
+// linearized_eval (all entries) plus, for the gate states (gate_states[i] = 1: rhs[i] = (A - y_i) r),
+// gate_target[i] = A.  A GRL stage of a gate is then y0_i + (y_i - A) expm1(linear[i] h).
+constexpr unsigned char gate_states[] = {0, 0};
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void linearized_eval_gates(const real* __restrict__ states, const treal time, const PA& pa,
+                                               real* __restrict__ linear, real* __restrict__ rhs,
+                                               real* __restrict__ gate_target)
+{
+  (void)time; (void)pa; (void)gate_target;
+  const real v = states[0];
+  const real s = states[1];
+  const real b = pa.template get<1>();
+  const real c_3 = pa.template get<4>();
+  const real v_peak = pa.template get<9>();
+  const real v_rest = pa.template get<10>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  const real I = c__2*s*(v - v_rest) + c__3*(v - c__1)*(v - v_rest)*(v_peak - v);
+  const treal i_Stim = m_div(pa.template getd<5>()*(treal(1.0) - m_rcp(treal(1.0) + m_exp_l2(treal(7.213475204444817)*time - treal(7.213475204444817)*pa.template getd<8>(), treal(5.0)*time - treal(5.0)*pa.template getd<8>()))), (treal(1.0) + m_exp_l2(treal(1.4426950408889634)*pa.template cstd<4>() + treal(7.213475204444817)*time, pa.template cstd<4>() + treal(5.0)*time)));
+  rhs[0] = I + real(i_Stim);
+  rhs[1] = b*(v - v_rest - c_3*s);
+  const real dI_dv = c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) + c__6*(v - c__1)*(v - v_rest);
+  linear[0] = dI_dv;
+  linear[1] = c__5;
+}
+
 // Discrete adjoints of one step of size h at `time` (one step per interval):
 //   *_vjp:  lam_prev = (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam;
 //   grl1: y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h (GRL1.h:72-84);

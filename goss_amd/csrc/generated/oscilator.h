@@ -62,6 +62,25 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   if (only_linear) return;
 }
 
+// linearized_eval (all entries) plus, for the gate states (gate_states[i] = 1: rhs[i] = (A - y_i) r),
+// gate_target[i] = A.  A GRL stage of a gate is then y0_i + (y_i - A) expm1(linear[i] h).
+constexpr unsigned char gate_states[] = {0, 0};
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void linearized_eval_gates(const real* __restrict__ states, const treal time, const PA& pa,
+                                               real* __restrict__ linear, real* __restrict__ rhs,
+                                               real* __restrict__ gate_target)
+{
+  (void)time; (void)pa; (void)gate_target;
+  const real x = states[0];
+  const real y = states[1];
+  const real a = pa.template get<0>();
+  const real b = pa.template get<1>();
+  rhs[0] = -a*y;
+  rhs[1] = b*x;
+  linear[0] = real(0.0);
+  linear[1] = real(0.0);
+}
+
 // Discrete adjoints of one step of size h at `time` (one step per interval):
 //   *_vjp:  lam_prev = (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam;
 //   grl1: y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h (GRL1.h:72-84);

@@ -191,6 +191,16 @@ GOSS_HD GOSS_INLINE float rl_increment_f32(const float lin, const float rhs, con
   return m_select(use_exp, inc_exp, inc_lin);
 }
 
+// The same increment for a gate state, rhs = (A - y) r with linear = -r, evaluated at y:
+// (y - A) expm1(x) with x = linear h, or (y - A) x (= rhs h) when the state is not updated
+// exponentially.  No reciprocal of the diagonal and no product with rhs.
+GOSS_HD GOSS_INLINE float rl_gate_increment_f32(const float y_minus_target, const float lin, const float h,
+                                                const bool use_exp)
+{
+  const float x = lin * h;
+  return y_minus_target * m_select(use_exp, expm1_f32(x), x);
+}
+
 // Whether state i takes the exponential update: |a| > delta (generalized) or the model's
 // linear_terms flag (classic).
 template <class M, bool GEN, class R>
@@ -275,7 +285,9 @@ struct Step<M, kRK4, R, COMP> {
 };
 
 // Rush-Larsen family: one stage evaluated at `at` and applied to y0 (the
-// y0 of RL1.h:68-77 / GRL1.h:72-84).  APPLY: the result goes to the node
+// y0 of RL1.h:68-77 / GRL1.h:72-84).  In FP32 the generalized stage updates a
+// gate state (rhs = (A - y) r, the model's gate_states) in the closed form
+// (y - A) expm1(linear h), which is rhs expm1(linear h) / linear.  APPLY: the result goes to the node
 // state n (whose values are y0); otherwise to `out`.  GEN selects the
 // generalized form (full diagonal, |a| > delta) or the classic one
 // (linear_terms flags, only_linear evaluation).
@@ -284,8 +296,11 @@ __device__ GOSS_INLINE void rl_stage(R* out, NodeState<M, R, COMP>& n, const R* 
                                      const A& acc, const R delta)
 {
   constexpr int S = M::num_states;
-  R lin[S], rhs[S];
-  M::template linearized_eval<R>(at, tt, acc, lin, rhs, !GEN);
+  // FP32 generalized stages take the gate form for the model's gate states.
+  constexpr bool GATES = GEN && sizeof(R) == 4;
+  R lin[S], rhs[S], tgt[S];
+  if constexpr (GATES) M::template linearized_eval_gates<R>(at, tt, acc, lin, rhs, tgt);
+  else M::template linearized_eval<R>(at, tt, acc, lin, rhs, !GEN);
 #pragma unroll
   for (int i = 0; i < S; ++i) {
     const bool use_exp = rl_use_exp<M, GEN>(i, lin[i], delta);
@@ -298,7 +313,9 @@ __device__ GOSS_INLINE void rl_stage(R* out, NodeState<M, R, COMP>& n, const R* 
       const double v = m_select(use_exp, v_exp, v_lin);
       if constexpr (APPLY) n.y[i] = v; else out[i] = v;
     } else {
-      const float inc = rl_increment_f32(lin[i], rhs[i], h, use_exp);
+      float inc;
+      if (GATES && M::gate_states[i]) inc = rl_gate_increment_f32(at[i] - tgt[i], lin[i], h, use_exp);
+      else inc = rl_increment_f32(lin[i], rhs[i], h, use_exp);
       if constexpr (APPLY) n.add(i, inc); else out[i] = y0 + inc;
     }
   }
@@ -311,8 +328,10 @@ GOSS_HD GOSS_INLINE void rl_point(R* out, const R* y0, const R* at, const double
                                   const R delta)
 {
   constexpr int S = M::num_states;
-  R lin[S], rhs[S];
-  M::template linearized_eval<R>(at, tt, acc, lin, rhs, !GEN);
+  constexpr bool GATES = GEN && sizeof(R) == 4;
+  R lin[S], rhs[S], tgt[S];
+  if constexpr (GATES) M::template linearized_eval_gates<R>(at, tt, acc, lin, rhs, tgt);
+  else M::template linearized_eval<R>(at, tt, acc, lin, rhs, !GEN);
 #pragma unroll
   for (int i = 0; i < S; ++i) {
     const bool use_exp = rl_use_exp<M, GEN>(i, lin[i], delta);
@@ -320,6 +339,8 @@ GOSS_HD GOSS_INLINE void rl_point(R* out, const R* y0, const R* at, const double
       const double v_exp = rl_update(y0[i], lin[i], rhs[i], h);
       const double v_lin = y0[i] + rhs[i] * h;
       out[i] = m_select(use_exp, v_exp, v_lin);
+    } else if (GATES && M::gate_states[i]) {
+      out[i] = y0[i] + rl_gate_increment_f32(at[i] - tgt[i], lin[i], h, use_exp);
     } else {
       out[i] = y0[i] + rl_increment_f32(lin[i], rhs[i], h, use_exp);
     }

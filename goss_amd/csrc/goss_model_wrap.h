@@ -16,6 +16,7 @@
     static constexpr const int* field_state_indices = NSN::field_state_indices;                                      \
     static constexpr const int* field_parameter_indices = NSN::field_parameter_indices;                              \
     static constexpr const unsigned char* linear_terms = NSN::linear_terms;                                          \
+    static constexpr const unsigned char* gate_states = NSN::gate_states;                                            \
     template <typename A>                                                                                            \
     static void compute_constants(const A& a, double* c)                                                             \
     {                                                                                                                \
@@ -30,6 +31,12 @@
     GOSS_HD GOSS_INLINE static void linearized_eval(const R* y, const T t, const A& a, R* lin, R* rhs, bool ol)      \
     {                                                                                                                \
       NSN::linearized_eval<R>(y, t, a, lin, rhs, ol);                                                                \
+    }                                                                                                                \
+    template <typename R, typename A, typename T>                                                                    \
+    GOSS_HD GOSS_INLINE static void linearized_eval_gates(const R* y, const T t, const A& a, R* lin, R* rhs,         \
+                                                          R* gate_target)                                            \
+    {                                                                                                                \
+      NSN::linearized_eval_gates<R>(y, t, a, lin, rhs, gate_target);                                                 \
     }                                                                                                                \
     template <typename R, typename A, typename T, typename J>                                                        \
     GOSS_HD GOSS_INLINE static void compute_jacobian(const R* y, const T t, const A& a, J jac)                       \
