@@ -296,8 +296,8 @@ struct StepVjp<M, kRK2, R> {
 // y' = y + inc(y2; h), both stages evaluated at time t.  With V(x, k, mu) the
 // one-stage VJP mu + (d inc(x; k)/dx)^T mu (rl1_vjp / grl1_vjp):
 //   lam2 = V(y2, h, lam) - lam,   lp = lam + V(y, h/2, lam2),
-// and the parameter gradients of both calls add up.  In FP64 y2, lam2 and copies
-// of lam and y are work rows (0..4S).
+// and the parameter gradients of both calls add up.  y2, lam2 and copies of lam
+// and y are work rows (0..4S), so that few vectors are live in registers at once.
 template <class M, class R, bool GEN>
 struct TwoStageRLVjp {
   template <class A, class TP, class MV, class OV>
@@ -350,17 +350,40 @@ struct TwoStageRLVjp {
 #pragma unroll
       for (int i = 0; i < S; ++i) lp[i] += lam_v[i];
     } else {
-      R lam2[S];
+      // The register VJPs take arrays: the same rows, copied into registers call by call.
+      const TapeVec<R> lam2_v{work, S}, lam_v{work, 2 * S}, y_v{work, 3 * S};
+#pragma unroll
+      for (int i = 0; i < S; ++i) lam_v[i] = lam[i];
       {
-        R y2[S], g[S];
+        R y2[S];
         rl_point<M, R, GEN>(y2, y, y, t, hh, acc, delta);
-        vjp(y2, t, h, acc, delta, lam, g, pb, tape);
 #pragma unroll
-        for (int i = 0; i < S; ++i) lam2[i] = g[i] - lam[i];
+        for (int i = 0; i < S; ++i) work.put(i, y2[i]);
       }
-      vjp(y, t, hh, acc, delta, lam2, lp, pb, tape);
 #pragma unroll
-      for (int i = 0; i < S; ++i) lp[i] += lam[i];
+      for (int i = 0; i < S; ++i) y_v[i] = y[i];
+      {
+        R y2[S], lr[S], g[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          y2[i] = work.get(i);
+          lr[i] = lam_v[i];
+        }
+        vjp(y2, t, h, acc, delta, lr, g, pb, tape);
+#pragma unroll
+        for (int i = 0; i < S; ++i) lam2_v[i] = g[i] - lr[i];
+      }
+      {
+        R yr[S], l2[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          yr[i] = y_v[i];
+          l2[i] = lam2_v[i];
+        }
+        vjp(yr, t, hh, acc, delta, l2, lp, pb, tape);
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) lp[i] += lam_v[i];
     }
   }
 };

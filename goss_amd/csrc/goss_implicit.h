@@ -717,8 +717,15 @@ GOSS_HD GOSS_INLINE NodeViews node_views(double* y, const ImplicitBuffers& B, in
 // The fused implicit kernel: `nouter` outer steps (t_k, interval_k).  JAC
 // selects the Jacobian (JacobianMode); spf, the FP32 shared parameters, is
 // read only by the FP32 Jacobian evaluations.
+// Waves per SIMD of the implicit kernels.  They wait on dependent memory most of the time, so more
+// waves beat the extra scratch spills: TP06 epi at 262,144 nodes, ImplicitEuler 3.65e7 node-steps/s
+// with 1 wave (256 VGPRs + 256 AGPRs), 5.15e7 with 2, 6.64e7 with 4, 6.05e7 with 6, 5.69e7 with 8.
+#ifndef GOSS_IMPLICIT_WAVES
+#define GOSS_IMPLICIT_WAVES 4
+#endif
+
 template <class M, int SOLVER, int JAC>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GOSS_IMPLICIT_WAVES)))
 implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const double2* __restrict__ steps,
                 const int nouter, const SharedParamsOf<M, double> sp, const SharedParamsOf<M, float> spf,
                 const ImplicitParams ip, const ImplicitBuffers B, const unsigned N, const unsigned ld,

@@ -457,8 +457,16 @@ __device__ GOSS_INLINE void explicit_node(const unsigned node, R* __restrict__ y
   }
 }
 
+// Waves per SIMD the FP64 explicit kernels are compiled for: the register budget per lane is
+// 512 / waves (VGPRs and AGPRs).  With 2 the TP06 GRL1 kernel spills ~250 B per lane to scratch
+// but gains 22 % over 1 (5.84e9 -> 7.13e9 node-steps/s at 1M nodes); 3 and 4 lose (4.1e9, 3.3e9).
+// The FP32 kernels fit two waves without a bound.
+#ifndef GOSS_EXPLICIT_WAVES_FP64
+#define GOSS_EXPLICIT_WAVES_FP64 2
+#endif
+
 template <class M, int SOLVER, class R, bool COMP>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? GOSS_EXPLICIT_WAVES_FP64 : 1)))
 explicit_kernel(R* __restrict__ y, R* __restrict__ ylo, const R* __restrict__ fp, const double2* __restrict__ steps,
                 const int nouter, const SharedParamsOf<M, R> sp, const double ldt_0, const R delta,
                 const unsigned N, const unsigned ld, const FieldOut fo)
