@@ -487,8 +487,14 @@ adjoint_recompute_kernel(const R* __restrict__ ckpt, const R* __restrict__ fp, c
 // ([S][ld], precision R) and the parameter gradients ([NFP][ld], FP64) carry
 // over between segments in HBM; `first` starts them at zero, `last` writes
 // the results node-major.
+// Waves per SIMD of the reverse sweep.  It is VALU-bound; a 2-wave register budget spills and
+// was slower (FP32 TP06 GRL1 at 65,536 nodes x 20,000 steps: backward 137.5 ms against 102.5 ms).
+#ifndef GOSS_REVERSE_WAVES
+#define GOSS_REVERSE_WAVES 1
+#endif
+
 template <class M, class R, int SOLVER>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GOSS_REVERSE_WAVES)))
 adjoint_reverse_kernel(const R* __restrict__ fp, const double2* __restrict__ steps, const SharedParamsOf<M, R> sp,
                        const R delta, const unsigned N, const unsigned ld, const AdjointShape shp, const int seg_index,
                        const double* __restrict__ grad_samples, const R* __restrict__ seg, R* __restrict__ tape,
