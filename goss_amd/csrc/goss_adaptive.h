@@ -251,8 +251,14 @@ __device__ GOSS_INLINE void rkf32_node(const unsigned node, R* __restrict__ y, c
   B.is[size_t(kFailed) * ld + node] = failed;
 }
 
+// Waves per SIMD of the RKF32 kernel.  More waves spill and were slower (TP06 M, 262,144 nodes,
+// 20 intervals of 0.5 ms: FP64 1004 / 1069 / 1515 ms, FP32 364 / 436 / 576 ms with 1 / 2 / 4 waves).
+#ifndef GOSS_RKF32_WAVES
+#define GOSS_RKF32_WAVES 1
+#endif
+
 template <class M, class R>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GOSS_RKF32_WAVES)))
 rkf32_kernel(R* __restrict__ y, const R* __restrict__ fp, const double2* __restrict__ steps, const int nouter,
              const SharedParamsOf<M, R> sp, const AdaptiveParams ap, const ImplicitBuffers B, const unsigned N,
              const unsigned ld, const FieldOut fo)
