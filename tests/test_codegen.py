@@ -174,3 +174,70 @@ def test_all_variants_generate():
         assert v.num_states > 0
         y = v.initial_conditions
         assert np.all(np.isfinite(v.eval(y, 0.0)))
+
+
+def test_log2e_scaled_is_the_argument_times_log2e():
+    """exp(x) prints as m_exp_l2(x log2(e), x); the FP32 kernels evaluate exp2 of the first argument.
+    The fold into literal coefficients must leave the value x * log2(e)."""
+    from goss_amd.codegen.printer import LOG2E, log2e_scaled
+
+    V, c, y = sp.symbols("V c__5 y", real=True)
+    args = [-sp.Float(0.1) * V - 4.5, sp.Float(0.3485) * V, c * V, -(V + 45) / 10, V, -V * c + 2 * y,
+            sp.Float(0.0438) * (V - y) * c, sp.Float(3.0)]
+    rng = np.random.Generator(np.random.PCG64(3))
+    for a in args:
+        s = log2e_scaled(a)
+        for _ in range(3):
+            sub = {V: rng.uniform(-90, 40), c: rng.uniform(0.01, 0.1), y: rng.uniform(-2, 2)}
+            want = float(a.subs(sub)) * LOG2E
+            assert abs(float(s.subs(sub)) - want) <= 1e-12 * max(1.0, abs(want)), (a, s)
+    assert abs(LOG2E - 1.0 / np.log(2.0)) < 1e-15
+
+
+@pytest.mark.parametrize("variant", ["tp06_m_fp6", "tp06_epi", "lorentz"])
+def test_gate_states_are_the_closed_form_the_fp32_stage_uses(variant):
+    """For every state that the generator marks as a gate (gate_states = 1), rhs_i = (A - y_i) * (-linear_i)
+    with A = the gate target written by linearized_eval_gates: then rhs expm1(linear h) / linear =
+    (y - A) expm1(linear h), the FP32 GRL stage's gate form.  TP06 has 12 gates (the Hodgkin-Huxley
+    gating variables); Lorenz's x' = sigma (y - x) is one."""
+    from goss_amd.codegen.adjoint import gate_states
+    from goss_amd.codegen.analysis import linearize
+    from goss_amd.codegen.generate import make_variant
+
+    var = make_variant(variant)
+    model = var.model
+    lin = linearize(model)
+    body = dict(model.body)
+    rhs = [body[model.derivatives[s]] for s in model.state_names]
+    gates = gate_states(model.body, rhs, model.state_names, lin.linear_terms)
+    assert len(gates) == {"tp06_m_fp6": 12, "tp06_epi": 12, "lorentz": 1}[variant]
+
+    def expand(e):
+        # substitute intermediates until only states, parameters and time remain
+        for _ in range(50):
+            names = {s.name for s in e.free_symbols}
+            sub = {sp.Symbol(n, real=True): body[n] for n in names if n in body and n not in model.derivatives.values()}
+            if not sub:
+                return e
+            e = e.xreplace(sub)
+        raise AssertionError("intermediate expansion did not terminate")
+
+    rng = np.random.Generator(np.random.PCG64(4))
+    ic = {sp.Symbol(s.name, real=True): s.init * (1 + 0.01 * rng.uniform(-1, 1)) for s in model.states}
+    prm = {sp.Symbol(p.name, real=True): p.value for p in model.parameters}
+    for i, a_sym in gates.items():
+        y = sp.Symbol(model.state_names[i], real=True)
+        r = expand(rhs[i])
+        a = expand(a_sym)
+        d = sp.diff(r, y)
+        assert sp.simplify(sp.diff(a, y)) == 0
+        for V in (-85.0, -41.0, -39.0, 20.0):
+            sub = dict(ic)
+            sub.update(prm)
+            vs = sp.Symbol("V", real=True)
+            if vs in sub:
+                sub[vs] = V
+            sub[sp.Symbol("time", real=True)] = 1.0
+            lhs = float(r.subs(sub))
+            rhs_closed = float(((a - y) * (-d)).subs(sub))
+            assert abs(lhs - rhs_closed) <= 1e-12 * max(1.0, abs(lhs)), (variant, model.state_names[i], V)
