@@ -258,7 +258,7 @@ def adjoint_run(N=65536, steps=20000, dt=0.02, sample_every=50, checkpoint_every
 
 
 def implicit_run(N=262144, steps=150, dt=0.1, seed=2027):
-    """Config-3 shape: TP06 epi with the .ode's default parameters (stim_amplitude 0: the cells stay
+    """Implicit workload: TP06 epi with the .ode's default parameters (stim_amplitude 0: the cells stay
     near rest), ImplicitEuler (relative_tolerance 1e-6) and ESDIRK23a (its fixed tol 1e-5), N nodes,
     dt 0.1 ms, initial states x (1 + U(-0.01, 0.01)), `steps` steps from t = 0.  Each solver runs with
     the reference's FP64 forward-difference Jacobian and with the FP32 forward-difference Jacobian

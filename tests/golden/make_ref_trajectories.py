@@ -4,7 +4,7 @@ Runs in a container where oracle/_ref/libgoss_ref.so was built from the
 reference sources (cpp/goss/*.cpp) and writes the fixtures that the tests
 compare the oracle and the HIP path against:
 
-* ref_tp06_grl1_cfg1.npz -- TP06 M-cell, GRL1, FP64, 1,024 nodes,
+* ref_tp06_grl1_1024x10000.npz -- TP06 M-cell, GRL1, FP64, 1,024 nodes,
   dt = 0.01 ms, 10,000 steps; y0 = defaults * (1 + U(-0.01, 0.01)), seeded;
   default stimulus (-52 pA/pF on [10, 11] ms).  Final states and V of nodes
   0..7 every 100 steps.
@@ -23,7 +23,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 import oracle  # noqa: E402
 
-SEEDS = {"cfg1": 1001, "fp3_window": 1002}
+SEEDS = {"tp06_1024": 1001, "fp3_window": 1002}
 
 
 def init_states(sys_, rng):
@@ -34,8 +34,8 @@ def init_states(sys_, rng):
 def main():
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libgoss_ref.so is not built")
-    # Config 1.
-    rng = np.random.Generator(np.random.PCG64(SEEDS["cfg1"]))
+    # TP06 M-cell GRL1: 1,024 nodes, dt 0.01 ms, 10,000 steps.
+    rng = np.random.Generator(np.random.PCG64(SEEDS["tp06_1024"]))
     N, dt, steps = 1024, 0.01, 10000
     r = oracle.RefSystem("tp06_m", "GRL1", N)
     r.set_num_threads(8)
@@ -46,9 +46,9 @@ def main():
     for k in range(0, steps, 100):
         r.forward_times(t[k:k + 101])
         trace.append(r.states()[:8, 17].copy())
-    np.savez_compressed(os.path.join(HERE, "ref_tp06_grl1_cfg1.npz"), y0=y0, t=t, final=r.states(),
+    np.savez_compressed(os.path.join(HERE, "ref_tp06_grl1_1024x10000.npz"), y0=y0, t=t, final=r.states(),
                         V_trace=np.array(trace))
-    # Config 2 window, node subset.
+    # TP06 M-cell with 3 field parameters, a 5,000-step window of 128 nodes at dt 0.02 ms.
     rng = np.random.Generator(np.random.PCG64(SEEDS["fp3_window"]))
     N, dt, steps = 128, 0.02, 5000
     out = {}

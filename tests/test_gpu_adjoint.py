@@ -149,7 +149,7 @@ def test_record_samples_equal_forward(prec):
 
 
 def test_adjoint_fp32_close_to_fp64():
-    # Config-5 shape on a subset: 20 ms through the stimulus and upstroke,
+    # The calibration workload's shape on a subset: 20 ms through the stimulus and upstroke,
     # MSE of V sampled every 1 ms against a target from conductances x lognormal(0, 0.1).
     N, nsteps, se, ce = 256, 1000, 50, 200
     y0, fp = _inputs(N, 68)

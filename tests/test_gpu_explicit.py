@@ -43,9 +43,9 @@ def perturbed(sys_states, seed, spread=0.01):
     return sys_states * (1.0 + rng.uniform(-spread, spread, sys_states.shape))
 
 
-def test_cfg1_tp06_grl1_fp64_matches_reference():
-    """Config 1: 1,024 nodes, dt 0.01 ms, 10,000 steps (through a full AP)."""
-    d = np.load(golden("ref_tp06_grl1_cfg1.npz"))
+def test_tp06_grl1_fp64_1024_nodes_matches_reference():
+    """TP06 M-cell GRL1 FP64: 1,024 nodes, dt 0.01 ms, 10,000 steps (through a full AP)."""
+    d = np.load(golden("ref_tp06_grl1_1024x10000.npz"))
     dev = dev_system("tp06_m", "GRL1", 1024, "fp64")
     dev.set_states(d["y0"])
     t = d["t"]
@@ -53,9 +53,9 @@ def test_cfg1_tp06_grl1_fp64_matches_reference():
     for k in range(0, len(t) - 1, 100):
         dev.forward_times(t[k:k + 101])
         trace.append(dev.states()[:8, 17])
-    err = assert_parity(dev.states(), d["final"], what="cfg1 final")
+    err = assert_parity(dev.states(), d["final"], what="1,024-node final states")
     np.testing.assert_allclose(np.array(trace), d["V_trace"], rtol=0, atol=1e-8)
-    print(f"cfg1 max rel err {err:.3e}")
+    print(f"1,024 nodes x 10,000 steps: max rel err {err:.3e}")
 
 
 @pytest.mark.parametrize("solver", ["GRL1", "RL2"])

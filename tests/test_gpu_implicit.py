@@ -69,8 +69,8 @@ def test_implicit_matches_oracle_per_node(solver, variant, dt, steps, ldt):
         assert list(oc[1:]) == list(dc[1:])
 
 
-def test_config3_epi_subset_and_full_size_equality():
-    """Config 3 shape: TP06 epi, dt 0.1 ms, init +-1%.  A 1,024-node subset
+def test_tp06_epi_implicit_subset_and_full_size_equality():
+    """TP06 epi, dt 0.1 ms, init +-1%.  A 1,024-node subset
     for 100 ms against the oracle, and the full 262,144 nodes from identical
     initial states for 20 ms: all nodes stay bit-identical."""
     for solver in ("ESDIRK23a", "ImplicitEuler"):
@@ -139,8 +139,8 @@ def _diff(a, b):
 
 
 @pytest.mark.parametrize("solver,rtol", [("ImplicitEuler", 1e-6), ("ImplicitEuler", None), ("ESDIRK23a", None)])
-def test_fp32_jacobian_config3_precision(solver, rtol):
-    """Config 3 precision pattern: the Jacobian from forward differences of the
+def test_fp32_jacobian_precision(solver, rtol):
+    """FP32 right-hand side in the Jacobian, FP64 Newton: the Jacobian from forward differences of the
     FP32 right-hand side, the Newton iteration (residual, LU, substitutions)
     in FP64.  TP06 epi, dt 0.1 ms, 30 ms through the stimulus and upstroke,
     against the FP64 oracle with the reference's Jacobian; ImplicitEuler at

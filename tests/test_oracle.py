@@ -18,8 +18,8 @@ from conftest import golden
 EXPLICIT = ["ExplicitEuler", "RK2", "RK4", "RL1", "RL2", "GRL1", "GRL2"]
 
 
-def test_oracle_matches_reference_cfg1_fixture():
-    d = np.load(golden("ref_tp06_grl1_cfg1.npz"))
+def test_oracle_matches_reference_tp06_1024_node_fixture():
+    d = np.load(golden("ref_tp06_grl1_1024x10000.npz"))
     o = oracle.OracleSystem("tp06_m", "GRL1", 1024)
     o.set_num_threads(8)
     o.set_states(d["y0"])
