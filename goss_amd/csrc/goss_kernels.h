@@ -464,9 +464,14 @@ __device__ GOSS_INLINE void explicit_node(const unsigned node, R* __restrict__ y
 #ifndef GOSS_EXPLICIT_WAVES_FP64
 #define GOSS_EXPLICIT_WAVES_FP64 2
 #endif
+// FP32 / FP32C: no bound (the TP06 GRL1 kernel fits two waves); 3 or 4 waves measured within 1 %.
+#ifndef GOSS_EXPLICIT_WAVES_FP32
+#define GOSS_EXPLICIT_WAVES_FP32 1
+#endif
 
 template <class M, int SOLVER, class R, bool COMP>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? GOSS_EXPLICIT_WAVES_FP64 : 1)))
+__global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? GOSS_EXPLICIT_WAVES_FP64 : GOSS_EXPLICIT_WAVES_FP32)))
 explicit_kernel(R* __restrict__ y, R* __restrict__ ylo, const R* __restrict__ fp, const double2* __restrict__ steps,
                 const int nouter, const SharedParamsOf<M, R> sp, const double ldt_0, const R delta,
                 const unsigned N, const unsigned ld, const FieldOut fo)
