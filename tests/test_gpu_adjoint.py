@@ -1,7 +1,7 @@
-"""Discrete adjoint of GRL1 on the device (goss_amd.adjoint, goss_adjoint.h).
+"""Discrete adjoint of the explicit steps on the device (goss_amd.adjoint, goss_adjoint.h).
 
 References:
-* plain-PyTorch FP64 autograd through the same GRL1 scheme (tests/torch_reference.py,
+* plain-PyTorch FP64 autograd through the same scheme (tests/torch_reference.py,
   run on the CPU): the FP64 device gradients must agree to 1e-8 relative per entry
   (floor: 1e-9 of the entry's column maximum);
 * central finite differences of the device FP64 loss;
@@ -236,3 +236,18 @@ def test_explicit_adjoint_fp32_close_to_fp64_and_checkpoint_invariant(solver, dt
         cols = np.linalg.norm(b, axis=0) > 0
         print(f"{solver} FP32 vs FP64, d loss / d {name}: max relative L2 per column {rel_l2[cols].max():.2e}")
         assert np.all(np.isfinite(a)) and rel_l2[cols].max() <= 5e-3
+
+
+@pytest.mark.parametrize("prec,elem", [("fp64", 8), ("fp32", 4)])
+def test_every_state_checkpoint_sizes(prec, elem):
+    """Checkpoint interval 1: one state copy before every step, (n - 1) x S x ld values; the segment
+    buffer still holds one state row block, the work rows, lam and the FP64 parameter gradients."""
+    N, n = 100, 41  # ld = 128 (N rounded up to a multiple of 64)
+    s = dev_system(VARIANT, "GRL1", N, prec)
+    ckpt, seg, ns = s.adjoint_buffer_sizes(n, 10, 1)
+    S = s.model.num_states
+    assert ckpt == (n - 1) * S * 128 * elem
+    assert ns == (n - 1) // 10
+    ckpt200, seg200, _ = s.adjoint_buffer_sizes(n, 10, 200)
+    assert ckpt200 == S * 128 * elem  # one checkpoint: the 40 steps fit in one segment
+    assert seg200 - seg == 199 * S * 128 * elem  # the segment's recomputed states: 200 rows blocks against 1
