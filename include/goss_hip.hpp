@@ -146,7 +146,76 @@ class ODESolver {
     params_.rtol = rtol;
   }
 
+  // ODESolver::attach (ODESolver.h:62-67): the model that forward and solve advance.  They run on a
+  // one-node FP64 device system, created at the first call with the model's current parameter values
+  // (shared and field) and the solver's tunables as they are then.
+  void attach(std::shared_ptr<ParameterizedODE> ode)
+  {
+    release();
+    ode_ = std::move(ode);
+  }
+  // ODESolver::forward (ODESolver.h:75): the states y of one node advanced over `interval` from t.
+  void forward(double* y, double t, double interval)
+  {
+    goss_hip_system* sys = one_node();
+    check(goss_hip_system_set_states(sys, y));
+    check(goss_hip_system_forward(sys, t, interval));
+    check(goss_hip_system_get_states(sys, y));
+  }
+  // ODESolver::solve (ODESolver.h:78-98): y0 advanced interval by interval over t[0 .. n-1]; row it of
+  // y[n][num_states] receives the states at t[it] (row 0 is left as it is, as in the reference).
+  void solve(double* y, double* y0, const double* t, unsigned long n)
+  {
+    const std::size_t S = one_node_states();
+    for (unsigned long it = 1; it < n; ++it) {
+      forward(y0, t[it - 1], t[it] - t[it - 1]);
+      for (std::size_t j = 0; j < S; ++j) y[it * S + j] = y0[j];
+    }
+  }
+
+  ODESolver(const ODESolver& o) : name_(o.name_), kind_(o.kind_), params_(o.params_), ode_(o.ode_) {}
+  ODESolver& operator=(const ODESolver& o)
+  {
+    if (this != &o) {
+      release();
+      name_ = o.name_;
+      kind_ = o.kind_;
+      params_ = o.params_;
+      ode_ = o.ode_;
+    }
+    return *this;
+  }
+  ~ODESolver() { release(); }
+
  private:
+  goss_hip_system* one_node()
+  {
+    if (sys_) return sys_;
+    if (!ode_) throw Error(GOSS_HIP_EINVAL, "ODESolver: no ODE attached");
+    check(goss_hip_system_create(ode_->variant().c_str(), kind_, &params_, GOSS_HIP_FP64, 1, &sys_));
+    try {
+      for (const std::string& p : ode_->get_parameter_names())
+        check(goss_hip_system_set_parameter(sys_, p.c_str(), ode_->get_parameter(p)));
+      std::vector<double> fp;
+      for (const std::string& p : ode_->get_field_parameter_names()) fp.push_back(ode_->get_parameter(p));
+      if (!fp.empty()) check(goss_hip_system_set_field_parameters(sys_, fp.data(), 1));
+    } catch (...) {
+      release();
+      throw;
+    }
+    return sys_;
+  }
+  std::size_t one_node_states()
+  {
+    one_node();
+    return ode_->num_states();
+  }
+  void release()
+  {
+    if (sys_) goss_hip_system_destroy(sys_);
+    sys_ = nullptr;
+  }
+
   static int kind_of(const std::string& name)
   {
     static const char* const names[] = {"ExplicitEuler", "RK2",       "RK4",           "RL1",       "RL2",
@@ -159,6 +228,8 @@ class ODESolver {
   std::string name_;
   int kind_;
   goss_hip_solver_params params_{};
+  std::shared_ptr<ParameterizedODE> ode_;
+  goss_hip_system* sys_ = nullptr;
 };
 
 // goss::ODESystemSolver on the device.

@@ -83,6 +83,24 @@ int run(const char* out_path)
   std::vector<double> rows(size_t(51) * N * F, 0.0);
   system.solve(rows.data(), times.data(), times.size());
   const double* st = system.states();
+
+  // ODESolver::solve on one node (ODESolver.h:78-98) against a one-node system over the same times:
+  // the same kernel, so the same states bit for bit.
+  {
+    auto single = std::make_shared<goss_hip::ODESolver>("GRL1");
+    single->attach(ode);
+    std::vector<double> y0(S), traj(size_t(41) * S, 0.0), times1(41);
+    ode->get_ic(y0.data());
+    for (int k = 0; k <= 40; ++k) times1[k] = k * dt;
+    single->solve(traj.data(), y0.data(), times1.data(), times1.size());
+    goss_hip::ODESystemSolver one(1, solver, ode);
+    for (int k = 0; k < 40; ++k) one.forward(times1[k], times1[k + 1] - times1[k]);
+    const double* s1 = one.states();
+    for (goss_hip::uint j = 0; j < S; ++j)
+      if (traj[size_t(40) * S + j] != s1[j] || y0[j] != s1[j]) return 7;
+    std::cout << "single-node solve equals the one-node system\n";
+  }
+
   std::ofstream f(out_path, std::ios::binary);
   const double header[3] = {double(N), double(S), double(F)};
   f.write(reinterpret_cast<const char*>(header), sizeof(header));
