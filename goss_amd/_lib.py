@@ -34,7 +34,7 @@ EXPORTS = (
     "goss_hip_last_error", "goss_hip_device_count", "goss_hip_model_count", "goss_hip_model_name",
     "goss_hip_model_info_get", "goss_hip_solver_params_default", "goss_hip_solver_is_adaptive",
     "goss_hip_system_create", "goss_hip_system_destroy", "goss_hip_system_forward",
-    "goss_hip_system_forward_times", "goss_hip_system_solve", "goss_hip_system_get_field_states",
+    "goss_hip_system_forward_times", "goss_hip_system_forward_times_noalloc", "goss_hip_system_solve", "goss_hip_system_get_field_states",
     "goss_hip_system_set_field_states", "goss_hip_system_get_field_state_components",
     "goss_hip_system_set_field_state_components", "goss_hip_system_get_field_parameters",
     "goss_hip_system_set_field_parameters", "goss_hip_system_get_field_states_device",
@@ -127,6 +127,7 @@ def lib() -> ctypes.CDLL:
     L.goss_hip_system_destroy.argtypes = [V]
     L.goss_hip_system_forward.argtypes = [V, ctypes.c_double, ctypes.c_double]
     L.goss_hip_system_forward_times.argtypes = [V, _dp, ctypes.c_uint64]
+    L.goss_hip_system_forward_times_noalloc.argtypes = [V, _dp, ctypes.c_uint64]
     L.goss_hip_system_solve.argtypes = [V, _dp, _dp, ctypes.c_uint64, ctypes.c_int]
     for fn in ("get_field_states", "set_field_states", "get_field_parameters", "set_field_parameters",
                "get_field_states_device", "set_field_states_device"):

@@ -14,6 +14,7 @@
 #include <numeric>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -153,6 +154,7 @@ struct goss_hip_system {
   uint32_t num_threads = 0;
   uint32_t steps_per_launch = 200;
   IndexMap map{};
+  void* scratch = nullptr;  // device address of the library's scratch array on this device
   ImplicitBuffers ib{};  // implicit solvers only
   int* h_error = nullptr;  // pinned copy of ib.error
 
@@ -370,6 +372,12 @@ struct goss_hip_system {
   }
 };
 
+// Staging of goss_hip_system_forward_times_noalloc: time points, then (t, interval) pairs.
+constexpr uint64_t kNoallocMaxPoints = 32768;  // time points per call (32,767 steps)
+constexpr size_t kNoallocTBytes = kNoallocMaxPoints * sizeof(double);  // 256 KiB
+constexpr size_t kNoallocScratchBytes = kNoallocTBytes + (kNoallocMaxPoints - 1) * sizeof(double2) + 16;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 namespace {
 
 ComponentList all_components(int F)
@@ -395,6 +403,44 @@ ComponentList component_list(const goss_hip_system* s, uint32_t nc, const uint32
 void check_sys(const goss_hip_system* s)
 {
   if (!s) throw Error(GOSS_HIP_EINVAL, "null system");
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free stepping (goss_hip_system_forward_times_noalloc).
+//
+// The entry point stages the caller's time points and the (t, interval) pairs
+// the kernels read in `scratch`, a fixed device array of the library (one copy
+// per GPU, held until the process ends), instead of the system's growable
+// step buffers.  Layout: [0, kNoallocTBytes) the time points (double),
+// [kNoallocTBytes, kNoallocScratchBytes) the pairs (double2).
+// ---------------------------------------------------------------------------
+// The device address of `scratch`, per device.  `scratch` in host code names a
+// host-side stand-in, so the kernels and copies take these addresses only.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves the address on the current device (once per device); called at
+// system creation, with the system's device current.
+void* resolve_scratch(int device)
+{
+  if (device < 0 || device >= kMaxDevices) throw Error(GOSS_HIP_EUNSUPPORTED, "device index beyond 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw Error(GOSS_HIP_EDEVICE, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+// (t[i], t[i+1] - t[i]) for i < n - 1: the interval of ODESystemSolver::solve
+// (ODESystemSolver.cpp:91-97), one IEEE subtraction as on the host.
+__global__ void noalloc_pairs_kernel(const double* __restrict__ t, double2* __restrict__ pairs, const unsigned n_steps)
+{
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_steps) pairs[i] = make_double2(t[i], t[i + 1] - t[i]);
 }
 
 }  // namespace
@@ -493,6 +539,7 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
     s->ld = (num_nodes + 63u) & ~63u;
     if (s->ld == 0) s->ld = 64;
     hip_check(hipGetDevice(&s->device), "hipGetDevice");
+    s->scratch = resolve_scratch(s->device);
     hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&s->steps_free, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventRecord(s->steps_free, s->stream), "hipEventRecord");
@@ -581,6 +628,37 @@ int goss_hip_system_forward_times(goss_hip_system* s, const double* t, uint64_t 
     s->check_steps(st.data(), st.size());
     const double2* d = s->upload_steps(st.data(), st.size());
     s->launch_steps(d, st.size(), nullptr, 1);
+  });
+}
+
+int goss_hip_system_forward_times_noalloc(goss_hip_system* s, const double* t, uint64_t n)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (n < 2) return;
+    if (n > kNoallocMaxPoints)
+      throw Error(GOSS_HIP_EINVAL, "goss_hip_system_forward_times_noalloc takes at most " +
+                                       std::to_string(kNoallocMaxPoints) + " time points per call, got " +
+                                       std::to_string(n));
+    if (!s->scratch) throw Error(GOSS_HIP_EDEVICE, "the scratch array of this device was not resolved");
+    s->use_device();
+    // The checks of check_steps, on the caller's array (no host copy).
+    const double ldt_0 = s->sp.internal_time_step;
+    for (uint64_t i = 0; i + 1 < n; ++i) {
+      const double dt = t[i + 1] - t[i];
+      if (!std::isfinite(t[i]) || !std::isfinite(dt) || dt < 0)
+        throw Error(GOSS_HIP_EINVAL, "time and interval must be finite and the interval non-negative");
+      if (ldt_0 > 0 && !(dt / ldt_0 < 1e9))
+        throw Error(GOSS_HIP_EINVAL, "interval / internal_time_step exceeds 1e9 sub-steps");
+    }
+    double* d_t = static_cast<double*>(s->scratch);
+    double2* d_pairs = reinterpret_cast<double2*>(static_cast<unsigned char*>(s->scratch) + kNoallocTBytes);
+    hip_check(hipMemcpyAsync(d_t, t, n * sizeof(double), hipMemcpyHostToDevice, s->stream),
+              "hipMemcpyAsync(times)");
+    const unsigned steps = unsigned(n - 1);
+    noalloc_pairs_kernel<<<(steps + 255) / 256, 256, 0, s->stream>>>(d_t, d_pairs, steps);
+    hip_check(hipGetLastError(), "kernel launch");
+    s->launch_steps(d_pairs, steps, nullptr, 1);
   });
 }
 

@@ -94,6 +94,12 @@ class DeviceSystem:
         t = _f64(t)
         check(lib().goss_hip_system_forward_times(self._h, _ptr(t), len(t)))
 
+    def forward_times_noalloc(self, t) -> None:
+        """forward_times with the steps staged in the library's fixed device
+        array (no allocation; at most 32,768 time points per call)."""
+        t = _f64(t)
+        check(lib().goss_hip_system_forward_times_noalloc(self._h, _ptr(t), len(t)))
+
     def solve(self, t, out: np.ndarray, tangled: bool = True) -> None:
         t = _f64(t)
         if not (out.flags.c_contiguous and out.dtype == np.float64):

@@ -135,6 +135,18 @@ int goss_hip_system_forward(goss_hip_system* sys, double t, double interval);
  * loop of ODESystemSolver::solve (ODESystemSolver.cpp:81-99) without the
  * per-step copy-out. */
 int goss_hip_system_forward_times(goss_hip_system* sys, const double* t, uint64_t n);
+/* No counterpart in the reference.  goss_hip_system_forward_times without any
+ * device or pinned-host allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the time points and the (t, interval) pairs the kernels read are
+ * staged in a fixed device array of the library (768 KiB per GPU, held until
+ * the process ends) instead of the system's growable step buffers.  At most
+ * 32,768 time points (32,767 steps) per call; more gives GOSS_HIP_EINVAL.
+ * Results equal those of goss_hip_system_forward_times bit for bit.  The
+ * array is shared by every system on one GPU: calls that overlap on one GPU
+ * race (calls on one system are ordered by its stream; before calling this on
+ * another system of the same GPU, synchronize the previous one). */
+int goss_hip_system_forward_times_noalloc(goss_hip_system* sys, const double* t, uint64_t n);
 /* ODESystemSolver::solve(field_states, t, n, tangled) (ODESystemSolver.cpp:81-99):
  * rows 1..n-1 of field_states[n][...] receive the field states after each
  * step; row 0 is left to the caller (python/goss/systemsolver.py:130-132). */
