@@ -7,7 +7,12 @@ the CPU-baseline leg of bench.py, never by the product package.
   solvers (``goss_oracle.cpp``), built into ``oracle/liboracle.so``;
 * :class:`RefSystem` -- the reference's own C++ (``cpp/goss/*.cpp``) built into
   ``oracle/_ref/libgoss_ref.so`` when the reference tree was available at build
-  time.
+  time (with our generated model plugins), and into
+  ``oracle/_ref/libgoss_refpin.so`` with the reference's own gotran-generated
+  test models (``cpp/tests/*.h``; variants ``ref_fhn``,
+  ``ref_beeler_reuter_1977``, ``ref_winslow``, ``ref_panfilov``);
+* :func:`register_reference_models` -- makes those reference classes oracle
+  variants of the same names, so the oracle's solvers run on them.
 """
 from __future__ import annotations
 
@@ -21,6 +26,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libgoss_ref.so")
+REFPIN_SO = os.path.join(HERE, "_ref", "libgoss_refpin.so")
+REFERENCE_MODELS = ("ref_fhn", "ref_beeler_reuter_1977", "ref_winslow", "ref_panfilov")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -45,7 +52,7 @@ def _f64(a) -> np.ndarray:
 
 
 _lib = None
-_ref = None
+_ref = {}
 
 
 def lib():
@@ -79,22 +86,26 @@ def lib():
         L.oracle_model_fd_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
         L.oracle_model_step_vjp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, _dp,
                                             ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.oracle_register_external_model.argtypes = [ctypes.c_void_p]
         L.oracle_lu_factorize.argtypes = [ctypes.c_int, _dp]
         L.oracle_forward_backward_subst.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
 
-def ref_available() -> bool:
-    return os.path.exists(REF_SO)
+def ref_available(path: str = REF_SO) -> bool:
+    return os.path.exists(path)
 
 
-def ref_lib():
-    global _ref
-    if _ref is None:
-        if not ref_available():
-            raise RuntimeError(f"{REF_SO} is not built")
-        L = ctypes.CDLL(REF_SO)
+def refpin_available() -> bool:
+    return os.path.exists(REFPIN_SO)
+
+
+def ref_lib(path: str = REF_SO):
+    if path not in _ref:
+        if not ref_available(path):
+            raise RuntimeError(f"{path} is not built")
+        L = ctypes.CDLL(path)
         L.ref_last_error.restype = ctypes.c_char_p
         L.ref_system_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint,
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
@@ -123,8 +134,24 @@ def ref_lib():
         L.ref_model_set_field_parameters.argtypes = [ctypes.c_void_p, _dp]
         L.ref_model_linear_term.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         L.ref_model_destroy.argtypes = [ctypes.c_void_p]
-        _ref = L
-    return _ref
+        if path == REFPIN_SO:
+            L.refpin_model_ops.restype = ctypes.c_void_p
+            L.refpin_model_ops.argtypes = [ctypes.c_char_p]
+        _ref[path] = L
+    return _ref[path]
+
+
+def register_reference_models() -> None:
+    """Registers the reference's gotran-generated test models (built into
+    _ref/libgoss_refpin.so) as oracle variants of the same names."""
+    R = ref_lib(REFPIN_SO)
+    L = lib()
+    for name in REFERENCE_MODELS:
+        ops = R.refpin_model_ops(name.encode())
+        if not ops:
+            raise RuntimeError(R.ref_last_error().decode())
+        if L.oracle_register_external_model(ops) < 0:
+            raise RuntimeError(L.oracle_last_error().decode())
 
 
 class Variant:
@@ -323,8 +350,10 @@ class RefSystem(_SystemBase):
     """The reference's ODESystemSolver compiled from its own sources."""
 
     def __init__(self, variant, solver, num_nodes, solver_params=None, model_params=None):
+        if variant in REFERENCE_MODELS:
+            register_reference_models()  # the Variant metadata comes from the oracle's registry
         super().__init__(variant, solver, num_nodes)
-        self._L = ref_lib()
+        self._L = ref_lib(REFPIN_SO if variant in REFERENCE_MODELS else REF_SO)
         n, names, vals = _names(solver_params or {})
         m, mnames, mvals = _names(model_params or {})
         self._h = ctypes.c_void_p()
@@ -352,7 +381,9 @@ class RefModel:
     """A reference-side goss::ParameterizedODE instance (generated plugin)."""
 
     def __init__(self, variant: str):
-        self._L = ref_lib()
+        if variant in REFERENCE_MODELS:
+            register_reference_models()
+        self._L = ref_lib(REFPIN_SO if variant in REFERENCE_MODELS else REF_SO)
         self._h = ctypes.c_void_p()
         if self._L.ref_model_create(variant.encode(), ctypes.byref(self._h)) != 0:
             raise RuntimeError(self._L.ref_last_error().decode())

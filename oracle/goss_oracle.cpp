@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -52,13 +53,29 @@ struct VariantOps {
   const char* const* pnames;
   int K;  // number of parameter-only constants
   void (*constants)(const double* p, double* c);
-  void (*eval)(const double*, double, const double*, const double*, double*);
-  void (*linearized)(const double*, double, const double*, const double*, double*, double*, bool);
+  void (*eval_fn)(const double*, double, const double*, const double*, double*);
+  void (*linearized_fn)(const double*, double, const double*, const double*, double*, double*, bool);
   void (*jacobian)(const double*, double, const double*, const double*, double*);
   // Generated step adjoints (test infrastructure for the device adjoint): kind 0 grl1_vjp, 1 rl1_vjp,
   // 2 eval_vjp (lam is the seed, out accumulates y_bar); tape selects the *_tape form.
   void (*step_vjp)(int kind, bool tape, const double*, double, const double*, const double*, double, double,
                    const double*, double*, double*);
+  // A model registered at run time (oracle_register_external_model): its
+  // right-hand side is called through these entries with the full parameter
+  // vector; the generated entries above are then unused.
+  const oracle_external_model* ext = nullptr;
+
+  void eval(const double* y, double t, const double* p, const double* c, double* out) const
+  {
+    if (ext) ext->eval(ext->ctx, y, t, p, out);
+    else eval_fn(y, t, p, c, out);
+  }
+  void linearized(const double* y, double t, const double* p, const double* c, double* lin, double* rhs,
+                  bool only_linear) const
+  {
+    if (ext) ext->linearized_eval(ext->ctx, y, t, p, lin, rhs, only_linear ? 1 : 0);
+    else linearized_fn(y, t, p, c, lin, rhs, only_linear);
+  }
 };
 
 // Host tape for grl1_vjp_tape (slot k at v[k]).
@@ -101,14 +118,21 @@ struct HostTape {
                if (kind == 2 && tape) NS::eval_vjp_tape<double>(y, t, a, lam, out, fp_bar, T);         \
              }}
 
-const VariantOps kVariants[] = {
-    GOSS_ORACLE_VARIANT(tp06_m),         GOSS_ORACLE_VARIANT(tp06_m_fp3),
-    GOSS_ORACLE_VARIANT(tp06_m_fp6),     GOSS_ORACLE_VARIANT(tp06_epi),
-    GOSS_ORACLE_VARIANT(fitzhughnagumo), GOSS_ORACLE_VARIANT(lorentz),
-    GOSS_ORACLE_VARIANT(fhn),            GOSS_ORACLE_VARIANT(beeler_reuter_1977),
-    GOSS_ORACLE_VARIANT(oscilator),      GOSS_ORACLE_VARIANT(linear_decay),
-};
-constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+// The generated models, then the models registered at run time (a deque:
+// registration never moves the entries systems point to).
+std::deque<VariantOps>& variant_table()
+{
+  static std::deque<VariantOps> t{
+      GOSS_ORACLE_VARIANT(tp06_m),         GOSS_ORACLE_VARIANT(tp06_m_fp3),
+      GOSS_ORACLE_VARIANT(tp06_m_fp6),     GOSS_ORACLE_VARIANT(tp06_epi),
+      GOSS_ORACLE_VARIANT(fitzhughnagumo), GOSS_ORACLE_VARIANT(lorentz),
+      GOSS_ORACLE_VARIANT(fhn),            GOSS_ORACLE_VARIANT(beeler_reuter_1977),
+      GOSS_ORACLE_VARIANT(oscilator),      GOSS_ORACLE_VARIANT(linear_decay),
+  };
+  return t;
+}
+#define kVariants variant_table()
+#define kNumVariants int(variant_table().size())
 
 // ---------------------------------------------------------------------------
 // Dense helpers of ODE.cpp
@@ -920,6 +944,10 @@ void oracle_model_analytic_jacobian(int v, const double* y, double t, const doub
 {
   double c[kMaxConstants];
   kVariants[v].constants(p, c);
+  if (!kVariants[v].jacobian) {  // a model registered at run time has none
+    for (int i = 0; i < kVariants[v].S * kVariants[v].S; ++i) jac[i] = std::nan("");
+    return;
+  }
   kVariants[v].jacobian(y, t, p, c, jac);
 }
 void oracle_model_fd_jacobian(int v, double* y, double t, const double* p, double* jac)
@@ -937,10 +965,45 @@ void oracle_model_step_vjp(int v, int kind, int tape, const double* y, double t,
 {
   double c[kMaxConstants];
   kVariants[v].constants(p, c);
+  if (!kVariants[v].step_vjp) {  // a model registered at run time has none
+    for (int i = 0; i < kVariants[v].S; ++i) out[i] = std::nan("");
+    return;
+  }
   kVariants[v].step_vjp(kind, tape != 0, y, t, p, c, h, delta, lam, out, fp_bar);
 }
 void oracle_lu_factorize(int n, double* mat) { lu_factorize(n, mat); }
 void oracle_forward_backward_subst(int n, const double* mat, const double* b, double* x) { fb_subst(n, mat, b, x); }
+
+int oracle_register_external_model(const oracle_external_model* e)
+{
+  if (!e || !e->name || !e->eval || !e->linearized_eval || e->num_states <= 0 || e->num_parameters < 0) {
+    g_err = "incomplete external model";
+    return -1;
+  }
+  const int found = oracle_variant_find(e->name);
+  if (found >= 0) {
+    if (kVariants[found].ext == e) return found;
+    g_err = std::string("a model named ") + e->name + " is already registered";
+    return -1;
+  }
+  VariantOps m{};
+  m.name = e->name;
+  m.S = e->num_states;
+  m.P = e->num_parameters;
+  m.NFS = e->num_field_states;
+  m.NFP = e->num_field_parameters;
+  m.fsi = e->field_state_indices;
+  m.fpi = e->field_parameter_indices;
+  m.defaults = e->default_parameters;
+  m.ic = e->initial_conditions;
+  m.linear_terms = e->linear_terms;
+  m.pnames = e->parameter_names;
+  m.K = 0;
+  m.constants = [](const double*, double*) {};
+  m.ext = e;
+  variant_table().push_back(m);
+  return kNumVariants - 1;
+}
 
 int oracle_system_create(const char* variant, const char* solver, unsigned num_nodes, int n_params,
                          const char* const* names, const double* values, int per_node_state, oracle_system** out)

@@ -34,6 +34,29 @@ void oracle_variant_defaults(int v, double* parameters, double* initial_conditio
 void oracle_variant_field_indices(int v, int* field_state_indices, int* field_parameter_indices);
 void oracle_variant_linear_terms(int v, unsigned char* flags);
 
+/* A model supplied at run time by another library (the reference-side
+ * checker registers the reference's own gotran-generated classes this way):
+ * sizes, metadata and its right-hand side, called with the full parameter
+ * vector (P values, field parameters substituted per node).  The struct and
+ * everything it points to must outlive the oracle's use of it.  Analytic
+ * Jacobians and step adjoints are not available for such a model. */
+typedef struct oracle_external_model {
+  const char* name;
+  int num_states, num_parameters, num_field_states, num_field_parameters;
+  const int* field_state_indices;
+  const int* field_parameter_indices;
+  const double* default_parameters;
+  const double* initial_conditions;
+  const unsigned char* linear_terms;
+  const char* const* parameter_names;
+  void* ctx;
+  void (*eval)(void* ctx, const double* states, double t, const double* params, double* values);
+  void (*linearized_eval)(void* ctx, const double* states, double t, const double* params, double* linear,
+                          double* rhs, int only_linear);
+} oracle_external_model;
+/* Returns the new variant index (or the existing one for the same struct), -1 on error. */
+int oracle_register_external_model(const oracle_external_model* model);
+
 /* Single-model evaluation with a full parameter vector (P values). */
 void oracle_model_eval(int v, const double* states, double t, const double* params, double* values);
 void oracle_model_linearized_eval(int v, const double* states, double t, const double* params,

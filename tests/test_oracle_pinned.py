@@ -1,0 +1,193 @@
+"""The oracle's solvers pinned against the reference with no code of ours in
+the model, and against what the mathematics fixes.
+
+* The reference's own gotran-generated test models (cpp/tests/fhn.h,
+  beeler_reuter_1977.h, Winslow.h, Panfilov.h) are compiled with the
+  reference's solvers (cpp/goss/*.cpp) into oracle/_ref/libgoss_refpin.so.
+  The same model classes are registered with the oracle, whose restated
+  solvers then run on them: states after every solver must equal the
+  reference's bit for bit, serially and with OpenMP threads, and the oracle's
+  per-node solver state must equal one reference system per node.  The FD
+  Jacobian (ODE.cpp:47-67) must equal ODE::compute_jacobian bit for bit.
+* Closed forms on y' = a*y + b: one backward-Euler step is
+  (y0 + h*b) / (1 - h*a); ESDIRK23a's single-step amplification R(z) has a
+  local error of order z^4 (a third-order method) and tends to 0 as z -> -inf
+  (L-stability); the FD Jacobian of the linear oscillator is its matrix to
+  forward-difference rounding.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+EXPLICIT = ["ExplicitEuler", "RK2", "RK4", "RL1", "RL2", "GRL1", "GRL2"]
+ALL = EXPLICIT + ["ImplicitEuler", "ESDIRK23a", "RKF32"]
+
+needs_refpin = pytest.mark.skipif(not oracle.refpin_available(),
+                                  reason="reference build oracle/_ref/libgoss_refpin.so not present")
+
+# (model, solvers, dt, steps).  Winslow and Panfilov have no linearized_eval:
+# the Rush-Larsen schemes do not apply to them.
+CASES = [
+    ("ref_fhn", ALL, 0.1, 400),
+    ("ref_beeler_reuter_1977", ALL, 0.05, 400),
+    ("ref_winslow", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a"], 0.01, 200),
+    ("ref_winslow", ["RKF32"], 0.01, 20),
+    ("ref_panfilov", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "RKF32"], 0.1, 200),
+]
+
+
+def _inputs(v, N, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y0 = v.initial_conditions * (1 + rng.uniform(-0.01, 0.01, (N, v.num_states)))
+    fp = None
+    if v.num_field_parameters:
+        g = v.default_parameters[v.field_parameter_indices]
+        fp = g * np.exp(rng.normal(0.0, 0.1, (N, v.num_field_parameters)))
+    return y0, fp
+
+
+def _params(solver, dt):
+    # Fixed-step solvers with sub-steps, so the internal time step is exercised too.
+    return {"internal_time_step": dt / 4} if solver in EXPLICIT and solver != "GRL2" else {}
+
+
+@needs_refpin
+@pytest.mark.parametrize("threads", [0, 3])
+@pytest.mark.parametrize("model,solvers,dt,steps", CASES)
+def test_oracle_solvers_equal_reference_on_reference_models(model, solvers, dt, steps, threads):
+    oracle.register_reference_models()
+    v = oracle.Variant(model)
+    N = 7
+    y0, fp = _inputs(v, N, 5)
+    t = np.arange(steps + 1) * dt
+    for solver in solvers:
+        if threads and solver == "RKF32":
+            # The threaded reference runs copies of the solver (ODESystemSolver.cpp:236-241), and RKF32's copy
+            # constructor (RKF32.cpp:52-61) leaves `first` uninitialised: a copy may skip the k1 evaluation
+            # (RKF32.cpp:127-131) and start a node from the previous node's derivative.  That read is undefined;
+            # the oracle keeps the serial semantics (k1 evaluated on every call), pinned with threads = 0.
+            continue
+        o = oracle.OracleSystem(model, solver, N, _params(solver, dt))
+        r = oracle.RefSystem(model, solver, N, _params(solver, dt))
+        for s in (o, r):
+            if threads:
+                s.set_num_threads(threads)
+            s.set_states(y0)
+            if fp is not None:
+                s.set_field_parameters(fp)
+            s.forward_times(t)
+        a, b = o.states(), r.states()
+        assert np.all(np.isfinite(b)), (model, solver)
+        assert np.array_equal(a, b), (model, solver, np.abs(a - b).max())
+        if fp is not None:
+            assert np.array_equal(o.field_states(), r.field_states())
+
+
+@needs_refpin
+@pytest.mark.parametrize("model,solver,dt,steps", [
+    ("ref_fhn", "ImplicitEuler", 0.5, 60), ("ref_fhn", "ESDIRK23a", 0.5, 60), ("ref_fhn", "RKF32", 0.5, 60),
+    ("ref_beeler_reuter_1977", "ImplicitEuler", 0.2, 60), ("ref_beeler_reuter_1977", "ESDIRK23a", 0.2, 60),
+    ("ref_winslow", "ImplicitEuler", 0.05, 40), ("ref_winslow", "ESDIRK23a", 0.05, 40),
+])
+def test_oracle_per_node_state_equals_one_reference_system_per_node(model, solver, dt, steps):
+    """The device keeps solver state per node; the oracle's per-node mode is
+    that semantics, and equals a separate reference system for each node."""
+    oracle.register_reference_models()
+    v = oracle.Variant(model)
+    N = 5
+    y0, fp = _inputs(v, N, 9)
+    t = np.arange(steps + 1) * dt
+    o = oracle.OracleSystem(model, solver, N, {}, per_node_state=True)
+    o.set_states(y0)
+    if fp is not None:
+        o.set_field_parameters(fp)
+    o.forward_times(t)
+    got = o.states()
+    for n in range(N):
+        r = oracle.RefSystem(model, solver, 1)
+        r.set_states(y0[n:n + 1])
+        if fp is not None:
+            r.set_field_parameters(fp[n:n + 1])
+        r.forward_times(t)
+        assert np.array_equal(got[n], r.states()[0]), (model, solver, n)
+
+
+@needs_refpin
+@pytest.mark.parametrize("model", oracle.REFERENCE_MODELS)
+def test_fd_jacobian_equals_reference_compute_jacobian(model):
+    oracle.register_reference_models()
+    v = oracle.Variant(model)
+    m = oracle.RefModel(model)
+    rng = np.random.Generator(np.random.PCG64(2))
+    for k in range(5):
+        y = v.initial_conditions * (1 + rng.uniform(-0.05, 0.05, v.num_states))
+        t = float(rng.uniform(0, 20))
+        assert np.array_equal(v.eval(y, t), m.eval(y, t))
+        assert np.array_equal(v.fd_jacobian(y, t), m.compute_jacobian(y, t))
+
+
+def test_backward_euler_step_is_the_closed_form():
+    # y' = a*y + b: one ImplicitEuler step of size h solves y1 = y0 + h*(a*y1 + b).
+    a = np.array([-2.0, -50.0, -0.3, 1.5, -1e3])
+    b = np.array([3.0, 1.0, -0.7, 0.2, 5.0])
+    h = 0.01
+    N = len(a)
+    o = oracle.OracleSystem("linear_decay", "ImplicitEuler", N, {}, per_node_state=True)
+    y0 = np.array([1.0, -0.5, 2.0, 0.3, 4.0])
+    o.set_states(y0[:, None])
+    o.set_field_parameters(np.stack([a, b], 1))
+    o.forward_times(np.array([0.0, h]))
+    expect = (y0 + h * b) / (1 - h * a)
+    np.testing.assert_allclose(o.states()[:, 0], expect, rtol=1e-12, atol=0)
+    # One step each: no refinement, no rejection.
+    assert o.counters()[2] == 0
+
+
+def _esdirk_single_step(z, h=1e-3, y0=1e-6):
+    """Amplification y1/y0 of one ESDIRK23a step of size h on y' = (z/h) y.
+    The first step of ESDIRK23a is at least min_dt = 1e-3 (ESDIRK23a.h:38), so
+    an interval of h = 1e-3 is one step; y0 is small enough that the step's
+    absolute error estimate is below tol = 1e-5 and it is accepted."""
+    z = np.asarray(z, dtype=float)
+    N = z.size
+    o = oracle.OracleSystem("linear_decay", "ESDIRK23a", N, {}, per_node_state=True)
+    o.set_states(np.full((N, 1), y0))
+    o.set_field_parameters(np.stack([z / h, np.zeros(N)], 1))
+    o.forward_times(np.array([0.0, h]))
+    c = o.counters()
+    assert c[2] == 0 and c[3] == N, c  # no rejection, one accepted step per node
+    return o.states()[:, 0] / y0
+
+
+def test_esdirk23a_is_third_order():
+    z = -np.array([0.0125, 0.025, 0.05, 0.1, 0.2])
+    err = _esdirk_single_step(z) - np.exp(z)
+    # Local error C z^4 of a third-order method: halving z divides it by 16.
+    ratios = err[:-1] / err[1:]
+    np.testing.assert_allclose(ratios, 1 / 16, rtol=0.2)
+    assert np.all(np.abs(err) < 1e-3)
+
+
+def test_esdirk23a_is_l_stable():
+    z = np.array([-1e2, -1e3, -1e4, -1e5])
+    R = _esdirk_single_step(z)
+    # R(z) -> 0 like c/z (stiffly accurate, A-stable with R(inf) = 0).
+    assert np.all(np.abs(R) < 0.05)
+    zr = z * R
+    np.testing.assert_allclose(zr, zr[-1], rtol=0.1)
+
+
+def test_fd_jacobian_of_linear_model_is_its_matrix():
+    v = oracle.Variant("oscilator")
+    a, b = 1.7, -0.6
+    p = v.default_parameters.copy()
+    p[0], p[1] = a, b
+    for y in ([1.0, 0.0], [0.3, -2.0], [1e-7, 5.0]):
+        J = v.fd_jacobian(np.array(y), 0.0, p)
+        exact = np.array([[0.0, -a], [b, 0.0]])
+        # delta_i = sqrt(1e-15 max(1e-5, |y_i|)); the rounding error of the
+        # difference quotient is about eps * |f| / delta.
+        delta = np.sqrt(1e-15 * np.maximum(1e-5, np.abs(y)))
+        f = np.abs(v.eval(np.array(y), 0.0, p)).max() + 1.0
+        assert np.all(np.abs(J - exact) <= 4 * 2.2e-16 * f / delta.min())
