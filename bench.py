@@ -130,6 +130,7 @@ def timed_run(prec, N, y0, fp, t_warm, t_timed, spl, ev):
     dev.synchronize()
     ev.device_synchronize()
     stream = dev.stream()
+    dev.set_kernel_timing(True)
     e0, e1 = ev.create(), ev.create()
     w0 = time.perf_counter()
     ev.record(e0, stream)
@@ -138,9 +139,12 @@ def timed_run(prec, N, y0, fp, t_warm, t_timed, spl, ev):
     dev.synchronize()
     ev.device_synchronize()
     wall = time.perf_counter() - w0
-    kernel_ms = ev.elapsed_ms(e0, e1)
+    call_ms = ev.elapsed_ms(e0, e1)
+    # HIP events the library records on the same stream right before the first
+    # and right after the last kernel launch of the call.
+    kernel_ms, launches = dev.kernel_timing()
     dev.close()
-    return wall, kernel_ms
+    return wall, call_ms, kernel_ms, launches
 
 
 def accuracy_gate(y0, fp, dt, horizon_ms=GATE_HORIZON_MS, sample_ms=GATE_SAMPLE_MS, oracle_nodes=64):
@@ -577,10 +581,11 @@ def main():
 
     res = {}
     for prec in ("fp32c", "fp64"):
-        wall, kms = timed_run(prec, N, y0, fp, t_warm, t_timed, args.steps_per_launch, ev)
-        launches = math.ceil(K / args.steps_per_launch)
-        res[prec] = {"wall": wall, "kernel_ms": kms, "launches": launches, "value": N * K / wall}
-        log(f"{prec}: {wall:.3f} s wall, {kms:.1f} ms events, {N * K / wall:.4e} node-steps/s")
+        wall, call_ms, kms, launches = timed_run(prec, N, y0, fp, t_warm, t_timed, args.steps_per_launch, ev)
+        assert launches == math.ceil(K / args.steps_per_launch)
+        res[prec] = {"wall": wall, "call_ms": call_ms, "kernel_ms": kms, "launches": launches, "value": N * K / wall}
+        log(f"{prec}: {wall:.3f} s wall, {call_ms:.3f} ms call events, {kms:.3f} ms kernel events over {launches} "
+            f"launches, {N * K / wall:.4e} node-steps/s")
 
     flops, rhs_ops = flops_per_node_step()
     spl = min(args.steps_per_launch, K)
@@ -592,7 +597,7 @@ def main():
                 "traffic": traffic.get("bytes_per_launch") if traffic else None,
                 "kernel": "goss_hip::explicit_kernel<tp06_m_fp3, GRL1, float, COMP=true>",
                 "flops_per_node_step": flops, "node_steps_per_launch": N * spl,
-                "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches": res["fp32c"]["launches"],
                 "algorithmic_bytes_per_launch": 212 * N}
     valu = (traffic or {}).get("valu", {}).get("f32c")
     if valu and valu.get("insts_per_node_step"):

@@ -32,7 +32,10 @@ JACOBIANS = {"fd": 0, "fd_fp32": 1}
 # the exported symbols).
 EXPORTS = (
     "goss_hip_last_error", "goss_hip_device_count", "goss_hip_model_count", "goss_hip_model_name",
-    "goss_hip_model_info_get", "goss_hip_solver_params_default", "goss_hip_solver_is_adaptive",
+    "goss_hip_model_info_get", "goss_hip_model_load", "goss_hip_model_eval", "goss_hip_model_linearized_eval",
+    "goss_hip_model_compute_jacobian", "goss_hip_model_intermediates", "goss_hip_model_eval_monitored",
+    "goss_hip_lu_factorize", "goss_hip_forward_backward_subst",
+    "goss_hip_solver_params_default", "goss_hip_solver_is_adaptive",
     "goss_hip_system_create", "goss_hip_system_destroy", "goss_hip_system_forward",
     "goss_hip_system_forward_times", "goss_hip_system_forward_times_noalloc", "goss_hip_system_solve", "goss_hip_system_get_field_states",
     "goss_hip_system_set_field_states", "goss_hip_system_get_field_state_components",
@@ -42,6 +45,7 @@ EXPORTS = (
     "goss_hip_system_reset_default", "goss_hip_system_set_parameter", "goss_hip_system_get_parameter",
     "goss_hip_system_set_num_threads", "goss_hip_system_get_num_threads", "goss_hip_system_num_nodes",
     "goss_hip_system_set_steps_per_launch", "goss_hip_system_synchronize", "goss_hip_system_stream",
+    "goss_hip_system_set_kernel_timing", "goss_hip_system_get_kernel_timing",
     "goss_hip_system_get_solver_counters", "goss_hip_adjoint_buffer_sizes", "goss_hip_adjoint_record",
     "goss_hip_adjoint_backward", "goss_hip_batch_create", "goss_hip_batch_destroy", "goss_hip_batch_set_config",
     "goss_hip_batch_get_config", "goss_hip_batch_forward", "goss_hip_batch_autotune",
@@ -110,16 +114,41 @@ class GossHipError(RuntimeError):
 
 
 def lib() -> ctypes.CDLL:
+    """The product library, goss_amd/libgoss_hip.so."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C goss_amd/csrc")
-    L = ctypes.CDLL(LIB_PATH)
+    _lib = load(LIB_PATH)
+    return _lib
+
+
+_loaded = {}
+
+
+def load(path: str) -> ctypes.CDLL:
+    """A library with the C ABI of include/goss_hip.h at `path`, with its
+    signatures bound (the product library, or a test build of it)."""
+    if path in _loaded:
+        return _loaded[path]
+    L = ctypes.CDLL(path)
     L.goss_hip_last_error.restype = ctypes.c_char_p
     L.goss_hip_model_name.restype = ctypes.c_char_p
     L.goss_hip_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     L.goss_hip_model_info_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ModelInfo)]
+    if hasattr(L, "goss_hip_model_load"):
+        L.goss_hip_model_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.goss_hip_model_eval.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, ctypes.c_uint64, _dp]
+        L.goss_hip_model_linearized_eval.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, ctypes.c_uint64, ctypes.c_int,
+                                                     _dp, _dp]
+        L.goss_hip_model_compute_jacobian.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, ctypes.c_uint64, _dp]
+        L.goss_hip_model_intermediates.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                                   ctypes.POINTER(ctypes.POINTER(ctypes.c_char_p))]
+        L.goss_hip_model_eval_monitored.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                    _dp, _dp, ctypes.c_uint64, _dp]
+        L.goss_hip_lu_factorize.argtypes = [ctypes.c_int, ctypes.c_uint64, _dp]
+        L.goss_hip_forward_backward_subst.argtypes = [ctypes.c_int, ctypes.c_uint64, _dp, _dp, _dp]
     L.goss_hip_solver_params_default.argtypes = [ctypes.c_int, ctypes.POINTER(SolverParams)]
     L.goss_hip_system_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(SolverParams), ctypes.c_int,
                                          ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
@@ -145,22 +174,26 @@ def lib() -> ctypes.CDLL:
     L.goss_hip_system_set_steps_per_launch.argtypes = [V, ctypes.c_uint32]
     L.goss_hip_system_synchronize.argtypes = [V]
     L.goss_hip_system_stream.argtypes = [V, ctypes.POINTER(ctypes.c_void_p)]
+    L.goss_hip_system_set_kernel_timing.argtypes = [V, ctypes.c_int]
+    L.goss_hip_system_get_kernel_timing.argtypes = [V, _dp, ctypes.POINTER(ctypes.c_uint64)]
     L.goss_hip_system_get_solver_counters.argtypes = [V, ctypes.POINTER(ctypes.c_int64)]
     u64p = ctypes.POINTER(ctypes.c_uint64)
     L.goss_hip_adjoint_buffer_sizes.argtypes = [V, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p, u64p]
     L.goss_hip_adjoint_record.argtypes = [V, _dp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, V, V]
     L.goss_hip_adjoint_backward.argtypes = [V, _dp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, V, V, V, V, V]
-    L.goss_hip_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32,
-                                        ctypes.POINTER(ctypes.c_void_p)]
-    L.goss_hip_batch_destroy.argtypes = [V]
-    L.goss_hip_batch_set_config.argtypes = [V, ctypes.POINTER(BatchConfig)]
-    L.goss_hip_batch_get_config.argtypes = [V, ctypes.POINTER(BatchConfig)]
-    L.goss_hip_batch_forward.argtypes = [V, ctypes.c_double, ctypes.c_double, _dp]
-    L.goss_hip_batch_autotune.argtypes = [V, ctypes.c_double, ctypes.c_double, _dp, ctypes.POINTER(BatchConfig)]
-    _lib = L
+    if hasattr(L, "goss_hip_batch_create"):
+        L.goss_hip_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32,
+                                            ctypes.POINTER(ctypes.c_void_p)]
+        L.goss_hip_batch_destroy.argtypes = [V]
+        L.goss_hip_batch_set_config.argtypes = [V, ctypes.POINTER(BatchConfig)]
+        L.goss_hip_batch_get_config.argtypes = [V, ctypes.POINTER(BatchConfig)]
+        L.goss_hip_batch_forward.argtypes = [V, ctypes.c_double, ctypes.c_double, _dp]
+        L.goss_hip_batch_autotune.argtypes = [V, ctypes.c_double, ctypes.c_double, _dp,
+                                              ctypes.POINTER(BatchConfig)]
+    _loaded[path] = L
     return L
 
 
-def check(rc: int) -> None:
+def check(rc: int, L: Optional[ctypes.CDLL] = None) -> None:
     if rc != 0:
-        raise GossHipError(rc, lib().goss_hip_last_error().decode())
+        raise GossHipError(rc, (L or lib()).goss_hip_last_error().decode())

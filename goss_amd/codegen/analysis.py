@@ -321,3 +321,41 @@ def census_state_ops(stmts: Sequence[Tuple[str, sp.Expr]], outputs: Sequence[sp.
     for e in red:
         census_expr(e, c, keep)
     return {k: int(c.get(k, 0)) for k in _CENSUS_KEYS}
+
+
+def dependence_pattern(stmts: Sequence[Tuple[str, sp.Expr]], outputs: Sequence[sp.Expr],
+                       state_names: Sequence[str]) -> List[Set[int]]:
+    """rows[j] = indices of the states that output j reads, through the
+    statements as emitted (every operand of every statement counts, both arms
+    of a piecewise included).  A forward difference in state i changes output
+    j only if i is in rows[j]; every other entry of a difference-quotient
+    Jacobian is exactly zero."""
+    index = {s: k for k, s in enumerate(state_names)}
+    deps: Dict[str, Set[int]] = {}
+
+    def of(e: sp.Expr) -> Set[int]:
+        d: Set[int] = set()
+        for s in e.free_symbols:
+            if s.name in index:
+                d.add(index[s.name])
+            elif s.name in deps:
+                d |= deps[s.name]
+        return d
+
+    for name, e in stmts:
+        deps[name] = of(e)
+    return [of(e) for e in outputs]
+
+
+def lu_fill_pattern(rows: Sequence[Set[int]]) -> List[Set[int]]:
+    """Structure of the no-pivot LU factors of a matrix with structure rows
+    plus the diagonal: entry (i, j) is nonzero if it is in the matrix or
+    some r < min(i, j) has (i, r) and (r, j) nonzero (symbolic Gaussian
+    elimination in the natural order)."""
+    S = len(rows)
+    pat = [set(r) | {i} for i, r in enumerate(rows)]
+    for k in range(S):
+        for i in range(k + 1, S):
+            if k in pat[i]:
+                pat[i] |= {j for j in pat[k] if j > k}
+    return pat

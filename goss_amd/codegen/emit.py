@@ -26,7 +26,8 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import sympy as sp
 
 from .adjoint import gate_states, lift_calls, reverse_sweep
-from .analysis import Jacobian, Linearized, census_body, census_state_ops, jacobian, linearize, needed_names
+from .analysis import (Jacobian, Linearized, census_body, census_state_ops, dependence_pattern, jacobian, linearize,
+                       needed_names)
 from .hoist import ConstantTable, hoist_statements
 from .ode_model import ODEModel, TIME
 from .printer import CPrinter, assignments, c_name, fmt_double
@@ -76,6 +77,7 @@ class Emitted:
     linear_terms: List[int]
     jac_nnz: int
     num_constants: int
+    structure_nnz: Tuple[int, int] = (0, 0)  # Jacobian and LU structure sizes (implicit solvers)
 
 
 def _closure(stmts: Dict[str, sp.Expr], roots: Sequence[sp.Expr]) -> set:
@@ -198,6 +200,7 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         hjac_stmts = hoist_statements(jac.stmts, table)
         hjac = [[table.rewrite(e) for e in row] for row in jac.entries]
     derivs = set(model.derivatives.values())
+    derivs_names = set(model.derivatives.values())
 
     all_stmts = list(hbody) + hlin_first + hlin_second + (hjac_stmts if jac is not None else [])
     g = _Gen(var, table, all_stmts)
@@ -269,6 +272,33 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         L.append(f"{ind}values[{i}] = {pr.p(rhs_out[i])};")
     L.append("}")
     L.append("")
+
+    # -- eval_intermediates: every named intermediate (monitored values) -------
+    inter_names = [n for n, _ in model.body if n not in derivs_names]
+    inter_out = [out_expr(n) for n in inter_names]
+    mon_stmts = select(body_inter, inter_out)
+    mused = g.used_main(mon_stmts, inter_out)
+    L.append(f"constexpr int num_intermediates = {len(inter_names)};")
+    L.append("static const char* const intermediate_names[] = {" +
+             (", ".join(f'"{n}"' for n in inter_names) or "nullptr") + "};")
+    L.append("// Every named intermediate of the model at (states, time): the values a caller may monitor")
+    L.append("// (ParameterizedODE::eval_monitored, ParameterizedODE.h:90-105, selects among them).")
+    L.append("template <typename real, typename PA, typename treal>")
+    L.append("GOSS_HD GOSS_INLINE void eval_intermediates(const real* __restrict__ states, const treal time, const PA& pa,")
+    L.append("                                            real* __restrict__ out)")
+    L.append("{")
+    L.append(f"{ind}(void)time; (void)pa; (void)states; (void)out;")
+    L += g.bind(mused, ind)
+    L += g.lines(mon_stmts, ind)
+    for k, e in enumerate(inter_out):
+        L.append(f"{ind}out[{k}] = real({pr.p(e)});")
+    L.append("}")
+    L.append("")
+
+    # -- structure of the Jacobian and of its LU factors (implicit solvers) ----
+    struct_rows = dependence_pattern(eval_stmts, rhs_out, model.state_names)
+    sL, struct_jnnz, struct_lnnz = emit_sparse_structure(S, struct_rows, ind)
+    L += sL
 
     # -- linearized_eval ----------------------------------------------------
     lin_roots = [e for _, e in hlin_first + hlin_second] + hlinear
@@ -582,7 +612,7 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("}}  // namespace goss_models::" + var.name)
     L.append("")
     return Emitted(header="\n".join(L), census=census, linear_terms=lin.linear_terms, jac_nnz=jac_nnz,
-                   num_constants=len(table.items))
+                   num_constants=len(table.items), structure_nnz=(struct_jnnz, struct_lnnz))
 
 
 def emit_parameterized_ode(var: Variant, header_include: str) -> str:
@@ -709,5 +739,133 @@ def variant_metadata(var: Variant, em: Emitted) -> dict:
         "linear_terms": em.linear_terms,
         "census": em.census,
         "jacobian_nnz": em.jac_nnz,
+        "jacobian_structure_nnz": em.structure_nnz[0],
+        "lu_structure_nnz": em.structure_nnz[1],
         "num_constants": em.num_constants,
     }
+
+
+def emit_sparse_structure(S: int, rows: Sequence[set], ind: str = "  ") -> Tuple[List[str], int, int]:
+    """Structure of the Jacobian (rows[j] = states output j reads) and of its
+    no-pivot LU factors, with straight-line LU factorisation and substitution
+    over that structure.  The code follows ODE::lu_factorize (ODE.cpp:69-100)
+    and ODE::forward_backward_subst (ODE.cpp:102-132) term by term, leaving out
+    only products with a structurally zero factor: for finite entries the
+    results equal the dense routines' (a skipped term adds an exact zero)."""
+    from .analysis import lu_fill_pattern
+
+    lu = lu_fill_pattern(rows)
+    # Jacobian storage: column-major over the structure.
+    cols = [sorted(j for j in range(S) if i in rows[j]) for i in range(S)]
+    jac_slot = {}
+    for i in range(S):
+        for j in cols[i]:
+            jac_slot[(j, i)] = len(jac_slot)
+    # LU storage: row-major over the fill structure.
+    lu_slot = {}
+    for j in range(S):
+        for i in sorted(lu[j]):
+            lu_slot[(j, i)] = len(lu_slot)
+    jn, ln = len(jac_slot), len(lu_slot)
+    L: List[str] = []
+    L.append("// Structure of the Jacobian d values[j] / d states[i]: values[j] reads states[i] only for the")
+    L.append("// entries below (through any operand of the emitted right-hand side), so every other entry of a")
+    L.append("// forward-difference Jacobian is exactly zero.  And the structure of its no-pivot LU factors")
+    L.append("// (the entries, the diagonal and the fill).")
+    L.append(f"constexpr int jac_nnz = {jn};")
+    L.append(f"constexpr int lu_nnz = {ln};")
+    L.append("// jac_cols[i]: bit j set when values[j] reads states[i].  Jacobian storage is column-major over the")
+    L.append("// structure: entry (j, i) at jac_col_start[i] + (number of bits of jac_cols[i] below j).")
+    masks = [sum(1 << j for j in cols[i]) for i in range(S)]
+    starts, acc = [], 0
+    for i in range(S):
+        starts.append(acc)
+        acc += len(cols[i])
+    L.append("constexpr unsigned jac_cols[] = {" + ", ".join(f"0x{m:x}u" for m in masks) + "};")
+    L.append("constexpr int jac_col_start[] = {" + ", ".join(str(s) for s in starts) + "};")
+    L.append("// LU storage is row-major over its structure: lu_slot[j * num_states + i], -1 outside.")
+    L.append("constexpr short lu_slot[] = {" + ", ".join(str(lu_slot.get((j, i), -1)) for j in range(S)
+                                                       for i in range(S)) + "};")
+    L.append("")
+    L.append("// a = I + scale * J on the LU structure (fill entries zero): ODE.cpp's dense scale-and-add-identity")
+    L.append("// of ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105).")
+    L.append("template <typename real>")
+    L.append("GOSS_HD GOSS_INLINE void lu_from_jacobian(const real* __restrict__ J, const real scale, real* __restrict__ a)")
+    L.append("{")
+    for j in range(S):
+        for i in sorted(lu[j]):
+            s = lu_slot[(j, i)]
+            if (j, i) in jac_slot:
+                e = f"J[{jac_slot[(j, i)]}] * scale"
+                if i == j:
+                    e += " + real(1.0)"
+            else:
+                e = "real(1.0)" if i == j else "real(0.0)"
+            L.append(f"{ind}a[{s}] = {e};")
+    L.append("}")
+    L.append("")
+    L.append("// ODE::lu_factorize (ODE.cpp:69-100) in place on a, over the structure.")
+    L.append("template <typename real>")
+    L.append("GOSS_HD GOSS_INLINE void lu_factorize_sparse(real* __restrict__ a)")
+    L.append("{")
+    L.append(f"{ind}real acc;")
+
+    def A(r, c):
+        return f"a[{lu_slot[(r, c)]}]"
+
+    def has(r, c):
+        return (r, c) in lu_slot
+
+    for k in range(1, S):
+        for i in range(k):
+            if has(i, k):
+                terms = [r for r in range(i) if has(i, r) and has(r, k)]
+                if terms:
+                    L.append(f"{ind}acc = {A(i, terms[0])} * {A(terms[0], k)};")
+                    for r in terms[1:]:
+                        L.append(f"{ind}acc = acc + {A(i, r)} * {A(r, k)};")
+                    L.append(f"{ind}{A(i, k)} = {A(i, k)} - acc;")
+            if has(k, i):
+                terms = [r for r in range(i) if has(k, r) and has(r, i)]
+                if terms:
+                    L.append(f"{ind}acc = {A(k, terms[0])} * {A(terms[0], i)};")
+                    for r in terms[1:]:
+                        L.append(f"{ind}acc = acc + {A(k, r)} * {A(r, i)};")
+                    L.append(f"{ind}{A(k, i)} = ({A(k, i)} - acc) / {A(i, i)};")
+                else:
+                    L.append(f"{ind}{A(k, i)} = {A(k, i)} / {A(i, i)};")
+        terms = [r for r in range(k) if has(k, r) and has(r, k)]
+        if terms:
+            L.append(f"{ind}acc = {A(k, terms[0])} * {A(terms[0], k)};")
+            for r in terms[1:]:
+                L.append(f"{ind}acc = acc + {A(k, r)} * {A(r, k)};")
+            L.append(f"{ind}{A(k, k)} = {A(k, k)} - acc;")
+    L.append(f"{ind}(void)acc;")
+    L.append("}")
+    L.append("")
+    L.append("// ODE::forward_backward_subst (ODE.cpp:102-132) in place on x, over the structure.")
+    L.append("template <typename real>")
+    L.append("GOSS_HD GOSS_INLINE void lu_solve_sparse(const real* __restrict__ a, real* __restrict__ x)")
+    L.append("{")
+    L.append(f"{ind}real acc;")
+    for i in range(1, S):
+        terms = [j for j in range(i) if has(i, j)]
+        if terms:
+            L.append(f"{ind}acc = {A(i, terms[0])} * x[{terms[0]}];")
+            for j in terms[1:]:
+                L.append(f"{ind}acc = acc + {A(i, j)} * x[{j}];")
+            L.append(f"{ind}x[{i}] = x[{i}] - acc;")
+    L.append(f"{ind}x[{S - 1}] = x[{S - 1}] / {A(S - 1, S - 1)};")
+    for i in range(S - 2, -1, -1):
+        terms = [j for j in range(i + 1, S) if has(i, j)]
+        if terms:
+            L.append(f"{ind}acc = {A(i, terms[0])} * x[{terms[0]}];")
+            for j in terms[1:]:
+                L.append(f"{ind}acc = acc + {A(i, j)} * x[{j}];")
+            L.append(f"{ind}x[{i}] = (x[{i}] - acc) / {A(i, i)};")
+        else:
+            L.append(f"{ind}x[{i}] = x[{i}] / {A(i, i)};")
+    L.append(f"{ind}(void)acc;")
+    L.append("}")
+    L.append("")
+    return L, jn, ln

@@ -66,6 +66,80 @@ GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time,
   values[1] = b*(V - V_rest - c_3*s);
 }
 
+constexpr int num_intermediates = 4;
+static const char* const intermediate_names[] = {"V_amp", "V_th", "I", "i_Stim"};
+// Every named intermediate of the model at (states, time): the values a caller may monitor
+// (ParameterizedODE::eval_monitored, ParameterizedODE.h:90-105, selects among them).
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval_intermediates(const real* __restrict__ states, const treal time, const PA& pa,
+                                            real* __restrict__ out)
+{
+  (void)time; (void)pa; (void)states; (void)out;
+  const real V = states[0];
+  const real s = states[1];
+  const real V_peak = pa.template get<0>();
+  const real V_rest = pa.template get<1>();
+  const real stim_amplitude = pa.template get<7>();
+  const real stim_start = pa.template get<10>();
+  const real c__0 = pa.template cst<0>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__4 = pa.template cst<4>();
+  out[0] = real(c__0);
+  out[1] = real(c__1);
+  out[2] = real(c__2*s*(V - V_rest) + c__3*(V - V_rest)*(V - c__1)*(V_peak - V));
+  out[3] = real(m_div(stim_amplitude*(real(1.0) - m_rcp(real(1.0) + m_exp_l2(real(7.213475204444817)*real(time) - real(7.213475204444817)*stim_start, real(5.0)*real(time) - real(5.0)*stim_start))), (real(1.0) + m_exp_l2(real(1.4426950408889634)*c__4 + real(7.213475204444817)*real(time), c__4 + real(5.0)*real(time)))));
+}
+
+// Structure of the Jacobian d values[j] / d states[i]: values[j] reads states[i] only for the
+// entries below (through any operand of the emitted right-hand side), so every other entry of a
+// forward-difference Jacobian is exactly zero.  And the structure of its no-pivot LU factors
+// (the entries, the diagonal and the fill).
+constexpr int jac_nnz = 4;
+constexpr int lu_nnz = 4;
+// jac_cols[i]: bit j set when values[j] reads states[i].  Jacobian storage is column-major over the
+// structure: entry (j, i) at jac_col_start[i] + (number of bits of jac_cols[i] below j).
+constexpr unsigned jac_cols[] = {0x3u, 0x3u};
+constexpr int jac_col_start[] = {0, 2};
+// LU storage is row-major over its structure: lu_slot[j * num_states + i], -1 outside.
+constexpr short lu_slot[] = {0, 1, 2, 3};
+
+// a = I + scale * J on the LU structure (fill entries zero): ODE.cpp's dense scale-and-add-identity
+// of ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105).
+template <typename real>
+GOSS_HD GOSS_INLINE void lu_from_jacobian(const real* __restrict__ J, const real scale, real* __restrict__ a)
+{
+  a[0] = J[0] * scale + real(1.0);
+  a[1] = J[2] * scale;
+  a[2] = J[1] * scale;
+  a[3] = J[3] * scale + real(1.0);
+}
+
+// ODE::lu_factorize (ODE.cpp:69-100) in place on a, over the structure.
+template <typename real>
+GOSS_HD GOSS_INLINE void lu_factorize_sparse(real* __restrict__ a)
+{
+  real acc;
+  a[2] = a[2] / a[0];
+  acc = a[2] * a[1];
+  a[3] = a[3] - acc;
+  (void)acc;
+}
+
+// ODE::forward_backward_subst (ODE.cpp:102-132) in place on x, over the structure.
+template <typename real>
+GOSS_HD GOSS_INLINE void lu_solve_sparse(const real* __restrict__ a, real* __restrict__ x)
+{
+  real acc;
+  acc = a[2] * x[0];
+  x[1] = x[1] - acc;
+  x[1] = x[1] / a[3];
+  acc = a[1] * x[1];
+  x[0] = (x[0] - acc) / a[0];
+  (void)acc;
+}
+
 // Linearized right-hand side: rhs[i] as in eval and linear[i] = d rhs[i] / d states[i].
 // Entries with linear_terms[i] == 1 are written before the only_linear return.
 template <typename real, typename PA, typename treal>

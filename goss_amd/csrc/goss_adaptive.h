@@ -138,7 +138,7 @@ struct Rkf32Node {
         }
       }
       nfevals += 3;
-      double errd = sqrt(double(err) / S);
+      double errd = m_sqrt(double(err) / S);
       if (__builtin_isnan(errd) || __builtin_isinf(errd)) errd = 2000.0;
       bool accepted;
       if (errd <= 1.0) {
@@ -148,7 +148,7 @@ struct Rkf32Node {
       } else {
         accepted = false;
       }
-      double fac = stabfac * pow(1.0 / errd, 1.0 / (iord + 1));
+      double fac = stabfac * m_pow(1.0 / errd, 1.0 / (iord + 1));
       if (facmin > fac) fac = facmin;
       if (fac > facmax) fac = facmax;
       facmax = accepted ? facmaxb : 1.0;

@@ -7,6 +7,7 @@
 // with ld = num_nodes rounded up to a multiple of 64, so that every state row
 // starts on a 512-byte (double) / 256-byte (float) boundary and one
 // wavefront's 64 lanes read one contiguous segment.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,6 +15,7 @@
 #include <numeric>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <sstream>
 #include <stdexcept>
@@ -23,20 +25,37 @@
 #include "../../include/goss_hip.h"
 #include "generated/variant_list.h"
 #include "goss_batch.h"
+
+// Test builds may compile a subset of the variants (GOSS_VARIANT_SUBSET, an
+// X-macro list) and only their FP64 family (GOSS_F64_ONLY), without the
+// mixed-population batch (GOSS_NO_BATCH).
+#if defined(GOSS_VARIANT_SUBSET)
+#undef GOSS_VARIANT_LIST
+#define GOSS_VARIANT_LIST(X) GOSS_VARIANT_SUBSET(X)
+#endif
 #include "goss_layout.h"
 #include "goss_registry.h"
 
 namespace goss_hip {
 
+#if defined(GOSS_F64_ONLY)
+#define GOSS_DECLARE_REGISTER(V) void goss_register_##V##_f64(VariantInfo&);
+#define GOSS_REGISTER_F32(V, info)
+#else
 #define GOSS_DECLARE_REGISTER(V) \
   void goss_register_##V##_f64(VariantInfo&); \
   void goss_register_##V##_f32(VariantInfo&);
+#define GOSS_REGISTER_F32(V, info) goss_register_##V##_f32(info);
+#endif
 GOSS_VARIANT_LIST(GOSS_DECLARE_REGISTER)
 
 namespace {
 
 struct Registry {
-  std::vector<VariantInfo> v;
+  // A deque: variants loaded at run time are appended without moving the
+  // entries that systems point to.
+  std::deque<VariantInfo> v;
+  std::mutex mutex;
   Registry()
   {
 #define GOSS_DO_REGISTER(V)              \
@@ -44,7 +63,7 @@ struct Registry {
     VariantInfo info;                    \
     std::memset(&info, 0, sizeof(info)); \
     goss_register_##V##_f64(info);       \
-    goss_register_##V##_f32(info);       \
+    GOSS_REGISTER_F32(V, info)           \
     v.push_back(info);                   \
   }
     GOSS_VARIANT_LIST(GOSS_DO_REGISTER)
@@ -59,10 +78,18 @@ Registry& registry()
 
 }  // namespace
 
-VariantInfo* variants(int* count)
+VariantInfo* variant_at(int index)
 {
-  *count = int(registry().v.size());
-  return registry().v.data();
+  Registry& r = registry();
+  std::lock_guard<std::mutex> lock(r.mutex);
+  return (index >= 0 && index < int(r.v.size())) ? &r.v[size_t(index)] : nullptr;
+}
+
+int variant_count()
+{
+  Registry& r = registry();
+  std::lock_guard<std::mutex> lock(r.mutex);
+  return int(r.v.size());
 }
 
 }  // namespace goss_hip
@@ -100,10 +127,11 @@ int guarded(F&& f)
 
 const VariantInfo* find_variant(const char* name)
 {
-  int n = 0;
-  VariantInfo* v = variants(&n);
-  for (int i = 0; i < n; ++i)
-    if (name && std::strcmp(v[i].name, name) == 0) return &v[i];
+  const int n = variant_count();
+  for (int i = 0; i < n; ++i) {
+    const VariantInfo* v = variant_at(i);
+    if (name && std::strcmp(v->name, name) == 0) return v;
+  }
   throw Error(GOSS_HIP_EINVAL, std::string("unknown model variant ") + (name ? name : "(null)"));
 }
 
@@ -155,6 +183,9 @@ struct goss_hip_system {
   uint32_t steps_per_launch = 200;
   IndexMap map{};
   void* scratch = nullptr;  // device address of the library's scratch array on this device
+  // Kernel timing of the last stepping call (goss_hip_system_set_kernel_timing).
+  hipEvent_t t_begin = nullptr, t_end = nullptr;
+  uint64_t timed_launches = 0;
   ImplicitBuffers ib{};  // implicit solvers only
   int* h_error = nullptr;  // pinned copy of ib.error
 
@@ -215,6 +246,18 @@ struct goss_hip_system {
     }
   }
 
+  // Events around the kernel launches of one stepping call, recorded right
+  // before the first and right after the last launch on the system's stream.
+  void timing_begin()
+  {
+    timed_launches = 0;
+    if (t_begin) hip_check(hipEventRecord(t_begin, stream), "hipEventRecord");
+  }
+  void timing_end()
+  {
+    if (t_end) hip_check(hipEventRecord(t_end, stream), "hipEventRecord");
+  }
+
   void launch_steps(const double2* d, size_t n, double* field_out, int tangled)
   {
     if (is_implicit(solver)) {
@@ -227,7 +270,8 @@ struct goss_hip_system {
     }
     const ExplicitLaunch launch = v->explicit_launch[solver][prec];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not available for this model");
-    for (size_t off = 0; off < n; off += steps_per_launch) {
+    timing_begin();
+    for (size_t off = 0; off < n; off += steps_per_launch, ++timed_launches) {
       const int cnt = int(std::min<size_t>(steps_per_launch, n - off));
       ExplicitArgs a;
       a.y = y;
@@ -246,6 +290,7 @@ struct goss_hip_system {
       a.stream = stream;
       hip_check(launch(a), "kernel launch");
     }
+    timing_end();
   }
 
   ImplicitParams implicit_params() const
@@ -270,7 +315,8 @@ struct goss_hip_system {
   {
     const ImplicitLaunch launch = v->implicit_launch[solver == kImplicitEuler ? 0 : 1][sp.jacobian];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
-    for (size_t off = 0; off < n; off += steps_per_launch) {
+    timing_begin();
+    for (size_t off = 0; off < n; off += steps_per_launch, ++timed_launches) {
       const int cnt = int(std::min<size_t>(steps_per_launch, n - off));
       ImplicitArgs a;
       a.y = static_cast<double*>(y);
@@ -288,6 +334,7 @@ struct goss_hip_system {
       a.stream = stream;
       hip_check(launch(a), "kernel launch");
     }
+    timing_end();
     if (failed()) {
       std::ostringstream msg;
       msg << "Newtons solver failed to converge as dt become smaller than 'min_dt' " << sp.min_dt << std::endl;
@@ -300,7 +347,8 @@ struct goss_hip_system {
   {
     const AdaptiveLaunch launch = v->rkf32_launch[prec];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "RKF32 not compiled for this model and precision");
-    for (size_t off = 0; off < n; off += steps_per_launch) {
+    timing_begin();
+    for (size_t off = 0; off < n; off += steps_per_launch, ++timed_launches) {
       const int cnt = int(std::min<size_t>(steps_per_launch, n - off));
       AdaptiveArgs a;
       a.y = y;
@@ -319,6 +367,7 @@ struct goss_hip_system {
       a.stream = stream;
       hip_check(launch(a), "kernel launch");
     }
+    timing_end();
     if (failed()) throw Error(GOSS_HIP_ESOLVER, "RKF32 did not reach the end of the interval");
   }
 
@@ -361,6 +410,8 @@ struct goss_hip_system {
     if (h_steps) (void)hipHostFree(h_steps);
     if (d_stage) (void)hipFree(d_stage);
     if (steps_free) (void)hipEventDestroy(steps_free);
+    if (t_begin) (void)hipEventDestroy(t_begin);
+    if (t_end) (void)hipEventDestroy(t_end);
     if (ib.jac) (void)hipFree(ib.jac);
     if (ib.jfac) (void)hipFree(ib.jfac);
     if (ib.work) (void)hipFree(ib.work);
@@ -458,18 +509,38 @@ int goss_hip_device_count(int* count)
   });
 }
 
-int goss_hip_model_count(void)
-{
-  int n = 0;
-  variants(&n);
-  return n;
-}
+int goss_hip_model_count(void) { return variant_count(); }
 
 const char* goss_hip_model_name(int index)
 {
-  int n = 0;
-  VariantInfo* v = variants(&n);
-  return (index >= 0 && index < n) ? v[index].name : nullptr;
+  const VariantInfo* v = variant_at(index);
+  return v ? v->name : nullptr;
+}
+
+int goss_hip_model_load(const char* library_path, const char* variant)
+{
+  return guarded([&] {
+    if (!library_path || !variant) throw Error(GOSS_HIP_EINVAL, "null argument");
+    for (int i = 0; i < variant_count(); ++i)
+      if (std::strcmp(variant_at(i)->name, variant) == 0)
+        throw Error(GOSS_HIP_EINVAL, std::string("a model variant named ") + variant + " is already registered");
+    // The library stays loaded until the process ends: systems and kernels refer to it.
+    void* h = dlopen(library_path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) throw Error(GOSS_HIP_EINVAL, std::string("cannot load ") + library_path + ": " + dlerror());
+    using Reg = int (*)(void*, size_t);
+    auto r64 = reinterpret_cast<Reg>(dlsym(h, "goss_hip_plugin_register_f64"));
+    auto r32 = reinterpret_cast<Reg>(dlsym(h, "goss_hip_plugin_register_f32"));
+    if (!r64 || !r32) throw Error(GOSS_HIP_EINVAL, std::string(library_path) + " is not a goss_hip model library");
+    VariantInfo info;
+    std::memset(&info, 0, sizeof(info));
+    if (r64(&info, sizeof(info)) != 0 || r32(&info, sizeof(info)) != 0)
+      throw Error(GOSS_HIP_EINVAL, std::string(library_path) + " was built against another version of this library");
+    if (std::strcmp(info.name, variant) != 0)
+      throw Error(GOSS_HIP_EINVAL, std::string(library_path) + " holds variant " + info.name + ", not " + variant);
+    Registry& reg = registry();
+    std::lock_guard<std::mutex> lock(reg.mutex);
+    reg.v.push_back(info);
+  });
 }
 
 int goss_hip_model_info_get(const char* variant, goss_hip_model_info* out)
@@ -504,6 +575,188 @@ int goss_hip_solver_is_adaptive(int solver)
 {
   // AdaptiveExplicitSolver.h / AdaptiveImplicitSolver.h:116-119.
   return solver == kESDIRK23a || solver == kRKF32;
+}
+
+// ---------------------------------------------------------------------------
+// Model evaluation and dense LU over many points (ParameterizedODE / ODE
+// methods, ODE.h:55-77).  Host arrays in and out; the work runs on the device.
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+// RAII device buffer of doubles.
+struct DevBuf {
+  double* p = nullptr;
+  explicit DevBuf(size_t n)
+  {
+    if (n) hip_check(hipMalloc(&p, n * sizeof(double)), "hipMalloc");
+  }
+  ~DevBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+  void in(const double* h, size_t n) { hip_check(hipMemcpy(p, h, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"); }
+  void out(double* h, size_t n) const { hip_check(hipMemcpy(h, p, n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy"); }
+};
+
+// ODE::lu_factorize (ODE.cpp:69-100), one S x S row-major matrix per lane.
+__global__ void dense_lu_kernel(const int S, const unsigned long n, double* __restrict__ mats)
+{
+  const unsigned long p = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double* a = mats + p * size_t(S) * S;
+  for (int k = 1; k < S; k++) {
+    for (int i = 0; i <= k - 1; ++i) {
+      double sum = 0.0;
+      for (int r = 0; r <= i - 1; r++) sum += a[i * S + r] * a[r * S + k];
+      a[i * S + k] -= sum;
+      sum = 0.0;
+      for (int r = 0; r <= i - 1; r++) sum += a[k * S + r] * a[r * S + i];
+      a[k * S + i] = (a[k * S + i] - sum) / a[i * S + i];
+    }
+    double sum = 0.0;
+    for (int r = 0; r <= k - 1; r++) sum += a[k * S + r] * a[r * S + k];
+    a[k * S + k] -= sum;
+  }
+}
+
+// ODE::forward_backward_subst (ODE.cpp:102-132), one system per lane.
+__global__ void dense_fb_subst_kernel(const int S, const unsigned long n, const double* __restrict__ mats,
+                                      const double* __restrict__ b, double* __restrict__ x)
+{
+  const unsigned long p = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double* a = mats + p * size_t(S) * S;
+  const double* bp = b + p * S;
+  double* dx = x + p * S;
+  dx[0] = bp[0];
+  for (int i = 1; i < S; ++i) {
+    double sum = 0.0;
+    for (int j = 0; j <= i - 1; ++j) sum = sum + a[i * S + j] * dx[j];
+    dx[i] = bp[i] - sum;
+  }
+  dx[S - 1] = dx[S - 1] / a[(S - 1) * S + (S - 1)];
+  for (int i = S - 2; i >= 0; i--) {
+    double sum = 0;
+    for (int j = i + 1; j < S; ++j) sum = sum + a[i * S + j] * dx[j];
+    dx[i] = (dx[i] - sum) / a[i * S + i];
+  }
+}
+
+// Runs one model-evaluation kind at n points.
+void model_eval(const char* variant, const double* params, const double* states, const double* times, uint64_t n,
+                int kind, int only_linear, double* out0, double* out1, const MonitoredList& mon = MonitoredList{})
+{
+  const VariantInfo* v = find_variant(variant);
+  if (!v->model_eval) throw Error(GOSS_HIP_EUNSUPPORTED, "model evaluation not compiled for this variant");
+  if (n == 0) return;
+  const size_t S = size_t(v->S);
+  const size_t w0 = kind == 2 ? S * S : (kind == 3 ? size_t(mon.n) : S);
+  if (kind == 3 && mon.n == 0) return;
+  DevBuf y(n * S), t(n), o0(n * w0), o1(kind == 1 ? n * S : 0);
+  y.in(states, n * S);
+  t.in(times, n);
+  ModelEvalArgs a{};
+  a.kind = kind;
+  a.only_linear = only_linear;
+  a.monitored = mon;
+  a.states = y.p;
+  a.times = t.p;
+  a.n = n;
+  a.params = params ? params : v->defaults;
+  a.out0 = o0.p;
+  a.out1 = o1.p;
+  a.stream = nullptr;
+  if (kind == 1 && only_linear) {
+    // linearized_eval leaves the entries of the non-linear states untouched: start from the caller's.
+    o0.in(out0, n * S);
+  }
+  hip_check(v->model_eval(a), "kernel launch");
+  o0.out(out0, n * w0);
+  if (kind == 1) o1.out(out1, n * S);
+}
+
+}  // namespace
+
+extern "C" {
+
+int goss_hip_model_eval(const char* variant, const double* params, const double* states, const double* times,
+                        uint64_t n, double* values)
+{
+  return guarded([&] { model_eval(variant, params, states, times, n, 0, 0, values, nullptr); });
+}
+
+int goss_hip_model_linearized_eval(const char* variant, const double* params, const double* states,
+                                   const double* times, uint64_t n, int only_linear, double* linear, double* rhs)
+{
+  return guarded([&] { model_eval(variant, params, states, times, n, 1, only_linear, linear, rhs); });
+}
+
+int goss_hip_model_compute_jacobian(const char* variant, const double* params, const double* states,
+                                    const double* times, uint64_t n, double* jac)
+{
+  return guarded([&] { model_eval(variant, params, states, times, n, 2, 0, jac, nullptr); });
+}
+
+int goss_hip_model_intermediates(const char* variant, int* count, const char* const** names)
+{
+  return guarded([&] {
+    const VariantInfo* v = find_variant(variant);
+    *count = v->num_intermediates;
+    *names = v->intermediate_names;
+  });
+}
+
+int goss_hip_model_eval_monitored(const char* variant, const double* params, int num_monitored,
+                                  const int* monitored, const double* states, const double* times, uint64_t n,
+                                  double* values)
+{
+  return guarded([&] {
+    const VariantInfo* v = find_variant(variant);
+    if (num_monitored < 0 || num_monitored > kMaxMonitored)
+      throw Error(GOSS_HIP_EINVAL, "at most " + std::to_string(kMaxMonitored) + " monitored values");
+    MonitoredList mon{};
+    mon.n = num_monitored;
+    for (int k = 0; k < num_monitored; ++k) {
+      if (monitored[k] < 0 || monitored[k] >= v->num_intermediates)
+        throw Error(GOSS_HIP_EINVAL, "monitored index out of range");
+      mon.index[k] = monitored[k];
+    }
+    model_eval(variant, params, states, times, n, 3, 0, values, nullptr, mon);
+  });
+}
+
+int goss_hip_lu_factorize(int num_states, uint64_t n, double* mats)
+{
+  return guarded([&] {
+    if (num_states <= 0 || num_states > kMaxStates) throw Error(GOSS_HIP_EINVAL, "num_states out of range");
+    if (n == 0) return;
+    const size_t m = size_t(num_states) * num_states;
+    DevBuf d(n * m);
+    d.in(mats, n * m);
+    hipLaunchKernelGGL(dense_lu_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, nullptr, num_states,
+                       (unsigned long)n, d.p);
+    hip_check(hipGetLastError(), "kernel launch");
+    d.out(mats, n * m);
+  });
+}
+
+int goss_hip_forward_backward_subst(int num_states, uint64_t n, const double* mats, const double* b, double* x)
+{
+  return guarded([&] {
+    if (num_states <= 0 || num_states > kMaxStates) throw Error(GOSS_HIP_EINVAL, "num_states out of range");
+    if (n == 0) return;
+    const size_t m = size_t(num_states) * num_states;
+    DevBuf dm(n * m), db(n * num_states), dx(n * num_states);
+    dm.in(mats, n * m);
+    db.in(b, n * num_states);
+    hipLaunchKernelGGL(dense_fb_subst_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, nullptr, num_states,
+                       (unsigned long)n, dm.p, db.p, dx.p);
+    hip_check(hipGetLastError(), "kernel launch");
+    dx.out(x, n * num_states);
+  });
 }
 
 int goss_hip_system_create(const char* variant, int solver, const goss_hip_solver_params* params, int precision,
@@ -562,13 +815,15 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
     hip_check(hipMemsetAsync(s->fp, 0, size_t(std::max(v->NFP, 1)) * s->ld * s->elem(), s->stream), "hipMemset");
     s->reset_default();
     if (is_implicit(solver)) {
-      const size_t mat = size_t(v->S) * v->S * s->ld * sizeof(double);
-      hip_check(hipMalloc(&s->ib.jac, mat), "hipMalloc(jacobian)");
-      if (solver == kESDIRK23a) hip_check(hipMalloc(&s->ib.jfac, mat), "hipMalloc(factorized jacobian)");
+      // The Jacobian and the LU factors over their structure (jac_nnz, lu_nnz entries per node).
+      const size_t jbytes = size_t(v->jac_nnz) * s->ld * sizeof(double);
+      const size_t fbytes = size_t(v->lu_nnz) * s->ld * sizeof(double);
+      hip_check(hipMalloc(&s->ib.jac, jbytes), "hipMalloc(jacobian)");
+      hip_check(hipMalloc(&s->ib.jfac, fbytes), "hipMalloc(factorized jacobian)");
       const size_t work = size_t(implicit_work_vectors(solver)) * v->S * s->ld * sizeof(double);
       hip_check(hipMalloc(&s->ib.work, work), "hipMalloc(work vectors)");
-      hip_check(hipMemsetAsync(s->ib.jac, 0, mat, s->stream), "hipMemset");
-      if (s->ib.jfac) hip_check(hipMemsetAsync(s->ib.jfac, 0, mat, s->stream), "hipMemset");
+      hip_check(hipMemsetAsync(s->ib.jac, 0, jbytes, s->stream), "hipMemset");
+      hip_check(hipMemsetAsync(s->ib.jfac, 0, fbytes, s->stream), "hipMemset");
       hip_check(hipMemsetAsync(s->ib.work, 0, work, s->stream), "hipMemset");
     }
     if (solver == kRKF32) {
@@ -1047,6 +1302,40 @@ int goss_hip_system_synchronize(goss_hip_system* s)
   });
 }
 
+int goss_hip_system_set_kernel_timing(goss_hip_system* s, int enable)
+{
+  return guarded([&] {
+    check_sys(s);
+    s->use_device();
+    if (enable && !s->t_begin) {
+      hip_check(hipEventCreate(&s->t_begin), "hipEventCreate");
+      hip_check(hipEventCreate(&s->t_end), "hipEventCreate");
+    } else if (!enable && s->t_begin) {
+      hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+      hip_check(hipEventDestroy(s->t_begin), "hipEventDestroy");
+      hip_check(hipEventDestroy(s->t_end), "hipEventDestroy");
+      s->t_begin = s->t_end = nullptr;
+    }
+    s->timed_launches = 0;
+  });
+}
+
+int goss_hip_system_get_kernel_timing(goss_hip_system* s, double* kernel_ms, uint64_t* launches)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (!s->t_begin) throw Error(GOSS_HIP_EINVAL, "kernel timing is not enabled on this system");
+    s->use_device();
+    *launches = s->timed_launches;
+    *kernel_ms = 0.0;
+    if (s->timed_launches == 0) return;
+    hip_check(hipEventSynchronize(s->t_end), "hipEventSynchronize");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, s->t_begin, s->t_end), "hipEventElapsedTime");
+    *kernel_ms = double(ms);
+  });
+}
+
 int goss_hip_system_stream(goss_hip_system* s, void** stream)
 {
   return guarded([&] {
@@ -1057,6 +1346,7 @@ int goss_hip_system_stream(goss_hip_system* s, void** stream)
 
 }  // extern "C"
 
+#if !defined(GOSS_NO_BATCH)
 // ---------------------------------------------------------------------------
 // Mixed-population batch (goss_batch.h).
 // ---------------------------------------------------------------------------
@@ -1490,3 +1780,4 @@ int goss_hip_batch_autotune(goss_hip_batch* b, double t0, double t1, const doubl
 }
 
 }  // extern "C"
+#endif  // !GOSS_NO_BATCH

@@ -4,18 +4,22 @@
 //
 // One node per lane.  Each node owns its solver state (Newton eta, Jacobian
 // reuse flags, ESDIRK step-size history, held in registers during a launch),
-// its two S x S matrices -- the finite-difference Jacobian
-// (ODE::compute_jacobian, ODE.cpp:47-67) and its LU factorisation
-// (ODE::lu_factorize, ODE.cpp:69-100) -- and a few work vectors (stage
-// values, Newton increments).  States, matrices and work vectors live in HBM
-// as structure of arrays: element i of every node in one row of stride ld,
-// so a wavefront's 64 lanes touch 512 contiguous bytes per element.  Only
-// the input and output of one right-hand-side evaluation are held in
-// registers (the FP64 RHS of a cell model needs most of the register file by
-// itself); the norms and vector updates run over the HBM vectors in unrolled
-// loops (all loads of a vector in flight at once, no value kept beyond the
-// loop), the substitutions hold their vector in registers only while they
-// run, and no long-lived vector competes with the RHS for registers.
+// its finite-difference Jacobian (ODE::compute_jacobian, ODE.cpp:47-67) and
+// the LU factors of I - h a J (ODE::lu_factorize, ODE.cpp:69-100), and a few
+// work vectors (stage values, Newton increments).  The Jacobian and the
+// factors are stored over their structure only (the generated model's
+// jac_nnz / lu_nnz entries: 76 and 82 of 361 for TP06): the entries a
+// state's right-hand side does not read are exact zeros of the difference
+// quotient, and the LU fill is fixed by that structure, so the dense
+// routines' results are reproduced with the zero terms left out.  The
+// factorisation and the substitutions are generated straight-line code over
+// a register copy of the factors: all loads of an operand are issued at
+// once and no load waits on the previous row.
+// States, factors and work vectors live in HBM as structure of arrays:
+// element i of every node in one row of stride ld, so a wavefront's 64 lanes
+// touch 512 contiguous bytes per element.  Only the input and output of one
+// right-hand-side evaluation are held in registers across the solver logic
+// (the FP64 RHS of a cell model needs most of the register file by itself).
 // The RHS is instantiated at one site per algorithmic role (Newton residual,
 // first ESDIRK stage, Jacobian columns).
 //
@@ -122,8 +126,8 @@ __host__ __device__ constexpr int implicit_work_vectors(int solver)
 }
 
 struct ImplicitBuffers {
-  double* jac;   // [S*S][ld]
-  double* jfac;  // [S*S][ld] (ESDIRK23a only)
+  double* jac;   // [jac_nnz][ld]: the Jacobian over its structure
+  double* jfac;  // [lu_nnz][ld]: the LU factors of I - h a J over their structure
   double* work;  // [implicit_work_vectors(solver) * S][ld]
   double* ds;    // [kNumDState][ld]
   int* is;       // [kNumIState][ld]
@@ -137,21 +141,14 @@ struct VecView {
   GOSS_HD GOSS_INLINE double& operator[](int i) const { return a[size_t(i) * ld]; }
 };
 
-// View of one node's matrix.
-struct MatView {
-  double* a;
-  unsigned ld;
-  int S;
-  GOSS_HD GOSS_INLINE double& operator()(int i, int j) const { return a[size_t(i * S + j) * ld]; }
-};
-
-// Everything one node works on: its states, work vectors and matrices.
+// Everything one node works on: its states, work vectors, Jacobian J and LU
+// factors F (each a vector over its structure).
 struct NodeViews {
   VecView y;
   double* work;
   unsigned ld;
   int S;
-  MatView J, F;
+  VecView J, F;
   GOSS_HD GOSS_INLINE VecView w(int k) const { return VecView{work + size_t(k) * S * ld, ld}; }
 };
 
@@ -162,9 +159,24 @@ struct Implicit {
   // ODE::compute_jacobian (ODE.cpp:47-67): forward differences, column by
   // column, delta_i = sqrt(1e-15 * max(1e-5, |y_i|)).  Pass i = -1 evaluates
   // f(y) into f1, pass i >= 0 the perturbed RHS of column i.
+  // Stores column i's entries (rows of the structure) of (f - f1) / d.
+  GOSS_HD static GOSS_INLINE void store_column(const VecView& J, const int i, const double* f, const VecView& f1,
+                                               const double d)
+  {
+    const unsigned rows = M::jac_cols[i];
+    int slot = M::jac_col_start[i];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      if (rows & (1u << j)) {
+        J[slot] = (f[j] - f1[j]) / d;
+        ++slot;
+      }
+    }
+  }
+
   template <class A>
   GOSS_HD static void fd_jacobian(const VecView& y, const double t, const A& acc, const VecView& f1,
-                                  const MatView& J)
+                                  const VecView& J)
   {
 #pragma unroll 1
     for (int i = -1; i < S; ++i) {
@@ -172,7 +184,7 @@ struct Implicit {
       if (i >= 0) {
         const double ysafe = y[i];
         const double mx = 1e-5 > fabs(ysafe) ? 1e-5 : fabs(ysafe);
-        delta = sqrt(1e-15 * mx);
+        delta = m_sqrt(1e-15 * mx);
       }
       double yp[S], f[S];
 #pragma unroll
@@ -185,8 +197,7 @@ struct Implicit {
 #pragma unroll
         for (int j = 0; j < S; ++j) f1[j] = f[j];
       } else {
-#pragma unroll
-        for (int j = 0; j < S; ++j) J(j, i) = (f[j] - f1[j]) / delta;
+        store_column(J, i, f, f1, delta);
       }
     }
   }
@@ -197,7 +208,7 @@ struct Implicit {
   // represented in FP32.  The base evaluation, also FP32, is kept in f1.
   template <class A>
   GOSS_HD static void fd_jacobian_f32(const VecView& y, const double t, const A& acc, const VecView& f1,
-                                      const MatView& J)
+                                      const VecView& J)
   {
     constexpr float kSqrtEps = 3.4526698e-4f;  // sqrt(2^-23)
 #pragma unroll 1
@@ -221,15 +232,17 @@ struct Implicit {
 #pragma unroll
         for (int j = 0; j < S; ++j) f1[j] = double(f[j]);
       } else {
+        double fd[S];
 #pragma unroll
-        for (int j = 0; j < S; ++j) J(j, i) = (double(f[j]) - f1[j]) / double(dy);
+        for (int j = 0; j < S; ++j) fd[j] = double(f[j]);
+        store_column(J, i, fd, f1, double(dy));
       }
     }
   }
 
   // The Jacobian at y into J, as the accessor's mode says (f1 is overwritten).
   template <class A>
-  GOSS_HD static void jacobian(const VecView& y, const double t, const A& acc, const VecView& f1, const MatView& J)
+  GOSS_HD static void jacobian(const VecView& y, const double t, const A& acc, const VecView& f1, const VecView& J)
   {
     if constexpr (JacobianModeOf<A>::value == kJacobianFDFP32)
       fd_jacobian_f32(y, t, acc, f1, J);
@@ -237,80 +250,30 @@ struct Implicit {
       fd_jacobian(y, t, acc, f1, J);
   }
 
-  // dst <- scale * src + I, then in-place LU (ODE.cpp:69-100); scale is
-  // -dt*alpha (ImplicitODESolver.cpp:92-105) or -dt*gamma (ESDIRK23a.cpp:101-115).
-  GOSS_HD static void scale_add_identity(const MatView& src, const MatView& dst, const double scale)
+  // F <- LU factors of I + scale * J (ImplicitODESolver.cpp:92-105 with
+  // ODE.cpp:69-100); scale is -dt*alpha or -dt*gamma (ESDIRK23a.cpp:101-115).
+  // J is read into registers, factorised there and stored once.
+  GOSS_HD static void factorize(const VecView& J, const VecView& F, const double scale)
   {
-#pragma unroll 1
-    for (int i = 0; i < S; ++i)
+    double j[M::jac_nnz], a[M::lu_nnz];
 #pragma unroll
-      for (int j = 0; j < S; ++j) dst(i, j) = src(i, j) * scale;
+    for (int k = 0; k < M::jac_nnz; ++k) j[k] = J[k];
+    M::lu_from_jacobian(j, scale, a);
+    M::lu_factorize_sparse(a);
 #pragma unroll
-    for (int i = 0; i < S; ++i) dst(i, i) += 1.0 * 1.0;
+    for (int k = 0; k < M::lu_nnz; ++k) F[k] = a[k];
   }
 
-  GOSS_HD static void lu_factorize(const MatView& a)
+  // ODE::forward_backward_subst (ODE.cpp:102-132) in place on xv, with the
+  // factors and the vector read into registers first.
+  GOSS_HD static void fb_subst(const VecView& F, const VecView& xv)
   {
-#pragma unroll 1
-    for (int k = 1; k < S; k++) {
-#pragma unroll 1
-      for (int i = 0; i <= k - 1; ++i) {
-        double acc = 0.0;
-#pragma unroll 4
-        for (int r = 0; r <= i - 1; r++) acc += a(i, r) * a(r, k);
-        a(i, k) -= acc;
-        acc = 0.0;
-#pragma unroll 4
-        for (int r = 0; r <= i - 1; r++) acc += a(k, r) * a(r, i);
-        a(k, i) = (a(k, i) - acc) / a(i, i);
-      }
-      double acc = 0.0;
-#pragma unroll 4
-      for (int r = 0; r <= k - 1; r++) acc += a(k, r) * a(r, k);
-      a(k, k) -= acc;
-    }
-  }
-
-  // Row i of a as a pointer that the compiler cannot compute before `after`
-  // is known: the substitution then loads one row at a time (its loads all in
-  // flight together) instead of hoisting every factor into registers at once.
-  GOSS_HD static GOSS_INLINE const double* row_after(const MatView& a, const int i, const double after)
-  {
-    const double* r = a.a + size_t(i) * S * a.ld;
-#ifdef __HIP_DEVICE_COMPILE__
-    asm volatile("" : "+v"(r) : "v"(after));
-#else
-    (void)after;
-#endif
-    return r;
-  }
-
-  // ODE::forward_backward_subst (ODE.cpp:102-132), in place on xv.  The
-  // vector is held in registers while the substitution runs (it lives only
-  // here); each row of factors is read from HBM once its predecessor's
-  // unknown is known, and the arithmetic keeps the reference's order.
-  GOSS_HD static void fb_subst(const MatView& a, const VecView& xv)
-  {
-    double x[S];
+    double a[M::lu_nnz], x[S];
+#pragma unroll
+    for (int k = 0; k < M::lu_nnz; ++k) a[k] = F[k];
 #pragma unroll
     for (int i = 0; i < S; ++i) x[i] = xv[i];
-#pragma unroll
-    for (int i = 1; i < S; ++i) {
-      const double* r = row_after(a, i, x[i - 1]);
-      double acc = 0.0;
-#pragma unroll
-      for (int j = 0; j <= i - 1; ++j) acc = acc + r[size_t(j) * a.ld] * x[j];
-      x[i] = x[i] - acc;
-    }
-    x[S - 1] = x[S - 1] / a(S - 1, S - 1);
-#pragma unroll
-    for (int i = S - 2; i >= 0; i--) {
-      const double* r = row_after(a, i, x[i + 1]);
-      double acc = 0;
-#pragma unroll
-      for (int j = i + 1; j < S; ++j) acc = acc + r[size_t(j) * a.ld] * x[j];
-      x[i] = (x[i] - acc) / r[size_t(i) * a.ld];
-    }
+    M::lu_solve_sparse(a, x);
 #pragma unroll
     for (int i = 0; i < S; ++i) xv[i] = x[i];
   }
@@ -320,7 +283,7 @@ struct Implicit {
     double acc = 0;
 #pragma unroll
     for (int i = 0; i < S; ++i) acc += v[i] * v[i];
-    return sqrt(acc);
+    return m_sqrt(acc);
   }
 
   GOSS_HD static double norm(const VecView& v)
@@ -328,7 +291,7 @@ struct Implicit {
     double acc = 0;
 #pragma unroll
     for (int i = 0; i < S; ++i) acc += v[i] * v[i];
-    return sqrt(acc);
+    return m_sqrt(acc);
   }
 };
 
@@ -414,16 +377,15 @@ struct ImplicitEulerNode {
       if (relative_residual < p.relative_tolerance) break;
       if (st.recompute || always) {
         I::jacobian(yzv, t, acc, f1, v.J);
-        I::scale_add_identity(v.J, v.J, -dt * 1.0);
-        I::lu_factorize(v.J);
+        I::factorize(v.J, v.F, -dt * 1.0);
         st.jac_comp += 1;
         st.recompute = 0;
       }
-      I::fb_subst(v.J, bv);  // bv <- dz
+      I::fb_subst(v.F, bv);  // bv <- dz
       if (st.newton_its == 0) {
         initial_residual = residual;
         st.eta = st.eta > kGossEps ? st.eta : kGossEps;
-        st.eta = pow(st.eta, 0.8);
+        st.eta = m_pow(st.eta, 0.8);
       } else {
         relative_previous_residual = residual / previous_residual;
         st.recompute = relative_previous_residual >= p.max_relative_previous_residual;
@@ -433,7 +395,7 @@ struct ImplicitEulerNode {
           st.recompute = 1;
           break;
         }
-        const double scaled = smax(pow(relative_previous_residual, double(p.max_iterations - st.newton_its)),
+        const double scaled = smax(m_pow(relative_previous_residual, double(p.max_iterations - st.newton_its)),
                                    kGossEps);
         if (residual > (p.kappa * p.relative_tolerance * (1 - relative_previous_residual) / scaled))
           st.recompute = 1;
@@ -541,8 +503,7 @@ struct EsdirkNode {
 
   GOSS_HD static void factorized(const EsdirkCoefficients& k, const double dt, const NodeViews& v)
   {
-    I::scale_add_identity(v.J, v.F, -dt * k.a22);
-    I::lu_factorize(v.F);
+    I::factorize(v.J, v.F, -dt * k.a22);
   }
 
   // compute_stage_val (ESDIRK23a.cpp:201-258).  The Newton increment of the
@@ -554,7 +515,7 @@ struct EsdirkNode {
     const VecView prev = v.w(kEsPrev), dz = v.w(kEsDZ);
     st.newton_its = 0;
     bool conv_ok = true;
-    st.es_eta = pow(smax(st.es_eta, etamin), 0.8);
+    st.es_eta = m_pow(smax(st.es_eta, etamin), 0.8);
     double ndz, prev_ndz = 0.0;
 #pragma unroll 1
     do {
@@ -663,7 +624,7 @@ struct EsdirkNode {
   {
     const EsdirkCoefficients k = esdirk_coefficients();
     double local_t = 0;
-    double dt = smax(s_fac * st.prev_dt * pow((tol / st.loc_error), 0.25), p.min_dt);
+    double dt = smax(s_fac * st.prev_dt * m_pow((tol / st.loc_error), 0.25), p.min_dt);
     dt = smin(dt, interval);
     if (st.first_step) {
       st.prev_dt = dt;
@@ -696,7 +657,7 @@ struct EsdirkNode {
       st.new_jacobian = 0;
       st.prev_dt = dt;
       local_t += dt;
-      dt = smax(s_fac * st.prev_dt * pow((tol / st.loc_error), 0.25), p.min_dt);
+      dt = smax(s_fac * st.prev_dt * m_pow((tol / st.loc_error), 0.25), p.min_dt);
     }
     st.first_step = 0;
   }
@@ -709,8 +670,8 @@ GOSS_HD GOSS_INLINE NodeViews node_views(double* y, const ImplicitBuffers& B, in
   v.work = B.work + node;
   v.ld = ld;
   v.S = S;
-  v.J = MatView{B.jac + node, ld, S};
-  v.F = MatView{B.jfac ? B.jfac + node : nullptr, ld, S};
+  v.J = VecView{B.jac + node, ld};
+  v.F = VecView{B.jfac + node, ld};
   return v;
 }
 

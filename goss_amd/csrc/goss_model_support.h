@@ -13,6 +13,10 @@
 #include <cmath>
 #include <type_traits>
 
+#if defined(GOSS_PARITY_MATH)
+#include "goss_parity_math.h"
+#endif
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define GOSS_HD __host__ __device__
@@ -47,15 +51,25 @@ template <typename T> GOSS_HD GOSS_INLINE T m_heaviside(const T x) { return x > 
 
 GOSS_HD GOSS_INLINE double m_div(const double a, const double b) { return a / b; }
 GOSS_HD GOSS_INLINE double m_rcp(const double b) { return 1.0 / b; }
+#if defined(GOSS_PARITY_MATH)
+// Parity build (tests only): exp, log, pow and sqrt that round identically on
+// the host and the GPU (goss_parity_math.h).
+GOSS_HD GOSS_INLINE double m_exp(const double x) { return goss_pm::exp(x); }
+GOSS_HD GOSS_INLINE double m_exp_l2(const double x2, const double x) { (void)x2; return goss_pm::exp(x); }
+GOSS_HD GOSS_INLINE double m_log(const double x) { return goss_pm::log(x); }
+GOSS_HD GOSS_INLINE double m_sqrt(const double x) { return goss_pm::sqrt(x); }
+GOSS_HD GOSS_INLINE double m_pow(const double x, const double y) { return goss_pm::pow(x, y); }
+#else
 GOSS_HD GOSS_INLINE double m_exp(const double x) { return std::exp(x); }
 // m_exp_l2(x * log2(e), x): the FP32 device kernels use the first argument (see below); double uses x.
 GOSS_HD GOSS_INLINE double m_exp_l2(const double x2, const double x) { (void)x2; return std::exp(x); }
 GOSS_HD GOSS_INLINE double m_log(const double x) { return std::log(x); }
 GOSS_HD GOSS_INLINE double m_sqrt(const double x) { return std::sqrt(x); }
+GOSS_HD GOSS_INLINE double m_pow(const double x, const double y) { return std::pow(x, y); }
+#endif
 GOSS_HD GOSS_INLINE double m_fabs(const double x) { return std::fabs(x); }
 GOSS_HD GOSS_INLINE double m_floor(const double x) { return std::floor(x); }
 GOSS_HD GOSS_INLINE double m_ceil(const double x) { return std::ceil(x); }
-GOSS_HD GOSS_INLINE double m_pow(const double x, const double y) { return std::pow(x, y); }
 GOSS_HD GOSS_INLINE double m_fmax(const double x, const double y) { return std::fmax(x, y); }
 GOSS_HD GOSS_INLINE double m_fmin(const double x, const double y) { return std::fmin(x, y); }
 GOSS_HD GOSS_INLINE double m_sin(const double x) { return std::sin(x); }

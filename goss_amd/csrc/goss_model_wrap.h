@@ -43,6 +43,25 @@
     {                                                                                                                \
       NSN::compute_jacobian<R>(y, t, a, jac);                                                                        \
     }                                                                                                                \
+    static constexpr int num_intermediates = NSN::num_intermediates;                                                 \
+    template <typename R, typename A, typename T>                                                                    \
+    GOSS_HD GOSS_INLINE static void eval_intermediates(const R* y, const T t, const A& a, R* out)                    \
+    {                                                                                                                \
+      NSN::eval_intermediates<R>(y, t, a, out);                                                                      \
+    }                                                                                                                \
+    static constexpr int jac_nnz = NSN::jac_nnz;                                                                     \
+    static constexpr int lu_nnz = NSN::lu_nnz;                                                                       \
+    static constexpr const unsigned* jac_cols = NSN::jac_cols;                                                       \
+    static constexpr const int* jac_col_start = NSN::jac_col_start;                                                  \
+    GOSS_HD GOSS_INLINE static void lu_from_jacobian(const double* J, const double scale, double* a)                 \
+    {                                                                                                                \
+      NSN::lu_from_jacobian<double>(J, scale, a);                                                                    \
+    }                                                                                                                \
+    GOSS_HD GOSS_INLINE static void lu_factorize_sparse(double* a) { NSN::lu_factorize_sparse<double>(a); }         \
+    GOSS_HD GOSS_INLINE static void lu_solve_sparse(const double* a, double* x)                                     \
+    {                                                                                                                \
+      NSN::lu_solve_sparse<double>(a, x);                                                                            \
+    }                                                                                                                \
     static constexpr int grl1_vjp_tape_slots = NSN::grl1_vjp_tape_slots;                                             \
     static constexpr int rl1_vjp_tape_slots = NSN::rl1_vjp_tape_slots;                                               \
     static constexpr int eval_vjp_tape_slots = NSN::eval_vjp_tape_slots;                                             \

@@ -86,6 +86,33 @@ struct AdjointArgs {
 
 typedef hipError_t (*AdjointLaunch)(const AdjointArgs&);
 
+// Model evaluation at n points, one per lane (ODE.h:55-77): kind 0 eval
+// (out0 = values [n][S]), 1 linearized_eval (out0 = linear, out1 = rhs, both
+// [n][S]), 2 the forward-difference Jacobian of ODE::compute_jacobian
+// (ODE.cpp:47-67; out0 = [n][S*S] row-major), 3 eval_monitored (out0 =
+// [n][num_monitored], the intermediates listed in `monitored`).  Array
+// pointers are device pointers; params holds the P model parameters (host),
+// field parameters included.
+constexpr int kMaxMonitored = 64;
+struct MonitoredList {
+  int n;
+  int index[kMaxMonitored];  // into the variant's intermediates
+};
+struct ModelEvalArgs {
+  int kind;
+  int only_linear;
+  MonitoredList monitored;
+  const double* states;  // [n][S]
+  const double* times;   // [n]
+  unsigned long n;
+  const double* params;  // host, P values
+  double* out0;
+  double* out1;
+  hipStream_t stream;
+};
+
+typedef hipError_t (*ModelEvalLaunch)(const ModelEvalArgs&);
+
 // Solvers with a device adjoint (every explicit fixed-step solver), in the order of the adjoint tables.
 constexpr int kAdjointSolvers[] = {kGRL1, kRL1, kExplicitEuler, kRK4, kRK2, kRL2, kGRL2};
 constexpr int kNumAdjointSolvers = 7;
@@ -99,6 +126,7 @@ inline int adjoint_index(const int solver)
 struct VariantInfo {
   const char* name;
   int S, P, NFS, NFP, NCOMP;
+  int jac_nnz, lu_nnz;  // structure of the Jacobian and of its LU factors (implicit solvers)
   int vjp_tape_slots[kNumAdjointSolvers];  // slots of the FP64 adjoint's per-node tape
   int vjp_work_rows[kNumAdjointSolvers];   // per-node work rows of the step adjoint (precision R)
   const int* fsi;
@@ -115,9 +143,13 @@ struct VariantInfo {
   AdaptiveLaunch rkf32_launch[kNumPrecisions];  // FP64 and FP32
   AdjointLaunch adjoint_record[kNumAdjointSolvers][kNumPrecisions];  // FP64 and FP32
   AdjointLaunch adjoint_backward[kNumAdjointSolvers][kNumPrecisions];
+  ModelEvalLaunch model_eval;  // FP64
+  int num_intermediates;       // named intermediates (the values eval_monitored can select)
+  const char* const* intermediate_names;
 };
 
-// All variants, in registration order.
-VariantInfo* variants(int* count);
+// The variants, in registration order (compiled in, then loaded at run time).
+int variant_count();
+VariantInfo* variant_at(int index);
 
 }  // namespace goss_hip

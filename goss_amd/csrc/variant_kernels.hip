@@ -5,6 +5,7 @@
 //                          or -DGOSS_PREC_F32 (FP32 and FP32C).
 // Exports goss_register_<name>_<family>(VariantInfo&), called by the registry.
 #include "goss_model_wrap.h"
+#include "goss_layout.h"
 #include "goss_registry.h"
 
 #define GOSS_STR2(x) #x
@@ -106,6 +107,95 @@ hipError_t launch_adjoint_backward(const AdjointArgs& a)
   return hipSuccess;
 }
 
+#if defined(GOSS_PREC_F64)
+// ParameterizedODE::eval / linearized_eval and ODE::compute_jacobian
+// (ODE.h:55-77, ODE.cpp:47-67) at n points, one per lane, node-major in and out.
+struct FieldValues {
+  double v[kMaxFieldParams > 0 ? kMaxFieldParams : 1];
+};
+
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) model_eval_kernel(const double* __restrict__ states,
+                                                            const double* __restrict__ times, const unsigned long n,
+                                                            const SharedParamsOf<M, double> sp, const FieldValues fv,
+                                                            const int only_linear, const MonitoredList mon,
+                                                            double* __restrict__ out0, double* __restrict__ out1)
+{
+  constexpr int S = M::num_states;
+  const unsigned long p = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double fpv[M::num_field_parameters > 0 ? M::num_field_parameters : 1];
+#pragma unroll
+  for (int j = 0; j < M::num_field_parameters; ++j) fpv[j] = fv.v[j];
+  const DevAcc<M, double> acc{sp, fpv};
+  double y[S], f[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) y[i] = states[p * S + i];
+  const double t = times[p];
+  if constexpr (KIND == 0) {
+    M::template eval<double>(y, t, acc, f);
+#pragma unroll
+    for (int i = 0; i < S; ++i) out0[p * S + i] = f[i];
+  } else if constexpr (KIND == 1) {
+    double lin[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) lin[i] = 0.0;
+    M::template linearized_eval<double>(y, t, acc, lin, f, only_linear != 0);
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      out0[p * S + i] = lin[i];
+      out1[p * S + i] = f[i];
+    }
+  } else if constexpr (KIND == 3) {
+    // ParameterizedODE::eval_monitored (ParameterizedODE.h:90): the selected intermediates.
+    double all[M::num_intermediates > 0 ? M::num_intermediates : 1];
+    M::template eval_intermediates<double>(y, t, acc, all);
+    for (int k = 0; k < mon.n; ++k) out0[p * mon.n + k] = all[mon.index[k]];
+  } else {
+    // ODE::compute_jacobian (ODE.cpp:47-67): forward differences column by column.
+    double f2[S];
+    M::template eval<double>(y, t, acc, f);
+#pragma unroll 1
+    for (int i = 0; i < S; ++i) {
+      double yp[S];
+#pragma unroll
+      for (int k = 0; k < S; ++k) yp[k] = y[k];
+      const double ysafe = y[i];
+      const double mx = 1e-5 > fabs(ysafe) ? 1e-5 : fabs(ysafe);
+      const double delta = m_sqrt(1e-15 * mx);
+#pragma unroll
+      for (int k = 0; k < S; ++k) yp[k] = (k == i) ? ysafe + delta : yp[k];
+      M::template eval<double>(yp, t, acc, f2);
+#pragma unroll
+      for (int j = 0; j < S; ++j) out0[p * S * S + j * S + i] = (f2[j] - f[j]) / delta;
+    }
+  }
+}
+
+hipError_t launch_model_eval(const ModelEvalArgs& a)
+{
+  const SharedParamsOf<M, double> sp = make_shared_params<M, double>(a.params);
+  FieldValues fv{};
+  for (int j = 0; j < M::num_field_parameters; ++j) fv.v[j] = a.params[M::field_parameter_indices[j]];
+  const dim3 grid(unsigned((a.n + kBlock - 1) / kBlock));
+#define GOSS_MODEL_EVAL(K)                                                                                    \
+  hipLaunchKernelGGL((model_eval_kernel<K>), grid, dim3(kBlock), 0, a.stream, a.states, a.times, a.n, sp, fv, \
+                     a.only_linear, a.monitored, a.out0, a.out1)
+  if (a.kind == 0)
+    GOSS_MODEL_EVAL(0);
+  else if (a.kind == 1)
+    GOSS_MODEL_EVAL(1);
+  else if (a.kind == 2)
+    GOSS_MODEL_EVAL(2);
+  else if (a.kind == 3)
+    GOSS_MODEL_EVAL(3);
+  else
+    return hipErrorInvalidValue;
+#undef GOSS_MODEL_EVAL
+  return hipGetLastError();
+}
+#endif
+
 template <class R, bool COMP>
 void register_family(VariantInfo& v, int prec)
 {
@@ -145,6 +235,10 @@ void fill_info(VariantInfo& v)
   v.NFS = NS::num_field_states;
   v.NFP = NS::num_field_parameters;
   v.NCOMP = num_comp<M>();
+  v.jac_nnz = M::jac_nnz;
+  v.lu_nnz = M::lu_nnz;
+  v.num_intermediates = NS::num_intermediates;
+  v.intermediate_names = NS::intermediate_names;
 #define GOSS_ADJOINT_ROWS(SOLVER)                                                   \
   v.vjp_tape_slots[adjoint_index(SOLVER)] = step_vjp_tape_slots<M, SOLVER>();     \
   v.vjp_work_rows[adjoint_index(SOLVER)] = step_vjp_work_rows<M, SOLVER>();
@@ -169,7 +263,17 @@ void fill_info(VariantInfo& v)
 
 }  // namespace
 
-#if defined(GOSS_PREC_F64)
+#if defined(GOSS_PREC_F64) && defined(GOSS_PARITY_MATH)
+// The parity build (tests only): the implicit solvers with the reference's FD
+// Jacobian and RKF32, with host-identical FP64 elementary functions.
+void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
+{
+  fill_info(v);
+  v.implicit_launch[0][kJacobianFD] = &launch_implicit<kImplicitEuler, kJacobianFD>;
+  v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
+  v.rkf32_launch[kFP64] = &launch_rkf32<double>;
+}
+#elif defined(GOSS_PREC_F64)
 void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
 {
   fill_info(v);
@@ -180,7 +284,18 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   v.implicit_launch[1][kJacobianFDFP32] = &launch_implicit<kESDIRK23a, kJacobianFDFP32>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
   register_adjoint<double>(v, kFP64);
+  v.model_eval = &launch_model_eval;
 }
+#if defined(GOSS_PLUGIN)
+// A model compiled at run time into its own library: the main library loads it
+// (goss_hip_model_load) and calls this to fill the variant's FP64 entries.
+extern "C" int goss_hip_plugin_register_f64(void* info, size_t info_size)
+{
+  if (info_size != sizeof(VariantInfo)) return -1;
+  GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(*static_cast<VariantInfo*>(info));
+  return 0;
+}
+#endif
 #elif defined(GOSS_PREC_F32)
 void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
 {
@@ -190,6 +305,14 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(VariantInfo& v)
   v.rkf32_launch[kFP32] = &launch_rkf32<float>;
   register_adjoint<float>(v, kFP32);
 }
+#if defined(GOSS_PLUGIN)
+extern "C" int goss_hip_plugin_register_f32(void* info, size_t info_size)
+{
+  if (info_size != sizeof(VariantInfo)) return -1;
+  GOSS_CAT3(goss_register_, GOSS_VARIANT, _f32)(*static_cast<VariantInfo*>(info));
+  return 0;
+}
+#endif
 #else
 #error "define GOSS_PREC_F64 or GOSS_PREC_F32"
 #endif

@@ -6,7 +6,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from ._lib import JACOBIANS, PRECISIONS, SOLVERS, GossHipError, ModelInfo, SolverParams, check, lib
+from ._lib import JACOBIANS, PRECISIONS, SOLVERS, GossHipError, ModelInfo, SolverParams, check, lib, load
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -27,9 +27,10 @@ def model_names() -> List[str]:
 class ModelVariant:
     """Metadata of a compiled model variant (sizes, names, defaults)."""
 
-    def __init__(self, name: str):
+    def __init__(self, name: str, library: Optional[str] = None):
+        L = load(library) if library else lib()
         info = ModelInfo()
-        check(lib().goss_hip_model_info_get(name.encode(), ctypes.byref(info)))
+        check(L.goss_hip_model_info_get(name.encode(), ctypes.byref(info)), L)
         self.name = name
         self.num_states = info.num_states
         self.num_parameters = info.num_parameters
@@ -47,9 +48,10 @@ class ModelVariant:
         self.linear_terms = [int(info.linear_terms[i]) for i in range(self.num_states)]
 
 
-def solver_params(solver: str, **overrides) -> SolverParams:
+def solver_params(solver: str, library: Optional[str] = None, **overrides) -> SolverParams:
+    L = load(library) if library else lib()
     sp = SolverParams()
-    check(lib().goss_hip_solver_params_default(SOLVERS[solver], ctypes.byref(sp)))
+    check(L.goss_hip_solver_params_default(SOLVERS[solver], ctypes.byref(sp)), L)
     for k, v in overrides.items():
         if v is None:
             continue
@@ -65,19 +67,21 @@ class DeviceSystem:
     """One ODESystemSolver on the GPU (wraps goss_hip_system_*)."""
 
     def __init__(self, variant: str, solver: str, num_nodes: int, precision: str = "fp64",
-                 params: Optional[Dict[str, float]] = None):
-        self.model = ModelVariant(variant)
+                 params: Optional[Dict[str, float]] = None, library: Optional[str] = None):
+        """`library`: path of another build of the C ABI (tests); default the product library."""
+        self._L = load(library) if library else lib()
+        self.model = ModelVariant(variant, library)
         self.solver = solver
         self.precision = precision
         self.num_nodes = int(num_nodes)
-        self._sp = solver_params(solver, **(params or {}))
+        self._sp = solver_params(solver, library, **(params or {}))
         self._h = ctypes.c_void_p()
-        check(lib().goss_hip_system_create(variant.encode(), SOLVERS[solver], ctypes.byref(self._sp),
-                                           PRECISIONS[precision], self.num_nodes, ctypes.byref(self._h)))
+        check(self._L.goss_hip_system_create(variant.encode(), SOLVERS[solver], ctypes.byref(self._sp),
+                                             PRECISIONS[precision], self.num_nodes, ctypes.byref(self._h)), self._L)
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
-            lib().goss_hip_system_destroy(self._h)
+            self._L.goss_hip_system_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):
@@ -88,117 +92,127 @@ class DeviceSystem:
 
     # -- stepping -----------------------------------------------------------
     def forward(self, t: float, interval: float) -> None:
-        check(lib().goss_hip_system_forward(self._h, float(t), float(interval)))
+        check(self._L.goss_hip_system_forward(self._h, float(t), float(interval)), self._L)
 
     def forward_times(self, t) -> None:
         t = _f64(t)
-        check(lib().goss_hip_system_forward_times(self._h, _ptr(t), len(t)))
+        check(self._L.goss_hip_system_forward_times(self._h, _ptr(t), len(t)), self._L)
 
     def forward_times_noalloc(self, t) -> None:
         """forward_times with the steps staged in the library's fixed device
         array (no allocation; at most 32,768 time points per call)."""
         t = _f64(t)
-        check(lib().goss_hip_system_forward_times_noalloc(self._h, _ptr(t), len(t)))
+        check(self._L.goss_hip_system_forward_times_noalloc(self._h, _ptr(t), len(t)), self._L)
 
     def solve(self, t, out: np.ndarray, tangled: bool = True) -> None:
         t = _f64(t)
         if not (out.flags.c_contiguous and out.dtype == np.float64):
             raise ValueError("out must be a C-contiguous float64 array")
-        check(lib().goss_hip_system_solve(self._h, _ptr(out), _ptr(t), len(t), int(tangled)))
+        check(self._L.goss_hip_system_solve(self._h, _ptr(out), _ptr(t), len(t), int(tangled)), self._L)
 
     def synchronize(self) -> None:
-        check(lib().goss_hip_system_synchronize(self._h))
+        check(self._L.goss_hip_system_synchronize(self._h), self._L)
 
     def set_steps_per_launch(self, n: int) -> None:
-        check(lib().goss_hip_system_set_steps_per_launch(self._h, int(n)))
+        check(self._L.goss_hip_system_set_steps_per_launch(self._h, int(n)), self._L)
 
     # -- data ----------------------------------------------------------------
     def states(self) -> np.ndarray:
         out = np.zeros((self.num_nodes, self.model.num_states))
-        check(lib().goss_hip_system_get_states(self._h, _ptr(out)))
+        check(self._L.goss_hip_system_get_states(self._h, _ptr(out)), self._L)
         return out
 
     def set_states(self, states) -> None:
         a = _f64(states).reshape(self.num_nodes, self.model.num_states)
-        check(lib().goss_hip_system_set_states(self._h, _ptr(a)))
+        check(self._L.goss_hip_system_set_states(self._h, _ptr(a)), self._L)
 
     def get_field_states(self, out: np.ndarray, tangled: bool = True) -> None:
-        check(lib().goss_hip_system_get_field_states(self._h, out.ctypes.data, int(tangled)))
+        check(self._L.goss_hip_system_get_field_states(self._h, out.ctypes.data, int(tangled)), self._L)
 
     def set_field_states(self, a: np.ndarray, tangled: bool = True) -> None:
         a = _f64(a)
-        check(lib().goss_hip_system_set_field_states(self._h, a.ctypes.data, int(tangled)))
+        check(self._L.goss_hip_system_set_field_states(self._h, a.ctypes.data, int(tangled)), self._L)
 
     def get_field_state_components(self, out: np.ndarray, components: Sequence[int], tangled: bool = True) -> None:
         c = np.ascontiguousarray(components, dtype=np.uint32)
-        check(lib().goss_hip_system_get_field_state_components(
-            self._h, _ptr(out), len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(tangled)))
+        check(self._L.goss_hip_system_get_field_state_components(
+            self._h, _ptr(out), len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(tangled)), self._L)
 
     def set_field_state_components(self, a: np.ndarray, components: Sequence[int], tangled: bool = True) -> None:
         a = _f64(a)
         c = np.ascontiguousarray(components, dtype=np.uint32)
-        check(lib().goss_hip_system_set_field_state_components(
-            self._h, _ptr(a), len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(tangled)))
+        check(self._L.goss_hip_system_set_field_state_components(
+            self._h, _ptr(a), len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(tangled)), self._L)
 
     def get_field_parameters(self, out: np.ndarray, tangled: bool = True) -> None:
-        check(lib().goss_hip_system_get_field_parameters(self._h, out.ctypes.data, int(tangled)))
+        check(self._L.goss_hip_system_get_field_parameters(self._h, out.ctypes.data, int(tangled)), self._L)
 
     def set_field_parameters(self, a: np.ndarray, tangled: bool = True) -> None:
         a = _f64(a)
-        check(lib().goss_hip_system_set_field_parameters(self._h, a.ctypes.data, int(tangled)))
+        check(self._L.goss_hip_system_set_field_parameters(self._h, a.ctypes.data, int(tangled)), self._L)
 
     def get_field_states_device(self, dev_ptr: int, tangled: bool = True) -> None:
-        check(lib().goss_hip_system_get_field_states_device(self._h, ctypes.c_void_p(dev_ptr), int(tangled)))
+        check(self._L.goss_hip_system_get_field_states_device(self._h, ctypes.c_void_p(dev_ptr), int(tangled)), self._L)
 
     def set_field_states_device(self, dev_ptr: int, tangled: bool = True) -> None:
-        check(lib().goss_hip_system_set_field_states_device(self._h, ctypes.c_void_p(dev_ptr), int(tangled)))
+        check(self._L.goss_hip_system_set_field_states_device(self._h, ctypes.c_void_p(dev_ptr), int(tangled)), self._L)
 
     def reset_default(self) -> None:
-        check(lib().goss_hip_system_reset_default(self._h))
+        check(self._L.goss_hip_system_reset_default(self._h), self._L)
 
     def set_parameter(self, name: str, value: float) -> None:
-        check(lib().goss_hip_system_set_parameter(self._h, name.encode(), float(value)))
+        check(self._L.goss_hip_system_set_parameter(self._h, name.encode(), float(value)), self._L)
 
     def get_parameter(self, name: str) -> float:
         v = ctypes.c_double()
-        check(lib().goss_hip_system_get_parameter(self._h, name.encode(), ctypes.byref(v)))
+        check(self._L.goss_hip_system_get_parameter(self._h, name.encode(), ctypes.byref(v)), self._L)
         return v.value
 
     def set_num_threads(self, n: int) -> None:
-        check(lib().goss_hip_system_set_num_threads(self._h, int(n)))
+        check(self._L.goss_hip_system_set_num_threads(self._h, int(n)), self._L)
 
     def get_num_threads(self) -> int:
         v = ctypes.c_uint32()
-        check(lib().goss_hip_system_get_num_threads(self._h, ctypes.byref(v)))
+        check(self._L.goss_hip_system_get_num_threads(self._h, ctypes.byref(v)), self._L)
         return v.value
 
     def solver_counters(self) -> np.ndarray:
         out = (ctypes.c_int64 * 4)()
-        check(lib().goss_hip_system_get_solver_counters(self._h, out))
+        check(self._L.goss_hip_system_get_solver_counters(self._h, out), self._L)
         return np.array(list(out))
 
     # -- adjoint of GRL1 stepping (device buffers passed as integer addresses) --
     def adjoint_buffer_sizes(self, n: int, sample_every: int, checkpoint_every: int):
         """(checkpoint bytes, segment bytes, number of samples) for n time points."""
         c, g, m = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        check(lib().goss_hip_adjoint_buffer_sizes(self._h, int(n), int(sample_every), int(checkpoint_every),
-                                                  ctypes.byref(c), ctypes.byref(g), ctypes.byref(m)))
+        check(self._L.goss_hip_adjoint_buffer_sizes(self._h, int(n), int(sample_every), int(checkpoint_every),
+                                                  ctypes.byref(c), ctypes.byref(g), ctypes.byref(m)), self._L)
         return c.value, g.value, m.value
 
     def adjoint_record(self, t, sample_every: int, checkpoint_every: int, ckpt_ptr: int, samples_ptr: int) -> None:
         t = _f64(t)
-        check(lib().goss_hip_adjoint_record(self._h, _ptr(t), len(t), int(sample_every), int(checkpoint_every),
-                                            ctypes.c_void_p(ckpt_ptr), ctypes.c_void_p(samples_ptr)))
+        check(self._L.goss_hip_adjoint_record(self._h, _ptr(t), len(t), int(sample_every), int(checkpoint_every),
+                                            ctypes.c_void_p(ckpt_ptr), ctypes.c_void_p(samples_ptr)), self._L)
 
     def adjoint_backward(self, t, sample_every: int, checkpoint_every: int, ckpt_ptr: int, grad_samples_ptr: int,
                          segment_ptr: int, grad_states_ptr: int, grad_fp_ptr: int) -> None:
         t = _f64(t)
-        check(lib().goss_hip_adjoint_backward(self._h, _ptr(t), len(t), int(sample_every), int(checkpoint_every),
+        check(self._L.goss_hip_adjoint_backward(self._h, _ptr(t), len(t), int(sample_every), int(checkpoint_every),
                                               ctypes.c_void_p(ckpt_ptr), ctypes.c_void_p(grad_samples_ptr),
                                               ctypes.c_void_p(segment_ptr), ctypes.c_void_p(grad_states_ptr),
-                                              ctypes.c_void_p(grad_fp_ptr)))
+                                              ctypes.c_void_p(grad_fp_ptr)), self._L)
+
+    def set_kernel_timing(self, enable: bool = True) -> None:
+        """Bracket the solver kernels of each stepping call with HIP events."""
+        check(self._L.goss_hip_system_set_kernel_timing(self._h, int(enable)), self._L)
+
+    def kernel_timing(self):
+        """(milliseconds, launches) of the solver kernels of the last stepping call."""
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        check(self._L.goss_hip_system_get_kernel_timing(self._h, ctypes.byref(ms), ctypes.byref(n)), self._L)
+        return ms.value, n.value
 
     def stream(self) -> int:
         v = ctypes.c_void_p()
-        check(lib().goss_hip_system_stream(self._h, ctypes.byref(v)))
+        check(self._L.goss_hip_system_stream(self._h, ctypes.byref(v)), self._L)
         return v.value or 0

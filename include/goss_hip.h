@@ -117,6 +117,38 @@ int goss_hip_device_count(int* count);
 int goss_hip_model_count(void);
 const char* goss_hip_model_name(int index);
 int goss_hip_model_info_get(const char* variant, goss_hip_model_info* out);
+/* Adds the model variant compiled into library_path (a model library built
+ * at run time from a .ode file by goss_amd.jit; the counterpart of the
+ * reference's JIT module, python/goss/compilemodule.py:19-58, whose plugin
+ * pointer Python hands to _gosscpp.make_ode, wrapper.cpp:259-266).  The
+ * library stays loaded until the process ends. */
+int goss_hip_model_load(const char* library_path, const char* variant);
+
+/* Model methods at n points, evaluated on the device (host arrays in and
+ * out; params: the P model parameter values, NULL for the defaults):
+ * ODE::eval (ODE.h:55), ODE::linearized_eval (ODE.h:61; with only_linear the
+ * entries of the non-linear states keep the values passed in `linear`) and
+ * ODE::compute_jacobian, the forward-difference Jacobian (ODE.cpp:47-67,
+ * jac[p][i*S + j] = d values[i] / d states[j]).  states [n][S], times [n]. */
+int goss_hip_model_eval(const char* variant, const double* params, const double* states, const double* times,
+                        uint64_t n, double* values);
+int goss_hip_model_linearized_eval(const char* variant, const double* params, const double* states,
+                                   const double* times, uint64_t n, int only_linear, double* linear, double* rhs);
+int goss_hip_model_compute_jacobian(const char* variant, const double* params, const double* states,
+                                    const double* times, uint64_t n, double* jac);
+/* The model's named intermediates (the values eval_monitored can select),
+ * and ParameterizedODE::eval_monitored (ParameterizedODE.h:90-105) of the
+ * intermediates listed in `monitored` (indices into those names; at most 64)
+ * at n points: values [n][num_monitored]. */
+int goss_hip_model_intermediates(const char* variant, int* count, const char* const** names);
+int goss_hip_model_eval_monitored(const char* variant, const double* params, int num_monitored,
+                                  const int* monitored, const double* states, const double* times, uint64_t n,
+                                  double* values);
+/* ODE::lu_factorize (ODE.cpp:69-100) and ODE::forward_backward_subst
+ * (ODE.cpp:102-132) of n row-major num_states x num_states matrices, in place
+ * / into x [n][num_states], on the device. */
+int goss_hip_lu_factorize(int num_states, uint64_t n, double* mats);
+int goss_hip_forward_backward_subst(int num_states, uint64_t n, const double* mats, const double* b, double* x);
 
 /* Default tunables of a solver (GRL1.h:39, ImplicitODESolver.h:36-41, ...). */
 int goss_hip_solver_params_default(int solver, goss_hip_solver_params* out);
@@ -265,6 +297,15 @@ int goss_hip_batch_autotune(goss_hip_batch* batch, double t0, double t1, const d
                             goss_hip_batch_config* best);
 /* Wait for all work of the system. */
 int goss_hip_system_synchronize(goss_hip_system* sys);
+/* Kernel timing, no counterpart in the reference (benchmarks and profiling):
+ * when enabled, the solver kernels of each stepping call (forward,
+ * forward_times, forward_times_noalloc, solve) are bracketed by HIP events on
+ * the system's stream, recorded right before the first and right after the
+ * last launch (solve: of its last chunk of rows).  get_kernel_timing returns
+ * the elapsed milliseconds between them and the number of launches of the
+ * last stepping call (it waits for that call's kernels). */
+int goss_hip_system_set_kernel_timing(goss_hip_system* sys, int enable);
+int goss_hip_system_get_kernel_timing(goss_hip_system* sys, double* kernel_ms, uint64_t* launches);
 /* The system's HIP stream (hipStream_t) for callers that order their own work. */
 int goss_hip_system_stream(goss_hip_system* sys, void** stream);
 

@@ -6,8 +6,10 @@
 // difference of the device exp/log/pow.  The step adjoints of
 // goss_adjoint.h (StepVjp, the composition the reverse-sweep kernel runs) are
 // built here too, in FP64 (tape form, as on the device) and FP32.
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "goss_adaptive.h"
@@ -26,33 +28,27 @@ using namespace goss_hip;
 
 namespace {
 
-template <class NSW>
-struct HostModel;
+// Runs body(first, last, slot) over node ranges on up to 16 host threads
+// (nodes are independent); slot indexes per-thread accumulators.
+template <class F>
+void parallel_nodes(unsigned N, F&& body)
+{
+  const unsigned T = std::max(1u, std::min({16u, std::thread::hardware_concurrency(), N}));
+  std::vector<std::thread> th;
+  for (unsigned k = 0; k < T; ++k) {
+    const unsigned a = unsigned(size_t(N) * k / T), b = unsigned(size_t(N) * (k + 1) / T);
+    th.emplace_back([=, &body] { body(a, b, k); });
+  }
+  for (auto& t : th) t.join();
+}
 
-#define GOSS_HARNESS_MODEL(V)                                                                           \
-  struct M_##V {                                                                                        \
-    static constexpr int num_states = goss_models::V::num_states;                                       \
-    static constexpr int num_parameters = goss_models::V::num_parameters;                               \
-    static constexpr int num_constants = goss_models::V::num_constants;                                 \
-    static constexpr int num_field_states = goss_models::V::num_field_states;                           \
-    static constexpr int num_field_parameters = goss_models::V::num_field_parameters;                   \
-    static constexpr const int* field_state_indices = goss_models::V::field_state_indices;              \
-    static constexpr const int* field_parameter_indices = goss_models::V::field_parameter_indices;      \
-    static constexpr const unsigned char* linear_terms = goss_models::V::linear_terms;                  \
-    template <typename R, typename A, typename T>                                                       \
-    static void eval(const R* y, const T t, const A& a, R* out) { goss_models::V::eval<R>(y, t, a, out); } \
-    struct HostAcc {                                                                                    \
-      const double* v;                                                                                  \
-      template <int k> double getd() const { return v[k]; }                                             \
-    };                                                                                                  \
-    static void constants(const double* p, double* c) { goss_models::V::compute_constants<double>(HostAcc{p}, c); } \
-  };
-GOSS_HARNESS_MODEL(tp06_m)
-GOSS_HARNESS_MODEL(tp06_epi)
-GOSS_HARNESS_MODEL(beeler_reuter_1977)
-GOSS_HARNESS_MODEL(fhn)
-GOSS_HARNESS_MODEL(lorentz)
-GOSS_HARNESS_MODEL(fitzhughnagumo)
+// The generated namespaces wrapped as kernel model types (as in variant_kernels.hip).
+GOSS_MODEL_TYPE(M_tp06_m, goss_models::tp06_m);
+GOSS_MODEL_TYPE(M_tp06_epi, goss_models::tp06_epi);
+GOSS_MODEL_TYPE(M_beeler_reuter_1977, goss_models::beeler_reuter_1977);
+GOSS_MODEL_TYPE(M_fhn, goss_models::fhn);
+GOSS_MODEL_TYPE(M_lorentz, goss_models::lorentz);
+GOSS_MODEL_TYPE(M_fitzhughnagumo, goss_models::fitzhughnagumo);
 
 // RKF32 per node with the device logic (goss_adaptive.h); counters are
 // nfevals, ndtsa, ndtsr summed over nodes.
@@ -61,57 +57,66 @@ int run_rkf32(unsigned N, const double* y0, const double* t, unsigned long n, co
               double rtol, double* out_states, long* counters)
 {
   constexpr int S = M::num_states;
-  SharedParamsOf<M, double> sp;
-  double c[M::num_constants > 0 ? M::num_constants : 1];
-  M::constants(prm, c);
-  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = prm[k];
-  for (int k = 0; k < M::num_constants; ++k) sp.c[k] = c[k];
+  const SharedParamsOf<M, double> sp = make_shared_params<M, double>(prm);
   const double fpv[1] = {0.0};
-  const DevAcc<M, double> acc{sp, fpv};
+  const DevAcc<M, double> dacc{sp, fpv};
   const AdaptiveParams ap{atol, rtol};
   for (int k = 0; k < 4; ++k) counters[k] = 0;
-  int failed_any = 0;
-  for (unsigned node = 0; node < N; ++node) {
+  std::vector<long> acc(16 * 4, 0);
+  std::vector<int> failed(16, 0);
+  parallel_nodes(N, [&](unsigned first, unsigned last, unsigned slot) {
+  for (unsigned node = first; node < last; ++node) {
     double ys[S], w[3 * S];
     for (int i = 0; i < S; ++i) ys[i] = y0[size_t(node) * S + i];
     const RVec<double> yv{ys, 1}, k1{w, 1}, yn{w + S, 1}, ep{w + 2 * S, 1};
     double ldt = 0.1, facmax = 2.0;
     int nfevals = 0, ndtsa = 0, ndtsr = 0;
     for (unsigned long k = 0; k + 1 < n; ++k) {
-      if (!Rkf32Node<M, double>::forward(yv, k1, yn, ep, t[k], t[k + 1] - t[k], acc, ap, ldt, facmax, nfevals, ndtsa,
-                                         ndtsr)) {
-        failed_any = 1;
+      if (!Rkf32Node<M, double>::forward(yv, k1, yn, ep, t[k], t[k + 1] - t[k], dacc, ap, ldt, facmax, nfevals,
+                                         ndtsa, ndtsr)) {
+        failed[slot] = 1;
         break;
       }
     }
     for (int i = 0; i < S; ++i) out_states[size_t(node) * S + i] = ys[i];
-    counters[1] += nfevals;
-    counters[2] += ndtsr;
-    counters[3] += ndtsa;
+    acc[slot * 4 + 1] += nfevals;
+    acc[slot * 4 + 2] += ndtsr;
+    acc[slot * 4 + 3] += ndtsa;
   }
-  return failed_any;
+  });
+  for (unsigned k = 0; k < 16; ++k)
+    for (int j = 0; j < 4; ++j) counters[j] += acc[k * 4 + j];
+  return *std::max_element(failed.begin(), failed.end());
 }
 
-template <class M>
+template <class M, int JAC = kJacobianFD>
 int run(int solver, unsigned N, const double* y0, const double* t, unsigned long n, const double* prm,
         const ImplicitParams& ip, double* out_states, long* counters)
 {
   constexpr int S = M::num_states;
-  SharedParamsOf<M, double> sp;
-  double c[M::num_constants > 0 ? M::num_constants : 1];
-  M::constants(prm, c);
-  for (int k = 0; k < M::num_parameters; ++k) sp.v[k] = prm[k];
-  for (int k = 0; k < M::num_constants; ++k) sp.c[k] = c[k];
+  const SharedParamsOf<M, double> sp = make_shared_params<M, double>(prm);
+  const SharedParamsOf<M, float> spf = make_shared_params<M, float>(prm);
   const double fpv[1] = {0.0};
-  const DevAcc<M, double> acc{sp, fpv};
-  std::vector<double> jac(size_t(S) * S), jfac(size_t(S) * S), work(size_t(implicit_work_vectors(solver)) * S);
+  const float fpvf[1] = {0.0f};
+  const DevAcc<M, double> acc64{sp, fpv};
+  const DevAcc<M, float> accf{spf, fpvf};
+  const JacAcc<M, JAC> jacc{acc64, accf};
+  using Acc = std::conditional_t<JAC == kJacobianFD, DevAcc<M, double>, JacAcc<M, JAC>>;
+  const Acc& acc = [&]() -> const Acc& {
+    if constexpr (JAC == kJacobianFD) return acc64;
+    else return jacc;
+  }();
+  (void)jacc;
+  for (int k = 0; k < 4; ++k) counters[k] = 0;
+  std::vector<long> cnt(16 * 4, 0);
+  std::vector<int> failed(16, 0);
+  parallel_nodes(N, [&](unsigned first, unsigned last, unsigned slot) {
+  std::vector<double> jac(M::jac_nnz), jfac(M::lu_nnz), work(size_t(implicit_work_vectors(solver)) * S);
   ImplicitBuffers B{};
   B.jac = jac.data();
   B.jfac = jfac.data();
   B.work = work.data();
-  for (int k = 0; k < 4; ++k) counters[k] = 0;
-  int failed_any = 0;
-  for (unsigned node = 0; node < N; ++node) {
+  for (unsigned node = first; node < last; ++node) {
     double ys[S];
     for (int i = 0; i < S; ++i) ys[i] = y0[size_t(node) * S + i];
     NodeImplicitState st{};
@@ -130,19 +135,22 @@ int run(int solver, unsigned N, const double* y0, const double* t, unsigned long
       if (solver == kImplicitEuler) {
         if (!ImplicitEulerNode<M>::forward(v, tk, dt, acc, ip, st)) {
           st.failed = 1;
-          failed_any = 1;
+          failed[slot] = 1;
         }
       } else {
         EsdirkNode<M>::forward(v, tk, dt, acc, ip, st);
       }
     }
     for (int i = 0; i < S; ++i) out_states[size_t(node) * S + i] = ys[i];
-    counters[0] += st.newton_its;
-    counters[1] += st.jac_comp;
-    counters[2] += st.rejects;
-    counters[3] += st.accepted;
+    cnt[slot * 4 + 0] += st.newton_its;
+    cnt[slot * 4 + 1] += st.jac_comp;
+    cnt[slot * 4 + 2] += st.rejects;
+    cnt[slot * 4 + 3] += st.accepted;
   }
-  return failed_any;
+  });
+  for (unsigned k = 0; k < 16; ++k)
+    for (int j = 0; j < 4; ++j) counters[j] += cnt[k * 4 + j];
+  return *std::max_element(failed.begin(), failed.end());
 }
 
 GOSS_MODEL_TYPE(MW_tp06_m_fp6, goss_models::tp06_m_fp6);
@@ -195,6 +203,20 @@ extern "C" int kernel_logic_step_vjp(int solver, int fp64, const double* y, doub
   GOSS_HARNESS_VJP(kRL2)
   GOSS_HARNESS_VJP(kGRL2)
 #undef GOSS_HARNESS_VJP
+  return -2;
+}
+
+// The implicit solvers with the Jacobian from forward differences of the FP32 right-hand side
+// (host float arithmetic), for tp06_m and tp06_epi.
+extern "C" int kernel_logic_run_fp32_jacobian(const char* variant, int solver, unsigned N, const double* y0,
+                                              const double* t, unsigned long n, const double* params,
+                                              double relative_tolerance, double* out_states, long* counters)
+{
+  const double min_dt = solver == kESDIRK23a ? 1e-3 : 1e-4;
+  ImplicitParams ip{-1.0, 1.0, 0.1, relative_tolerance, 0.01, min_dt, 30, 0, 2};
+  const std::string v(variant);
+  if (v == "tp06_m") return run<M_tp06_m, kJacobianFDFP32>(solver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "tp06_epi") return run<M_tp06_epi, kJacobianFDFP32>(solver, N, y0, t, n, params, ip, out_states, counters);
   return -2;
 }
 
