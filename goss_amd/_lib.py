@@ -24,6 +24,7 @@ SOLVERS = {
     "ImplicitEuler": 7,
     "ESDIRK23a": 8,
     "RKF32": 9,
+    "ThetaSolver": 10,
 }
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp32c": 2}
 JACOBIANS = {"fd": 0, "fd_fp32": 1}
@@ -42,10 +43,13 @@ EXPORTS = (
     "goss_hip_system_set_field_state_components", "goss_hip_system_get_field_parameters",
     "goss_hip_system_set_field_parameters", "goss_hip_system_get_field_states_device",
     "goss_hip_system_set_field_states_device", "goss_hip_system_get_states", "goss_hip_system_set_states",
+    "goss_hip_system_get_states_device", "goss_hip_system_set_states_device", "goss_hip_system_set_field_dofs",
+    "goss_hip_system_gather_field_states_device", "goss_hip_system_scatter_field_states_device",
     "goss_hip_system_reset_default", "goss_hip_system_set_parameter", "goss_hip_system_get_parameter",
     "goss_hip_system_set_num_threads", "goss_hip_system_get_num_threads", "goss_hip_system_num_nodes",
     "goss_hip_system_set_steps_per_launch", "goss_hip_system_synchronize", "goss_hip_system_stream",
     "goss_hip_system_set_kernel_timing", "goss_hip_system_get_kernel_timing",
+    "goss_hip_system_set_solver_params", "goss_hip_system_get_solver_params",
     "goss_hip_system_get_solver_counters", "goss_hip_adjoint_buffer_sizes", "goss_hip_adjoint_record",
     "goss_hip_adjoint_backward", "goss_hip_batch_create", "goss_hip_batch_destroy", "goss_hip_batch_set_config",
     "goss_hip_batch_get_config", "goss_hip_batch_forward", "goss_hip_batch_autotune",
@@ -67,6 +71,7 @@ class SolverParams(ctypes.Structure):
         ("always_recompute_jacobian", ctypes.c_int),
         ("num_refinements_without_always_recomputing_jacobian", ctypes.c_int),
         ("jacobian", ctypes.c_int),
+        ("theta", ctypes.c_double),
     ]
 
 
@@ -127,11 +132,28 @@ def lib() -> ctypes.CDLL:
 _loaded = {}
 
 
+def _share_the_hsa_runtime_with_torch() -> None:
+    """Load torch's ROCm libraries before this library when torch is installed.
+
+    The torch wheel carries its own HSA runtime, with the soname of ROCm's
+    (libhsa-runtime64.so.1) but required by torch under another name.  Loaded
+    after this library, torch would bring up a second HSA runtime in the
+    process, which cannot open the GPU again (torch then reports no device).
+    Loaded first, this library's HIP runtime binds to that same HSA runtime,
+    so torch tensors (device buffers of the Python API) and the library's
+    kernels use one device context."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path: str) -> ctypes.CDLL:
     """A library with the C ABI of include/goss_hip.h at `path`, with its
     signatures bound (the product library, or a test build of it)."""
     if path in _loaded:
         return _loaded[path]
+    _share_the_hsa_runtime_with_torch()
     L = ctypes.CDLL(path)
     L.goss_hip_last_error.restype = ctypes.c_char_p
     L.goss_hip_model_name.restype = ctypes.c_char_p
@@ -165,6 +187,11 @@ def load(path: str) -> ctypes.CDLL:
         getattr(L, f"goss_hip_system_{fn}").argtypes = [V, _dp, ctypes.c_uint32, _u32p, ctypes.c_int]
     L.goss_hip_system_get_states.argtypes = [V, _dp]
     L.goss_hip_system_set_states.argtypes = [V, _dp]
+    L.goss_hip_system_get_states_device.argtypes = [V, ctypes.c_void_p]
+    L.goss_hip_system_set_states_device.argtypes = [V, ctypes.c_void_p]
+    L.goss_hip_system_set_field_dofs.argtypes = [V, ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64]
+    L.goss_hip_system_gather_field_states_device.argtypes = [V, ctypes.c_void_p, ctypes.c_uint64]
+    L.goss_hip_system_scatter_field_states_device.argtypes = [V, ctypes.c_void_p, ctypes.c_uint64]
     L.goss_hip_system_reset_default.argtypes = [V]
     L.goss_hip_system_set_parameter.argtypes = [V, ctypes.c_char_p, ctypes.c_double]
     L.goss_hip_system_get_parameter.argtypes = [V, ctypes.c_char_p, _dp]
@@ -175,6 +202,8 @@ def load(path: str) -> ctypes.CDLL:
     L.goss_hip_system_synchronize.argtypes = [V]
     L.goss_hip_system_stream.argtypes = [V, ctypes.POINTER(ctypes.c_void_p)]
     L.goss_hip_system_set_kernel_timing.argtypes = [V, ctypes.c_int]
+    L.goss_hip_system_set_solver_params.argtypes = [V, ctypes.POINTER(SolverParams)]
+    L.goss_hip_system_get_solver_params.argtypes = [V, ctypes.POINTER(SolverParams)]
     L.goss_hip_system_get_kernel_timing.argtypes = [V, _dp, ctypes.POINTER(ctypes.c_uint64)]
     L.goss_hip_system_get_solver_counters.argtypes = [V, ctypes.POINTER(ctypes.c_int64)]
     u64p = ctypes.POINTER(ctypes.c_uint64)

@@ -6,9 +6,12 @@
   intermediates (``d<name>_d<state>``) instead of a fully substituted tree.
   This is what gotran's ``linearized_derivatives`` emits for goss
   (``python/goss/codegeneration.py:584-620``; example ``cpp/tests/fhn.h:92-126``).
-* ``linear_terms``: ``linear_terms[i] = 1`` when ``d f_i / d x_i`` does not
-  depend on ``x_i`` (``f_i`` affine in ``x_i``).  Pinned by ``cpp/tests/fhn.h:54``
-  and ``cpp/tests/beeler_reuter_1977.h:60-65``.
+* ``linear_terms``: ``linear_terms[i] = 1`` when ``d f_i / d x_i`` is not
+  zero and does not depend on ``x_i`` (``f_i`` affine in ``x_i`` with a
+  nonzero slope).  Pinned by ``cpp/tests/fhn.h:54``,
+  ``cpp/tests/beeler_reuter_1977.h:60-65`` and, for a derivative that does
+  not contain its state (x' = y), ``cpp/tests/VDP.h:59-63``: the Rush-Larsen
+  update divides by the slope (``RL1.h:76``).
 * ``jacobian``: the full analytic Jacobian (same chain rule, all states).
 * ``census``: operation counts of an emitted body.
 """
@@ -155,7 +158,8 @@ def linearize(model: ODEModel) -> Linearized:
         for nm, ex in stmts:
             extra_deps[nm] = _transitive_state_deps(ex, model, deps, extra_deps)
         per_state.append((lin, stmts))
-        flags.append(0 if st in _transitive_state_deps(lin, model, deps, extra_deps) else 1)
+        zero = lin.is_number and lin == 0
+        flags.append(0 if zero or st in _transitive_state_deps(lin, model, deps, extra_deps) else 1)
 
     # Derivative intermediates: those reachable from linear (flag 1) states
     # first, then the rest; each list in creation (dependency) order.

@@ -7,6 +7,7 @@
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import sys
@@ -54,6 +55,9 @@ def generate(names: List[str] = None, out_dir: str = OUT_DIR) -> Dict[str, dict]
         _write(os.path.join(out_dir, f"{n}.h"), em.header)
         _write(os.path.join(out_dir, f"{n}_plugin.h"), emit_parameterized_ode(var, f"{n}.h"))
         meta[n] = variant_metadata(var, em)
+        stem = VARIANTS[n][0]
+        with open(os.path.join(ODE_DIR, stem + ".ode"), "rb") as f:
+            meta[n]["source_sha256"] = hashlib.sha256(f.read()).hexdigest()
     _write(os.path.join(out_dir, "variants.json"), json.dumps(meta, indent=1, sort_keys=True) + "\n")
     lines = ["// Generated by goss_amd.codegen.generate -- do not edit.",
              "// X-macro list of the model variants compiled into the library.",

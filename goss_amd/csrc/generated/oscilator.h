@@ -18,7 +18,7 @@ static const char* const field_state_names[] = {"x", "y"};
 static const char* const field_parameter_names[] = {"a", "b"};
 constexpr double default_parameters[] = {1.0, 1.0};
 constexpr double initial_conditions[] = {1.0, 0.0};
-constexpr unsigned char linear_terms[] = {1, 1};
+constexpr unsigned char linear_terms[] = {0, 0};
 
 // Subexpressions of shared (non-field) parameters only, evaluated once per launch.
 // c[k] may use c[j] for j < k.  Called with real = double on the host.
@@ -121,9 +121,9 @@ GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const 
   const real b = pa.template get<1>();
   rhs[0] = -a*y;
   rhs[1] = b*x;
+  if (only_linear) return;
   linear[0] = real(0.0);
   linear[1] = real(0.0);
-  if (only_linear) return;
 }
 
 // linearized_eval (all entries) plus, for the gate states (gate_states[i] = 1: rhs[i] = (A - y_i) r),
@@ -232,13 +232,8 @@ GOSS_HD GOSS_INLINE void rl1_vjp(const real* __restrict__ states, const treal ti
   const real b = pa.template get<1>();
   const real rhs__0 = -a*y;
   const real rhs__1 = b*x;
-  const real lin__0 = real(0.0);
-  const real lin__1 = real(0.0);
-  const real hinv__ = m_rcp(h__);
-  real bbar__0, abar__0;
-  m_grl1_seeds(real(lin__0), m_rcp(real(lin__0)), real(rhs__0), lam__[0], h__, hinv__, real(-1.0), bbar__0, abar__0);
-  real bbar__1, abar__1;
-  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
+  const real bbar__0 = lam__[0]*h__;
+  const real bbar__1 = lam__[1]*h__;
   real a__b = real(0.0);
   real b__b = real(0.0);
   real x__b = real(0.0);
@@ -266,25 +261,20 @@ GOSS_HD GOSS_INLINE void rl1_vjp_tape(const real* __restrict__ states, const tre
   const real b = pa.template get<1>();
   tape.put(0, -a*y);
   tape.put(1, b*x);
+  const real bbar__0 = lam__[0]*h__;
+  const real bbar__1 = lam__[1]*h__;
   tape.put(2, real(0.0));
   tape.put(3, real(0.0));
-  const real hinv__ = m_rcp(h__);
-  real bbar__0, abar__0;
-  m_grl1_seeds(real(tape.get(2)), m_rcp(real(tape.get(2))), real(tape.get(0)), lam__[0], h__, hinv__, real(-1.0), bbar__0, abar__0);
-  real bbar__1, abar__1;
-  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(1)), lam__[1], h__, hinv__, real(-1.0), bbar__1, abar__1);
   tape.put(4, real(0.0));
   tape.put(5, real(0.0));
-  tape.put(6, real(0.0));
-  tape.put(7, real(0.0));
-  tape.add(5, bbar__1*(x));
-  tape.add(6, bbar__1*(b));
-  tape.add(4, bbar__0*(-y));
-  tape.add(7, bbar__0*(-a));
-  lam_prev__[0] = lam__[0] + tape.get(6);
-  lam_prev__[1] = lam__[1] + tape.get(7);
-  fp_bar__[0] += tape.get(4);
-  fp_bar__[1] += tape.get(5);
+  tape.add(3, bbar__1*(x));
+  tape.add(4, bbar__1*(b));
+  tape.add(2, bbar__0*(-y));
+  tape.add(5, bbar__0*(-a));
+  lam_prev__[0] = lam__[0] + tape.get(4);
+  lam_prev__[1] = lam__[1] + tape.get(5);
+  fp_bar__[0] += tape.get(2);
+  fp_bar__[1] += tape.get(3);
 }
 
 template <typename real, typename PA, typename treal>
@@ -340,7 +330,7 @@ GOSS_HD GOSS_INLINE void eval_vjp_tape(const real* __restrict__ states, const tr
 }
 
 constexpr int grl1_vjp_tape_slots = 8;
-constexpr int rl1_vjp_tape_slots = 8;
+constexpr int rl1_vjp_tape_slots = 6;
 constexpr int eval_vjp_tape_slots = 6;
 
 // Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].

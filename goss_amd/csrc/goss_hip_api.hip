@@ -136,7 +136,7 @@ const VariantInfo* find_variant(const char* name)
 }
 
 bool is_explicit(int solver) { return solver >= 0 && solver <= kGRL2; }
-bool is_implicit(int solver) { return solver == kImplicitEuler || solver == kESDIRK23a; }
+bool is_implicit(int solver) { return solver == kImplicitEuler || solver == kESDIRK23a || solver == kThetaSolver; }
 // Solvers with per-node state buffers (ds / is / error flag).
 bool has_node_state(int solver) { return is_implicit(solver) || solver == kRKF32; }
 
@@ -156,6 +156,7 @@ void default_params(int solver, goss_hip_solver_params* p)
   p->atol = 1e-5;  // AdaptiveExplicitSolver.cpp:52-53
   p->rtol = 1e-8;
   p->jacobian = GOSS_HIP_JACOBIAN_FD;
+  p->theta = 0.5;  // ThetaSolver.h:37
 }
 
 }  // namespace
@@ -188,6 +189,10 @@ struct goss_hip_system {
   uint64_t timed_launches = 0;
   ImplicitBuffers ib{};  // implicit solvers only
   int* h_error = nullptr;  // pinned copy of ib.error
+  // Field dof map of goss_hip_system_set_field_dofs: [F][N] indices into a
+  // field vector of dof_values doubles.
+  int* d_dofs = nullptr;
+  uint64_t dof_values = 0;
 
   size_t elem() const { return prec == kFP64 ? sizeof(double) : sizeof(float); }
   bool is_f64() const { return prec == kFP64; }
@@ -305,6 +310,7 @@ struct goss_hip_system {
     ip.max_iterations = sp.max_iterations;
     ip.always_recompute_jacobian = sp.always_recompute_jacobian;
     ip.num_refinements = sp.num_refinements_without_always_recomputing_jacobian;
+    ip.theta = sp.theta;
     return ip;
   }
 
@@ -313,7 +319,7 @@ struct goss_hip_system {
   // with the reference's message (ImplicitEuler.cpp:132-141).
   void launch_implicit(const double2* d, size_t n, double* field_out, int tangled)
   {
-    const ImplicitLaunch launch = v->implicit_launch[solver == kImplicitEuler ? 0 : 1][sp.jacobian];
+    const ImplicitLaunch launch = v->implicit_launch[implicit_slot(solver)][sp.jacobian];
     if (!launch) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
     timing_begin();
     for (size_t off = 0; off < n; off += steps_per_launch, ++timed_launches) {
@@ -419,6 +425,7 @@ struct goss_hip_system {
     if (ib.is) (void)hipFree(ib.is);
     if (ib.error) (void)hipFree(ib.error);
     if (h_error) (void)hipHostFree(h_error);
+    if (d_dofs) (void)hipFree(d_dofs);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -604,6 +611,7 @@ struct DevBuf {
 // ODE::lu_factorize (ODE.cpp:69-100), one S x S row-major matrix per lane.
 __global__ void dense_lu_kernel(const int S, const unsigned long n, double* __restrict__ mats)
 {
+#pragma clang fp contract(off)
   const unsigned long p = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
   if (p >= n) return;
   double* a = mats + p * size_t(S) * S;
@@ -626,6 +634,7 @@ __global__ void dense_lu_kernel(const int S, const unsigned long n, double* __re
 __global__ void dense_fb_subst_kernel(const int S, const unsigned long n, const double* __restrict__ mats,
                                       const double* __restrict__ b, double* __restrict__ x)
 {
+#pragma clang fp contract(off)
   const unsigned long p = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
   if (p >= n) return;
   const double* a = mats + p * size_t(S) * S;
@@ -774,7 +783,7 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
       if (precision != kFP64) throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solvers run in FP64 only");
       const int jac = params ? params->jacobian : GOSS_HIP_JACOBIAN_FD;
       if (jac < 0 || jac >= kNumJacobianModes) throw Error(GOSS_HIP_EINVAL, "unknown Jacobian mode");
-      if (!v->implicit_launch[solver == kImplicitEuler ? 0 : 1][jac])
+      if (!v->implicit_launch[implicit_slot(solver)][jac])
         throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
     } else if (solver == kRKF32) {
       if (!v->rkf32_launch[precision]) throw Error(GOSS_HIP_EUNSUPPORTED, "RKF32 runs in FP64 and FP32 only");
@@ -1096,6 +1105,93 @@ int goss_hip_system_set_states(goss_hip_system* s, const double* in)
   });
 }
 
+int goss_hip_system_get_states_device(goss_hip_system* s, double* dev_out)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (!dev_out) throw Error(GOSS_HIP_EINVAL, "null device pointer");
+    s->use_device();
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_states<R>(LayoutOp::kGet, s->y, s->ylo, dev_out, s->map, s->N, s->ld, s->stream),
+                "states_to_aos");
+    });
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_set_states_device(goss_hip_system* s, const double* dev_in)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (!dev_in) throw Error(GOSS_HIP_EINVAL, "null device pointer");
+    s->use_device();
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_states<R>(LayoutOp::kSet, s->y, s->ylo, const_cast<double*>(dev_in), s->map, s->N, s->ld,
+                                 s->stream),
+                "states_from_aos");
+    });
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_set_field_dofs(goss_hip_system* s, const int32_t* dofs, uint64_t num_values)
+{
+  return guarded([&] {
+    check_sys(s);
+    const size_t n = size_t(s->N) * s->v->NFS;
+    if (n && !dofs) throw Error(GOSS_HIP_EINVAL, "null dof map");
+    if (num_values > uint64_t(INT32_MAX)) throw Error(GOSS_HIP_EINVAL, "a field vector has at most 2^31 - 1 values");
+    // Every index in range and none repeated: the scatter then writes each
+    // value once and the kernels stay inside the caller's vector.
+    std::vector<int32_t> sorted(dofs, dofs + n);
+    std::sort(sorted.begin(), sorted.end());
+    if (n && (sorted.front() < 0 || uint64_t(sorted.back()) >= num_values))
+      throw Error(GOSS_HIP_EINVAL, "dof index out of range of the field vector");
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      throw Error(GOSS_HIP_EINVAL, "a dof index appears twice in the dof map");
+    s->use_device();
+    if (s->d_dofs) hip_check(hipFree(s->d_dofs), "hipFree");
+    s->d_dofs = nullptr;
+    s->dof_values = 0;
+    if (n) {
+      hip_check(hipMalloc(&s->d_dofs, n * sizeof(int32_t)), "hipMalloc(dofs)");
+      hip_check(hipMemcpy(s->d_dofs, dofs, n * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy(dofs)");
+    }
+    s->dof_values = num_values;
+  });
+}
+
+static int field_dofs_io(goss_hip_system* s, double* dev_values, uint64_t num_values, LayoutOp op)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (s->v->NFS && !s->d_dofs) throw Error(GOSS_HIP_EINVAL, "no dof map set (goss_hip_system_set_field_dofs)");
+    if (num_values != s->dof_values)
+      throw Error(GOSS_HIP_EINVAL, "field vector length differs from the dof map's (" +
+                                       std::to_string(s->dof_values) + ")");
+    if (s->v->NFS && !dev_values) throw Error(GOSS_HIP_EINVAL, "null device pointer");
+    s->use_device();
+    s->dispatch_layout([&](auto r) {
+      using R = decltype(r);
+      hip_check(layout_field_dofs<R>(op, s->y, s->ylo, dev_values, s->d_dofs, s->map, s->N, s->ld, s->stream),
+                "field_dofs");
+    });
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_gather_field_states_device(goss_hip_system* s, const double* dev_values, uint64_t num_values)
+{
+  return field_dofs_io(s, const_cast<double*>(dev_values), num_values, LayoutOp::kSet);
+}
+
+int goss_hip_system_scatter_field_states_device(goss_hip_system* s, double* dev_values, uint64_t num_values)
+{
+  return field_dofs_io(s, dev_values, num_values, LayoutOp::kGet);
+}
+
 static int field_states_io(goss_hip_system* s, double* host, const double* host_in, double* dev_ptr, bool device_io,
                            LayoutOp op, const ComponentList& cl, int tangled)
 {
@@ -1299,6 +1395,30 @@ int goss_hip_system_synchronize(goss_hip_system* s)
     check_sys(s);
     s->use_device();
     hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_set_solver_params(goss_hip_system* s, const goss_hip_solver_params* params)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (!params) throw Error(GOSS_HIP_EINVAL, "null parameters");
+    if (params->jacobian != s->sp.jacobian) {
+      if (params->jacobian < 0 || params->jacobian >= kNumJacobianModes)
+        throw Error(GOSS_HIP_EINVAL, "unknown Jacobian mode");
+      if (is_implicit(s->solver) && !s->v->implicit_launch[implicit_slot(s->solver)][params->jacobian])
+        throw Error(GOSS_HIP_EUNSUPPORTED, "implicit solver not compiled for this model");
+    }
+    // Read by the host at the next launch; kernels already queued keep their values.
+    s->sp = *params;
+  });
+}
+
+int goss_hip_system_get_solver_params(goss_hip_system* s, goss_hip_solver_params* params)
+{
+  return guarded([&] {
+    check_sys(s);
+    *params = s->sp;
   });
 }
 

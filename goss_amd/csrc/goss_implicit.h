@@ -88,6 +88,7 @@ struct ImplicitParams {
   int max_iterations;
   int always_recompute_jacobian;
   int num_refinements;
+  double theta;  // ThetaSolver.h:37
 };
 
 // Per-node solver state, structure of arrays [field][ld] (shared by the
@@ -118,11 +119,17 @@ enum IState : int {
 
 // Work vectors per node (each S rows of the work buffer).
 enum EsdirkWork : int { kEsZ1 = 0, kEsZ2, kEsZ3, kEsZ4, kEsPrev, kEsF1, kEsDZ, kNumEsdirkWork };
-enum EulerWork : int { kIeZ = 0, kIeYZ, kIeB, kIeF1, kNumEulerWork };
+enum EulerWork : int { kIeZ = 0, kIeYZ, kIeB, kIeF1, kIePrev, kNumEulerWork };  // kIePrev: ThetaSolver only
 
 __host__ __device__ constexpr int implicit_work_vectors(int solver)
 {
   return solver == kESDIRK23a ? int(kNumEsdirkWork) : int(kNumEulerWork);
+}
+
+// Slot of an implicit solver in VariantInfo::implicit_launch.
+__host__ __device__ constexpr int implicit_slot(int solver)
+{
+  return solver == kImplicitEuler ? 0 : (solver == kESDIRK23a ? 1 : 2);
 }
 
 struct ImplicitBuffers {
@@ -206,11 +213,14 @@ struct Implicit {
   // by sqrt(eps_FP32) * max(1e-5, |y_i|) (the reference's floor, with the FP32
   // epsilon in place of its 1e-15) and divides by the perturbation as it is
   // represented in FP32.  The base evaluation, also FP32, is kept in f1.
+  // Returns whether every FP32 evaluation was finite.
   template <class A>
-  GOSS_HD static void fd_jacobian_f32(const VecView& y, const double t, const A& acc, const VecView& f1,
+  GOSS_HD static bool fd_jacobian_f32(const VecView& y, const double t, const A& acc, const VecView& f1,
                                       const VecView& J)
   {
     constexpr float kSqrtEps = 3.4526698e-4f;  // sqrt(2^-23)
+    constexpr float kMaxFinite = 3.4028235e38f;
+    bool finite = true;
 #pragma unroll 1
     for (int i = -1; i < S; ++i) {
       float yp[S], f[S];
@@ -228,6 +238,8 @@ struct Implicit {
         }
       }
       M::template eval<float>(yp, t, acc.accf, f);
+#pragma unroll
+      for (int j = 0; j < S; ++j) finite = finite && (fabsf(f[j]) <= kMaxFinite);  // false for inf and NaN
       if (i < 0) {
 #pragma unroll
         for (int j = 0; j < S; ++j) f1[j] = double(f[j]);
@@ -238,16 +250,24 @@ struct Implicit {
         store_column(J, i, fd, f1, double(dy));
       }
     }
+    return finite;
   }
 
   // The Jacobian at y into J, as the accessor's mode says (f1 is overwritten).
+  // In the FP32 mode a node whose FP32 right-hand side is not finite at any of
+  // the difference points gets the FP64 forward-difference Jacobian instead:
+  // FP32 resolves removable singularities of a model only to its own width
+  // (the L-type calcium current of the TP06 epicardial model,
+  // (V - 15) / (exp(c (V - 15)) - 1), is 0/0 in FP32 for |V - 15 mV| below
+  // about 1e-6 mV), where FP64 is finite.
   template <class A>
   GOSS_HD static void jacobian(const VecView& y, const double t, const A& acc, const VecView& f1, const VecView& J)
   {
-    if constexpr (JacobianModeOf<A>::value == kJacobianFDFP32)
-      fd_jacobian_f32(y, t, acc, f1, J);
-    else
+    if constexpr (JacobianModeOf<A>::value == kJacobianFDFP32) {
+      if (!fd_jacobian_f32(y, t, acc, f1, J)) fd_jacobian(y, t, acc, f1, J);
+    } else {
       fd_jacobian(y, t, acc, f1, J);
+    }
   }
 
   // F <- LU factors of I + scale * J (ImplicitODESolver.cpp:92-105 with
@@ -347,18 +367,20 @@ struct ImplicitEulerNode {
   static constexpr int S = M::num_states;
   using I = Implicit<M>;
 
-  // ImplicitODESolver::newton_solve with prev = 0 and alpha = 1
-  // (ImplicitEuler.cpp:85), mass matrix = identity; z is the work vector kIeZ.
-  template <class A>
-  GOSS_HD static bool newton_solve(const NodeViews& v, const double t, const double dt, const A& acc,
-                                   const ImplicitParams& p, NodeImplicitState& st)
+  // ImplicitODESolver::newton_solve (ImplicitODESolver.cpp:114-245) with the
+  // identity mass matrix; z is the work vector kIeZ.  ImplicitEuler calls it
+  // with prev = 0 and alpha = 1 (ImplicitEuler.cpp:85), ThetaSolver with
+  // prev = (1 - theta) f(y, t) in kIePrev and alpha = theta
+  // (ThetaSolver.cpp:86).
+  template <bool PREV, class A>
+  GOSS_HD static bool newton_solve(const NodeViews& v, const double t, const double dt, const double alpha,
+                                   const A& acc, const ImplicitParams& p, NodeImplicitState& st, const bool always)
   {
     const VecView z = v.w(kIeZ), yzv = v.w(kIeYZ), bv = v.w(kIeB), f1 = v.w(kIeF1);
     bool step_ok = true;
     st.newton_its = 0;
     double relative_previous_residual = 1.0, relative_residual = 1.0, previous_residual = 1.0,
            initial_residual = 1.0, residual;
-    const bool always = st.always != 0;
 #pragma unroll 1
     do {
       {
@@ -368,7 +390,12 @@ struct ImplicitEulerNode {
         M::template eval<double>(yz, t, acc, f);
 #pragma unroll
         for (int i = 0; i < S; ++i) {
-          b[i] = -z[i] * 1.0 + dt * (0.0 + 1.0 * f[i]);
+          if constexpr (PREV) {
+            const VecView prev = v.w(kIePrev);
+            b[i] = -z[i] * 1.0 + dt * (prev[i] + alpha * f[i]);
+          } else {
+            b[i] = -z[i] * 1.0 + dt * (0.0 + 1.0 * f[i]);
+          }
           bv[i] = b[i];
           yzv[i] = yz[i];
         }
@@ -377,7 +404,7 @@ struct ImplicitEulerNode {
       if (relative_residual < p.relative_tolerance) break;
       if (st.recompute || always) {
         I::jacobian(yzv, t, acc, f1, v.J);
-        I::factorize(v.J, v.F, -dt * 1.0);
+        I::factorize(v.J, v.F, -dt * alpha);
         st.jac_comp += 1;
         st.recompute = 0;
       }
@@ -433,7 +460,7 @@ struct ImplicitEulerNode {
 #pragma unroll
       for (int i = 0; i < S; ++i) z[i] = 0.0;
       if (num_refinements > p.num_refinements) st.always = 1;
-      const bool step_ok = newton_solve(v, t + ldt, ldt, acc, p, st);
+      const bool step_ok = newton_solve<false>(v, t + ldt, ldt, 1.0, acc, p, st, st.always != 0);
       if (step_ok) {
 #pragma unroll
         for (int i = 0; i < S; ++i) v.y[i] += z[i];
@@ -453,6 +480,72 @@ struct ImplicitEulerNode {
         ldt /= 2.0;
         if (ldt < p.min_dt) return false;
         st.recompute = 1;
+        st.justrefined = 1;
+        num_refinements += 1;
+      }
+    }
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// ThetaSolver
+// ---------------------------------------------------------------------------
+template <class M>
+struct ThetaNode {
+  static constexpr int S = M::num_states;
+  using E = ImplicitEulerNode<M>;
+
+  // ThetaSolver::forward (ThetaSolver.cpp:48-155).  The sticky
+  // always_recompute_jacobian is set while num_refinements <=
+  // num_refinements_without_always_recomputing_jacobian (ThetaSolver.cpp:82-83),
+  // i.e. from the first step on.  Returns false when the step size falls
+  // below min_dt (the reference throws).
+  template <class A>
+  GOSS_HD static bool forward(const NodeViews& v, double t, const double dt, const A& acc, const ImplicitParams& p,
+                              NodeImplicitState& st)
+  {
+    const VecView z = v.w(kIeZ), prev = v.w(kIePrev);
+    const double theta = p.theta;
+    const double t_end = t + dt;
+    const double ldt_0 = p.ldt_0;
+    double ldt = ldt_0 > 0 ? ldt_0 : dt;
+    int num_refinements = 0;
+    const double eps = kGossEps * 1000;
+#pragma unroll
+    for (int i = 0; i < S; ++i) prev[i] = 0.0;
+#pragma unroll 1
+    while (true) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) z[i] = 0.0;
+      if (fabs(theta - 1.0) > kGossEps) {
+        double y[S], f[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) y[i] = v.y[i];
+        M::template eval<double>(y, t, acc, f);
+#pragma unroll
+        for (int i = 0; i < S; ++i) prev[i] = (1 - theta) * f[i];
+      }
+      if (num_refinements <= p.num_refinements) st.always = 1;
+      const bool step_ok = E::template newton_solve<true>(v, t + theta * ldt, ldt, theta, acc, p, st, st.always != 0);
+      if (step_ok) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) v.y[i] += z[i];
+        t += ldt;
+        if (fabs(t - t_end) < eps) break;
+        if (!st.justrefined) {
+          const double tmp = 2.0 * ldt;
+          if (ldt_0 > 0. && tmp >= ldt_0)
+            ldt = ldt_0;
+          else
+            ldt = tmp;
+        } else {
+          st.justrefined = 0;
+        }
+        if ((t + ldt + kGossEps) > t_end) ldt = t_end - t;
+      } else {
+        ldt /= 2.0;
+        if (ldt < p.min_dt) return false;
         st.justrefined = 1;
         num_refinements += 1;
       }
@@ -722,6 +815,11 @@ implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const dou
       const double dt = steps[k].y;
       if constexpr (SOLVER == kImplicitEuler) {
         if (!ImplicitEulerNode<M>::forward(v, t, dt, acc, ip, st)) {
+          st.failed = 1;
+          atomicOr(B.error, 1);
+        }
+      } else if constexpr (SOLVER == kThetaSolver) {
+        if (!ThetaNode<M>::forward(v, t, dt, acc, ip, st)) {
           st.failed = 1;
           atomicOr(B.error, 1);
         }

@@ -38,7 +38,8 @@ enum SolverKind : int {
   kImplicitEuler = 7,
   kESDIRK23a = 8,
   kRKF32 = 9,
-  kNumSolvers = 10,
+  kThetaSolver = 10,
+  kNumSolvers = 11,
 };
 
 enum Precision : int { kFP64 = 0, kFP32 = 1, kFP32C = 2, kNumPrecisions = 3 };

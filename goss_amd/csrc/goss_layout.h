@@ -35,6 +35,11 @@ hipError_t layout_states(LayoutOp op, void* y, void* ylo, double* buf, const Ind
 template <class R>
 hipError_t layout_field_states(LayoutOp op, void* y, void* ylo, double* buf, const IndexMap& m,
                                const ComponentList& cl, unsigned N, unsigned ld, int tangled, hipStream_t s);
+// kGet: field states -> values[dofs] (scatter); kSet: values[dofs] -> field
+// states (gather).  dofs [F][N], device memory.
+template <class R>
+hipError_t layout_field_dofs(LayoutOp op, void* y, void* ylo, double* values, const int* dofs, const IndexMap& m,
+                             unsigned N, unsigned ld, hipStream_t s);
 template <class R>
 hipError_t layout_field_params(LayoutOp op, void* fp, double* buf, unsigned NFP, unsigned N, unsigned ld,
                                int tangled, hipStream_t s);

@@ -74,6 +74,32 @@ __global__ void field_states_set(R* __restrict__ y, R* __restrict__ ylo, const d
   store_state(y, ylo, m, ld, m.fsi[c], n, in[idx]);
 }
 
+// Field states <-> a field vector through a dof map: dofs[c * N + n] is the
+// index in `values` of field state c of node n (validated on the host: in
+// range and without repeats).  One thread per (component, node), so the dof
+// map and the state rows are read and written contiguously.
+template <class R>
+__global__ void field_states_gather(R* __restrict__ y, R* __restrict__ ylo, const double* __restrict__ values,
+                                    const int* __restrict__ dofs, const IndexMap m, unsigned N, unsigned ld)
+{
+  const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= size_t(N) * m.F) return;
+  const int c = int(k / N);
+  const unsigned n = unsigned(k % N);
+  store_state(y, ylo, m, ld, m.fsi[c], n, values[dofs[k]]);
+}
+
+template <class R>
+__global__ void field_states_scatter(const R* __restrict__ y, const R* __restrict__ ylo, double* __restrict__ values,
+                                     const int* __restrict__ dofs, const IndexMap m, unsigned N, unsigned ld)
+{
+  const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= size_t(N) * m.F) return;
+  const int c = int(k / N);
+  const unsigned n = unsigned(k % N);
+  values[dofs[k]] = load_state(y, ylo, m, ld, m.fsi[c], n);
+}
+
 // Field parameters [NFP][ld] <-> caller layout.
 template <class R>
 __global__ void field_params_get(const R* __restrict__ fp, double* __restrict__ out, unsigned NFP, unsigned N,
@@ -174,6 +200,21 @@ hipError_t layout_field_states(LayoutOp op, void* y, void* ylo, double* buf, con
 }
 
 template <class R>
+hipError_t layout_field_dofs(LayoutOp op, void* y, void* ylo, double* values, const int* dofs, const IndexMap& m,
+                             unsigned N, unsigned ld, hipStream_t s)
+{
+  const size_t work = size_t(N) * m.F;
+  if (work == 0) return hipSuccess;
+  if (op == LayoutOp::kGet)
+    hipLaunchKernelGGL(field_states_scatter<R>, grid_for(work), dim3(256), 0, s, static_cast<const R*>(y),
+                       static_cast<const R*>(ylo), values, dofs, m, N, ld);
+  else
+    hipLaunchKernelGGL(field_states_gather<R>, grid_for(work), dim3(256), 0, s, static_cast<R*>(y),
+                       static_cast<R*>(ylo), values, dofs, m, N, ld);
+  return hipGetLastError();
+}
+
+template <class R>
 hipError_t layout_field_params(LayoutOp op, void* fp, double* buf, unsigned NFP, unsigned N, unsigned ld,
                                int tangled, hipStream_t s)
 {
@@ -205,6 +246,8 @@ hipError_t layout_fill_defaults(void* y, void* ylo, void* fp, const IndexMap& m,
                                              const ComponentList&, unsigned, unsigned, int, hipStream_t);       \
   template hipError_t layout_field_params<R>(LayoutOp, void*, double*, unsigned, unsigned, unsigned, int,       \
                                              hipStream_t);                                                      \
+  template hipError_t layout_field_dofs<R>(LayoutOp, void*, void*, double*, const int*, const IndexMap&,        \
+                                           unsigned, unsigned, hipStream_t);                                    \
   template hipError_t layout_fill_defaults<R>(void*, void*, void*, const IndexMap&, const DefaultValues&,       \
                                               unsigned, unsigned, hipStream_t);
 GOSS_LAYOUT_INST(float)

@@ -271,6 +271,7 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   fill_info(v);
   v.implicit_launch[0][kJacobianFD] = &launch_implicit<kImplicitEuler, kJacobianFD>;
   v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
+  v.implicit_launch[2][kJacobianFD] = &launch_implicit<kThetaSolver, kJacobianFD>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
 }
 #elif defined(GOSS_PREC_F64)
@@ -282,6 +283,8 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   v.implicit_launch[0][kJacobianFDFP32] = &launch_implicit<kImplicitEuler, kJacobianFDFP32>;
   v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
   v.implicit_launch[1][kJacobianFDFP32] = &launch_implicit<kESDIRK23a, kJacobianFDFP32>;
+  v.implicit_launch[2][kJacobianFD] = &launch_implicit<kThetaSolver, kJacobianFD>;
+  v.implicit_launch[2][kJacobianFDFP32] = &launch_implicit<kThetaSolver, kJacobianFDFP32>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
   register_adjoint<double>(v, kFP64);
   v.model_eval = &launch_model_eval;

@@ -9,9 +9,13 @@ with the library's kernel templates for gfx950 into a small model library;
 ``goss_hip_model_load`` registers it with the main library, after which the
 variant is used like the compiled-in ones.
 
+A variant's metadata (sizes, names, defaults, linear terms, intermediates)
+comes from the code generator without compiling (:func:`metadata`), so a
+model can be inspected before its first device use, which compiles it.
+
 Libraries are cached under ``goss_amd/_jit/<key>/`` by a key over the model
-text, the field selection and the library's sources, so each combination is
-compiled once.  Monitored values need no compilation: every named
+text, the field selection, the code generator and the library's sources, so
+each combination is compiled once.  Monitored values need no compilation: every named
 intermediate is evaluated by the generated ``eval_intermediates``.
 """
 from __future__ import annotations
@@ -50,7 +54,62 @@ def _key(ode_text: str, field_states: Sequence[str], field_parameters: Sequence[
     for f in SOURCES:
         with open(os.path.join(CSRC, f), "rb") as fh:
             h.update(fh.read())
+    codegen = os.path.join(HERE, "codegen")
+    for f in sorted(os.listdir(codegen)):
+        if f.endswith(".py"):
+            with open(os.path.join(codegen, f), "rb") as fh:
+                h.update(fh.read())
     return h.hexdigest()[:16]
+
+
+def variant_name(ode_path: str, field_states: Sequence[str] = (), field_parameters: Sequence[str] = ()) -> str:
+    """Name of the run-time variant for this selection (no compilation)."""
+    with open(ode_path) as f:
+        text = f.read()
+    return "jit_" + _key(text, field_states, field_parameters)
+
+
+class ModelMeta:
+    """The attributes of :class:`goss_amd.system.ModelVariant`, computed by the
+    code generator for a variant that is not compiled yet."""
+
+    def __init__(self, ode_path: str, field_states: Sequence[str], field_parameters: Sequence[str]):
+        import numpy as np
+
+        from .codegen.analysis import linearize
+        from .codegen.emit import Variant
+        from .codegen.ode_model import load_ode
+
+        model = load_ode(ode_path)
+        var = Variant(model=model, name=variant_name(ode_path, field_states, field_parameters),
+                      field_states=list(field_states), field_parameters=list(field_parameters))
+        self.name = var.name
+        self.num_states = model.num_states
+        self.num_parameters = model.num_parameters
+        self.num_field_states = len(var.field_states)
+        self.num_field_parameters = len(var.field_parameters)
+        self.state_names = model.state_names
+        self.parameter_names = model.parameter_names
+        self.field_state_names = list(var.field_states)
+        self.field_parameter_names = list(var.field_parameters)
+        self.field_state_indices = list(var.field_state_indices)
+        self.field_parameter_indices = list(var.field_parameter_indices)
+        self.default_parameters = np.array([float(p.value) for p in model.parameters])
+        self.initial_conditions = np.array([float(st.init) for st in model.states])
+        self.linear_terms = list(linearize(model).linear_terms)
+        derivs = set(model.derivatives.values())
+        # The order of the generated eval_intermediates (intermediate_names).
+        self.intermediate_names = [n for n, _ in model.body if n not in derivs]
+
+
+_meta: Dict[str, ModelMeta] = {}
+
+
+def metadata(ode_path: str, field_states: Sequence[str] = (), field_parameters: Sequence[str] = ()) -> ModelMeta:
+    name = variant_name(ode_path, field_states, field_parameters)
+    if name not in _meta:
+        _meta[name] = ModelMeta(ode_path, field_states, field_parameters)
+    return _meta[name]
 
 
 def build(ode_path: str, field_states: Sequence[str] = (), field_parameters: Sequence[str] = (),
@@ -60,9 +119,7 @@ def build(ode_path: str, field_states: Sequence[str] = (), field_parameters: Seq
     from .codegen.emit import Variant, emit_model_header
     from .codegen.ode_model import load_ode
 
-    with open(ode_path) as f:
-        text = f.read()
-    name = "jit_" + _key(text, field_states, field_parameters)
+    name = variant_name(ode_path, field_states, field_parameters)
     out_dir = os.path.join(CACHE, name)
     so = os.path.join(out_dir, f"libgoss_model_{name}.so")
     if os.path.exists(so):

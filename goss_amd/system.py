@@ -157,6 +157,31 @@ class DeviceSystem:
     def set_field_states_device(self, dev_ptr: int, tangled: bool = True) -> None:
         check(self._L.goss_hip_system_set_field_states_device(self._h, ctypes.c_void_p(dev_ptr), int(tangled)), self._L)
 
+    def get_states_device(self, dev_ptr: int) -> None:
+        """All states, node-major [N][S] FP64, into device memory at dev_ptr."""
+        check(self._L.goss_hip_system_get_states_device(self._h, ctypes.c_void_p(dev_ptr)), self._L)
+
+    def set_states_device(self, dev_ptr: int) -> None:
+        check(self._L.goss_hip_system_set_states_device(self._h, ctypes.c_void_p(dev_ptr)), self._L)
+
+    def set_field_dofs(self, dofs: np.ndarray, num_values: int) -> None:
+        """dofs [F][N]: index in a field vector of num_values doubles of field state c of node n."""
+        d = np.ascontiguousarray(dofs, dtype=np.int32)
+        if d.size != self.num_nodes * self.model.num_field_states:
+            raise ValueError(f"expected {self.model.num_field_states} x {self.num_nodes} dofs, got {d.shape}")
+        check(self._L.goss_hip_system_set_field_dofs(self._h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                     int(num_values)), self._L)
+
+    def gather_field_states_device(self, dev_ptr: int, num_values: int) -> None:
+        """Field states <- values[dofs] of the device field vector at dev_ptr."""
+        check(self._L.goss_hip_system_gather_field_states_device(self._h, ctypes.c_void_p(dev_ptr),
+                                                                 int(num_values)), self._L)
+
+    def scatter_field_states_device(self, dev_ptr: int, num_values: int) -> None:
+        """values[dofs] <- field states, into the device field vector at dev_ptr."""
+        check(self._L.goss_hip_system_scatter_field_states_device(self._h, ctypes.c_void_p(dev_ptr),
+                                                                  int(num_values)), self._L)
+
     def reset_default(self) -> None:
         check(self._L.goss_hip_system_reset_default(self._h), self._L)
 
@@ -201,6 +226,24 @@ class DeviceSystem:
                                               ctypes.c_void_p(ckpt_ptr), ctypes.c_void_p(grad_samples_ptr),
                                               ctypes.c_void_p(segment_ptr), ctypes.c_void_p(grad_states_ptr),
                                               ctypes.c_void_p(grad_fp_ptr)), self._L)
+
+    def set_solver_params(self, **overrides) -> None:
+        """Changes solver tunables (names of goss_hip_solver_params) from the next launch on."""
+        sp = SolverParams()
+        check(self._L.goss_hip_system_get_solver_params(self._h, ctypes.byref(sp)), self._L)
+        for k, v in overrides.items():
+            if not hasattr(sp, k):
+                raise KeyError(f"unknown solver parameter {k!r}")
+            if k == "jacobian" and isinstance(v, str):
+                v = JACOBIANS[v]
+            setattr(sp, k, type(getattr(sp, k))(v))
+        check(self._L.goss_hip_system_set_solver_params(self._h, ctypes.byref(sp)), self._L)
+        self._sp = sp
+
+    def solver_params(self) -> SolverParams:
+        sp = SolverParams()
+        check(self._L.goss_hip_system_get_solver_params(self._h, ctypes.byref(sp)), self._L)
+        return sp
 
     def set_kernel_timing(self, enable: bool = True) -> None:
         """Bracket the solver kernels of each stepping call with HIP events."""

@@ -45,13 +45,14 @@ typedef enum goss_hip_solver {
   GOSS_HIP_GRL2 = 6,           /* GRL2.cpp:63-86 */
   GOSS_HIP_IMPLICIT_EULER = 7, /* ImplicitEuler.cpp:57-152 */
   GOSS_HIP_ESDIRK23A = 8,      /* ESDIRK23a.cpp:280-316 */
-  GOSS_HIP_RKF32 = 9           /* RKF32.cpp:105-222 */
+  GOSS_HIP_RKF32 = 9,          /* RKF32.cpp:105-222 */
+  GOSS_HIP_THETA = 10          /* ThetaSolver.cpp:48-155 */
 } goss_hip_solver;
 
 /* Arithmetic of the per-node work.  FP64 is the reference's arithmetic.
  * FP32C computes in FP32 and carries the states whose derivative is not
  * affine in themselves (linear_terms == 0) as FP32 hi + lo pairs.
- * ImplicitEuler and ESDIRK23a run in FP64; RKF32 in FP64 or FP32 (its step
+ * ImplicitEuler, ESDIRK23a and ThetaSolver run in FP64; RKF32 in FP64 or FP32 (its step
  * sizes and local time stay FP64). */
 typedef enum goss_hip_precision {
   GOSS_HIP_FP64 = 0,
@@ -81,6 +82,7 @@ typedef struct goss_hip_solver_params {
    * forward differences of the model's FP32 right-hand side.  The Newton
    * iteration, its residual, the LU and the substitutions are FP64 in both. */
   int jacobian;
+  double theta; /* ThetaSolver::theta (ThetaSolver.h:37) */
 } goss_hip_solver_params;
 
 #define GOSS_HIP_JACOBIAN_FD 0      /* forward differences of the FP64 right-hand side */
@@ -184,6 +186,14 @@ int goss_hip_system_forward_times_noalloc(goss_hip_system* sys, const double* t,
  * step; row 0 is left to the caller (python/goss/systemsolver.py:130-132). */
 int goss_hip_system_solve(goss_hip_system* sys, double* field_states, const double* t, uint64_t n, int tangled);
 
+/* The solver's public tunables after creation (GRL1::delta, ImplicitODESolver's
+ * eta_0 ..., ODESolver::set_internal_time_step; ODESolver.h:119-128): the
+ * reference's system holds the solver object, so changes made to it apply to
+ * the next forward; here they apply from the next launch.  Per-node solver
+ * state (Newton eta, step-size history) is kept. */
+int goss_hip_system_set_solver_params(goss_hip_system* sys, const goss_hip_solver_params* params);
+int goss_hip_system_get_solver_params(goss_hip_system* sys, goss_hip_solver_params* params);
+
 /* get/set_field_states (ODESystemSolver.cpp:134-160). */
 int goss_hip_system_get_field_states(goss_hip_system* sys, double* out, int tangled);
 int goss_hip_system_set_field_states(goss_hip_system* sys, const double* in, int tangled);
@@ -198,6 +208,26 @@ int goss_hip_system_set_field_parameters(goss_hip_system* sys, const double* in,
 /* Device-pointer variants of the field-state accessors (no PCIe copy). */
 int goss_hip_system_get_field_states_device(goss_hip_system* sys, double* dev_out, int tangled);
 int goss_hip_system_set_field_states_device(goss_hip_system* sys, const double* dev_in, int tangled);
+/* All states, node-major [num_nodes][num_states] FP64, to and from device
+ * memory (no PCIe copy): the saved copies of DOLFINODESystemSolver.save_states
+ * and restore_states (python/goss/dolfinutils.py:782-810).  For the FP32C
+ * precision the FP64 value is the sum of the two parts, and setting it back
+ * restores both exactly. */
+int goss_hip_system_get_states_device(goss_hip_system* sys, double* dev_out);
+int goss_hip_system_set_states_device(goss_hip_system* sys, const double* dev_in);
+/* The operator-splitting exchange of DOLFINODESystemSolver (to_field_states,
+ * from_field_states: python/goss/dolfinutils.py:670-748) on the device.
+ * dofs [num_field_states][num_nodes] (host memory): the index, in a field
+ * vector of num_values doubles, of field state c of node n -- the reference's
+ * dof_maps[label][field_name] (dolfinutils.py:290-296) stacked in field-state
+ * order.  Checked once on the host (every index in [0, num_values), none
+ * repeated) and kept on the device with the system.  gather: field state c of
+ * node n <- values[dofs[c][n]]; scatter: values[dofs[c][n]] <- field state c
+ * of node n.  values is device memory of num_values doubles, which must equal
+ * the dof map's; both calls return after the device work is done. */
+int goss_hip_system_set_field_dofs(goss_hip_system* sys, const int32_t* dofs, uint64_t num_values);
+int goss_hip_system_gather_field_states_device(goss_hip_system* sys, const double* dev_values, uint64_t num_values);
+int goss_hip_system_scatter_field_states_device(goss_hip_system* sys, double* dev_values, uint64_t num_values);
 
 /* states(): all states, node-major [num_nodes][num_states] as in
  * ODESystemSolver::_states (ODESystemSolver.h:74-96), copied out / in. */

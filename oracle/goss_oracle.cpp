@@ -196,12 +196,12 @@ void fd_jacobian(const VariantOps& m, double* y, double t, const double* p, cons
 // Solvers
 // ---------------------------------------------------------------------------
 
-enum SolverKind { EE, RK2, RK4, RL1, RL2, GRL1, GRL2, IMPLICIT_EULER, ESDIRK23A, RKF32, NUM_SOLVERS };
+enum SolverKind { EE, RK2, RK4, RL1, RL2, GRL1, GRL2, IMPLICIT_EULER, ESDIRK23A, RKF32, THETA, NUM_SOLVERS };
 
 int solver_kind(const std::string& s)
 {
   static const char* names[] = {"ExplicitEuler", "RK2", "RK4", "RL1", "RL2", "GRL1", "GRL2",
-                                "ImplicitEuler", "ESDIRK23a", "RKF32"};
+                                "ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver"};
   for (int i = 0; i < NUM_SOLVERS; ++i)
     if (s == names[i]) return i;
   return -1;
@@ -223,6 +223,7 @@ struct SolverParams {
   // AdaptiveExplicitSolver::reset (AdaptiveExplicitSolver.cpp:46-65), set_tol (AdaptiveExplicitSolver.h:71-75)
   double atol = 1.0e-5;
   double rtol = 1.0e-8;
+  double theta = 0.5;  // ThetaSolver.h:37
 };
 
 constexpr double GOSS_EPS = 3.0e-16;  // constants.h:22
@@ -500,6 +501,64 @@ void implicit_euler_forward(const VariantOps& m, const SolverParams& sp, Implici
         throw std::runtime_error(msg.str());
       }
       st.recompute_jacobian = true;
+      st.justrefined = true;
+      num_refinements += 1;
+    }
+  }
+}
+
+// ThetaSolver::forward (ThetaSolver.cpp:48-155): a theta step
+// z = dt ((1 - theta) f(y, t) + theta f(y + z, t + theta dt)) solved by
+// newton_solve with prev = (1 - theta) f(y, t) and alpha = theta.  The
+// public always_recompute_jacobian is set while num_refinements <=
+// num_refinements_without_always_recomputing_jacobian (ThetaSolver.cpp:82-83,
+// the opposite test of ImplicitEuler's), so from the first step on the
+// Jacobian is recomputed at every Newton iteration.
+void theta_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* y, double t, double dt,
+                   const double* p, const double* c, Work& w)
+{
+  const int S = m.S;
+  const double theta = sp.theta;
+  const double t_end = t + dt;
+  const double ldt_0 = sp.ldt;
+  double ldt = ldt_0 > 0 ? ldt_0 : dt;
+  int num_refinements = 0;
+  const double eps = GOSS_EPS * 1000;
+  double* z1 = w.z1.data();
+  double* prev = w.prev.data();
+  double* ft1 = w.ff1.data();
+  for (int i = 0; i < S; ++i) prev[i] = 0.0;
+  while (true) {
+    for (int i = 0; i < S; ++i) z1[i] = 0.0;
+    if (std::abs(theta - 1.0) > GOSS_EPS) {
+      m.eval(y, t, p, c, ft1);
+      for (int i = 0; i < S; ++i) prev[i] = (1 - theta) * ft1[i];
+    }
+    if (num_refinements <= sp.num_refinements_without_always_recomputing_jacobian)
+      st.always_recompute_jacobian = true;
+    const bool step_ok = newton_solve(m, sp, st, z1, prev, y, t + theta * ldt, ldt, theta,
+                                      st.always_recompute_jacobian, p, c, w);
+    if (step_ok) {
+      for (int i = 0; i < S; ++i) y[i] += z1[i];
+      t += ldt;
+      if (std::fabs(t - t_end) < eps) break;
+      if (!st.justrefined) {
+        const double tmp = 2.0 * ldt;
+        if (ldt_0 > 0. && tmp >= ldt_0)
+          ldt = ldt_0;
+        else
+          ldt = tmp;
+      } else {
+        st.justrefined = false;
+      }
+      if ((t + ldt + GOSS_EPS) > t_end) ldt = t_end - t;
+    } else {
+      ldt /= 2.0;
+      if (ldt < sp.min_dt) {
+        std::ostringstream msg;  // ThetaSolver.cpp:131-141
+        msg << "Newtons solver failed to converge as dt become smaller than 'min_dt' " << sp.min_dt << std::endl;
+        throw std::runtime_error(msg.str());
+      }
       st.justrefined = true;
       num_refinements += 1;
     }
@@ -820,12 +879,14 @@ void forward_node(oracle_system* s, unsigned node, double t, double interval, Wo
   double c[kMaxConstants];
   m.constants(prm, c);
   double* y = &s->states[size_t(node) * m.S];
-  if (s->kind == IMPLICIT_EULER || s->kind == ESDIRK23A || s->kind == RKF32) {
+  if (s->kind == IMPLICIT_EULER || s->kind == ESDIRK23A || s->kind == RKF32 || s->kind == THETA) {
     ImplicitState& st = s->per_node_state ? s->node_state[node] : w.carried;
     if (s->kind == IMPLICIT_EULER)
       implicit_euler_forward(m, s->sp, st, y, t, interval, prm, c, w);
     else if (s->kind == ESDIRK23A)
       esdirk_forward(m, s->sp, st, y, t, interval, prm, c, w);
+    else if (s->kind == THETA)
+      theta_forward(m, s->sp, st, y, t, interval, prm, c, w);
     else
       rkf32_forward(m, s->sp, st, y, t, interval, prm, c, w);
     return;
@@ -878,6 +939,7 @@ int set_param(oracle_system* s, const std::string& name, double v)
   if (name == "min_dt") { s->sp.min_dt = v; return 0; }
   if (name == "atol") { s->sp.atol = v; return 0; }
   if (name == "rtol") { s->sp.rtol = v; return 0; }
+  if (name == "theta") { s->sp.theta = v; return 0; }
   if (name == "num_refinements_without_always_recomputing_jacobian") {
     s->sp.num_refinements_without_always_recomputing_jacobian = int(v);
     return 0;

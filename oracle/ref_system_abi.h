@@ -24,6 +24,7 @@
 #include "goss/ImplicitEuler.h"
 #include "goss/ESDIRK23a.h"
 #include "goss/RKF32.h"
+#include "goss/ThetaSolver.h"
 
 
 namespace {
@@ -44,6 +45,7 @@ std::shared_ptr<ODESolver> make_solver(const std::string& s)
   if (s == "ImplicitEuler") return std::make_shared<ImplicitEuler>();
   if (s == "ESDIRK23a") return std::make_shared<ESDIRK23a>();
   if (s == "RKF32") return std::make_shared<RKF32>();
+  if (s == "ThetaSolver") return std::make_shared<ThetaSolver>();
   throw std::runtime_error("unknown solver " + s);
 }
 
@@ -58,6 +60,12 @@ void set_solver_param(ODESolver& s, const std::string& name, double value)
     if (name == "min_dt") { ie->min_dt = value; return; }
     if (name == "num_refinements_without_always_recomputing_jacobian") {
       ie->num_refinements_without_always_recomputing_jacobian = (int)value; return; }
+  }
+  if (auto* th = dynamic_cast<ThetaSolver*>(&s)) {
+    if (name == "theta") { th->theta = value; return; }
+    if (name == "min_dt") { th->min_dt = value; return; }
+    if (name == "num_refinements_without_always_recomputing_jacobian") {
+      th->num_refinements_without_always_recomputing_jacobian = (int)value; return; }
   }
   if (auto* es = dynamic_cast<ESDIRK23a*>(&s)) {
     if (name == "min_dt") { es->min_dt = value; return; }

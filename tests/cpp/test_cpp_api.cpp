@@ -33,17 +33,34 @@ int info()
   } catch (const goss_hip::Error& e) {
     std::cout << "error " << e.what() << "\n";
   }
-  goss_hip::ODESolver grl1("GRL1");
-  goss_hip::ODESolver rkf("RKF32");
+  goss_hip::GRL1 grl1;
+  goss_hip::RKF32 rkf;
   std::cout << "adaptive " << grl1.is_adaptive() << rkf.is_adaptive() << " delta " << grl1.parameters().delta
             << "\n";
+  // Public tunables with the reference's defaults (GRL1.h:39, ImplicitODESolver.h:36-41,
+  // ImplicitEuler.h:36-37, ThetaSolver.h:37, ESDIRK23a.h:37-38) reach the parameter block.
+  grl1.delta = 1e-7;
+  goss_hip::ThetaSolver theta;
+  goss_hip::ESDIRK23a esdirk;
+  std::cout << "tunables " << grl1.parameters().delta << " " << theta.theta << " " << theta.min_dt << " "
+            << theta.eta_0 << " " << theta.kappa << " " << theta.max_iterations << " " << esdirk.min_dt << " "
+            << rkf.get_atol() << " " << rkf.get_iord() << "\n";
+  theta.theta = 0.75;
+  theta.max_iterations = 12;
+  theta.always_recompute_jacobian = true;
+  theta.set_internal_time_step(0.02);
+  const goss_hip_solver_params p = theta.parameters();
+  rkf.set_tol(1e-6, 1e-9);
+  std::cout << "block " << p.theta << " " << p.max_iterations << " " << p.always_recompute_jacobian << " "
+            << p.internal_time_step << " " << rkf.parameters().atol << " " << rkf.parameters().rtol << "\n";
+  std::cout << "solver " << (goss_hip::make_solver("ThetaSolver")->kind() == GOSS_HIP_THETA) << "\n";
   return found ? 0 : 2;
 }
 
 int create()
 {
   auto ode = std::make_shared<goss_hip::ParameterizedODE>("tp06_m_fp3");
-  auto solver = std::make_shared<goss_hip::ODESolver>("GRL1");
+  auto solver = std::make_shared<goss_hip::GRL1>();
   try {
     goss_hip::ODESystemSolver system(64, solver, ode);
     std::cout << "created\n";
@@ -62,7 +79,7 @@ int run(const char* out_path)
   const goss_hip::uint N = 1000;
   auto ode = std::make_shared<goss_hip::ParameterizedODE>("tp06_m_fp3");
   ode->set_parameter("stim_start", 0.5);
-  auto solver = std::make_shared<goss_hip::ODESolver>("GRL1");
+  auto solver = std::make_shared<goss_hip::GRL1>();
   goss_hip::ODESystemSolver system(N, solver, ode);
   if (system.get_parameter("stim_start") != 0.5) return 5;
   const goss_hip::uint NFP = ode->num_field_parameters(), S = ode->num_states(), F = ode->num_field_states();
@@ -87,7 +104,7 @@ int run(const char* out_path)
   // ODESolver::solve on one node (ODESolver.h:78-98) against a one-node system over the same times:
   // the same kernel, so the same states bit for bit.
   {
-    auto single = std::make_shared<goss_hip::ODESolver>("GRL1");
+    auto single = std::make_shared<goss_hip::GRL1>();
     single->attach(ode);
     std::vector<double> y0(S), traj(size_t(41) * S, 0.0), times1(41);
     ode->get_ic(y0.data());
@@ -99,6 +116,71 @@ int run(const char* out_path)
     for (goss_hip::uint j = 0; j < S; ++j)
       if (traj[size_t(40) * S + j] != s1[j] || y0[j] != s1[j]) return 7;
     std::cout << "single-node solve equals the one-node system\n";
+  }
+
+  // The system and a one-node solver read the solver's tunables and the model's parameter
+  // values again at every call: changes made between two calls equal a fresh object created
+  // with them.
+  {
+    auto m1 = std::make_shared<goss_hip::ParameterizedODE>("tp06_m");
+    auto m2 = std::make_shared<goss_hip::ParameterizedODE>("tp06_m");
+    auto a = std::make_shared<goss_hip::GRL1>();
+    a->attach(m1);
+    std::vector<double> ya(S), yb(S);
+    m1->get_ic(ya.data());
+    a->forward(ya.data(), 0.0, 1.0);
+    a->set_internal_time_step(0.01);
+    a->delta = 1e-6;
+    m1->set_parameter("stim_amplitude", 60.0);
+    m1->set_parameter("stim_start", 1.5);
+    a->forward(ya.data(), 1.0, 4.0);
+    auto b = std::make_shared<goss_hip::GRL1>();
+    b->attach(m2);
+    m2->get_ic(yb.data());
+    b->forward(yb.data(), 0.0, 1.0);
+    auto c = std::make_shared<goss_hip::GRL1>();
+    c->set_internal_time_step(0.01);
+    c->delta = 1e-6;
+    m2->set_parameter("stim_amplitude", 60.0);
+    m2->set_parameter("stim_start", 1.5);
+    c->attach(m2);
+    c->forward(yb.data(), 1.0, 4.0);
+    for (goss_hip::uint j = 0; j < S; ++j)
+      if (ya[j] != yb[j]) return 8;
+    if (!(ya[17] > 0)) return 9;  // the stimulus moved to 1.5 ms fired the upstroke
+
+    auto sa = std::make_shared<goss_hip::GRL1>();
+    auto m3 = std::make_shared<goss_hip::ParameterizedODE>("tp06_m");
+    goss_hip::ODESystemSolver s1(8, sa, m3);
+    s1.forward(0.0, 1.0);
+    sa->set_internal_time_step(0.01);
+    m3->set_parameter("stim_start", 1.5);
+    s1.forward(1.0, 4.0);
+    auto sb = std::make_shared<goss_hip::GRL1>();
+    auto m4 = std::make_shared<goss_hip::ParameterizedODE>("tp06_m");
+    goss_hip::ODESystemSolver s2(8, sb, m4);
+    s2.forward(0.0, 1.0);
+    std::vector<double> mid(size_t(8) * S), e1(size_t(8) * S), e2(size_t(8) * S);
+    s2.get_states(mid.data());
+    auto sc = std::make_shared<goss_hip::GRL1>();
+    sc->set_internal_time_step(0.01);
+    auto m5 = std::make_shared<goss_hip::ParameterizedODE>("tp06_m");
+    m5->set_parameter("stim_start", 1.5);
+    goss_hip::ODESystemSolver s3(8, sc, m5);
+    s3.set_states(mid.data());
+    s3.forward(1.0, 4.0);
+    s1.get_states(e1.data());
+    s3.get_states(e2.data());
+    if (e1 != e2) return 10;
+    // Writes through the states() mirror reach the device before the next step.
+    double* w = s1.states();
+    w[17] = -80.0;
+    s1.forward(5.0, 0.1);
+    std::vector<double> chk(size_t(8) * S);
+    s1.get_states(chk.data());
+    for (size_t k = 0; k < chk.size(); ++k)
+      if (chk[k] != w[k]) return 11;
+    std::cout << "tunables and parameters changed between calls apply\n";
   }
 
   std::ofstream f(out_path, std::ios::binary);

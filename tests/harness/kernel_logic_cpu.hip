@@ -137,6 +137,11 @@ int run(int solver, unsigned N, const double* y0, const double* t, unsigned long
           st.failed = 1;
           failed[slot] = 1;
         }
+      } else if (solver == kThetaSolver) {
+        if (!ThetaNode<M>::forward(v, tk, dt, acc, ip, st)) {
+          st.failed = 1;
+          failed[slot] = 1;
+        }
       } else {
         EsdirkNode<M>::forward(v, tk, dt, acc, ip, st);
       }
@@ -213,10 +218,24 @@ extern "C" int kernel_logic_run_fp32_jacobian(const char* variant, int solver, u
                                               double relative_tolerance, double* out_states, long* counters)
 {
   const double min_dt = solver == kESDIRK23a ? 1e-3 : 1e-4;
-  ImplicitParams ip{-1.0, 1.0, 0.1, relative_tolerance, 0.01, min_dt, 30, 0, 2};
+  ImplicitParams ip{-1.0, 1.0, 0.1, relative_tolerance, 0.01, min_dt, 30, 0, 2, 0.5};
   const std::string v(variant);
   if (v == "tp06_m") return run<M_tp06_m, kJacobianFDFP32>(solver, N, y0, t, n, params, ip, out_states, counters);
   if (v == "tp06_epi") return run<M_tp06_epi, kJacobianFDFP32>(solver, N, y0, t, n, params, ip, out_states, counters);
+  return -2;
+}
+
+extern "C" int kernel_logic_run_theta(const char* variant, unsigned N, const double* y0, const double* t,
+                                      unsigned long n, const double* params, double ldt_0, double theta,
+                                      double* out_states, long* counters)
+{
+  ImplicitParams ip{ldt_0, 1.0, 0.1, 1e-12, 0.01, 1e-4, 30, 0, 2, theta};
+  const std::string v(variant);
+  if (v == "tp06_m") return run<M_tp06_m>(kThetaSolver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "tp06_epi") return run<M_tp06_epi>(kThetaSolver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "beeler_reuter_1977")
+    return run<M_beeler_reuter_1977>(kThetaSolver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "fhn") return run<M_fhn>(kThetaSolver, N, y0, t, n, params, ip, out_states, counters);
   return -2;
 }
 
@@ -226,7 +245,7 @@ extern "C" int kernel_logic_run(const char* variant, int solver, unsigned N, con
                                 int always, int num_refinements, double* out_states, long* counters)
 {
   ImplicitParams ip{ldt_0, eta_0, kappa, relative_tolerance, max_rel_prev, min_dt, max_iterations, always,
-                    num_refinements};
+                    num_refinements, 0.5};
   const std::string v(variant);
   if (solver == kRKF32) {
     // atol / rtol arrive in the eta_0 / kappa slots.

@@ -1,5 +1,6 @@
-"""C++ host API (include/goss_hip.hpp): goss_hip::ODESystemSolver, ODESolver and
-ParameterizedODE over the C ABI, driven by tests/cpp/test_cpp_api.cpp.
+"""C++ host API (include/goss_hip.hpp): goss_hip::ODESystemSolver, the solver
+classes with their public tunables and ParameterizedODE over the C ABI, driven
+by tests/cpp/test_cpp_api.cpp.
 
 CPU: the model metadata and solver objects work without a GPU, and building
 a system without one fails loudly (no CPU fallback).  GPU: a TP06 GRL1 run
@@ -29,6 +30,11 @@ def test_cpp_model_and_solver_objects():
     assert "states 19 parameters 53 field_states 1 field_parameters 3 first_state Xr1 g_Kr 0.153" in r.stdout
     assert '"no_such_parameter" is not a parameter in  this ODE.' in r.stdout  # ParameterizedODE.h:185-195
     assert "adaptive 01 delta 1e-08" in r.stdout
+    # GRL1.h:39, ThetaSolver.h:37, ImplicitEuler.h:37, ImplicitODESolver.h:36-41, ESDIRK23a.h:38,
+    # AdaptiveExplicitSolver.cpp:52-53, RKF32 iord 3.
+    assert "tunables 1e-07 0.5 0.0001 1 0.1 30 0.001 1e-05 3" in r.stdout
+    assert "block 0.75 12 1 0.02 1e-06 1e-09" in r.stdout
+    assert "solver 1" in r.stdout
 
 
 def test_cpp_system_without_gpu_fails_loudly():
@@ -52,6 +58,8 @@ def test_cpp_run_equals_python_binding_and_oracle(tmp_path):
     out = tmp_path / "cpp.bin"
     r = subprocess.run([_exe(), "run", str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    # Tunables and model parameters changed between calls apply to the next call.
+    assert "tunables and parameters changed between calls apply" in r.stdout
     raw = np.fromfile(out, dtype=np.float64)
     N, S, F = (int(x) for x in raw[:3])
     NFP = 3

@@ -144,3 +144,30 @@ def test_batch_rejects_what_it_cannot_run():
     b2 = DeviceBatch([d, e])
     with pytest.raises(GossHipError, match="permutation"):
         b2.config = Config(256, 1, 1.0, (1, 1))
+
+
+def test_batch_fp32_populations_against_the_fp64_oracle():
+    """The batch's FitzHugh-Nagumo (RK4) and TP06 epi (GRL1) populations after 20 ms in FP32,
+    against the FP64 oracle: V within 1e-3 of its range for FitzHugh-Nagumo and within
+    0.05 mV for TP06 epi."""
+    import oracle
+    from goss_amd.batch import DeviceBatch
+
+    sizes = [256, 256, 512, 32]
+    systems, y0 = _make(sizes, 21)
+    b = DeviceBatch(systems)
+    b.forward(0.0, 20.0, _dts())
+    b.synchronize()
+    for k in (1, 2):
+        v, solver, dt = POPS[k]
+        o = oracle.OracleSystem(v, solver, sizes[k])
+        o.set_num_threads(8)
+        o.set_states(y0[k])
+        o.forward_times(np.arange(int(round(20.0 / dt)) + 1) * dt)
+        got, ref = systems[k].states(), o.states()
+        vi = 17 if v == "tp06_epi" else 0
+        dv = np.abs(got[:, vi] - ref[:, vi]).max()
+        span = np.ptp(np.concatenate([y0[k][:, vi], ref[:, vi]]))
+        print(f"batch {v} {solver} FP32 vs FP64 oracle: max |dV| {dv:.3e} (V range {span:.3e})")
+        assert dv <= (0.05 if v == "tp06_epi" else 1e-3 * span)
+    b.close()

@@ -28,10 +28,14 @@ def dev_system(*a, **k):
     return DeviceSystem(*a, **k)
 
 
-def assert_parity(a, b, rtol=RTOL, what=""):
+def assert_parity(a, b, rtol=RTOL, what="", scale=None):
+    """max over nodes and states of |a - b| / max(|b|, 1e-6 * scale), scale per state: the
+    largest |b| over the nodes unless given (a state's magnitude over the run)."""
     assert a.shape == b.shape
     assert np.all(np.isfinite(a)) == np.all(np.isfinite(b))
-    floor = 1e-6 * np.max(np.abs(b), axis=0, keepdims=True)
+    if scale is None:
+        scale = np.max(np.abs(b), axis=0, keepdims=True)
+    floor = 1e-6 * np.asarray(scale)
     err = np.abs(a - b) / np.maximum(np.abs(b), np.maximum(floor, 1e-300))
     worst = np.unravel_index(np.argmax(err), err.shape)
     assert err.max() <= rtol, f"{what}: max rel err {err.max():.3e} at {worst}: {a[worst]} vs {b[worst]}"
@@ -271,3 +275,34 @@ def test_full_size_closed_form_linear_decay():
         dev.forward_times(np.linspace(0, 3.0, 301))
         exact = (y0 + b / a) * np.exp(a * 3.0) - b / a
         np.testing.assert_allclose(dev.states()[:, 0], exact, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", ["fp32c", "fp32"])
+@pytest.mark.parametrize("solver,variant,dt,t_end,N,max_dv", [
+    ("RL2", "tp06_m_fp3", 0.02, 100.0, 512, 0.05),
+    ("RK4", "tp06_m", 0.001, 30.0, 64, 0.05),
+    ("ExplicitEuler", "tp06_m", 0.001, 30.0, 64, 0.05),
+])
+def test_fp32_solvers_against_the_fp64_oracle(prec, solver, variant, dt, t_end, N, max_dv):
+    """FP32 and compensated FP32 for the explicit schemes besides GRL1, against the FP64 oracle:
+    RL2 over 100 ms (one action potential) at dt 0.02 ms, RK4 and ExplicitEuler through the
+    stimulus and upstroke at dt 0.001 ms (within their stability limit for TP06's fastest
+    gate).  Bound on max |dV| at the end, in mV; every state within 1e-3 relative of its
+    largest magnitude."""
+    o = oracle.OracleSystem(variant, solver, N)
+    o.set_num_threads(8)
+    dev = dev_system(variant, solver, N, prec)
+    y0 = perturbed(o.states(), 61)
+    o.set_states(y0)
+    dev.set_states(y0)
+    t = np.arange(int(round(t_end / dt)) + 1) * dt
+    o.forward_times(t)
+    dev.forward_times(t)
+    a, b = dev.states(), o.states()
+    if t_end < 50:
+        assert b[:, 17].min() > -20.0  # the 30 ms runs end on the plateau, after the upstroke
+    dv = np.abs(a[:, 17] - b[:, 17]).max()
+    rel = (np.abs(a - b) / np.max(np.abs(b), axis=0, keepdims=True)).max()
+    print(f"{solver} {prec}: max |dV| {dv:.3e} mV, max state error {rel:.3e} of the state's magnitude")
+    assert dv <= max_dv
+    assert rel <= 1e-3

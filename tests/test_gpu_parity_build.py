@@ -31,7 +31,7 @@ ROOT = os.path.dirname(HERE)
 DEV_LIB = os.path.join(HERE, "harness", "libgoss_hip_parity.so")
 HOST_LIB = os.path.join(HERE, "harness", "libkernel_logic_cpu_parity.so")
 META = os.path.join(ROOT, "goss_amd", "csrc", "generated", "variants.json")
-SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8, "RKF32": 9}
+SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8, "RKF32": 9, "ThetaSolver": 10}
 
 
 def _libs():
@@ -44,6 +44,8 @@ def _libs():
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp,
                                    ctypes.POINTER(ctypes.c_long)]
+    L.kernel_logic_run_theta.argtypes = [ctypes.c_char_p, ctypes.c_uint, dp, dp, ctypes.c_ulong, dp, ctypes.c_double,
+                                         ctypes.c_double, dp, ctypes.POINTER(ctypes.c_long)]
     return L
 
 
@@ -55,7 +57,10 @@ def _run_host(variant, solver, y0, t, params, relative_tolerance, atol=1e-5, rto
     L = _libs()
     out = np.zeros_like(y0)
     counters = (ctypes.c_long * 4)()
-    if solver == "RKF32":  # atol / rtol travel in the eta_0 / kappa slots of the harness call
+    if solver == "ThetaSolver":
+        rc = L.kernel_logic_run_theta(variant.encode(), y0.shape[0], _p(y0), _p(t), len(t), _p(params), -1.0, 0.5,
+                                      _p(out), counters)
+    elif solver == "RKF32":  # atol / rtol travel in the eta_0 / kappa slots of the harness call
         rc = L.kernel_logic_run(variant.encode(), 9, y0.shape[0], _p(y0), _p(t), len(t), _p(params), -1.0, atol, rtol,
                                 0.0, 0.0, 0.0, 0, 0, 0, _p(out), counters)
     else:
@@ -91,6 +96,8 @@ def _run_device(variant, solver, y0, t, params, relative_tolerance):
     # TP06 M-cell with its default stimulus, through the upstroke.
     ("tp06_m", "ImplicitEuler", 256, 0.1, 400, None, 1e-12),
     ("tp06_m", "ESDIRK23a", 256, 0.1, 400, None, None),
+    ("tp06_m", "ThetaSolver", 256, 0.1, 400, None, None),
+    ("tp06_epi", "ThetaSolver", 1024, 0.1, 1000, 52.0, None),
     ("tp06_m", "RKF32", 128, 0.1, 200, None, None),
     ("lorentz", "RKF32", 1024, 0.02, 500, None, None),
 ])

@@ -173,3 +173,31 @@ def test_step_adjoint_logic_matches_torch_autograd(solver, fp64, tol):
         err_y = np.abs(lp - ref_y).max() / np.abs(ref_y).max()
         err_p = np.abs(pb - ref_p).max() / np.abs(ref_p).max()
         assert err_y <= tol and err_p <= tol, (solver, fp64, V, err_y, err_p)
+
+
+@pytest.mark.parametrize("theta", [0.5, 1.0, 0.7])
+@pytest.mark.parametrize("variant,dt,nsteps,ldt", [("tp06_m", 0.05, 300, -1.0), ("beeler_reuter_1977", 0.1, 100, -1.0),
+                                                   ("fhn", 0.5, 60, -1.0), ("tp06_epi", 0.1, 60, 0.05)])
+def test_theta_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, ldt, theta):
+    """ThetaSolver (ThetaSolver.cpp:48-155): the kernel's per-node logic on the host against the oracle's
+    per-node mode."""
+    L = _harness()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.kernel_logic_run_theta.argtypes = [ctypes.c_char_p, ctypes.c_uint, dp, dp, ctypes.c_ulong, dp, ctypes.c_double,
+                                         ctypes.c_double, dp, ctypes.POINTER(ctypes.c_long)]
+    S = len(json.load(open(META))[variant]["state_names"])
+    o = oracle.OracleSystem(variant, "ThetaSolver", 7, {"internal_time_step": ldt, "theta": theta},
+                            per_node_state=True)
+    rng = np.random.Generator(np.random.PCG64(44))
+    y0 = o.states() * (1.0 + rng.uniform(-0.01, 0.01, (7, S)))
+    t = np.arange(nsteps + 1) * dt
+    o.set_states(y0)
+    o.forward_times(t)
+    params = np.ascontiguousarray(json.load(open(META))[variant]["default_parameters"], dtype=np.float64)
+    out = np.zeros_like(y0)
+    counters = (ctypes.c_long * 4)()
+    rc = L.kernel_logic_run_theta(variant.encode(), 7, _p(y0), _p(t), len(t), _p(params), ldt, theta, _p(out),
+                                  counters)
+    assert rc == 0
+    np.testing.assert_array_equal(out, o.states())
+    np.testing.assert_array_equal(np.array(list(counters))[1:3], o.counters()[1:3])

@@ -21,7 +21,7 @@ import pytest
 import oracle
 
 EXPLICIT = ["ExplicitEuler", "RK2", "RK4", "RL1", "RL2", "GRL1", "GRL2"]
-ALL = EXPLICIT + ["ImplicitEuler", "ESDIRK23a", "RKF32"]
+ALL = EXPLICIT + ["ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver"]
 
 needs_refpin = pytest.mark.skipif(not oracle.refpin_available(),
                                   reason="reference build oracle/_ref/libgoss_refpin.so not present")
@@ -31,9 +31,10 @@ needs_refpin = pytest.mark.skipif(not oracle.refpin_available(),
 CASES = [
     ("ref_fhn", ALL, 0.1, 400),
     ("ref_beeler_reuter_1977", ALL, 0.05, 400),
-    ("ref_winslow", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a"], 0.01, 200),
+    ("ref_winslow", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "ThetaSolver"], 0.01, 200),
     ("ref_winslow", ["RKF32"], 0.01, 20),
-    ("ref_panfilov", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "RKF32"], 0.1, 200),
+    ("ref_panfilov", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver"], 0.1,
+     200),
 ]
 
 
@@ -85,7 +86,30 @@ def test_oracle_solvers_equal_reference_on_reference_models(model, solvers, dt, 
 
 
 @needs_refpin
+@pytest.mark.parametrize("theta", [1.0, 0.7])
+@pytest.mark.parametrize("model", ["ref_fhn", "ref_beeler_reuter_1977", "ref_winslow"])
+def test_theta_solver_tunable_theta(model, theta):
+    """ThetaSolver with theta != 0.5, serially.  (The threaded reference runs
+    copies of the solver, and ThetaSolver's copy constructor (ThetaSolver.cpp:
+    27-31) does not copy the public theta, min_dt and refinement count, nor
+    ImplicitODESolver's tunables: the copies run with the defaults.)"""
+    oracle.register_reference_models()
+    v = oracle.Variant(model)
+    y0, fp = _inputs(v, 6, 3)
+    t = np.arange(61) * 0.1
+    o = oracle.OracleSystem(model, "ThetaSolver", 6, {"theta": theta})
+    r = oracle.RefSystem(model, "ThetaSolver", 6, {"theta": theta})
+    for s in (o, r):
+        s.set_states(y0)
+        if fp is not None:
+            s.set_field_parameters(fp)
+        s.forward_times(t)
+    assert np.array_equal(o.states(), r.states())
+
+
+@needs_refpin
 @pytest.mark.parametrize("model,solver,dt,steps", [
+    ("ref_fhn", "ThetaSolver", 0.5, 60), ("ref_winslow", "ThetaSolver", 0.05, 40),
     ("ref_fhn", "ImplicitEuler", 0.5, 60), ("ref_fhn", "ESDIRK23a", 0.5, 60), ("ref_fhn", "RKF32", 0.5, 60),
     ("ref_beeler_reuter_1977", "ImplicitEuler", 0.2, 60), ("ref_beeler_reuter_1977", "ESDIRK23a", 0.2, 60),
     ("ref_winslow", "ImplicitEuler", 0.05, 40), ("ref_winslow", "ESDIRK23a", 0.05, 40),

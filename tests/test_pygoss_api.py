@@ -1,0 +1,213 @@
+"""The goss Python API of goss_amd without device compute: models, field
+selections, monitored names and the solver classes.  Restates the
+expectations of the reference's tests/test_parameterized_ode_bindings.py and
+tests/test_ode_solver_bindings.py that need no GPU (the computing ones are in
+tests/test_gpu_pygoss.py)."""
+import numpy as np
+import pytest
+
+import goss_amd as goss
+from goss_amd import solvers
+
+OSC = "oscilator.ode"  # goss_amd/odes/oscilator.ode, the reference's tests/oscilator.ode
+
+
+@pytest.mark.parametrize("field_states", [[], ["x"], ["y"], ["x", "y"]])
+def test_parameterized_ode_field_states(field_states):
+    ode = goss.ParameterizedODE(OSC, field_states=field_states)
+    assert ode.num_field_states == len(field_states)
+    assert ode.field_state_names == field_states
+    assert ode.field_state_indices == [ode.state_names.index(s) for s in field_states]
+    assert ode.num_field_parameters == 0
+    assert ode.num_states == 2
+    assert ode.num_parameters == 2
+    assert ode.num_monitored == 0
+    assert ode.state_names == ["x", "y"]
+    assert ode.parameter_names == ["a", "b"]
+    assert ode.field_parameter_names == []
+    assert ode.monitored_names == []
+
+
+@pytest.mark.parametrize("field_parameters", [[], ["a"], ["b"], ["a", "b"]])
+def test_parameterized_ode_field_parameters(field_parameters):
+    ode = goss.ParameterizedODE(OSC, field_parameters=field_parameters)
+    assert ode.num_field_parameters == len(field_parameters)
+    assert ode.field_parameter_names == field_parameters
+    assert ode.num_field_states == 0
+    assert ode.field_state_names == []
+    assert ode.field_state_indices == []
+    assert ode.num_monitored == 0
+
+
+def test_compiled_variant_is_found_for_its_model_and_selection():
+    # The library's oscilator variant has field states x, y and field parameters a, b.
+    ode = goss.ParameterizedODE(OSC, field_states=["x", "y"], field_parameters=["a", "b"])
+    assert ode.variant == "oscilator"
+    # Other selections are compiled at run time, once.
+    other = goss.ParameterizedODE(OSC, field_parameters=["a"])
+    assert other.variant.startswith("jit_")
+    assert goss.ParameterizedODE(OSC, field_parameters=["a"]).variant == other.variant
+
+
+def test_parameterized_ode_parameters():
+    ode = goss.ParameterizedODE(OSC)
+    assert ode.num_parameters == 2
+    assert np.isclose(ode.get_parameter("a"), 1.0)
+    assert np.isclose(ode.get_parameter("b"), 1.0)
+    ode.set_parameter("a", 42.0)
+    assert np.isclose(ode.get_parameter("a"), 42.0)
+    assert ode.parameters == {"a": 42.0, "b": 1.0}
+    with pytest.raises(RuntimeError, match="is not a parameter"):
+        ode.get_parameter("c")
+
+
+def test_empty_parameterized_ode():
+    ode = goss.ParameterizedODE(num_states=3, num_parameters=1)
+    assert ode.num_states == 3 and ode.num_parameters == 1 and ode.num_monitored == 0
+
+
+def test_invalid_field_selection_and_monitored():
+    with pytest.raises(ValueError):
+        goss.ParameterizedODE(OSC, field_states=["a"])
+    with pytest.raises(ValueError):
+        goss.ParameterizedODE(OSC, field_parameters=["x"])
+    with pytest.raises(ValueError):
+        goss.ParameterizedODE(OSC, monitored=["invalid_parameter"])
+
+
+def test_monitored_names():
+    ode = goss.ParameterizedODE(OSC, monitored=["energy"])
+    assert ode.num_monitored == 1 and ode.monitored_names == ["energy"]
+
+
+def test_get_ic_and_copy():
+    ode = goss.ParameterizedODE(OSC)
+    assert np.array_equal(ode.get_ic(), [1.0, 0.0])
+    c = ode.copy()
+    c.set_parameter("b", 3.0)
+    assert ode.get_parameter("b") == 1.0 and c.get_parameter("b") == 3.0
+
+
+@pytest.mark.parametrize("Solver", solvers.GOSSSolvers)
+def test_solver_constructor_and_attach(Solver):
+    cls = solvers.solver_mapper[Solver.name]
+    ode = goss.ParameterizedODE(OSC)
+    assert cls(ode).num_states == 2
+    s = cls()
+    assert s.num_states == 0
+    s.attach(ode)
+    assert s.num_states == 2
+    assert s.get_ode().num_states == 2
+
+
+def test_invalid_argument_constructor():
+    with pytest.raises(ValueError):
+        solvers.ExplicitEuler("invalid_argument")
+
+
+@pytest.mark.parametrize("Solver", solvers.GOSSNonAdaptiveSolvers)
+def test_is_adaptive_false(Solver):
+    assert solvers.solver_mapper[Solver.name]().is_adaptive is False
+
+
+@pytest.mark.parametrize("Solver", solvers.GOSSAdaptiveSolvers)
+def test_is_adaptive_true(Solver):
+    assert solvers.solver_mapper[Solver.name]().is_adaptive is True
+
+
+def _new_values(cls, parameters):
+    out = {}
+    for name, default_value in cls.default_parameters().items():
+        if isinstance(default_value, bool):
+            out[name] = not parameters[name]
+        elif isinstance(default_value, int):
+            out[name] = 42
+        else:
+            out[name] = 42.0
+    return out
+
+
+@pytest.mark.parametrize("Solver", solvers.GOSSSolvers)
+def test_set_and_update_parameters(Solver):
+    cls = solvers.solver_mapper[Solver.name]
+    s = cls()
+    for name, value in _new_values(cls, s.parameters).items():
+        s.set_parameter(name, value)
+        assert np.isclose(s.parameters[name], value), name
+    t = cls()
+    new = _new_values(cls, t.parameters)
+    t.update_parameters(new)
+    for name, value in new.items():
+        assert np.isclose(t.get_parameter(name), value), name
+
+
+def test_set_invalid_parameter_type_and_name():
+    s = solvers.ThetaSolver()
+    with pytest.raises(TypeError):
+        s.set_parameter("theta", "hello")
+    with pytest.raises(KeyError):
+        s.set_parameter("invalid_parameter", 42.0)
+
+
+def test_default_parameters_are_the_reference_defaults():
+    # GRL1.h:39, ImplicitODESolver.h:36-41, ImplicitEuler.h:36-37, ThetaSolver.h:35-37, ESDIRK23a.h:37-38.
+    assert solvers.GRL1.default_parameters() == {"delta": 1e-8}
+    ie = solvers.ImplicitEuler.default_parameters()
+    assert ie["eta_0"] == 1.0 and ie["kappa"] == 0.1 and ie["relative_tolerance"] == 1e-12
+    assert ie["max_iterations"] == 30 and ie["max_relative_previous_residual"] == 0.01
+    assert ie["always_recompute_jacobian"] is False and ie["min_dt"] == 0.0001
+    assert solvers.ThetaSolver.default_parameters()["theta"] == 0.5
+    assert solvers.ESDIRK23a.default_parameters()["min_dt"] == 0.001
+    assert solvers.RKF32().iord == 3 and solvers.RKF32().atol == 1e-5
+
+
+def test_internal_time_step_and_device_params():
+    s = solvers.GRL1()
+    assert s.internal_time_step == -1.0
+    s.internal_time_step = 0.001
+    s.set_parameter("delta", 1e-7)
+    d = s.device_params()
+    assert d == {"internal_time_step": 0.001, "delta": 1e-7}
+    r = solvers.RKF32()
+    r.set_tol(1e-6, 1e-9)
+    assert r.device_params()["atol"] == 1e-6 and r.device_params()["rtol"] == 1e-9
+
+
+def test_has_openmp_reports_no_host_threads():
+    assert goss.has_openmp() is False
+
+
+@pytest.mark.parametrize("path,fs,fp,compiled", [
+    ("oscilator.ode", ["x", "y"], ["a", "b"], "oscilator"),
+    ("tentusscher_panfilov_2006_M_cell.ode", ["V"], ["g_Kr", "g_Ks", "g_CaL"], "tp06_m_fp3"),
+    ("beeler_reuter_1977.ode", ["V"], [], "beeler_reuter_1977"),
+])
+def test_run_time_metadata_equals_the_compiled_variant(path, fs, fp, compiled):
+    """A run-time variant is described by the code generator before it is
+    compiled; for a selection that is also compiled in, that description
+    equals the library's."""
+    import ctypes
+
+    from goss_amd import jit
+    from goss_amd._lib import check, lib
+    from goss_amd.ode import _resolve_path
+    from goss_amd.system import ModelVariant
+
+    meta = jit.metadata(str(_resolve_path(path)), fs, fp)
+    ref = ModelVariant(compiled)
+    for attr in ("num_states", "num_parameters", "num_field_states", "num_field_parameters", "state_names",
+                 "parameter_names", "field_state_names", "field_parameter_names", "field_state_indices",
+                 "field_parameter_indices", "linear_terms"):
+        assert getattr(meta, attr) == getattr(ref, attr), attr
+    np.testing.assert_array_equal(meta.default_parameters, ref.default_parameters)
+    np.testing.assert_array_equal(meta.initial_conditions, ref.initial_conditions)
+    n = ctypes.c_int()
+    names = ctypes.POINTER(ctypes.c_char_p)()
+    check(lib().goss_hip_model_intermediates(compiled.encode(), ctypes.byref(n), ctypes.byref(names)))
+    assert meta.intermediate_names == [names[i].decode() for i in range(n.value)]
+
+
+def test_run_time_variant_compiles_at_first_device_use_only():
+    ode = goss.ParameterizedODE(OSC, field_states=["y"])
+    assert ode.variant.startswith("jit_") and ode._pending is not None
+    assert ode.num_field_states == 1 and ode.field_state_indices == [1]
