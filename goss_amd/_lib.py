@@ -25,6 +25,8 @@ SOLVERS = {
     "ESDIRK23a": 8,
     "RKF32": 9,
     "ThetaSolver": 10,
+    "BasicImplicitEuler": 11,
+    "ESDIRK4O32": 12,
 }
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp32c": 2}
 JACOBIANS = {"fd": 0, "fd_fp32": 1}

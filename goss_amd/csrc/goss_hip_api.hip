@@ -136,7 +136,11 @@ const VariantInfo* find_variant(const char* name)
 }
 
 bool is_explicit(int solver) { return solver >= 0 && solver <= kGRL2; }
-bool is_implicit(int solver) { return solver == kImplicitEuler || solver == kESDIRK23a || solver == kThetaSolver; }
+bool is_implicit(int solver)
+{
+  return solver == kImplicitEuler || solver == kESDIRK23a || solver == kThetaSolver || solver == kBasicImplicitEuler ||
+         solver == kESDIRK4O32;
+}
 // Solvers with per-node state buffers (ds / is / error flag).
 bool has_node_state(int solver) { return is_implicit(solver) || solver == kRKF32; }
 
@@ -311,6 +315,8 @@ struct goss_hip_system {
     ip.always_recompute_jacobian = sp.always_recompute_jacobian;
     ip.num_refinements = sp.num_refinements_without_always_recomputing_jacobian;
     ip.theta = sp.theta;
+    ip.atol = sp.atol;
+    ip.rtol = sp.rtol;
     return ip;
   }
 
@@ -342,6 +348,7 @@ struct goss_hip_system {
     }
     timing_end();
     if (failed()) {
+      if (solver == kESDIRK4O32) throw Error(GOSS_HIP_ESOLVER, "ESDIRK4O32 did not reach the end of the interval");
       std::ostringstream msg;
       msg << "Newtons solver failed to converge as dt become smaller than 'min_dt' " << sp.min_dt << std::endl;
       throw Error(GOSS_HIP_ESOLVER, msg.str());
@@ -581,7 +588,7 @@ int goss_hip_solver_params_default(int solver, goss_hip_solver_params* out)
 int goss_hip_solver_is_adaptive(int solver)
 {
   // AdaptiveExplicitSolver.h / AdaptiveImplicitSolver.h:116-119.
-  return solver == kESDIRK23a || solver == kRKF32;
+  return solver == kESDIRK23a || solver == kRKF32 || solver == kESDIRK4O32;
 }
 
 // ---------------------------------------------------------------------------

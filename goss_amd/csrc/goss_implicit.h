@@ -1,6 +1,8 @@
 // Device kernels of the implicit solvers: ImplicitEuler (ImplicitEuler.cpp:57-152
-// over ImplicitODESolver::newton_solve, ImplicitODESolver.cpp:114-245) and
-// ESDIRK23a (ESDIRK23a.cpp:101-316).
+// over ImplicitODESolver::newton_solve, ImplicitODESolver.cpp:114-245),
+// ThetaSolver (ThetaSolver.cpp:48-155), BasicImplicitEuler
+// (BasicImplicitEuler.cpp:62-181), ESDIRK23a (ESDIRK23a.cpp:101-316) and
+// ESDIRK4O32 (ESDIRK4O32.cpp:92-298 over AdaptiveImplicitSolver.cpp:68-200).
 //
 // One node per lane.  Each node owns its solver state (Newton eta, Jacobian
 // reuse flags, ESDIRK step-size history, held in registers during a launch),
@@ -89,6 +91,7 @@ struct ImplicitParams {
   int always_recompute_jacobian;
   int num_refinements;
   double theta;  // ThetaSolver.h:37
+  double atol, rtol;  // AdaptiveImplicitSolver::set_tol (AdaptiveImplicitSolver.h), ESDIRK4O32's error test
 };
 
 // Per-node solver state, structure of arrays [field][ld] (shared by the
@@ -100,6 +103,10 @@ enum DState : int {
   kPrevDt,    // ESDIRK23a::prev_dt
   kRkLdt,     // AdaptiveExplicitSolver::_ldt (RKF32 initial step guess)
   kFacmax,    // AdaptiveExplicitSolver::facmax
+  kAiLdt,     // AdaptiveImplicitSolver::_ldt (ESDIRK4O32's step guess; -1 after reset)
+  kAiFacmax,  // AdaptiveImplicitSolver::facmax
+  kErrOld,    // AdaptiveImplicitSolver::err_old
+  kDtOld,     // AdaptiveImplicitSolver::dt_old
   kNumDState
 };
 enum IState : int {
@@ -120,16 +127,24 @@ enum IState : int {
 // Work vectors per node (each S rows of the work buffer).
 enum EsdirkWork : int { kEsZ1 = 0, kEsZ2, kEsZ3, kEsZ4, kEsPrev, kEsF1, kEsDZ, kNumEsdirkWork };
 enum EulerWork : int { kIeZ = 0, kIeYZ, kIeB, kIeF1, kIePrev, kNumEulerWork };  // kIePrev: ThetaSolver only
+// ESDIRK4O32: the Newton rows of EulerWork (kIeZ holds the stage being solved, z4 itself), then its own.
+enum Esdirk4Work : int { kE4Z1 = kNumEulerWork, kE4Z2, kE4Z3, kE4Yn, kE4Yh, kNumEsdirk4Work };
 
 __host__ __device__ constexpr int implicit_work_vectors(int solver)
 {
-  return solver == kESDIRK23a ? int(kNumEsdirkWork) : int(kNumEulerWork);
+  return solver == kESDIRK23a ? int(kNumEsdirkWork) : (solver == kESDIRK4O32 ? int(kNumEsdirk4Work) : int(kNumEulerWork));
 }
+
+constexpr int kNumImplicitSlots = 5;
 
 // Slot of an implicit solver in VariantInfo::implicit_launch.
 __host__ __device__ constexpr int implicit_slot(int solver)
 {
-  return solver == kImplicitEuler ? 0 : (solver == kESDIRK23a ? 1 : 2);
+  return solver == kImplicitEuler ? 0
+         : solver == kESDIRK23a   ? 1
+         : solver == kThetaSolver ? 2
+         : solver == kBasicImplicitEuler ? 3
+                                         : 4;
 }
 
 struct ImplicitBuffers {
@@ -318,7 +333,8 @@ struct Implicit {
 // Per-node solver state, loaded into registers for a launch.
 struct NodeImplicitState {
   double eta, es_eta, loc_error, prev_dt;
-  int recompute, newton_its, always, justrefined, first_step, new_jacobian, jac_comp, rejects, accepted, failed;
+  int recompute, newton_its, always, justrefined, first_step, new_jacobian, jac_comp, rejects, accepted, failed,
+      nfevals;
 };
 
 GOSS_HD GOSS_INLINE NodeImplicitState load_state(const ImplicitBuffers& B, unsigned ld, unsigned n)
@@ -338,6 +354,7 @@ GOSS_HD GOSS_INLINE NodeImplicitState load_state(const ImplicitBuffers& B, unsig
   s.rejects = B.is[size_t(kRejects) * ld + n];
   s.accepted = B.is[size_t(kAccepted) * ld + n];
   s.failed = B.is[size_t(kFailed) * ld + n];
+  s.nfevals = B.is[size_t(kNfevals) * ld + n];
   return s;
 }
 
@@ -357,6 +374,7 @@ GOSS_HD GOSS_INLINE void store_state(const ImplicitBuffers& B, unsigned ld, unsi
   B.is[size_t(kRejects) * ld + n] = s.rejects;
   B.is[size_t(kAccepted) * ld + n] = s.accepted;
   B.is[size_t(kFailed) * ld + n] = s.failed;
+  B.is[size_t(kNfevals) * ld + n] = s.nfevals;
 }
 
 // ---------------------------------------------------------------------------
@@ -555,6 +573,76 @@ struct ThetaNode {
 };
 
 // ---------------------------------------------------------------------------
+// BasicImplicitEuler
+// ---------------------------------------------------------------------------
+template <class M>
+struct BasicImplicitEulerNode {
+  static constexpr int S = M::num_states;
+  using I = Implicit<M>;
+
+  // BasicImplicitEuler::forward (BasicImplicitEuler.cpp:62-181) with its
+  // compute_factorized_jacobian (:46-60): full Newton, a new Jacobian at every
+  // iteration, on the residual (y - y_prev) - dt f(y, t + ldt).  As in the
+  // reference the residual and the clock advance use the whole interval dt and
+  // the Jacobian the sub-step ldt (equal with the default internal time step).
+  // A sub-step whose iteration does not converge within max_iterations keeps
+  // its last iterate and is counted as rejected (the reference prints a
+  // message and goes on).
+  template <class A>
+  GOSS_HD static void forward(const NodeViews& v, const double t, const double dt, const A& acc,
+                              const ImplicitParams& p, NodeImplicitState& st)
+  {
+    const VecView prev = v.w(kIePrev), bv = v.w(kIeB), f1 = v.w(kIeF1);
+    const double ldt_0 = p.ldt_0;
+    const unsigned long nsteps = ldt_0 > 0 ? (unsigned long)ceil(dt / ldt_0 - 1.0E-12) : 1UL;
+    const double ldt = dt / nsteps;
+    const double rtol = p.relative_tolerance;
+    double lt = t;
+#pragma unroll 1
+    for (unsigned long step = 0; step < nsteps; ++step) {
+      double relative_residual = 1.0, initial_residual = 1.0, residual;
+      bool converged = false;
+      st.newton_its = 0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) prev[i] = v.y[i];
+#pragma unroll 1
+      while (!converged && st.newton_its < p.max_iterations) {
+        {
+          double y[S], f[S];
+#pragma unroll
+          for (int i = 0; i < S; ++i) y[i] = v.y[i];
+          M::template eval<double>(y, lt + ldt, acc, f);
+          double b[S];
+#pragma unroll
+          for (int i = 0; i < S; ++i) {
+            b[i] = (y[i] - prev[i]) * 1.0 - dt * f[i];
+            bv[i] = b[i];
+          }
+          residual = I::norm(b);
+        }
+        if (st.newton_its == 0) initial_residual = residual;
+        relative_residual = residual / initial_residual;
+        if (relative_residual < rtol) {
+          converged = true;
+          break;
+        }
+        I::jacobian(v.y, lt + ldt, acc, f1, v.J);
+        I::factorize(v.J, v.F, -ldt * 1.0);
+        st.jac_comp += 1;
+        I::fb_subst(v.F, bv);  // bv <- dz
+#pragma unroll
+        for (int i = 0; i < S; ++i) v.y[i] -= bv[i];
+        ++st.newton_its;
+      }
+      if (converged)
+        lt += dt;
+      else
+        st.rejects++;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
 // ESDIRK23a
 // ---------------------------------------------------------------------------
 struct EsdirkCoefficients {
@@ -563,7 +651,7 @@ struct EsdirkCoefficients {
 
 __host__ __device__ inline EsdirkCoefficients esdirk_coefficients()
 {
-  // ESDIRK23a.cpp:27-44
+  // ESDIRK23a.cpp:27-44; ESDIRK4O32 uses the same tableau (ESDIRK4O32.cpp:29-37).
   EsdirkCoefficients k;
   const double gamma = 0.43586652150845899942;
   k.gamma = gamma;
@@ -579,6 +667,261 @@ __host__ __device__ inline EsdirkCoefficients esdirk_coefficients()
   k.c4 = 1.0;
   return k;
 }
+
+// ---------------------------------------------------------------------------
+// ESDIRK4O32
+// ---------------------------------------------------------------------------
+// Passes of the ESDIRK4O32 step loop after which a node gives up: the
+// reference loops until it reaches the interval end, which a non-finite error
+// or a step size halved to zero never does.
+constexpr long kEsdirk4MaxPasses = 1000000;
+
+template <class M>
+struct Esdirk4o32Node {
+  static constexpr int S = M::num_states;
+  using I = Implicit<M>;
+  using E = ImplicitEulerNode<M>;
+  // AdaptiveImplicitSolver constants (AdaptiveImplicitSolver.cpp:43-66) and _iord (ESDIRK4O32.cpp:38).
+  static constexpr double facmin = 0.5, facmaxb = 2.0, stabfac = 0.9, stabdown = 1.0, stabup = 1.2;
+  static constexpr double iord = 3.0;
+
+  // AdaptiveImplicitSolver::dtinit (AdaptiveImplicitSolver.cpp:68-128): f0 = f(y0, t) is given in z1;
+  // y1, f1 are scratch rows.
+  template <class A>
+  GOSS_HD static double dtinit(const NodeViews& v, const double t, const A& acc, const ImplicitParams& p)
+  {
+    const VecView f0 = v.w(kE4Z1), y1 = v.w(kE4Yn);
+    double dnf = 0.0, dny = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const double sk = p.atol + p.rtol * fabs(v.y[i]);
+      double tmp = f0[i] / sk;
+      dnf += tmp * tmp;
+      tmp = v.y[i] / sk;
+      dny += tmp * tmp;
+    }
+    double dt;
+    if (dnf <= 1.0e-10 || dny <= 1.0e-10)
+      dt = 1.0e-6;
+    else
+      dt = 0.01 * m_sqrt(dny / dnf);
+    double der2 = 0.0;
+    {
+      double y[S], f[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        y[i] = v.y[i] + dt * f0[i];
+        y1[i] = y[i];
+      }
+      M::template eval<double>(y, t + dt, acc, f);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const double sk = p.atol + p.rtol * fabs(y[i]);
+        const double tmp = ((f[i] - f0[i]) / sk);
+        der2 += tmp * tmp;
+      }
+    }
+    der2 = m_sqrt(der2) / dt;
+    const double der12 = fabs(der2) >= m_sqrt(dnf) ? fabs(der2) : m_sqrt(dnf);
+    double dt1;
+    if (der12 <= 1.0e-15)
+      dt1 = 1.0e-6 > fabs(dt) * 1.0e-3 ? 1.0e-6 : fabs(dt) * 1.0e-3;
+    else
+      dt1 = m_pow((0.01 / der12), (1.0 / iord));
+    if (100 * fabs(dt) < dt1) return 100 * fabs(dt);
+    return dt1;
+  }
+
+  // newton_solve (ImplicitODESolver.cpp:114-245) on stage zs (copied in and
+  // out of the Newton row kIeZ) with prev in kIePrev and y0 = y.
+  template <class A>
+  GOSS_HD static bool stage(const VecView& zs, const double t, const double dt, const double alpha, const NodeViews& v,
+                            const A& acc, const ImplicitParams& p, NodeImplicitState& st)
+  {
+    const VecView z = v.w(kIeZ);
+    if (zs.a != z.a) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) z[i] = zs[i];
+    }
+    const bool ok = E::template newton_solve<true>(v, t, dt, alpha, acc, p, st, p.always_recompute_jacobian != 0);
+    if (zs.a != z.a) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) zs[i] = z[i];
+    }
+    return ok;
+  }
+
+  // ESDIRK4O32::forward (ESDIRK4O32.cpp:92-298) with new_time_step
+  // (AdaptiveImplicitSolver.cpp:130-200), as the reference computes it: the
+  // second stage's right-hand side is evaluated at the increment z2, and the
+  // solution lives in two buffers, the node's states (the caller's y, A) and
+  // the row kE4Yn (the solver's member yn, B).  `y` starts on A; an accepted
+  // step swaps the pointers `y` and `yn0` while the new solution is always
+  // written to B, so once `y` is on B the next step overwrites it in place
+  // and is tested against the stale A; at the end B is copied to A if `y`
+  // is on B (ESDIRK4O32.cpp:255-297).  The solver's _t, _dt and step flags
+  // are local to the call; _ldt, facmax, err_old and dt_old persist in the
+  // node's rows `ds`.  Returns false when the pass limit is reached.
+  template <class A>
+  GOSS_HD static bool forward(const NodeViews& v, const VecView& ds, const double t, const double interval,
+                              const A& acc, const ImplicitParams& p, NodeImplicitState& st)
+  {
+    const EsdirkCoefficients k = esdirk_coefficients();
+    const double a33 = k.gamma, a44 = k.gamma, bh3 = k.gamma, b4 = k.gamma;
+    const VecView z1 = v.w(kE4Z1), z2 = v.w(kE4Z2), z3 = v.w(kE4Z3), z4 = v.w(kIeZ), ynb = v.w(kE4Yn),
+                  yh = v.w(kE4Yh), prev = v.w(kIePrev);
+    bool y_on_b = false;  // `y` points to B
+    double ti = t;
+    const double t_end = ti + interval;
+    double dt = ds[kAiLdt];
+    double facmax = ds[kAiFacmax], err_old = ds[kErrOld], dt_old = ds[kDtOld];
+    bool reached_tend = false;
+    if (dt < 0) {
+      {
+        double y[S], f[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) y[i] = v.y[i];
+        M::template eval<double>(y, ti, acc, f);
+#pragma unroll
+        for (int i = 0; i < S; ++i) z1[i] = f[i];
+      }
+      dt = dtinit(v, ti, acc, p);  // y1 = yn0 = B
+      st.nfevals += 2;
+    }
+    bool ok = true;
+    long passes = 0;
+#pragma unroll 1
+    while (!reached_tend) {
+      if (++passes > kEsdirk4MaxPasses) {
+        ok = false;
+        break;
+      }
+      NodeViews vy = v;
+      vy.y = y_on_b ? ynb : v.y;
+      const VecView yv = vy.y, yn0 = y_on_b ? v.y : ynb;
+      {
+        double y[S], f[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) y[i] = yv[i];
+        M::template eval<double>(y, ti, acc, f);
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          z1[i] = f[i];
+          prev[i] = k.a21 * f[i];
+          z2[i] = 0.0;
+        }
+      }
+      st.nfevals += 1;
+      if (!stage(z2, ti + k.c2 * dt, dt, k.a22, vy, acc, p, st)) {
+        st.recompute = 1;
+        dt /= 2.0;
+        continue;
+      }
+      {
+        // The stage evaluation reads z2 itself (ESDIRK4O32.cpp:169), as the reference does.
+        double zz[S], f[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          zz[i] = z2[i];
+          z3[i] = zz[i] + yv[i];
+        }
+        M::template eval<double>(zz, ti + k.c2 * dt, acc, f);
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          z2[i] = f[i];
+          prev[i] = k.bh1 * z1[i] + k.bh2 * f[i];
+        }
+      }
+      st.nfevals += 1;
+      if (!stage(z3, ti + k.c3 * dt, dt, a33, vy, acc, p, st)) {
+        st.recompute = 1;
+        dt /= 2.0;
+        continue;
+      }
+      {
+        double zz[S], f[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) zz[i] = z3[i] + yv[i];
+        M::template eval<double>(zz, ti + k.c3 * dt, acc, f);
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          z3[i] = f[i];
+          const double y0 = yv[i];
+          const double yhi = y0 + dt * (k.bh1 * z1[i] + k.bh2 * z2[i] + bh3 * f[i]);
+          yh[i] = yhi;
+          z4[i] = yhi - y0;
+          prev[i] = k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * f[i];
+        }
+      }
+      st.nfevals += 1;
+      if (!stage(z4, ti + k.c4 * dt, dt, a44, vy, acc, p, st)) {
+        st.recompute = 1;
+        dt /= 2.0;
+        continue;
+      }
+      double err = 0.0;
+      {
+        double zz[S], f[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) zz[i] = z4[i] + yv[i];
+        M::template eval<double>(zz, ti + k.c4 * dt, acc, f);
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const double yni = yv[i] + dt * (k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * z3[i] + b4 * f[i]);
+          ynb[i] = yni;  // B, which is y itself when y is on B
+          const double e = yh[i] - yni;
+          // new_time_step(y, yn0, e): sk = atol + rtol max(|y|, |yn0|) (std::max semantics).
+          const double yi_abs = fabs(yv[i]), yni_abs = fabs(yn0[i]);
+          const double mx = smax(yi_abs, yni_abs);
+          const double sk = p.atol + p.rtol * mx;
+          const double tmp = e / sk;
+          err += tmp * tmp;
+        }
+      }
+      st.nfevals += 1;
+      // new_time_step (AdaptiveImplicitSolver.cpp:130-200).
+      const double dt_prev = dt;
+      st.recompute = 1;
+      err = m_sqrt(err / S);
+      bool accepted;
+      if (err <= 1) {
+        ti += dt;
+        st.accepted++;
+        accepted = true;
+        if (fabs(ti - t_end) < 1e-14) reached_tend = true;
+      } else {
+        st.rejects++;
+        accepted = false;
+      }
+      const double lstabfac = stabfac * (2 * p.max_iterations + 1) / (2.0 * p.max_iterations + st.newton_its);
+      double fac = lstabfac * m_pow((1.0 / err), (1.0 / (iord + 1)));
+      if (facmin > fac)
+        fac = facmin;
+      else if (fac > facmax)
+        fac = facmax;
+      facmax = accepted ? facmaxb : 1.0;
+      if (err_old > 0) fac *= dt / dt_old * m_pow((err_old / err), (1.0 / (iord + 1)));
+      if (fac < stabup && fac > stabdown) {
+        fac = 1.0;
+        st.recompute = 0;
+      }
+      dt *= fac;
+      ds[kAiLdt] = dt;
+      if (ti + dt >= t_end) dt = t_end - ti;
+      dt_old = dt_prev;
+      err_old = err;
+      if (accepted) y_on_b = !y_on_b;
+    }
+    if (y_on_b) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) v.y[i] = ynb[i];
+    }
+    ds[kAiFacmax] = facmax;
+    ds[kErrOld] = err_old;
+    ds[kDtOld] = dt_old;
+    return ok;
+  }
+};
 
 template <class M>
 struct EsdirkNode {
@@ -820,6 +1163,13 @@ implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const dou
         }
       } else if constexpr (SOLVER == kThetaSolver) {
         if (!ThetaNode<M>::forward(v, t, dt, acc, ip, st)) {
+          st.failed = 1;
+          atomicOr(B.error, 1);
+        }
+      } else if constexpr (SOLVER == kBasicImplicitEuler) {
+        BasicImplicitEulerNode<M>::forward(v, t, dt, acc, ip, st);
+      } else if constexpr (SOLVER == kESDIRK4O32) {
+        if (!Esdirk4o32Node<M>::forward(v, VecView{B.ds + node, ld}, t, dt, acc, ip, st)) {
           st.failed = 1;
           atomicOr(B.error, 1);
         }

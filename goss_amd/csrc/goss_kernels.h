@@ -39,7 +39,9 @@ enum SolverKind : int {
   kESDIRK23a = 8,
   kRKF32 = 9,
   kThetaSolver = 10,
-  kNumSolvers = 11,
+  kBasicImplicitEuler = 11,
+  kESDIRK4O32 = 12,
+  kNumSolvers = 13,
 };
 
 enum Precision : int { kFP64 = 0, kFP32 = 1, kFP32C = 2, kNumPrecisions = 3 };

@@ -136,9 +136,9 @@ __global__ void fill_defaults(R* __restrict__ y, R* __restrict__ ylo, R* __restr
 }
 
 // Per-node implicit solver state at construction: ImplicitODESolver::reset
-// (ImplicitODESolver.cpp:80-90: recompute Jacobian, eta = eta_0) and the
+// (ImplicitODESolver.cpp:80-90: recompute Jacobian, eta = eta_0), the
 // ESDIRK23a constructor values (ESDIRK23a.cpp:27-44: eta 1, loc_error 1e9,
-// prev_dt 0.1, first_step, new_jacobian).
+// prev_dt 0.1, first_step, new_jacobian) and the adaptive controllers' state.
 __global__ void init_implicit_state(double* __restrict__ ds, int* __restrict__ is, unsigned N, unsigned ld,
                                     double eta_0, int always)
 {
@@ -151,6 +151,12 @@ __global__ void init_implicit_state(double* __restrict__ ds, int* __restrict__ i
   // AdaptiveExplicitSolver constructor and reset (AdaptiveExplicitSolver.cpp:28-31, 58-63).
   ds[size_t(kRkLdt) * ld + n] = 0.1;
   ds[size_t(kFacmax) * ld + n] = 2.0;
+  // ESDIRK4O32::reset and AdaptiveImplicitSolver::reset (ESDIRK4O32.cpp:84-94,
+  // AdaptiveImplicitSolver.cpp:43-66): no step guess yet, facmax 2, no previous error.
+  ds[size_t(kAiLdt) * ld + n] = -1.0;
+  ds[size_t(kAiFacmax) * ld + n] = 2.0;
+  ds[size_t(kErrOld) * ld + n] = -1.0;
+  ds[size_t(kDtOld) * ld + n] = 0.0;
   for (int k = 0; k < kNumIState; ++k) is[size_t(k) * ld + n] = 0;
   is[size_t(kRecompute) * ld + n] = 1;
   is[size_t(kAlways) * ld + n] = always;

@@ -139,7 +139,7 @@ struct VariantInfo {
   const char* const* field_state_names;
   const char* const* field_parameter_names;
   ExplicitLaunch explicit_launch[kNumSolvers][kNumPrecisions];
-  ImplicitLaunch implicit_launch[3][kNumJacobianModes];  // [implicit_slot(solver)][JacobianMode]
+  ImplicitLaunch implicit_launch[kNumImplicitSlots][kNumJacobianModes];  // [implicit_slot(solver)][JacobianMode]
   AdaptiveLaunch rkf32_launch[kNumPrecisions];  // FP64 and FP32
   AdjointLaunch adjoint_record[kNumAdjointSolvers][kNumPrecisions];  // FP64 and FP32
   AdjointLaunch adjoint_backward[kNumAdjointSolvers][kNumPrecisions];

@@ -272,6 +272,8 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   v.implicit_launch[0][kJacobianFD] = &launch_implicit<kImplicitEuler, kJacobianFD>;
   v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
   v.implicit_launch[2][kJacobianFD] = &launch_implicit<kThetaSolver, kJacobianFD>;
+  v.implicit_launch[3][kJacobianFD] = &launch_implicit<kBasicImplicitEuler, kJacobianFD>;
+  v.implicit_launch[4][kJacobianFD] = &launch_implicit<kESDIRK4O32, kJacobianFD>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
 }
 #elif defined(GOSS_PREC_F64)
@@ -285,6 +287,9 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
   v.implicit_launch[1][kJacobianFDFP32] = &launch_implicit<kESDIRK23a, kJacobianFDFP32>;
   v.implicit_launch[2][kJacobianFD] = &launch_implicit<kThetaSolver, kJacobianFD>;
   v.implicit_launch[2][kJacobianFDFP32] = &launch_implicit<kThetaSolver, kJacobianFDFP32>;
+  // BasicImplicitEuler and ESDIRK4O32 with the reference's Jacobian only.
+  v.implicit_launch[3][kJacobianFD] = &launch_implicit<kBasicImplicitEuler, kJacobianFD>;
+  v.implicit_launch[4][kJacobianFD] = &launch_implicit<kESDIRK4O32, kJacobianFD>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
   register_adjoint<double>(v, kFP64);
   v.model_eval = &launch_model_eval;

@@ -242,6 +242,13 @@ class ImplicitEuler(ImplicitODESolver):
         return names
 
 
+class BasicImplicitEuler(ImplicitODESolver):
+    """BasicImplicitEuler (BasicImplicitEuler.cpp:62-181): full Newton with a new
+    Jacobian at every iteration.  The reference's Python layer marks it as not
+    working as expected (python/goss/solvers.py:279-282) and leaves it out of
+    its solver lists; so does this package."""
+
+
 class ThetaSolver(ImplicitODESolver):
     @staticmethod
     def default_parameters() -> Dict[str, Any]:
@@ -313,6 +320,26 @@ class ESDIRK23a(AdaptiveImplicitSolver):
         names = AdaptiveImplicitSolver.default_parameters()
         names.update({"num_refinements_without_always_recomputing_jacobian": 2, "min_dt": 0.001})
         return names
+
+    @property
+    def ndtsa(self) -> int:
+        return int(self._counters()[3])
+
+    @property
+    def ndtsr(self) -> int:
+        return int(self._counters()[2])
+
+
+class ESDIRK4O32(AdaptiveImplicitSolver):
+    """ESDIRK4O32 (ESDIRK4O32.cpp:92-298) as the reference computes it, error
+    test with atol / rtol (set_tol) and its two-buffer solution update.  The
+    reference's Python layer marks it as not working as expected
+    (python/goss/solvers.py:389-392) and leaves it out of its solver lists; so
+    does this package."""
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self._iord = 3  # ESDIRK4O32.cpp:38
 
     @property
     def ndtsa(self) -> int:

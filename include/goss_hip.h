@@ -46,13 +46,15 @@ typedef enum goss_hip_solver {
   GOSS_HIP_IMPLICIT_EULER = 7, /* ImplicitEuler.cpp:57-152 */
   GOSS_HIP_ESDIRK23A = 8,      /* ESDIRK23a.cpp:280-316 */
   GOSS_HIP_RKF32 = 9,          /* RKF32.cpp:105-222 */
-  GOSS_HIP_THETA = 10          /* ThetaSolver.cpp:48-155 */
+  GOSS_HIP_THETA = 10,         /* ThetaSolver.cpp:48-155 */
+  GOSS_HIP_BASIC_IMPLICIT_EULER = 11, /* BasicImplicitEuler.cpp:62-181 (FP64, reference Jacobian) */
+  GOSS_HIP_ESDIRK4O32 = 12     /* ESDIRK4O32.cpp:92-298 (FP64, reference Jacobian) */
 } goss_hip_solver;
 
 /* Arithmetic of the per-node work.  FP64 is the reference's arithmetic.
  * FP32C computes in FP32 and carries the states whose derivative is not
  * affine in themselves (linear_terms == 0) as FP32 hi + lo pairs.
- * ImplicitEuler, ESDIRK23a and ThetaSolver run in FP64; RKF32 in FP64 or FP32 (its step
+ * The implicit solvers run in FP64; RKF32 in FP64 or FP32 (its step
  * sizes and local time stay FP64). */
 typedef enum goss_hip_precision {
   GOSS_HIP_FP64 = 0,
@@ -251,10 +253,14 @@ int goss_hip_system_num_nodes(goss_hip_system* sys, uint32_t* num_nodes);
 /* Outer steps fused into one kernel launch by forward_times / solve. */
 int goss_hip_system_set_steps_per_launch(goss_hip_system* sys, uint32_t steps);
 /* Sums over nodes of the per-node solver counters.
- * ImplicitEuler / ESDIRK23a: out[0] Newton iterations of the last stage/solve
+ * ImplicitEuler / ESDIRK23a / ThetaSolver / BasicImplicitEuler / ESDIRK4O32:
+ * out[0] Newton iterations of the last stage/solve
  * (ImplicitODESolver::_newton_iterations), out[1] Jacobian evaluations
  * (num_jac_comp, ImplicitODESolver.h:65-68), out[2] rejected Newton solves /
- * steps, out[3] accepted ESDIRK23a steps.
+ * steps (BasicImplicitEuler: sub-steps whose iteration did not converge;
+ * ESDIRK4O32: failed stage solves plus steps rejected by the error test,
+ * AdaptiveImplicitSolver::num_rejected), out[3] accepted ESDIRK23a /
+ * ESDIRK4O32 steps (num_accepted).
  * RKF32 (RKF32.h:70-72): out[0] 0, out[1] nfevals, out[2] ndtsr, out[3] ndtsa.
  * Zeros for the fixed-step solvers. */
 int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);

@@ -8,7 +8,8 @@
 //     ImplicitODESolver (eta_0, kappa, relative_tolerance, max_iterations,
 //       max_relative_previous_residual, always_recompute_jacobian; ImplicitODESolver.h:36-41),
 //     ImplicitEuler (num_refinements_without_always_recomputing_jacobian, min_dt; ImplicitEuler.h:36-37),
-//     ThetaSolver (+ theta; ThetaSolver.h:35-37), ESDIRK23a (ESDIRK23a.h:37-38),
+//     ThetaSolver (+ theta; ThetaSolver.h:35-37), ESDIRK23a (ESDIRK23a.h:37-38), BasicImplicitEuler,
+//     ESDIRK4O32 (set_tol; ESDIRK4O32.h:31-90),
 //     AdaptiveExplicitSolver / RKF32 (set_tol, get_atol, get_rtol; nfevals, ndtsa, ndtsr)
 //   goss_hip::ODESystemSolver    <- goss::ODESystemSolver (ODESystemSolver.h:38-186)
 // so that code written against goss's ODESystemSolver, e.g.
@@ -413,6 +414,12 @@ class ImplicitEuler : public ImplicitODESolver {
   }
 };
 
+// BasicImplicitEuler (BasicImplicitEuler.cpp:62-181): full Newton, a new Jacobian every iteration.
+class BasicImplicitEuler : public ImplicitODESolver {
+ public:
+  BasicImplicitEuler() : ImplicitODESolver(GOSS_HIP_BASIC_IMPLICIT_EULER) {}
+};
+
 class ThetaSolver : public ImplicitEuler {
  public:
   ThetaSolver() : ImplicitEuler(GOSS_HIP_THETA) {}
@@ -453,6 +460,34 @@ class ESDIRK23a : public ImplicitODESolver {
 
  private:
   double atol_ = 1.0e-5, rtol_ = 1.0e-8;
+};
+
+// ESDIRK4O32 (ESDIRK4O32.h:31-90) over AdaptiveImplicitSolver (AdaptiveImplicitSolver.h:31-150): the error
+// test uses set_tol's atol / rtol.
+class ESDIRK4O32 : public ImplicitODESolver {
+ public:
+  ESDIRK4O32() : ImplicitODESolver(GOSS_HIP_ESDIRK4O32) {}
+  void set_tol(double atol, double rtol = 1.0e-8)
+  {
+    atol_ = atol;
+    rtol_ = rtol;
+  }
+  double get_atol() const { return atol_; }
+  double get_rtol() const { return rtol_; }
+  int get_iord() const { return 3; }  // ESDIRK4O32.cpp:38
+  long get_num_accepted() const { return long(counter(3)); }
+  long get_num_rejected() const { return long(counter(2)); }
+
+ protected:
+  void fill(goss_hip_solver_params& p) const override
+  {
+    ImplicitODESolver::fill(p);
+    p.atol = atol_;
+    p.rtol = rtol_;
+  }
+
+ private:
+  double atol_ = 1.0e-5, rtol_ = 1.0e-8;  // AdaptiveImplicitSolver::reset (AdaptiveImplicitSolver.cpp:61-62)
 };
 
 // AdaptiveExplicitSolver (AdaptiveExplicitSolver.h:35-130) and RKF32 (RKF32.h:34-100).
@@ -503,6 +538,8 @@ inline std::shared_ptr<ODESolver> make_solver(const std::string& name)
   if (name == "ThetaSolver") return std::make_shared<ThetaSolver>();
   if (name == "ESDIRK23a") return std::make_shared<ESDIRK23a>();
   if (name == "RKF32") return std::make_shared<RKF32>();
+  if (name == "BasicImplicitEuler") return std::make_shared<BasicImplicitEuler>();
+  if (name == "ESDIRK4O32") return std::make_shared<ESDIRK4O32>();
   throw Error(GOSS_HIP_EINVAL, "unknown solver " + name);
 }
 

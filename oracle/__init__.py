@@ -80,6 +80,7 @@ def lib():
         L.oracle_system_destroy.argtypes = [ctypes.c_void_p]
         L.oracle_system_reset_default.argtypes = [ctypes.c_void_p]
         L.oracle_system_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        L.oracle_system_esdirk4o32_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
         L.oracle_model_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
         L.oracle_model_linearized_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int]
         L.oracle_model_analytic_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
@@ -124,6 +125,7 @@ def ref_lib(path: str = REF_SO):
         L.ref_system_destroy.argtypes = [ctypes.c_void_p]
         L.ref_system_reset_default.argtypes = [ctypes.c_void_p]
         L.ref_system_rkf32_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        L.ref_system_esdirk4o32_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
         L.ref_model_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
         L.ref_model_eval.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
         L.ref_model_linearized_eval.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp, ctypes.c_int]
@@ -345,6 +347,12 @@ class OracleSystem(_SystemBase):
         self._call("counters", out)
         return np.array(list(out))
 
+    def esdirk4o32_counters(self) -> np.ndarray:
+        """Accepted steps, error-test rejections and right-hand-side evaluations of ESDIRK4O32."""
+        out = (ctypes.c_long * 3)()
+        self._call("esdirk4o32_counters", out)
+        return np.array(list(out))
+
 
 class RefSystem(_SystemBase):
     """The reference's ODESystemSolver compiled from its own sources."""
@@ -374,6 +382,12 @@ class RefSystem(_SystemBase):
         """nfevals, ndtsa, ndtsr of the system's RKF32 solver (serial mode)."""
         out = (ctypes.c_long * 3)()
         self._check(self._L.ref_system_rkf32_counters(self._h, out))
+        return np.array(list(out))
+
+    def esdirk4o32_counters(self) -> np.ndarray:
+        """num_accepted, num_rejected, nfevals of the system's ESDIRK4O32 solver (serial mode)."""
+        out = (ctypes.c_long * 3)()
+        self._check(self._L.ref_system_esdirk4o32_counters(self._h, out))
         return np.array(list(out))
 
 

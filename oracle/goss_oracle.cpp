@@ -196,12 +196,16 @@ void fd_jacobian(const VariantOps& m, double* y, double t, const double* p, cons
 // Solvers
 // ---------------------------------------------------------------------------
 
-enum SolverKind { EE, RK2, RK4, RL1, RL2, GRL1, GRL2, IMPLICIT_EULER, ESDIRK23A, RKF32, THETA, NUM_SOLVERS };
+enum SolverKind {
+  EE, RK2, RK4, RL1, RL2, GRL1, GRL2, IMPLICIT_EULER, ESDIRK23A, RKF32, THETA, BASIC_IMPLICIT_EULER, ESDIRK4O32,
+  NUM_SOLVERS
+};
 
 int solver_kind(const std::string& s)
 {
   static const char* names[] = {"ExplicitEuler", "RK2", "RK4", "RL1", "RL2", "GRL1", "GRL2",
-                                "ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver"};
+                                "ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver", "BasicImplicitEuler",
+                                "ESDIRK4O32"};
   for (int i = 0; i < NUM_SOLVERS; ++i)
     if (s == names[i]) return i;
   return -1;
@@ -256,6 +260,15 @@ struct ImplicitState {
   double rk_ldt = 0.1;
   double facmax = 2.0;
   long rk_nfevals = 0, ndtsa = 0, ndtsr = 0;
+  // ESDIRK4O32 over AdaptiveImplicitSolver: ESDIRK4O32::reset sets the step
+  // guess _ldt to -1 (ESDIRK4O32.cpp:84-94); AdaptiveImplicitSolver::reset
+  // (AdaptiveImplicitSolver.cpp:43-66) facmax 2, err_old -1.
+  double ai_ldt = -1.0;
+  double ai_facmax = 2.0;
+  double err_old = -1.0;
+  double dt_old = 0.0;
+  long e4_nfevals = 0;
+  std::vector<double> e4_yn;  // ESDIRK4O32's member yn (ESDIRK4O32.h)
   void init(int S, const SolverParams& sp)
   {
     jac.assign(size_t(S) * S, 0.0);
@@ -752,6 +765,237 @@ void esdirk_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& 
   st.first_step = false;
 }
 
+// BasicImplicitEuler::forward (BasicImplicitEuler.cpp:62-181) with its
+// compute_factorized_jacobian (:46-60): full Newton with a new Jacobian at
+// every iteration (scaled by -ldt, identity added, LU), residual
+// (y - y_prev) - dt f(y, t + ldt).  The residual and the clock advance use the
+// whole interval dt, the Jacobian the sub-step ldt, as in the reference.  A
+// sub-step whose iteration does not converge within max_iterations keeps its
+// last iterate; the reference prints a message, the oracle counts a rejection.
+void basic_implicit_euler_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* y,
+                                  double t, double dt, const double* p, const double* c, Work& w)
+{
+  const int S = m.S;
+  const double ldt_0 = sp.ldt;
+  const unsigned long nsteps = ldt_0 > 0 ? (unsigned long)std::ceil(dt / ldt_0 - 1.0E-12) : 1UL;
+  const double ldt = dt / nsteps;
+  const double rtol = sp.relative_tolerance;
+  double lt = t;
+  double* prev = w.prev.data();
+  double* f1 = w.f1.data();
+  double* b = w.b.data();
+  double* dz = w.dz.data();
+  for (unsigned long step = 0; step < nsteps; ++step) {
+    double relative_residual = 1.0, initial_residual = 1.0, residual;
+    bool newton_converged = false;
+    st.newton_iterations = 0;
+    for (int i = 0; i < S; ++i) prev[i] = y[i];
+    while (!newton_converged && st.newton_iterations < sp.max_iterations) {
+      m.eval(y, lt + ldt, p, c, f1);
+      for (int i = 0; i < S; ++i) b[i] = (y[i] - prev[i]) * 1.0 - dt * f1[i];
+      residual = l2norm(b, S);
+      if (st.newton_iterations == 0) initial_residual = residual;
+      relative_residual = residual / initial_residual;
+      if (relative_residual < rtol) {
+        newton_converged = true;
+        break;
+      }
+      ie_factorized_jacobian(m, st, y, lt + ldt, ldt, 1.0, p, c, w);
+      fb_subst(S, st.jac.data(), b, dz);
+      for (int i = 0; i < S; ++i) y[i] -= dz[i];
+      ++st.newton_iterations;
+    }
+    if (!newton_converged)
+      st.rejects++;
+    else
+      lt += dt;
+  }
+}
+
+// AdaptiveImplicitSolver::dtinit (AdaptiveImplicitSolver.cpp:68-128).
+double ai_dtinit(const VariantOps& m, const SolverParams& sp, double t, const double* y0, double* y1,
+                 const double* f0, double* f1, double iord, const double* p, const double* c)
+{
+  const int S = m.S;
+  double dnf = 0.0, dny = 0.0, sk, dt, tmp;
+  for (int i = 0; i < S; ++i) {
+    sk = sp.atol + sp.rtol * std::fabs(y0[i]);
+    tmp = f0[i] / sk;
+    dnf += tmp * tmp;
+    tmp = y0[i] / sk;
+    dny += tmp * tmp;
+  }
+  if (dnf <= 1.0e-10 || dny <= 1.0e-10)
+    dt = 1.0e-6;
+  else
+    dt = 0.01 * std::sqrt(dny / dnf);
+  for (int i = 0; i < S; ++i) y1[i] = y0[i] + dt * f0[i];
+  m.eval(y1, t + dt, p, c, f1);
+  double der2 = 0.0;
+  for (int i = 0; i < S; ++i) {
+    sk = sp.atol + sp.rtol * std::fabs(y1[i]);
+    tmp = ((f1[i] - f0[i]) / sk);
+    der2 += tmp * tmp;
+  }
+  der2 = std::sqrt(der2) / dt;
+  double der12;
+  if (std::fabs(der2) >= std::sqrt(dnf))
+    der12 = std::fabs(der2);
+  else
+    der12 = std::sqrt(dnf);
+  double dt1;
+  if (der12 <= 1.0e-15) {
+    if (1.0e-6 > std::fabs(dt) * 1.0e-3)
+      dt1 = 1.0e-6;
+    else
+      dt1 = std::fabs(dt) * 1.0e-3;
+  } else {
+    dt1 = std::pow((0.01 / der12), (1.0 / iord));
+  }
+  if (100 * std::fabs(dt) < dt1) return 100 * std::fabs(dt);
+  return dt1;
+}
+
+// Passes of the ESDIRK4O32 step loop after which the oracle gives up (the
+// reference loops until it reaches the interval end, which a non-finite error
+// or a step size halved to zero never does).
+constexpr long kEsdirk4MaxPasses = 1000000;
+
+// ESDIRK4O32::forward (ESDIRK4O32.cpp:92-298) with
+// AdaptiveImplicitSolver::new_time_step (AdaptiveImplicitSolver.cpp:130-200),
+// as the reference computes it:
+//  * the second stage's right-hand side is evaluated at the stage increment z2
+//    and the third stage starts from z2 + y (ESDIRK4O32.cpp:163-183);
+//  * the solution lives in two buffers, the caller's y (A) and the solver's
+//    member yn (B).  `y` points to A at the start; each accepted step swaps
+//    the pointers `y` and `yn0` (:272-276), while the new solution is always
+//    written to B by name (:255-262).  Once `y` points to B, the next step
+//    overwrites y in place, the error test compares it with the stale A, and
+//    an accepted step swaps back to A.  At the end `y` is copied to the
+//    caller's buffer if it points to B (:290-297).
+// The member yn persists in the solver instance (st.e4_yn); it is written
+// before it is read in every call.
+void esdirk4o32_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* y, double t,
+                        double interval, const double* p, const double* c, Work& w)
+{
+  static const EsdirkCoef k;  // the same tableau (ESDIRK4O32.cpp:29-37)
+  const double iord = 3.0;    // ESDIRK4O32.cpp:38
+  const double facmin = 0.5, facmaxb = 2.0, stabfac = 0.9, stabdown = 1.0, stabup = 1.2;
+  const int S = m.S;
+  if (st.e4_yn.size() != size_t(S)) st.e4_yn.assign(S, 0.0);
+  double *z1 = w.z1.data(), *z2 = w.z2.data(), *z3 = w.z3.data(), *z4 = w.z4.data(), *yh = w.yh.data(),
+         *prev = w.prev.data(), *f1 = w.f1.data();
+  double* const ret = y;
+  double* yn = st.e4_yn.data();  // the member vector (buffer B)
+  double* yp = y;                // the pointer `y`
+  double* yn0 = yn;              // the pointer `yn0`
+  double ti = t;
+  const double t_end = ti + interval;
+  double dt = st.ai_ldt;
+  bool reached_tend = false;
+  if (dt < 0) {
+    m.eval(yp, ti, p, c, z1);
+    dt = ai_dtinit(m, sp, ti, yp, yn0, z1, z2, iord, p, c);
+    st.e4_nfevals += 2;
+  }
+  long passes = 0;
+  while (!reached_tend) {
+    if (++passes > kEsdirk4MaxPasses) throw std::runtime_error("ESDIRK4O32 did not reach the end of the interval");
+    m.eval(yp, ti, p, c, z1);
+    st.e4_nfevals += 1;
+    for (int i = 0; i < S; ++i) {
+      prev[i] = k.a21 * z1[i];
+      z2[i] = 0.0;
+    }
+    if (!newton_solve(m, sp, st, z2, prev, yp, ti + k.c2 * dt, dt, k.a22, sp.always_recompute_jacobian, p, c, w)) {
+      st.recompute_jacobian = true;
+      dt /= 2.0;
+      continue;
+    }
+    for (int i = 0; i < S; ++i) z3[i] = z2[i] + yp[i];
+    m.eval(z2, ti + k.c2 * dt, p, c, f1);
+    st.e4_nfevals += 1;
+    for (int i = 0; i < S; ++i) {
+      z2[i] = f1[i];
+      prev[i] = k.a31 * z1[i] + k.a32 * z2[i];
+    }
+    if (!newton_solve(m, sp, st, z3, prev, yp, ti + k.c3 * dt, dt, k.a33, sp.always_recompute_jacobian, p, c, w)) {
+      st.recompute_jacobian = true;
+      dt /= 2.0;
+      continue;
+    }
+    for (int i = 0; i < S; ++i) z3[i] += yp[i];
+    m.eval(z3, ti + k.c3 * dt, p, c, f1);
+    st.e4_nfevals += 1;
+    for (int i = 0; i < S; ++i) {
+      z3[i] = f1[i];
+      yh[i] = yp[i] + dt * (k.bh1 * z1[i] + k.bh2 * z2[i] + k.bh3 * z3[i]);
+    }
+    for (int i = 0; i < S; ++i) {
+      z4[i] = yh[i] - yp[i];
+      prev[i] = k.a41 * z1[i] + k.a42 * z2[i] + k.a43 * z3[i];
+    }
+    if (!newton_solve(m, sp, st, z4, prev, yp, ti + k.c4 * dt, dt, k.a44, sp.always_recompute_jacobian, p, c, w)) {
+      st.recompute_jacobian = true;
+      dt /= 2.0;
+      continue;
+    }
+    for (int i = 0; i < S; ++i) z4[i] += yp[i];
+    m.eval(z4, ti + k.c4 * dt, p, c, f1);
+    st.e4_nfevals += 1;
+    for (int i = 0; i < S; ++i) {
+      yn[i] = yp[i] + dt * (k.b1 * z1[i] + k.b2 * z2[i] + k.b3 * z3[i] + k.b4 * f1[i]);
+      yh[i] -= yn[i];
+    }
+    // new_time_step(y, yn0, yh, t_end)
+    const double dt_prev = dt;
+    st.recompute_jacobian = true;
+    double err = 0.0;
+    for (int i = 0; i < S; ++i) {
+      const double yi_abs = std::fabs(yp[i]);
+      const double yni_abs = std::fabs(yn0[i]);
+      const double mx = std::max(yi_abs, yni_abs);
+      const double sk = sp.atol + sp.rtol * mx;
+      const double tmp = yh[i] / sk;
+      err += tmp * tmp;
+    }
+    err = std::sqrt(err / S);
+    bool step_accepted;
+    if (err <= 1) {
+      ti += dt;
+      st.accepted += 1;
+      step_accepted = true;
+      if (std::fabs(ti - t_end) < 1e-14) reached_tend = true;
+    } else {
+      st.num_rejected += 1;
+      step_accepted = false;
+    }
+    const double lstabfac = stabfac * (2 * sp.max_iterations + 1) / (2.0 * sp.max_iterations + st.newton_iterations);
+    double fac = lstabfac * std::pow((1.0 / err), (1.0 / (iord + 1)));
+    if (facmin > fac)
+      fac = facmin;
+    else if (fac > st.ai_facmax)
+      fac = st.ai_facmax;
+    if (!step_accepted)
+      st.ai_facmax = 1.0;
+    else
+      st.ai_facmax = facmaxb;
+    if (st.err_old > 0) fac *= dt / st.dt_old * std::pow((st.err_old / err), (1.0 / (iord + 1)));
+    if (fac < stabup && fac > stabdown) {
+      fac = 1.0;
+      st.recompute_jacobian = false;
+    }
+    dt *= fac;
+    st.ai_ldt = dt;
+    if (ti + dt >= t_end) dt = t_end - ti;
+    st.dt_old = dt_prev;
+    st.err_old = err;
+    if (step_accepted) std::swap(yp, yn0);
+  }
+  if (ret != yp)
+    for (int i = 0; i < S; ++i) ret[i] = yp[i];
+}
+
 // ---------------------------------------------------------------------------
 // RKF32 (RKF32.cpp:105-222) with AdaptiveExplicitSolver::new_time_step
 // (AdaptiveExplicitSolver.cpp:168-244)
@@ -879,7 +1123,8 @@ void forward_node(oracle_system* s, unsigned node, double t, double interval, Wo
   double c[kMaxConstants];
   m.constants(prm, c);
   double* y = &s->states[size_t(node) * m.S];
-  if (s->kind == IMPLICIT_EULER || s->kind == ESDIRK23A || s->kind == RKF32 || s->kind == THETA) {
+  if (s->kind == IMPLICIT_EULER || s->kind == ESDIRK23A || s->kind == RKF32 || s->kind == THETA ||
+      s->kind == BASIC_IMPLICIT_EULER || s->kind == ESDIRK4O32) {
     ImplicitState& st = s->per_node_state ? s->node_state[node] : w.carried;
     if (s->kind == IMPLICIT_EULER)
       implicit_euler_forward(m, s->sp, st, y, t, interval, prm, c, w);
@@ -887,6 +1132,10 @@ void forward_node(oracle_system* s, unsigned node, double t, double interval, Wo
       esdirk_forward(m, s->sp, st, y, t, interval, prm, c, w);
     else if (s->kind == THETA)
       theta_forward(m, s->sp, st, y, t, interval, prm, c, w);
+    else if (s->kind == BASIC_IMPLICIT_EULER)
+      basic_implicit_euler_forward(m, s->sp, st, y, t, interval, prm, c, w);
+    else if (s->kind == ESDIRK4O32)
+      esdirk4o32_forward(m, s->sp, st, y, t, interval, prm, c, w);
     else
       rkf32_forward(m, s->sp, st, y, t, interval, prm, c, w);
     return;
@@ -1227,6 +1476,21 @@ void oracle_system_set_field_parameters(oracle_system* s, const double* in, int 
     for (unsigned i = 0; i < F; ++i) s->fparams[size_t(n) * F + i] = in[field_index(tangled, s->N, F, n, i)];
 }
 void oracle_system_reset_default(oracle_system* s) { reset_default(s); }
+// ESDIRK4O32: accepted steps, steps rejected by the error test, right-hand-side evaluations.
+void oracle_system_esdirk4o32_counters(const oracle_system* s, long* out3)
+{
+  for (int i = 0; i < 3; ++i) out3[i] = 0;
+  auto add = [&](const ImplicitState& st) {
+    out3[0] += st.accepted;
+    out3[1] += st.num_rejected;
+    out3[2] += st.e4_nfevals;
+  };
+  if (s->per_node_state)
+    for (const auto& st : s->node_state) add(st);
+  else
+    for (const auto& w : s->work) add(w.carried);
+}
+
 void oracle_system_counters(const oracle_system* s, long* out4)
 {
   for (int i = 0; i < 4; ++i) out4[i] = 0;

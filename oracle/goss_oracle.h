@@ -101,6 +101,8 @@ void oracle_system_reset_default(oracle_system* s);
 /* Per-node counters of the implicit/adaptive solvers (Newton iterations,
  * Jacobian evaluations, rejected steps, accepted steps); zeros otherwise. */
 void oracle_system_counters(const oracle_system* s, long* out4);
+/* ESDIRK4O32: accepted steps, steps rejected by the error test, right-hand-side evaluations. */
+void oracle_system_esdirk4o32_counters(const oracle_system* s, long* out3);
 
 #ifdef __cplusplus
 }

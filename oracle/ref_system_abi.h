@@ -25,6 +25,8 @@
 #include "goss/ESDIRK23a.h"
 #include "goss/RKF32.h"
 #include "goss/ThetaSolver.h"
+#include "goss/BasicImplicitEuler.h"
+#include "goss/ESDIRK4O32.h"
 
 
 namespace {
@@ -46,6 +48,8 @@ std::shared_ptr<ODESolver> make_solver(const std::string& s)
   if (s == "ESDIRK23a") return std::make_shared<ESDIRK23a>();
   if (s == "RKF32") return std::make_shared<RKF32>();
   if (s == "ThetaSolver") return std::make_shared<ThetaSolver>();
+  if (s == "BasicImplicitEuler") return std::make_shared<BasicImplicitEuler>();
+  if (s == "ESDIRK4O32") return std::make_shared<ESDIRK4O32>();
   throw std::runtime_error("unknown solver " + s);
 }
 
@@ -83,6 +87,10 @@ void set_solver_param(ODESolver& s, const std::string& name, double value)
   if (auto* ad = dynamic_cast<AdaptiveExplicitSolver*>(&s)) {
     if (name == "atol") { ad->set_tol(value, ad->get_rtol()); return; }
     if (name == "rtol") { ad->set_tol(ad->get_atol(), value); return; }
+  }
+  if (auto* ai = dynamic_cast<AdaptiveImplicitSolver*>(&s)) {
+    if (name == "atol") { ai->set_tol(value, ai->get_rtol()); return; }
+    if (name == "rtol") { ai->set_tol(ai->get_atol(), value); return; }
   }
   throw std::runtime_error("unknown solver parameter " + name);
 }
@@ -133,6 +141,20 @@ int ref_system_create(const char* variant, const char* solver, unsigned num_node
 }
 
 void ref_system_destroy(void* h) { delete static_cast<RefSystem*>(h); }
+
+// ESDIRK4O32 counters of the system's own solver (serial mode):
+// num_accepted, num_rejected (AdaptiveImplicitSolver.h) and nfevals
+// (ESDIRK4O32.h).  Returns -1 for other solvers.
+int ref_system_esdirk4o32_counters(void* h, long* out3)
+{
+  auto* r = static_cast<RefSystem*>(h);
+  auto* es = dynamic_cast<ESDIRK4O32*>(r->solver.get());
+  if (!es) return -1;
+  out3[0] = es->get_num_accepted();
+  out3[1] = es->get_num_rejected();
+  out3[2] = es->nfevals;
+  return 0;
+}
 
 // RKF32 counters of the system's own solver (serial mode): nfevals, ndtsa,
 // ndtsr (RKF32.h:70-72).  Returns -1 for other solvers.

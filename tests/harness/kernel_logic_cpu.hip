@@ -119,6 +119,12 @@ int run(int solver, unsigned N, const double* y0, const double* t, unsigned long
   for (unsigned node = first; node < last; ++node) {
     double ys[S];
     for (int i = 0; i < S; ++i) ys[i] = y0[size_t(node) * S + i];
+    // Adaptive-implicit controller rows of ESDIRK4O32 (its reset values).
+    double dsv[kNumDState] = {};
+    dsv[kAiLdt] = -1.0;
+    dsv[kAiFacmax] = 2.0;
+    dsv[kErrOld] = -1.0;
+    dsv[kDtOld] = 0.0;
     NodeImplicitState st{};
     st.eta = ip.eta_0;
     st.es_eta = 1.0;
@@ -139,6 +145,13 @@ int run(int solver, unsigned N, const double* y0, const double* t, unsigned long
         }
       } else if (solver == kThetaSolver) {
         if (!ThetaNode<M>::forward(v, tk, dt, acc, ip, st)) {
+          st.failed = 1;
+          failed[slot] = 1;
+        }
+      } else if (solver == kBasicImplicitEuler) {
+        BasicImplicitEulerNode<M>::forward(v, tk, dt, acc, ip, st);
+      } else if (solver == kESDIRK4O32) {
+        if (!Esdirk4o32Node<M>::forward(v, VecView{dsv, 1}, tk, dt, acc, ip, st)) {
           st.failed = 1;
           failed[slot] = 1;
         }
@@ -218,7 +231,7 @@ extern "C" int kernel_logic_run_fp32_jacobian(const char* variant, int solver, u
                                               double relative_tolerance, double* out_states, long* counters)
 {
   const double min_dt = solver == kESDIRK23a ? 1e-3 : 1e-4;
-  ImplicitParams ip{-1.0, 1.0, 0.1, relative_tolerance, 0.01, min_dt, 30, 0, 2, 0.5};
+  ImplicitParams ip{-1.0, 1.0, 0.1, relative_tolerance, 0.01, min_dt, 30, 0, 2, 0.5, 1.0e-5, 1.0e-8};
   const std::string v(variant);
   if (v == "tp06_m") return run<M_tp06_m, kJacobianFDFP32>(solver, N, y0, t, n, params, ip, out_states, counters);
   if (v == "tp06_epi") return run<M_tp06_epi, kJacobianFDFP32>(solver, N, y0, t, n, params, ip, out_states, counters);
@@ -229,7 +242,7 @@ extern "C" int kernel_logic_run_theta(const char* variant, unsigned N, const dou
                                       unsigned long n, const double* params, double ldt_0, double theta,
                                       double* out_states, long* counters)
 {
-  ImplicitParams ip{ldt_0, 1.0, 0.1, 1e-12, 0.01, 1e-4, 30, 0, 2, theta};
+  ImplicitParams ip{ldt_0, 1.0, 0.1, 1e-12, 0.01, 1e-4, 30, 0, 2, theta, 1.0e-5, 1.0e-8};
   const std::string v(variant);
   if (v == "tp06_m") return run<M_tp06_m>(kThetaSolver, N, y0, t, n, params, ip, out_states, counters);
   if (v == "tp06_epi") return run<M_tp06_epi>(kThetaSolver, N, y0, t, n, params, ip, out_states, counters);
@@ -245,7 +258,7 @@ extern "C" int kernel_logic_run(const char* variant, int solver, unsigned N, con
                                 int always, int num_refinements, double* out_states, long* counters)
 {
   ImplicitParams ip{ldt_0, eta_0, kappa, relative_tolerance, max_rel_prev, min_dt, max_iterations, always,
-                    num_refinements, 0.5};
+                    num_refinements, 0.5, 1.0e-5, 1.0e-8};
   const std::string v(variant);
   if (solver == kRKF32) {
     // atol / rtol arrive in the eta_0 / kappa slots.

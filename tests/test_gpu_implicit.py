@@ -52,7 +52,7 @@ def ulp_sensitivity(variant, solver, params, y0, t, threads=1):
     return float((np.abs(x - z) / np.maximum(np.abs(x), floor)).max()), a.counters(), b.counters()
 
 
-@pytest.mark.parametrize("solver", ["ImplicitEuler", "ESDIRK23a", "ThetaSolver"])
+@pytest.mark.parametrize("solver", ["ImplicitEuler", "ESDIRK23a", "ThetaSolver", "BasicImplicitEuler"])
 @pytest.mark.parametrize("variant,dt,steps,ldt", [
     ("tp06_epi", 0.1, 300, -1.0),
     ("tp06_epi", 0.1, 200, 0.05),
@@ -81,6 +81,33 @@ def test_implicit_matches_oracle_per_node(solver, variant, dt, steps, ldt):
     oc, dc = o.counters(), dev.solver_counters()
     print(f"{variant}/{solver}: max rel err {err:.3e} (one-ulp spread {sens:.3e}, bound {tol:.1e}), "
           f"counters oracle {oc} device {dc}")
+    if same_decisions:
+        assert list(oc[1:]) == list(dc[1:])
+
+
+def test_esdirk4o32_matches_oracle_per_node():
+    """ESDIRK4O32 as the reference computes it (1e4-1e5 steps per interval on FHN), 64 nodes,
+    three intervals of 0.05 ms: the oracle's one-ulp calibration of the other implicit solvers."""
+    N = 64
+    o = oracle.OracleSystem("fhn", "ESDIRK4O32", N, per_node_state=True)
+    o.set_num_threads(8)
+    dev = dev_system("fhn", "ESDIRK4O32", N, "fp64")
+    y0 = perturbed(o.states(), 44)
+    t = np.arange(4) * 0.05
+    sens, c_a, c_b = ulp_sensitivity("fhn", "ESDIRK4O32", {}, y0, t, threads=8)
+    o.set_states(y0)
+    dev.set_states(y0)
+    o.forward_times(t)
+    dev.forward_times(t)
+    same_decisions = list(c_a[1:]) == list(c_b[1:])
+    tol = max(1e-10, 10 * sens)
+    if same_decisions:
+        tol = min(tol, 1e-8)
+    err = assert_parity(dev.states(), o.states(), rtol=tol, what="fhn/ESDIRK4O32", scale=state_scale(y0, o.states()))
+    oc, dc = o.counters(), dev.solver_counters()
+    print(f"fhn/ESDIRK4O32: max rel err {err:.3e} (one-ulp spread {sens:.3e}, bound {tol:.1e}), "
+          f"counters oracle {oc} device {dc}")
+    assert dc[3] > 0
     if same_decisions:
         assert list(oc[1:]) == list(dc[1:])
 

@@ -31,7 +31,8 @@ ROOT = os.path.dirname(HERE)
 DEV_LIB = os.path.join(HERE, "harness", "libgoss_hip_parity.so")
 HOST_LIB = os.path.join(HERE, "harness", "libkernel_logic_cpu_parity.so")
 META = os.path.join(ROOT, "goss_amd", "csrc", "generated", "variants.json")
-SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8, "RKF32": 9, "ThetaSolver": 10}
+SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8, "RKF32": 9, "ThetaSolver": 10, "BasicImplicitEuler": 11,
+              "ESDIRK4O32": 12}
 
 
 def _libs():
@@ -100,6 +101,9 @@ def _run_device(variant, solver, y0, t, params, relative_tolerance):
     ("tp06_epi", "ThetaSolver", 1024, 0.1, 1000, 52.0, None),
     ("tp06_m", "RKF32", 128, 0.1, 200, None, None),
     ("lorentz", "RKF32", 1024, 0.02, 500, None, None),
+    ("tp06_epi", "BasicImplicitEuler", 256, 0.1, 300, 52.0, None),
+    # ESDIRK4O32 as the reference computes it takes 1e4-1e5 steps per interval on FHN.
+    ("fhn", "ESDIRK4O32", 16, 0.05, 3, None, None),
 ])
 def test_device_equals_host_bit_for_bit(variant, solver, N, dt, steps, stim, reltol):
     meta = json.load(open(META))[variant]

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(HERE)
 HARNESS = os.path.join(HERE, "harness", "libkernel_logic_cpu.so")
 META = os.path.join(ROOT, "goss_amd", "csrc", "generated", "variants.json")
 
-SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8}
+SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8, "BasicImplicitEuler": 11, "ESDIRK4O32": 12}
 # ImplicitODESolver / ImplicitEuler / ESDIRK23a defaults
 # (ImplicitODESolver.cpp:31-45, ImplicitEuler.h:36-37, ESDIRK23a.cpp:46-60).
 DEFAULTS = dict(eta_0=1.0, kappa=0.1, relative_tolerance=1e-12, max_relative_previous_residual=0.01,
@@ -83,7 +83,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("solver", ["ImplicitEuler", "ESDIRK23a"])
+@pytest.mark.parametrize("solver", ["ImplicitEuler", "ESDIRK23a", "BasicImplicitEuler"])
 @pytest.mark.parametrize("variant,dt,nsteps,ldt", CASES)
 def test_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, ldt, solver):
     S = len(json.load(open(META))[variant]["state_names"])
@@ -98,6 +98,23 @@ def test_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, ldt, solver)
     np.testing.assert_array_equal(got, want)
     # counters: Newton iterations of the last solve, Jacobians, rejects, accepted
     np.testing.assert_array_equal(got_c[1:], want_c[1:])
+
+
+@pytest.mark.parametrize("variant,dt,nsteps", [("fhn", 0.05, 3), ("fhn", 0.02, 5)])
+def test_esdirk4o32_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps):
+    """ESDIRK4O32 as the reference computes it (its stage-2 evaluation at the increment, its
+    two-buffer update): 1e4-1e5 steps per interval, so short runs on a small model."""
+    S = len(json.load(open(META))[variant]["state_names"])
+    o = oracle.OracleSystem(variant, "ESDIRK4O32", 1, per_node_state=True)
+    base = o.states()[0]
+    rng = np.random.Generator(np.random.PCG64(42))
+    y0 = base[None, :] * (1.0 + rng.uniform(-0.01, 0.01, (3, S)))
+    t = np.arange(nsteps + 1) * dt
+    want, want_c = run_oracle(variant, "ESDIRK4O32", y0, t)
+    got, got_c, _ = run_harness(variant, "ESDIRK4O32", y0, t)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(got_c[1:], want_c[1:])
+    assert want_c[3] > 0
 
 
 @pytest.mark.parametrize("variant,dt,nsteps,tol", [

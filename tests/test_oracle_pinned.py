@@ -21,7 +21,7 @@ import pytest
 import oracle
 
 EXPLICIT = ["ExplicitEuler", "RK2", "RK4", "RL1", "RL2", "GRL1", "GRL2"]
-ALL = EXPLICIT + ["ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver"]
+ALL = EXPLICIT + ["ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver", "BasicImplicitEuler"]
 
 needs_refpin = pytest.mark.skipif(not oracle.refpin_available(),
                                   reason="reference build oracle/_ref/libgoss_refpin.so not present")
@@ -31,10 +31,15 @@ needs_refpin = pytest.mark.skipif(not oracle.refpin_available(),
 CASES = [
     ("ref_fhn", ALL, 0.1, 400),
     ("ref_beeler_reuter_1977", ALL, 0.05, 400),
-    ("ref_winslow", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "ThetaSolver"], 0.01, 200),
+    ("ref_winslow", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "ThetaSolver",
+                     "BasicImplicitEuler"], 0.01, 200),
     ("ref_winslow", ["RKF32"], 0.01, 20),
-    ("ref_panfilov", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver"], 0.1,
-     200),
+    ("ref_panfilov", ["ExplicitEuler", "RK2", "RK4", "ImplicitEuler", "ESDIRK23a", "RKF32", "ThetaSolver",
+                      "BasicImplicitEuler"], 0.1, 200),
+    # ESDIRK4O32 as the reference computes it takes 1e4-1e6 steps per interval (its stage-2
+    # evaluation at the increment and its two-buffer update, ESDIRK4O32.cpp:163-297): short runs.
+    ("ref_fhn", ["ESDIRK4O32"], 0.05, 3),
+    ("ref_panfilov", ["ESDIRK4O32"], 0.1, 4),
 ]
 
 
@@ -215,3 +220,43 @@ def test_fd_jacobian_of_linear_model_is_its_matrix():
         delta = np.sqrt(1e-15 * np.maximum(1e-5, np.abs(y)))
         f = np.abs(v.eval(np.array(y), 0.0, p)).max() + 1.0
         assert np.all(np.abs(J - exact) <= 4 * 2.2e-16 * f / delta.min())
+
+
+@needs_refpin
+@pytest.mark.parametrize("model,dt,steps", [("ref_fhn", 0.05, 3), ("ref_panfilov", 0.1, 4)])
+def test_esdirk4o32_counters_equal_reference(model, dt, steps):
+    """ESDIRK4O32's accepted and rejected steps and right-hand-side evaluations
+    (AdaptiveImplicitSolver::num_accepted / num_rejected, ESDIRK4O32::nfevals)
+    equal the reference's, serially."""
+    oracle.register_reference_models()
+    v = oracle.Variant(model)
+    y0, _ = _inputs(v, 3, 8)
+    t = np.arange(steps + 1) * dt
+    o = oracle.OracleSystem(model, "ESDIRK4O32", 3, {"atol": 1e-6, "rtol": 1e-7})
+    r = oracle.RefSystem(model, "ESDIRK4O32", 3, {"atol": 1e-6, "rtol": 1e-7})
+    for s in (o, r):
+        s.set_states(y0)
+        s.forward_times(t)
+    assert np.array_equal(o.states(), r.states())
+    assert list(o.esdirk4o32_counters()) == list(r.esdirk4o32_counters())
+    assert o.esdirk4o32_counters()[0] > 0
+
+
+@needs_refpin
+def test_basic_implicit_euler_internal_time_step_equals_reference():
+    """BasicImplicitEuler with sub-steps: the reference's residual and clock use the whole
+    interval while the Jacobian uses the sub-step (BasicImplicitEuler.cpp:123,168); the
+    oracle follows it bit for bit."""
+    oracle.register_reference_models()
+    for model, dt in (("ref_panfilov", 0.1), ("ref_beeler_reuter_1977", 0.02)):
+        v = oracle.Variant(model)
+        y0, _ = _inputs(v, 4, 9)
+        t = np.arange(41) * dt
+        params = {"internal_time_step": dt / 2}
+        o = oracle.OracleSystem(model, "BasicImplicitEuler", 4, params)
+        r = oracle.RefSystem(model, "BasicImplicitEuler", 4, params)
+        for s in (o, r):
+            s.set_states(y0)
+            s.forward_times(t)
+        assert np.all(np.isfinite(r.states()))
+        assert np.array_equal(o.states(), r.states()), model

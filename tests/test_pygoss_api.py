@@ -211,3 +211,15 @@ def test_run_time_variant_compiles_at_first_device_use_only():
     ode = goss.ParameterizedODE(OSC, field_states=["y"])
     assert ode.variant.startswith("jit_") and ode._pending is not None
     assert ode.num_field_states == 1 and ode.field_state_indices == [1]
+
+
+def test_basic_implicit_euler_and_esdirk4o32_classes():
+    # Both exist in the reference's Python layer but are left out of its solver lists as not working as
+    # expected (python/goss/solvers.py:279-282, 389-392, 423-491); the same here.
+    assert "BasicImplicitEuler" not in solvers.solver_mapper and "ESDIRK4O32" not in solvers.solver_mapper
+    b = solvers.BasicImplicitEuler()
+    assert b.is_adaptive is False and b.parameters["relative_tolerance"] == 1e-12
+    e = solvers.ESDIRK4O32()
+    assert e.is_adaptive is True and e.iord == 3 and e.atol == 1e-5
+    e.set_tol(1e-6, 1e-9)
+    assert e.device_params()["atol"] == 1e-6 and e.device_params()["rtol"] == 1e-9
