@@ -197,6 +197,16 @@ struct goss_hip_system {
   // field vector of dof_values doubles.
   int* d_dofs = nullptr;
   uint64_t dof_values = 0;
+  // Adjoint backward pipeline (segment recomputes one segment ahead of the reverse sweeps).
+  hipStream_t aux = nullptr;
+  hipEvent_t adj_ev[5] = {};
+
+  void ensure_adjoint_pipeline()
+  {
+    if (aux) return;
+    hip_check(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipEvent_t& e : adj_ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  }
 
   size_t elem() const { return prec == kFP64 ? sizeof(double) : sizeof(float); }
   bool is_f64() const { return prec == kFP64; }
@@ -433,6 +443,10 @@ struct goss_hip_system {
     if (ib.error) (void)hipFree(ib.error);
     if (h_error) (void)hipHostFree(h_error);
     if (d_dofs) (void)hipFree(d_dofs);
+    if (aux) (void)hipStreamSynchronize(aux);
+    for (hipEvent_t e : adj_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (aux) (void)hipStreamDestroy(aux);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -933,6 +947,8 @@ int goss_hip_system_forward_times_noalloc(goss_hip_system* s, const double* t, u
   });
 }
 
+}  // extern "C"
+
 namespace {
 
 // Validates an adjoint request and returns the number of checkpoints and samples.
@@ -955,6 +971,29 @@ void adjoint_check(goss_hip_system* s, uint64_t n, uint32_t sample_every, uint32
   *n_samples = steps / sample_every;
 }
 
+// The caller's segment buffer of goss_hip_adjoint_backward: the recomputed states of a checkpoint
+// segment, twice when checkpoint_every > 1 (one buffer is recomputed while the reverse sweep reads
+// the other); then the reverse sweep's per-node tape (FP64) and work rows (RK4's stage inputs);
+// then lam between segments ([S][ld]) and the parameter gradients ([NFP][ld], FP64).
+struct SegmentLayout {
+  size_t seg2, tape, lam, pbar, total;
+};
+
+SegmentLayout adjoint_segment_layout(const goss_hip_system* s, uint32_t checkpoint_every)
+{
+  const size_t per = size_t(s->v->S) * s->ld * s->elem();
+  const int ai = adjoint_index(s->solver);
+  const size_t tape = (s->prec == kFP64 ? size_t(s->v->vjp_tape_slots[ai]) * s->ld * sizeof(double) : 0) +
+                      size_t(s->v->vjp_work_rows[ai]) * s->ld * s->elem();
+  SegmentLayout l;
+  l.seg2 = size_t(checkpoint_every) * per;
+  l.tape = l.seg2 * (checkpoint_every > 1 ? 2 : 1);
+  l.lam = l.tape + tape;
+  l.pbar = l.lam + per;
+  l.total = l.pbar + size_t(std::max(s->v->NFP, 1)) * s->ld * 8;
+  return l;
+}
+
 std::vector<double2> make_steps(goss_hip_system* s, const double* t, uint64_t n)
 {
   std::vector<double2> st(n - 1);
@@ -965,6 +1004,8 @@ std::vector<double2> make_steps(goss_hip_system* s, const double* t, uint64_t n)
 
 }  // namespace
 
+extern "C" {
+
 int goss_hip_adjoint_buffer_sizes(goss_hip_system* s, uint64_t n, uint32_t sample_every, uint32_t checkpoint_every,
                                   uint64_t* checkpoint_bytes, uint64_t* segment_bytes, uint64_t* num_samples)
 {
@@ -974,13 +1015,7 @@ int goss_hip_adjoint_buffer_sizes(goss_hip_system* s, uint64_t n, uint32_t sampl
     adjoint_check(s, n, sample_every, checkpoint_every, &nc, &ns);
     const uint64_t per = uint64_t(s->v->S) * s->ld * s->elem();
     *checkpoint_bytes = nc * per;
-    // The segment buffer holds the recomputed states of one checkpoint segment,
-    // then the reverse sweep's per-node tape (FP64) and work rows (RK4's stage inputs).
-    // Behind them: lam between segments ([S][ld]) and the parameter gradients ([NFP][ld], FP64).
-    const int ai = adjoint_index(s->solver);
-    const uint64_t tape = (s->prec == kFP64 ? uint64_t(s->v->vjp_tape_slots[ai]) * s->ld * sizeof(double) : 0) +
-                          uint64_t(s->v->vjp_work_rows[ai]) * s->ld * s->elem();
-    *segment_bytes = uint64_t(checkpoint_every) * per + tape + per + uint64_t(std::max(s->v->NFP, 1)) * s->ld * 8;
+    *segment_bytes = adjoint_segment_layout(s, checkpoint_every).total;
     *num_samples = ns;
   });
 }
@@ -1041,18 +1076,25 @@ int goss_hip_adjoint_backward(goss_hip_system* s, const double* t, uint64_t n, u
     a.shape = AdjointShape{int(st.size()), int(checkpoint_every), int(sample_every), s->v->fsi[0]};
     a.ckpt = const_cast<void*>(checkpoints);
     a.grad_samples = grad_samples;
-    a.seg = segment;
-    const size_t per = size_t(s->v->S) * s->ld * s->elem();
-    const int ai = adjoint_index(s->solver);
-    const size_t tape = (s->prec == kFP64 ? size_t(s->v->vjp_tape_slots[ai]) * s->ld * sizeof(double) : 0) +
-                        size_t(s->v->vjp_work_rows[ai]) * s->ld * s->elem();
+    const SegmentLayout l = adjoint_segment_layout(s, checkpoint_every);
     char* base = static_cast<char*>(segment);
-    a.tape = base + size_t(checkpoint_every) * per;
-    a.lam = base + size_t(checkpoint_every) * per + tape;
-    a.pbar = reinterpret_cast<double*>(base + size_t(checkpoint_every) * per + tape + per);
+    a.seg = segment;
+    a.seg2 = base + l.seg2;
+    a.tape = base + l.tape;
+    a.lam = base + l.lam;
+    a.pbar = reinterpret_cast<double*>(base + l.pbar);
     a.grad_y0 = grad_states;
     a.grad_fp = grad_field_parameters;
     a.stream = s->stream;
+    if (checkpoint_every > 1) {
+      s->ensure_adjoint_pipeline();
+      a.aux = s->aux;
+      a.ev_start = s->adj_ev[0];
+      a.ev_rc[0] = s->adj_ev[1];
+      a.ev_rc[1] = s->adj_ev[2];
+      a.ev_rev[0] = s->adj_ev[3];
+      a.ev_rev[1] = s->adj_ev[4];
+    }
     hip_check(launch(a), "kernel launch");
   });
 }

@@ -76,12 +76,20 @@ struct AdjointArgs {
   double* samples;         // record: [n_samples][N]
   const double* grad_samples;  // backward: [n_samples][N]
   void* seg;               // backward: [ckpt_every][S][ld]
+  void* seg2;              // backward, ckpt_every > 1: a second [ckpt_every][S][ld] buffer (the pipeline's other half)
   void* tape;              // backward: FP64 tape [vjp_tape_slots][ld], then work rows [vjp_work_rows][ld]
   void* lam;               // backward: adjoint states between segments [S][ld]
   double* pbar;            // backward: parameter gradients between segments [NFP][ld]
   double* grad_y0;         // backward: [N][S]
   double* grad_fp;         // backward: [N][NFP]
   hipStream_t stream;
+  // backward, ckpt_every > 1: the segment recomputes run on `aux`, one segment ahead of the
+  // reverse sweeps on `stream`.  ev_start: recorded on `stream` before the call; ev_rc[b] /
+  // ev_rev[b]: recompute into / reverse sweep out of segment buffer b done.
+  hipStream_t aux;
+  hipEvent_t ev_start;
+  hipEvent_t ev_rc[2];
+  hipEvent_t ev_rev[2];
 };
 
 typedef hipError_t (*AdjointLaunch)(const AdjointArgs&);

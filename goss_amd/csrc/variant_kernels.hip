@@ -92,17 +92,40 @@ hipError_t launch_adjoint_backward(const AdjointArgs& a)
     }
     return hipSuccess;
   }
+  // Checkpoint segments, last to first.  The recompute of segment s-1 (on `aux`, into the other
+  // segment buffer) runs while the reverse sweep of segment s (on `stream`) reads its own: the
+  // two kernels fit on one SIMD together (a reverse-sweep wave and a recompute wave), so the
+  // recompute fills issue slots the single reverse-sweep wave leaves idle.  Every kernel sees
+  // the same inputs as in a serial order, so the results are the same bit for bit.
   const int nseg = (a.shape.n + a.shape.ckpt_every - 1) / a.shape.ckpt_every;
-  for (int s = nseg - 1; s >= 0; --s) {
-    hipLaunchKernelGGL((adjoint_recompute_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.stream,
+  R* const seg[2] = {static_cast<R*>(a.seg), static_cast<R*>(a.seg2)};
+  hipError_t e = hipSuccess;
+  auto recompute = [&](const int s) {
+    const int b = s & 1;
+    if (s + 2 < nseg && (e = hipStreamWaitEvent(a.aux, a.ev_rev[b], 0)) != hipSuccess) return;
+    hipLaunchKernelGGL((adjoint_recompute_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.aux,
                        static_cast<const R*>(a.ckpt), static_cast<const R*>(a.fp), a.steps, sp, R(a.delta), a.N, a.ld,
-                       a.shape, s, static_cast<R*>(a.seg));
+                       a.shape, s, seg[b]);
+    if ((e = hipGetLastError()) != hipSuccess) return;
+    e = hipEventRecord(a.ev_rc[b], a.aux);
+  };
+  if ((e = hipEventRecord(a.ev_start, a.stream)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(a.aux, a.ev_start, 0)) != hipSuccess) return e;
+  recompute(nseg - 1);
+  if (e != hipSuccess) return e;
+  for (int s = nseg - 1; s >= 0; --s) {
+    const int b = s & 1;
+    if (s > 0) {
+      recompute(s - 1);
+      if (e != hipSuccess) return e;
+    }
+    if ((e = hipStreamWaitEvent(a.stream, a.ev_rc[b], 0)) != hipSuccess) return e;
     hipLaunchKernelGGL((adjoint_reverse_kernel<M, R, SOLVER>), grid, dim3(kBlock), 0, a.stream, static_cast<const R*>(a.fp),
                        a.steps, sp, R(a.delta), a.N, a.ld, a.shape, s, a.grad_samples,
-                       static_cast<const R*>(a.seg), static_cast<R*>(a.tape), static_cast<R*>(a.lam),
+                       static_cast<const R*>(seg[b]), static_cast<R*>(a.tape), static_cast<R*>(a.lam),
                        a.pbar, int(s == nseg - 1), int(s == 0), a.grad_y0, a.grad_fp);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipEventRecord(a.ev_rev[b], a.stream)) != hipSuccess) return e;
   }
   return hipSuccess;
 }

@@ -275,8 +275,10 @@ int goss_hip_system_get_solver_counters(goss_hip_system* sys, int64_t* out);
  *
  * goss_hip_adjoint_buffer_sizes: bytes of the caller-allocated device buffers
  * (checkpoints: one state copy every `checkpoint_every` steps; segment: the
- * backward sweep's recomputed states and, in FP64, its per-node tape of
- * intermediates) and the number of samples.
+ * backward sweep's recomputed states -- two segments' worth when
+ * checkpoint_every > 1, since the recompute of one segment overlaps the
+ * reverse sweep of the next on a second stream of the system -- and, in FP64,
+ * its per-node tape of intermediates) and the number of samples.
  * goss_hip_adjoint_record: advances the system like forward_times, writing the
  * checkpoints and samples[n_samples][N] (device, FP64).
  * goss_hip_adjoint_backward: given d loss / d samples (device, [n_samples][N]),

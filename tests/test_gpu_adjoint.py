@@ -180,6 +180,88 @@ def test_adjoint_fp32_close_to_fp64():
         assert rel_l2[cols].max() <= 5e-3  # measured 3.9e-4
 
 
+def test_calibration_workload_shape_65536_nodes():
+    """The calibration workload's shape on 2,000 of its 20,000 steps: TP06 M-cell GRL1 FP32,
+    65,536 nodes, dt 0.02 ms from t = 0 (stimulus at 10 ms), V sampled every 1 ms, MSE against a
+    target from conductances x lognormal(0, 0.1), checkpoints every 200 steps (10 segments, so the
+    pipelined backward alternates its two segment buffers).  Checks:
+    * interval 200 and interval 1 (every state stored, no recompute) give the same gradients bit
+      for bit at all 65,536 nodes;
+    * on 64 of the nodes, the FP64 adjoint of the same per-node losses agrees with central
+      differences of the FP64 loss (6 conductances, V, Ca_i), and the FP32 gradients agree with the FP64 ones.  Nodes are independent, so a 64-node system gives
+      those nodes' gradients."""
+    from goss_amd.adjoint import voltage_trace
+
+    N, nsteps, se, dt = 65536, 2000, 50, 0.02
+    y0, fp = _inputs(N, 71)
+    t = np.arange(nsteps + 1) * dt
+    rng = np.random.Generator(np.random.PCG64(72))
+    target_fp = fp * np.exp(rng.normal(0.0, 0.1, fp.shape))
+    with torch.no_grad():
+        tgt = voltage_trace(dev_system(VARIANT, "GRL1", N, "fp32"), torch.tensor(y0), torch.tensor(target_fp), t,
+                            se, 200).cpu().numpy()
+    tgt_t = torch.as_tensor(tgt)
+
+    def grads(prec, y, f, ce, target):
+        sys_ = dev_system(VARIANT, "GRL1", y.shape[0], prec)
+        yt = torch.tensor(y, requires_grad=True)
+        ft = torch.tensor(f, requires_grad=True)
+        v = voltage_trace(sys_, yt, ft, t, se, ce)
+        # sum over nodes of the per-node MSE over samples: a node's gradient does not depend on N
+        loss = ((v - target.to(v.device)) ** 2).mean(dim=0).sum()
+        loss.backward()
+        return v.detach().cpu().numpy(), yt.grad.numpy(), ft.grad.numpy()
+
+    v200, gy200, gp200 = grads("fp32", y0, fp, 200, tgt_t)
+    assert v200.max() > 0.0 and np.all(np.isfinite(gy200)) and np.all(np.isfinite(gp200))
+    _, gy1, gp1 = grads("fp32", y0, fp, 1, tgt_t)
+    assert np.array_equal(gy200, gy1) and np.array_equal(gp200, gp1)
+
+    n = 64
+    sub = np.arange(0, N, N // n)  # spread over the node range
+    tsub = tgt_t[:, sub]
+    _, gy64, gp64 = grads("fp64", y0[sub], fp[sub], 200, tsub)
+    fd_sys = dev_system(VARIANT, "GRL1", n, "fp64")
+
+    def loss(y, f):
+        with torch.no_grad():
+            v = voltage_trace(fd_sys, torch.tensor(y), torch.tensor(f), t, se, 200).cpu()
+        return ((v - tsub) ** 2).mean(dim=0).numpy()  # per node
+
+    # Central differences with a relative perturbation of 1e-6: at 1e-5 some nodes' perturbed runs move
+    # a step across the -40 mV switch of the gate rates, where the discrete loss jumps (measured: 1.3e-2
+    # of the column's largest gradient for g_Kr); at 1e-7 the quotient is rounding-limited (7e-4).  At
+    # 1e-6 the largest deviation is 7.8e-5 of the column's largest gradient.
+    def check(adj, perturb, what):
+        fd = perturb(1e-6)
+        scale = np.abs(adj).max()
+        print(f"{what}: max |adjoint - central difference| {np.abs(adj - fd).max() / scale:.2e} of the largest")
+        np.testing.assert_allclose(adj, fd, rtol=1e-3, atol=1e-3 * scale)
+
+    for j in range(fp.shape[1]):
+        def fd_p(rel, j=j):
+            e = rel * fp[sub, j]
+            fpp, fpm = fp[sub].copy(), fp[sub].copy()
+            fpp[:, j] += e
+            fpm[:, j] -= e
+            return (loss(y0[sub], fpp) - loss(y0[sub], fpm)) / (2 * e)
+        check(gp64[:, j], fd_p, f"field parameter {j}")
+    for i in (17, 12):
+        def fd_y(rel, i=i):
+            e = rel * np.abs(y0[sub, i])
+            yp, ym = y0[sub].copy(), y0[sub].copy()
+            yp[:, i] += e
+            ym[:, i] -= e
+            return (loss(yp, fp[sub]) - loss(ym, fp[sub])) / (2 * e)
+        check(gy64[:, i], fd_y, f"initial state {i}")
+    for a, b, name in ((gy200[sub], gy64, "y0"), (gp200[sub], gp64, "conductances")):
+        rel_l2 = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
+        cols = np.linalg.norm(b, axis=0) > 0
+        print(f"65,536 nodes x 2,000 steps: FP32 vs FP64 d loss / d {name} on 64 nodes, max relative L2 per column "
+              f"{rel_l2[cols].max():.2e}")
+        assert rel_l2[cols].max() <= 5e-3
+
+
 def test_adjoint_requires_a_supported_solver():
     from goss_amd._lib import GossHipError
 
@@ -250,7 +332,9 @@ def test_every_state_checkpoint_sizes(prec, elem):
     assert ns == (n - 1) // 10
     ckpt200, seg200, _ = s.adjoint_buffer_sizes(n, 10, 200)
     assert ckpt200 == S * 128 * elem  # one checkpoint: the 40 steps fit in one segment
-    assert seg200 - seg == 199 * S * 128 * elem  # the segment's recomputed states: 200 rows blocks against 1
+    # Interval 200: two buffers of a segment's recomputed states (the recompute of one segment overlaps
+    # the reverse sweep of the next), 2 x 200 row blocks against 1.
+    assert seg200 - seg == 399 * S * 128 * elem
 
 
 # RK4 at dt 0.002 ms is unstable for some of these 4,096 nodes (as the torch reference shows); 0.001 is not.
