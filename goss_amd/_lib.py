@@ -92,6 +92,8 @@ class ModelInfo(ctypes.Structure):
         ("default_parameters", ctypes.POINTER(ctypes.c_double)),
         ("initial_conditions", ctypes.POINTER(ctypes.c_double)),
         ("linear_terms", ctypes.POINTER(ctypes.c_ubyte)),
+        ("differential_states", ctypes.POINTER(ctypes.c_ubyte)),
+        ("is_dae", ctypes.c_int),
     ]
 
 

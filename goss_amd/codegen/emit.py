@@ -237,6 +237,10 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
     L.append("constexpr double default_parameters[] = {" + ", ".join(fmt_double(p.value) for p in model.parameters) + "};")
     L.append("constexpr double initial_conditions[] = {" + ", ".join(fmt_double(s.init) for s in model.states) + "};")
     L.append("constexpr unsigned char linear_terms[] = {" + ", ".join(str(f) for f in lin.linear_terms) + "};")
+    L.append("// Diagonal of the mass matrix (ODE::_differential_states): 0 for the algebraic states of a DAE.")
+    L.append("constexpr unsigned char differential_states[] = {"
+             + ", ".join(str(f) for f in model.differential_states) + "};")
+    L.append(f"constexpr bool is_dae = {'true' if model.is_dae else 'false'};")
     L.append("")
 
     # -- constants ------------------------------------------------------------
@@ -297,7 +301,7 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
 
     # -- structure of the Jacobian and of its LU factors (implicit solvers) ----
     struct_rows = dependence_pattern(eval_stmts, rhs_out, model.state_names)
-    sL, struct_jnnz, struct_lnnz = emit_sparse_structure(S, struct_rows, ind)
+    sL, struct_jnnz, struct_lnnz = emit_sparse_structure(S, struct_rows, ind, model.differential_states)
     L += sL
 
     # -- linearized_eval ----------------------------------------------------
@@ -663,6 +667,12 @@ def emit_parameterized_ode(var: Variant, header_include: str) -> str:
         L.append(f'    _field_parameter_names[{i}] = "{p}";')
     L.append("    _bind();")
     L.append(f"    for (int i = 0; i < {S}; ++i) _linear_terms[i] = {ns}::linear_terms[i];")
+    if model.is_dae:
+        # codegeneration.py:452-459: the algebraic states are not differential.
+        for i, d in enumerate(model.differential_states):
+            if not d:
+                L.append(f"    _differential_states[{i}] = 0;")
+        L.append("    _is_dae = true;")
     L.append("  }")
     L.append(f"  {N}(const {N}& ode) : ParameterizedODE(ode)")
     L.append("  {")
@@ -737,6 +747,8 @@ def variant_metadata(var: Variant, em: Emitted) -> dict:
         "field_parameters": var.field_parameters,
         "field_parameter_indices": var.field_parameter_indices,
         "linear_terms": em.linear_terms,
+        "differential_states": m.differential_states,
+        "is_dae": m.is_dae,
         "census": em.census,
         "jacobian_nnz": em.jac_nnz,
         "jacobian_structure_nnz": em.structure_nnz[0],
@@ -745,7 +757,8 @@ def variant_metadata(var: Variant, em: Emitted) -> dict:
     }
 
 
-def emit_sparse_structure(S: int, rows: Sequence[set], ind: str = "  ") -> Tuple[List[str], int, int]:
+def emit_sparse_structure(S: int, rows: Sequence[set], ind: str = "  ",
+                          differential: Sequence[int] = None) -> Tuple[List[str], int, int]:
     """Structure of the Jacobian (rows[j] = states output j reads) and of its
     no-pivot LU factors, with straight-line LU factorisation and substitution
     over that structure.  The code follows ODE::lu_factorize (ODE.cpp:69-100)
@@ -787,20 +800,24 @@ def emit_sparse_structure(S: int, rows: Sequence[set], ind: str = "  ") -> Tuple
     L.append("constexpr short lu_slot[] = {" + ", ".join(str(lu_slot.get((j, i), -1)) for j in range(S)
                                                        for i in range(S)) + "};")
     L.append("")
-    L.append("// a = I + scale * J on the LU structure (fill entries zero): ODE.cpp's dense scale-and-add-identity")
-    L.append("// of ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105).")
+    diff = list(differential) if differential is not None else [1] * S
+    L.append("// a = M + scale * J on the LU structure (fill entries zero), M the diagonal mass matrix")
+    L.append("// (differential_states): the dense scale and add_mass_matrix of")
+    L.append("// ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105, 258-262).  The")
+    L.append("// algebraic states of a DAE add 0.0, as the reference's weight * 0 does.")
     L.append("template <typename real>")
     L.append("GOSS_HD GOSS_INLINE void lu_from_jacobian(const real* __restrict__ J, const real scale, real* __restrict__ a)")
     L.append("{")
     for j in range(S):
         for i in sorted(lu[j]):
             s = lu_slot[(j, i)]
+            m = "real(1.0)" if diff[i] else "real(0.0)"
             if (j, i) in jac_slot:
                 e = f"J[{jac_slot[(j, i)]}] * scale"
                 if i == j:
-                    e += " + real(1.0)"
+                    e += f" + {m}"
             else:
-                e = "real(1.0)" if i == j else "real(0.0)"
+                e = m if i == j else "real(0.0)"
             L.append(f"{ind}a[{s}] = {e};")
     L.append("}")
     L.append("")

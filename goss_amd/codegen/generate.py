@@ -36,6 +36,8 @@ VARIANTS: Dict[str, tuple] = {
     "beeler_reuter_1977": ("beeler_reuter_1977", ["V"], []),
     "oscilator": ("oscilator", ["x", "y"], ["a", "b"]),
     "linear_decay": ("linear_decay", ["y"], ["a", "b"]),
+    "van_der_pol_dae": ("van_der_pol_dae", ["y"], []),
+    "markov_three_state": ("markov_three_state", ["O"], ["k_oi"]),
 }
 
 

@@ -23,7 +23,20 @@ Supported language: ``parameters``, ``states``, ``expressions``,
 ``component``, ``ScalarParam``, ``Conditional``, ``ContinuousConditional``,
 ``Lt/Le/Gt/Ge/Eq/Ne``, ``And/Or/Not``, ``exp/log/sqrt/abs/floor/ceil``,
 trigonometric and hyperbolic functions, ``**`` and the ``time`` (alias ``t``)
-symbol.
+symbol, and two forms that gotran offers besides ``d<X>_dt = ...``:
+
+* algebraic equations of a DAE, ``alg_<X>_0 = expr`` (the state ``X`` is
+  defined by ``0 = expr``).  Such a state is not differential: its entry of the
+  diagonal mass matrix is 0 (``ODE::_differential_states``, ``ODE.h:80-102``,
+  which the generated plugin sets as ``codegeneration.py:452-459`` does).
+  Explicit solvers refuse such a model (``GRL1.cpp:56-58`` and the other
+  explicit solvers); the implicit ones put the mass matrix into the Newton
+  residual and the iteration matrix (``ImplicitODESolver.cpp:140,258-262``);
+* Markov chains, ``rates[A, B] = expr``: a transition from state ``A`` to
+  state ``B`` at rate ``expr``.  Each state with rates and no derivative of its
+  own gets ``dX_dt = sum_Y rates[Y, X] * Y - (sum_Y rates[X, Y]) * X`` (inflows
+  in the order the rates are written, then the outflow), numbered after the
+  explicitly defined derivatives in the order the states were declared.
 """
 from __future__ import annotations
 
@@ -72,6 +85,16 @@ class ODEModel:
     body: List[Tuple[str, sp.Expr]]
     derivatives: Dict[str, str]
     symbols: Dict[str, sp.Symbol] = field(default_factory=dict)
+    algebraic: List[str] = field(default_factory=list)  # states defined by alg_<X>_0 = expr
+
+    @property
+    def is_dae(self) -> bool:
+        return bool(self.algebraic)
+
+    @property
+    def differential_states(self) -> List[int]:
+        """Diagonal of the mass matrix: 1 for a differential state, 0 for an algebraic one."""
+        return [0 if s.name in self.algebraic else 1 for s in self.states]
 
     @property
     def num_states(self) -> int:
@@ -302,6 +325,8 @@ def parse_ode_text(text: str, name: str) -> ODEModel:
     symbols: Dict[str, sp.Symbol] = {}
     derivative_order: List[str] = []
     derivatives: Dict[str, str] = {}
+    algebraic: List[str] = []
+    rates: Dict[str, List[Tuple[str, str]]] = {}  # state -> [(other state, rate name)], in and out
     component = name
 
     def declare(kind, call: ast.Call):
@@ -337,6 +362,32 @@ def parse_ode_text(text: str, name: str) -> ODEModel:
                 pass
             else:
                 raise ODEParseError(f"unsupported statement {fn!r} (line {stmt.lineno})")
+        elif (isinstance(stmt, ast.Assign) and len(stmt.targets) == 1
+              and isinstance(stmt.targets[0], ast.Subscript)):
+            tgt = stmt.targets[0]
+            sl = tgt.slice
+            if not (isinstance(tgt.value, ast.Name) and tgt.value.id == "rates" and isinstance(sl, ast.Tuple)
+                    and len(sl.elts) == 2 and all(isinstance(e, ast.Name) for e in sl.elts)):
+                raise ODEParseError(f"unsupported assignment (line {stmt.lineno}); rates[A, B] = ... is the only "
+                                    "subscripted form")
+            a, b = sl.elts[0].id, sl.elts[1].id
+            for x in (a, b):
+                if x not in state_decl:
+                    raise ODEParseError(f"rates[{a}, {b}]: {x!r} is not a state (line {stmt.lineno})")
+            if a == b:
+                raise ODEParseError(f"rates[{a}, {b}]: a transition needs two states (line {stmt.lineno})")
+            rname = f"rates_{a}_{b}"
+            if rname in names:
+                raise ODEParseError(f"rates[{a}, {b}] given twice (line {stmt.lineno})")
+            expr = _ExprBuilder(names).build(stmt.value)
+            if isinstance(expr, (_Cond, tuple)):
+                raise ODEParseError(f"rates[{a}, {b}] is not an arithmetic expression")
+            body.append((rname, sp.sympify(expr)))
+            sym = sp.Symbol(rname, real=True)
+            names[rname] = sym
+            symbols[rname] = sym
+            rates.setdefault(a, []).append((b, rname, "out"))
+            rates.setdefault(b, []).append((a, rname, "in"))
         elif isinstance(stmt, ast.Assign):
             if len(stmt.targets) != 1 or not isinstance(stmt.targets[0], ast.Name):
                 raise ODEParseError(f"unsupported assignment (line {stmt.lineno})")
@@ -351,21 +402,47 @@ def parse_ode_text(text: str, name: str) -> ODEModel:
             sym = sp.Symbol(lhs, real=True)
             names[lhs] = sym
             symbols[lhs] = sym
+            st = None
             if lhs.startswith("d") and lhs.endswith("_dt") and lhs[1:-3] in state_decl:
                 st = lhs[1:-3]
+            elif lhs.startswith("alg_") and lhs.endswith("_0") and lhs[4:-2] in state_decl:
+                st = lhs[4:-2]
+                algebraic.append(st)
+            if st is not None:
                 if st in derivatives:
-                    raise ODEParseError(f"second derivative for {st!r}")
+                    raise ODEParseError(f"second derivative or algebraic equation for {st!r}")
                 derivatives[st] = lhs
                 derivative_order.append(st)
         else:
             raise ODEParseError(f"unsupported statement at line {getattr(stmt, 'lineno', '?')}")
+
+    # Markov chains: a derivative for every state with rates and none of its own.
+    for st in state_decl_order:
+        if st not in rates:
+            continue
+        if st in derivatives:
+            raise ODEParseError(f"{st!r} has both rates and its own derivative")
+        ins = [names[rn] * names[other] for other, rn, d in rates[st] if d == "in"]
+        outs = [names[rn] for other, rn, d in rates[st] if d == "out"]
+        expr = sp.Add(*ins, evaluate=False) if ins else sp.Integer(0)
+        if outs:
+            expr = expr - sp.Add(*outs, evaluate=False) * names[st]
+        lhs = f"d{st}_dt"
+        if lhs in names:
+            raise ODEParseError(f"{lhs!r} is already a name")
+        body.append((lhs, expr))
+        sym = sp.Symbol(lhs, real=True)
+        names[lhs] = sym
+        symbols[lhs] = sym
+        derivatives[st] = lhs
+        derivative_order.append(st)
 
     missing = [s for s in state_decl_order if s not in derivatives]
     if missing:
         raise ODEParseError(f"states without a derivative: {missing}")
     states = [state_decl[s] for s in derivative_order]
     return ODEModel(name=name, states=states, parameters=params, body=body,
-                    derivatives=derivatives, symbols=symbols)
+                    derivatives=derivatives, symbols=symbols, algebraic=algebraic)
 
 
 def load_ode(path: str, name: Optional[str] = None) -> ODEModel:

@@ -19,6 +19,9 @@ static const char* const field_parameter_names[] = {nullptr};
 constexpr double default_parameters[] = {0.13, 0.013, 0.26, 0.1, 1.0, 50.0, 1.0, 1000.0, 1.0, 40.0, -85.0};
 constexpr double initial_conditions[] = {-85.0, 0.0};
 constexpr unsigned char linear_terms[] = {0, 1};
+// Diagonal of the mass matrix (ODE::_differential_states): 0 for the algebraic states of a DAE.
+constexpr unsigned char differential_states[] = {1, 1};
+constexpr bool is_dae = false;
 
 // Subexpressions of shared (non-field) parameters only, evaluated once per launch.
 // c[k] may use c[j] for j < k.  Called with real = double on the host.
@@ -105,8 +108,10 @@ constexpr int jac_col_start[] = {0, 2};
 // LU storage is row-major over its structure: lu_slot[j * num_states + i], -1 outside.
 constexpr short lu_slot[] = {0, 1, 2, 3};
 
-// a = I + scale * J on the LU structure (fill entries zero): ODE.cpp's dense scale-and-add-identity
-// of ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105).
+// a = M + scale * J on the LU structure (fill entries zero), M the diagonal mass matrix
+// (differential_states): the dense scale and add_mass_matrix of
+// ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105, 258-262).  The
+// algebraic states of a DAE add 0.0, as the reference's weight * 0 does.
 template <typename real>
 GOSS_HD GOSS_INLINE void lu_from_jacobian(const real* __restrict__ J, const real scale, real* __restrict__ a)
 {

@@ -1,0 +1,313 @@
+// Generated by goss_amd.codegen from van_der_pol_dae.ode -- do not edit.
+// Variant van_der_pol_dae: field states ['y'], field parameters [].
+#pragma once
+#include "goss_model_support.h"
+
+namespace goss_models { namespace van_der_pol_dae {
+
+constexpr int num_states = 2;
+constexpr int num_parameters = 0;
+constexpr int num_constants = 0;
+constexpr int num_field_states = 1;
+constexpr int num_field_parameters = 0;
+constexpr int field_state_indices[] = {0};
+constexpr int field_parameter_indices[] = {-1};
+static const char* const state_names[] = {"y", "z"};
+static const char* const parameter_names[] = {};
+static const char* const field_state_names[] = {"y"};
+static const char* const field_parameter_names[] = {nullptr};
+constexpr double default_parameters[] = {};
+constexpr double initial_conditions[] = {0.6666666666666665, 2.0};
+constexpr unsigned char linear_terms[] = {0, 0};
+// Diagonal of the mass matrix (ODE::_differential_states): 0 for the algebraic states of a DAE.
+constexpr unsigned char differential_states[] = {1, 0};
+constexpr bool is_dae = true;
+
+// Subexpressions of shared (non-field) parameters only, evaluated once per launch.
+// c[k] may use c[j] for j < k.  Called with real = double on the host.
+template <typename real, typename PA>
+GOSS_HD GOSS_INLINE void compute_constants(const PA& pa, real* __restrict__ c)
+{
+  (void)pa; (void)c;
+}
+
+// Right-hand side: values[i] = d states[i] / dt.  Intermediates that depend on time but on
+// no state are computed in the type of `time` (treal).
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval(const real* __restrict__ states, const treal time, const PA& pa,
+                              real* __restrict__ values)
+{
+  (void)time; (void)pa;
+  const real y = states[0];
+  const real z = states[1];
+  values[0] = -z;
+  values[1] = y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z));
+}
+
+constexpr int num_intermediates = 0;
+static const char* const intermediate_names[] = {nullptr};
+// Every named intermediate of the model at (states, time): the values a caller may monitor
+// (ParameterizedODE::eval_monitored, ParameterizedODE.h:90-105, selects among them).
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval_intermediates(const real* __restrict__ states, const treal time, const PA& pa,
+                                            real* __restrict__ out)
+{
+  (void)time; (void)pa; (void)states; (void)out;
+}
+
+// Structure of the Jacobian d values[j] / d states[i]: values[j] reads states[i] only for the
+// entries below (through any operand of the emitted right-hand side), so every other entry of a
+// forward-difference Jacobian is exactly zero.  And the structure of its no-pivot LU factors
+// (the entries, the diagonal and the fill).
+constexpr int jac_nnz = 3;
+constexpr int lu_nnz = 4;
+// jac_cols[i]: bit j set when values[j] reads states[i].  Jacobian storage is column-major over the
+// structure: entry (j, i) at jac_col_start[i] + (number of bits of jac_cols[i] below j).
+constexpr unsigned jac_cols[] = {0x2u, 0x3u};
+constexpr int jac_col_start[] = {0, 1};
+// LU storage is row-major over its structure: lu_slot[j * num_states + i], -1 outside.
+constexpr short lu_slot[] = {0, 1, 2, 3};
+
+// a = M + scale * J on the LU structure (fill entries zero), M the diagonal mass matrix
+// (differential_states): the dense scale and add_mass_matrix of
+// ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105, 258-262).  The
+// algebraic states of a DAE add 0.0, as the reference's weight * 0 does.
+template <typename real>
+GOSS_HD GOSS_INLINE void lu_from_jacobian(const real* __restrict__ J, const real scale, real* __restrict__ a)
+{
+  a[0] = real(1.0);
+  a[1] = J[1] * scale;
+  a[2] = J[0] * scale;
+  a[3] = J[2] * scale + real(0.0);
+}
+
+// ODE::lu_factorize (ODE.cpp:69-100) in place on a, over the structure.
+template <typename real>
+GOSS_HD GOSS_INLINE void lu_factorize_sparse(real* __restrict__ a)
+{
+  real acc;
+  a[2] = a[2] / a[0];
+  acc = a[2] * a[1];
+  a[3] = a[3] - acc;
+  (void)acc;
+}
+
+// ODE::forward_backward_subst (ODE.cpp:102-132) in place on x, over the structure.
+template <typename real>
+GOSS_HD GOSS_INLINE void lu_solve_sparse(const real* __restrict__ a, real* __restrict__ x)
+{
+  real acc;
+  acc = a[2] * x[0];
+  x[1] = x[1] - acc;
+  x[1] = x[1] / a[3];
+  acc = a[1] * x[1];
+  x[0] = (x[0] - acc) / a[0];
+  (void)acc;
+}
+
+// Linearized right-hand side: rhs[i] as in eval and linear[i] = d rhs[i] / d states[i].
+// Entries with linear_terms[i] == 1 are written before the only_linear return.
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void linearized_eval(const real* __restrict__ states, const treal time, const PA& pa,
+                                         real* __restrict__ linear, real* __restrict__ rhs,
+                                         const bool only_linear)
+{
+  (void)time; (void)pa;
+  const real y = states[0];
+  const real z = states[1];
+  rhs[0] = -z;
+  rhs[1] = y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z));
+  if (only_linear) return;
+  const real dalg_z_0_dz = real(1.0) - real(1.0)*m_sq(z);
+  linear[0] = real(0.0);
+  linear[1] = dalg_z_0_dz;
+}
+
+// linearized_eval (all entries) plus, for the gate states (gate_states[i] = 1: rhs[i] = (A - y_i) r),
+// gate_target[i] = A.  A GRL stage of a gate is then y0_i + (y_i - A) expm1(linear[i] h).
+constexpr unsigned char gate_states[] = {0, 0};
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void linearized_eval_gates(const real* __restrict__ states, const treal time, const PA& pa,
+                                               real* __restrict__ linear, real* __restrict__ rhs,
+                                               real* __restrict__ gate_target)
+{
+  (void)time; (void)pa; (void)gate_target;
+  const real y = states[0];
+  const real z = states[1];
+  rhs[0] = -z;
+  rhs[1] = y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z));
+  const real dalg_z_0_dz = real(1.0) - real(1.0)*m_sq(z);
+  linear[0] = real(0.0);
+  linear[1] = dalg_z_0_dz;
+}
+
+// Discrete adjoints of one step of size h at `time` (one step per interval):
+//   *_vjp:  lam_prev = (d y'/d y)^T lam,  fp_bar += (d y'/d p_field)^T lam;
+//   grl1: y'_i = y_i + rhs_i*h*phi1(linear_i*h) when |linear_i| > delta, else y_i + rhs_i*h (GRL1.h:72-84);
+//   rl1:  the same with the exponential update for the linear_terms states only (RL1.h:68-77);
+//   for a gate (rhs_i = (A - y_i) r) the exponential update is A + (y_i - A) exp(linear_i*h).
+//   eval_vjp: y_bar += (d rhs/d y)^T seed, fp_bar += (d rhs/d p_field)^T seed (the explicit schemes).
+// Reverse accumulation over the generated statements; the *_tape forms keep every intermediate
+// and adjoint in a caller-provided tape (for targets whose registers cannot hold the sweep).
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void grl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real y = states[0];
+  const real z = states[1];
+  const real dalg_z_0_dz = real(1.0) - real(1.0)*m_sq(z);
+  const real rhs__0 = -z;
+  const real rhs__1 = y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z));
+  const real lin__0 = real(0.0);
+  const real lin__1 = dalg_z_0_dz;
+  const real hinv__ = m_rcp(h__);
+  real bbar__0, abar__0;
+  m_grl1_seeds(real(lin__0), m_rcp(real(lin__0)), real(rhs__0), lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(lin__1), m_rcp(real(lin__1)), real(rhs__1), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
+  real dalg_z_0_dz__b = real(0.0);
+  real y__b = real(0.0);
+  real z__b = real(0.0);
+  dalg_z_0_dz__b += abar__1*(real(1.0));
+  y__b += bbar__1*(real(1.0));
+  z__b += bbar__1*(real(1.0) - real(1.0)*m_sq(z));
+  z__b += bbar__0*(-real(1.0));
+  z__b += dalg_z_0_dz__b*(-real(2.0)*z);
+  lam_prev__[0] = lam__[0] + y__b;
+  lam_prev__[1] = lam__[1] + z__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP, typename LV, typename OV>
+GOSS_HD GOSS_INLINE void grl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const LV& lam__,
+                                  const OV& lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real y = states[0];
+  const real z = states[1];
+  tape.put(0, real(1.0) - real(1.0)*m_sq(z));
+  tape.put(1, -z);
+  tape.put(2, y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z)));
+  tape.put(3, real(0.0));
+  tape.put(4, tape.get(0));
+  const real hinv__ = m_rcp(h__);
+  real bbar__0, abar__0;
+  m_grl1_seeds(real(tape.get(3)), m_rcp(real(tape.get(3))), real(tape.get(1)), lam__[0], h__, hinv__, delta__, bbar__0, abar__0);
+  real bbar__1, abar__1;
+  m_grl1_seeds(real(tape.get(4)), m_rcp(real(tape.get(4))), real(tape.get(2)), lam__[1], h__, hinv__, delta__, bbar__1, abar__1);
+  tape.put(5, real(0.0));
+  tape.put(6, real(0.0));
+  tape.put(7, real(0.0));
+  tape.add(5, abar__1*(real(1.0)));
+  tape.add(6, bbar__1*(real(1.0)));
+  tape.add(7, bbar__1*(real(1.0) - real(1.0)*m_sq(z)));
+  tape.add(7, bbar__0*(-real(1.0)));
+  tape.add(7, tape.get(5)*(-real(2.0)*z));
+  lam_prev__[0] = lam__[0] + tape.get(6);
+  lam_prev__[1] = lam__[1] + tape.get(7);
+}
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void rl1_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const real* __restrict__ lam__,
+                                  real* __restrict__ lam_prev__, real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real y = states[0];
+  const real z = states[1];
+  const real rhs__0 = -z;
+  const real rhs__1 = y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z));
+  const real bbar__0 = lam__[0]*h__;
+  const real bbar__1 = lam__[1]*h__;
+  real y__b = real(0.0);
+  real z__b = real(0.0);
+  y__b += bbar__1*(real(1.0));
+  z__b += bbar__1*(real(1.0) - real(1.0)*m_sq(z));
+  z__b += bbar__0*(-real(1.0));
+  lam_prev__[0] = lam__[0] + y__b;
+  lam_prev__[1] = lam__[1] + z__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP, typename LV, typename OV>
+GOSS_HD GOSS_INLINE void rl1_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real h__, const real delta__, const LV& lam__,
+                                  const OV& lam_prev__, real* __restrict__ fp_bar__,
+                                  const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__; (void)delta__;
+  const real y = states[0];
+  const real z = states[1];
+  tape.put(0, -z);
+  tape.put(1, y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z)));
+  const real bbar__0 = lam__[0]*h__;
+  const real bbar__1 = lam__[1]*h__;
+  tape.put(2, real(0.0));
+  tape.put(3, real(0.0));
+  tape.add(2, bbar__1*(real(1.0)));
+  tape.add(3, bbar__1*(real(1.0) - real(1.0)*m_sq(z)));
+  tape.add(3, bbar__0*(-real(1.0)));
+  lam_prev__[0] = lam__[0] + tape.get(2);
+  lam_prev__[1] = lam__[1] + tape.get(3);
+}
+
+template <typename real, typename PA, typename treal>
+GOSS_HD GOSS_INLINE void eval_vjp(const real* __restrict__ states, const treal time, const PA& pa,
+                                  const real* __restrict__ seed__, real* __restrict__ y_bar__,
+                                  real* __restrict__ fp_bar__)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real y = states[0];
+  const real z = states[1];
+  const real rhs__0 = -z;
+  const real rhs__1 = y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z));
+  real y__b = real(0.0);
+  real z__b = real(0.0);
+  y__b += seed__[1]*(real(1.0));
+  z__b += seed__[1]*(real(1.0) - real(1.0)*m_sq(z));
+  z__b += seed__[0]*(-real(1.0));
+  y_bar__[0] += y__b;
+  y_bar__[1] += z__b;
+}
+
+template <typename real, typename PA, typename treal, typename TP, typename SV, typename YV>
+GOSS_HD GOSS_INLINE void eval_vjp_tape(const real* __restrict__ states, const treal time, const PA& pa,
+                                       const SV& seed__, const YV& y_bar__, real* __restrict__ fp_bar__,
+                                       const TP& tape)
+{
+  (void)time; (void)pa; (void)fp_bar__;
+  const real y = states[0];
+  const real z = states[1];
+  tape.put(0, -z);
+  tape.put(1, y - z*(-real(1.0) + real(0.3333333333333333)*m_sq(z)));
+  tape.put(2, real(0.0));
+  tape.put(3, real(0.0));
+  tape.add(2, seed__[1]*(real(1.0)));
+  tape.add(3, seed__[1]*(real(1.0) - real(1.0)*m_sq(z)));
+  tape.add(3, seed__[0]*(-real(1.0)));
+  y_bar__[0] += tape.get(2);
+  y_bar__[1] += tape.get(3);
+}
+
+constexpr int grl1_vjp_tape_slots = 8;
+constexpr int rl1_vjp_tape_slots = 4;
+constexpr int eval_vjp_tape_slots = 4;
+
+// Analytic Jacobian: jac[i*num_states + j] = d values[i] / d states[j].
+// `jac` is a pointer or any object whose jac[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JAC>
+GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const treal time,
+                                          const PA& pa, JAC jac)
+{
+  (void)time; (void)pa;
+  const real z = states[1];
+  const real dalg_z_0_dz = real(1.0) - real(1.0)*m_sq(z);
+  jac[0] = real(0.0);
+  jac[1] = -real(1.0);
+  jac[2] = real(1.0);
+  jac[3] = dalg_z_0_dz;
+}
+
+}}  // namespace goss_models::van_der_pol_dae

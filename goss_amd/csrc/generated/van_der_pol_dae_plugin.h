@@ -1,0 +1,86 @@
+// Generated by goss_amd.codegen from van_der_pol_dae.ode -- do not edit.
+#ifndef VAN_DER_POL_DAE_PLUGIN_H_IS_INCLUDED
+#define VAN_DER_POL_DAE_PLUGIN_H_IS_INCLUDED
+#include <memory>
+#include <stdexcept>
+#include <cmath>
+#include "goss/ParameterizedODE.h"
+#include "van_der_pol_dae.h"
+
+namespace goss {
+
+class Van_der_pol_dae : public ParameterizedODE
+{
+  struct Acc {
+    const double* v;
+    const double* c;
+    template <int k> double get() const { return v[k]; }
+    template <int k> double getd() const { return v[k]; }
+    template <int k> double cst() const { return c[k]; }
+    template <int k> double cstd() const { return c[k]; }
+  };
+  static constexpr int K = goss_models::van_der_pol_dae::num_constants > 0 ? goss_models::van_der_pol_dae::num_constants : 1;
+public:
+  Van_der_pol_dae() : ParameterizedODE(2, 0, 1, 0, 0)
+  {
+    for (int i = 0; i < 2; ++i) _state_names[i] = goss_models::van_der_pol_dae::state_names[i];
+    for (int i = 0; i < 0; ++i) {
+      _parameter_names[i] = goss_models::van_der_pol_dae::parameter_names[i];
+      _p[i] = goss_models::van_der_pol_dae::default_parameters[i];
+    }
+    _field_state_names[0] = "y";
+    _field_state_indices[0] = 0;
+    _bind();
+    for (int i = 0; i < 2; ++i) _linear_terms[i] = goss_models::van_der_pol_dae::linear_terms[i];
+    _differential_states[1] = 0;
+    _is_dae = true;
+  }
+  Van_der_pol_dae(const Van_der_pol_dae& ode) : ParameterizedODE(ode)
+  {
+    for (int i = 0; i < 0; ++i) _p[i] = ode._p[i];
+    _bind();
+  }
+  // Parameters can change through _param_to_value at any time, so the
+  // parameter-only constants are recomputed on every evaluation.
+  void eval(const double* states, double time, double* values)
+  {
+    double c[K];
+    goss_models::van_der_pol_dae::compute_constants<double>(Acc{_p, nullptr}, c);
+    goss_models::van_der_pol_dae::eval<double>(states, time, Acc{_p, c}, values);
+  }
+  void linearized_eval(const double* states, double time, double* linear, double* rhs,
+                       bool only_linear) const
+  {
+    double c[K];
+    goss_models::van_der_pol_dae::compute_constants<double>(Acc{_p, nullptr}, c);
+    goss_models::van_der_pol_dae::linearized_eval<double>(states, time, Acc{_p, c}, linear, rhs, only_linear);
+  }
+  void get_ic(goss::DoubleVector* values) const
+  {
+    values->n = _num_states;
+    values->data.reset(new double[_num_states]);
+    for (int i = 0; i < 2; ++i) values->data[i] = goss_models::van_der_pol_dae::initial_conditions[i];
+  }
+  std::shared_ptr<ODE> copy() const
+  { return std::make_shared<Van_der_pol_dae>(*this); }
+  void eval_monitored(const double*, double, double*) const
+  { throw std::runtime_error("No monitored in the 'Van_der_pol_dae' model."); }
+  void set_field_parameters(const double* field_params)
+  {
+    (void)field_params;
+  }
+private:
+  double _p[0];
+  void _bind()
+  {
+    for (int i = 0; i < 0; ++i) _param_to_value[_parameter_names[i]] = &_p[i];
+  }
+};
+
+}  // namespace goss
+
+extern "C" __attribute__((visibility("default"))) goss::ParameterizedODE* create_Van_der_pol_dae()
+{
+  return new goss::Van_der_pol_dae;
+}
+#endif

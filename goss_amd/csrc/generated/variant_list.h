@@ -12,4 +12,6 @@
   X(beeler_reuter_1977) \
   X(oscilator) \
   X(linear_decay) \
+  X(van_der_pol_dae) \
+  X(markov_three_state) \
 

@@ -60,6 +60,50 @@ template <class M, class R>
 struct Rkf32Node {
   static constexpr int S = M::num_states;
 
+  // AdaptiveExplicitSolver::dtinit (AdaptiveExplicitSolver.cpp:67-166) with _itol = 0, iord = 3: an
+  // initial step from the scaled norms of y0 and f0 = k1 and a difference estimate of the second
+  // derivative after one explicit Euler step.  Sums in the precision R, step sizes in FP64.
+  template <class V, class A>
+  GOSS_HD static double dtinit(const V& y0, const V& f0, const double t, const A& acc, const AdaptiveParams& ap)
+  {
+    constexpr double iord = 3.0;
+    const R atol = R(ap.atol), rtol = R(ap.rtol);
+    R dnf = 0, dny = 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const R sk = atol + rtol * fabs(y0[i]);
+      R tmp = f0[i] / sk;
+      dnf += tmp * tmp;
+      tmp = y0[i] / sk;
+      dny += tmp * tmp;
+    }
+    double dt;
+    if (double(dnf) <= 1.0e-10 || double(dny) <= 1.0e-10)
+      dt = 1.0e-6;
+    else
+      dt = 0.01 * m_sqrt(double(dny / dnf));
+    R y1[S], f1[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) y1[i] = y0[i] + R(dt) * f0[i];
+    M::template eval<R>(y1, t + dt, acc, f1);
+    R der2 = 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const R sk = atol + rtol * fabs(y1[i]);
+      const R tmp = (f1[i] - f0[i]) / sk;
+      der2 += tmp * tmp;
+    }
+    const double der2d = m_sqrt(double(der2)) / dt;
+    const double sdnf = m_sqrt(double(dnf));
+    const double der12 = fabs(der2d) >= sdnf ? fabs(der2d) : sdnf;
+    double dt1;
+    if (der12 <= 1.0e-15)
+      dt1 = fabs(dt) * 1.0e-3 < 1.0e-6 ? 1.0e-6 : fabs(dt) * 1.0e-3;
+    else
+      dt1 = m_pow(0.01 / der12, 1.0 / iord);
+    return 100 * fabs(dt) < dt1 ? 100 * fabs(dt) : dt1;
+  }
+
   // RKF32::forward for one node.  y, k1, yn and ep are HBM views (RVec: only
   // the input and output of one RHS evaluation are held in registers) or, for
   // small models, register vectors (RegVec).  Returns false when the sub-step
@@ -89,8 +133,16 @@ struct Rkf32Node {
       for (int i = 0; i < S; ++i) k1[i] = out[i];
     }
     nfevals += 1;
-    // AdaptiveExplicitSolver::_ldt is never negative, so dtinit (RKF32.cpp:134-138) is not reachable.
-    double dt = ldt_state;
+    // AdaptiveExplicitSolver::_ldt turns negative when a step overshoots the end of an interval and the
+    // backward step t_end - _t is stored as the guess (AdaptiveExplicitSolver.cpp:234-241); the next
+    // call then starts from dtinit (RKF32.cpp:134-141).
+    double dt;
+    if (ldt_state < 0) {
+      dt = dtinit(y, k1, t, acc, ap);
+      nfevals += 1;
+    } else {
+      dt = ldt_state;
+    }
     long guard = 0;
 #pragma unroll 1
     while (!reached_tend) {

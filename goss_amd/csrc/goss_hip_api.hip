@@ -588,6 +588,8 @@ int goss_hip_model_info_get(const char* variant, goss_hip_model_info* out)
     out->default_parameters = v->defaults;
     out->initial_conditions = v->ic;
     out->linear_terms = v->linear_terms;
+    out->differential_states = v->differential_states;
+    out->is_dae = v->is_dae;
   });
 }
 
@@ -797,6 +799,13 @@ int goss_hip_system_create(const char* variant, int solver, const goss_hip_solve
     const VariantInfo* v = find_variant(variant);
     if (solver < 0 || solver >= kNumSolvers) throw Error(GOSS_HIP_EINVAL, "unknown solver");
     if (precision < 0 || precision >= kNumPrecisions) throw Error(GOSS_HIP_EINVAL, "unknown precision");
+    // The explicit solvers, RKF32 and ESDIRK4O32 refuse a DAE when the model is attached, with the
+    // reference's messages (RL1.cpp:54-56; GRL1.cpp:56-58, RKF32.cpp:75-77, ESDIRK4O32.cpp:72-74 and the
+    // other explicit solvers).
+    if (v->is_dae && solver == kRL1)
+      throw Error(GOSS_HIP_EUNSUPPORTED, "cannot integrate a DAE ode with Rush Larsen method.");
+    if (v->is_dae && (is_explicit(solver) || solver == kRKF32 || solver == kESDIRK4O32))
+      throw Error(GOSS_HIP_EUNSUPPORTED, "cannot integrate a DAE ode with an explicit solver.");
     if (is_explicit(solver)) {
       if (!v->explicit_launch[solver][precision])
         throw Error(GOSS_HIP_EUNSUPPORTED, "solver/precision not compiled for this model");

@@ -386,7 +386,8 @@ struct ImplicitEulerNode {
   using I = Implicit<M>;
 
   // ImplicitODESolver::newton_solve (ImplicitODESolver.cpp:114-245) with the
-  // identity mass matrix; z is the work vector kIeZ.  ImplicitEuler calls it
+  // diagonal mass matrix differential_states (the identity unless the model is
+  // a DAE); z is the work vector kIeZ.  ImplicitEuler calls it
   // with prev = 0 and alpha = 1 (ImplicitEuler.cpp:85), ThetaSolver with
   // prev = (1 - theta) f(y, t) in kIePrev and alpha = theta
   // (ThetaSolver.cpp:86).
@@ -410,9 +411,9 @@ struct ImplicitEulerNode {
         for (int i = 0; i < S; ++i) {
           if constexpr (PREV) {
             const VecView prev = v.w(kIePrev);
-            b[i] = -z[i] * 1.0 + dt * (prev[i] + alpha * f[i]);
+            b[i] = -z[i] * double(M::differential_states[i]) + dt * (prev[i] + alpha * f[i]);
           } else {
-            b[i] = -z[i] * 1.0 + dt * (0.0 + 1.0 * f[i]);
+            b[i] = -z[i] * double(M::differential_states[i]) + dt * (0.0 + 1.0 * f[i]);
           }
           bv[i] = b[i];
           yzv[i] = yz[i];
@@ -615,7 +616,7 @@ struct BasicImplicitEulerNode {
           double b[S];
 #pragma unroll
           for (int i = 0; i < S; ++i) {
-            b[i] = (y[i] - prev[i]) * 1.0 - dt * f[i];
+            b[i] = (y[i] - prev[i]) * double(M::differential_states[i]) - dt * f[i];
             bv[i] = b[i];
           }
           residual = I::norm(b);

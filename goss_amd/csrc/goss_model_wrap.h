@@ -16,6 +16,8 @@
     static constexpr const int* field_state_indices = NSN::field_state_indices;                                      \
     static constexpr const int* field_parameter_indices = NSN::field_parameter_indices;                              \
     static constexpr const unsigned char* linear_terms = NSN::linear_terms;                                          \
+    static constexpr const unsigned char* differential_states = NSN::differential_states;                            \
+    static constexpr bool is_dae = NSN::is_dae;                                                                      \
     static constexpr const unsigned char* gate_states = NSN::gate_states;                                            \
     template <typename A>                                                                                            \
     static void compute_constants(const A& a, double* c)                                                             \

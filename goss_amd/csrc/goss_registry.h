@@ -142,6 +142,8 @@ struct VariantInfo {
   const double* defaults;
   const double* ic;
   const unsigned char* linear_terms;
+  const unsigned char* differential_states;  // mass-matrix diagonal (0: algebraic state of a DAE)
+  int is_dae;
   const char* const* state_names;
   const char* const* parameter_names;
   const char* const* field_state_names;

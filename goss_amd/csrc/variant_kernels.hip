@@ -278,6 +278,8 @@ void fill_info(VariantInfo& v)
   v.defaults = NS::default_parameters;
   v.ic = NS::initial_conditions;
   v.linear_terms = NS::linear_terms;
+  v.differential_states = NS::differential_states;
+  v.is_dae = NS::is_dae;
   v.state_names = NS::state_names;
   v.parameter_names = NS::parameter_names;
   v.field_state_names = NS::field_state_names;

@@ -97,6 +97,8 @@ class ModelMeta:
         self.default_parameters = np.array([float(p.value) for p in model.parameters])
         self.initial_conditions = np.array([float(st.init) for st in model.states])
         self.linear_terms = list(linearize(model).linear_terms)
+        self.differential_states = list(model.differential_states)
+        self.is_dae = model.is_dae
         derivs = set(model.derivatives.values())
         # The order of the generated eval_intermediates (intermediate_names).
         self.intermediate_names = [n for n, _ in model.body if n not in derivs]

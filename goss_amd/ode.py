@@ -164,8 +164,15 @@ class ODE:
 
     @property
     def is_dae(self) -> bool:
-        """Differential-algebraic models are not supported by the device code generator."""
-        return False
+        """ODE::is_dae (python/goss/ode.py:74-77): the model has algebraic states
+        (``alg_<X>_0 = expr`` in the .ode file)."""
+        return bool(self._info.is_dae)
+
+    @property
+    def differential_states(self) -> np.ndarray:
+        """ODE::differential_states (ODE.h:85-89): the diagonal of the mass
+        matrix, 0 for the algebraic states."""
+        return np.array(self._info.differential_states, dtype=np.uint8)
 
     def get_ic(self) -> np.ndarray:
         return self._info.initial_conditions.copy()

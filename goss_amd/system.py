@@ -46,6 +46,8 @@ class ModelVariant:
         self.default_parameters = np.array([info.default_parameters[i] for i in range(self.num_parameters)])
         self.initial_conditions = np.array([info.initial_conditions[i] for i in range(self.num_states)])
         self.linear_terms = [int(info.linear_terms[i]) for i in range(self.num_states)]
+        self.differential_states = [int(info.differential_states[i]) for i in range(self.num_states)]
+        self.is_dae = bool(info.is_dae)
 
 
 def solver_params(solver: str, library: Optional[str] = None, **overrides) -> SolverParams:

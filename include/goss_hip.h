@@ -106,6 +106,9 @@ typedef struct goss_hip_model_info {
   const double* default_parameters; /* generated defaults */
   const double* initial_conditions; /* ODE::get_ic */
   const unsigned char* linear_terms;/* ODE::linear_term */
+  const unsigned char* differential_states; /* ODE::differential_states (ODE.h:85-89): 0 for an
+                                               algebraic state of a DAE */
+  int is_dae;                       /* ODE::is_dae (ODE.h:80-83) */
 } goss_hip_model_info;
 
 typedef struct goss_hip_system goss_hip_system;

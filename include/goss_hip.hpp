@@ -110,6 +110,12 @@ class ParameterizedODE {
   }
   // ODE::linear_term
   bool linear_term(uint i) const { return info_.linear_terms[i] != 0; }
+  // ODE::is_dae and ODE::differential_states (ODE.h:80-89)
+  bool is_dae() const { return info_.is_dae != 0; }
+  std::vector<unsigned char> differential_states() const
+  {
+    return std::vector<unsigned char>(info_.differential_states, info_.differential_states + info_.num_states);
+  }
   // ParameterizedODE::get_parameter / set_parameter (ParameterizedODE.h:110-120,
   // unknown names: ParameterizedODE.h:185-195).
   double get_parameter(const std::string& name) const { return values_[index_of(name)]; }

@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libgoss_ref.so")
 REFPIN_SO = os.path.join(HERE, "_ref", "libgoss_refpin.so")
-REFERENCE_MODELS = ("ref_fhn", "ref_beeler_reuter_1977", "ref_winslow", "ref_panfilov")
+REFERENCE_MODELS = ("ref_fhn", "ref_beeler_reuter_1977", "ref_winslow", "ref_panfilov", "ref_van_der_pol")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -63,6 +63,7 @@ def lib():
         L = ctypes.CDLL(ORACLE_SO)
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_variant_name.restype = ctypes.c_char_p
+        L.oracle_variant_differential_states.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)]
         L.oracle_system_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_void_p)]
@@ -139,8 +140,21 @@ def ref_lib(path: str = REF_SO):
         if path == REFPIN_SO:
             L.refpin_model_ops.restype = ctypes.c_void_p
             L.refpin_model_ops.argtypes = [ctypes.c_char_p]
+            L.refpin_ode_run.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp, _dp,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int]
         _ref[path] = L
     return _ref[path]
+
+
+def refpin_ode_run(solver: str, y0, t0: float, dt: float, nsteps: int, solver_params=None) -> np.ndarray:
+    """The reference's solver advancing the reference's van der Pol DAE test model
+    (cpp/tests/van_der_Pol.h) from y0: forward(y, t, dt); t += dt, nsteps times."""
+    R = ref_lib(REFPIN_SO)
+    y = _f64(y0).copy()
+    n, names, vals = _names(solver_params or {})
+    if R.refpin_ode_run(solver.encode(), n, names, _ptr(vals), _ptr(y), float(t0), float(dt), int(nsteps)) != 0:
+        raise RuntimeError(R.ref_last_error().decode())
+    return y
 
 
 def register_reference_models() -> None:
@@ -179,6 +193,9 @@ class Variant:
         lt = (ctypes.c_ubyte * self.num_states)()
         L.oracle_variant_linear_terms(self.index, lt)
         self.linear_terms = list(lt)
+        ds = (ctypes.c_ubyte * self.num_states)()
+        L.oracle_variant_differential_states(self.index, ds)
+        self.differential_states = list(ds)
 
     def eval(self, y, t, params=None):
         p = _f64(self.default_parameters if params is None else params)

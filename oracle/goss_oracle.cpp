@@ -24,6 +24,8 @@
 #include "beeler_reuter_1977.h"
 #include "oscilator.h"
 #include "linear_decay.h"
+#include "van_der_pol_dae.h"
+#include "markov_three_state.h"
 
 namespace {
 
@@ -50,6 +52,7 @@ struct VariantOps {
   const double* defaults;
   const double* ic;
   const unsigned char* linear_terms;
+  const unsigned char* diff;  // ODE::differential_states (ODE.h:85-89); nullptr: every state differential
   const char* const* pnames;
   int K;  // number of parameter-only constants
   void (*constants)(const double* p, double* c);
@@ -64,6 +67,16 @@ struct VariantOps {
   // right-hand side is called through these entries with the full parameter
   // vector; the generated entries above are then unused.
   const oracle_external_model* ext = nullptr;
+
+  // The mass matrix's diagonal entry of state i, as the reference reads it
+  // (_ode->differential_states()[i], an unsigned char).
+  double mass(const int i) const { return diff ? double(diff[i]) : 1.0; }
+  bool is_dae() const
+  {
+    for (int i = 0; diff && i < S; ++i)
+      if (!diff[i]) return true;
+    return false;
+  }
 
   void eval(const double* y, double t, const double* p, const double* c, double* out) const
   {
@@ -91,7 +104,8 @@ struct HostTape {
              goss_models::V::num_field_states, goss_models::V::num_field_parameters,                   \
              goss_models::V::field_state_indices, goss_models::V::field_parameter_indices,             \
              goss_models::V::default_parameters, goss_models::V::initial_conditions,                   \
-             goss_models::V::linear_terms, goss_models::V::parameter_names,                            \
+             goss_models::V::linear_terms, goss_models::V::differential_states,                        \
+             goss_models::V::parameter_names,                                                          \
              goss_models::V::num_constants,                                                            \
              [](const double* p, double* c) { goss_models::V::compute_constants<double>(Acc{p, nullptr}, c); }, \
              [](const double* y, double t, const double* p, const double* c, double* out) {            \
@@ -128,6 +142,7 @@ std::deque<VariantOps>& variant_table()
       GOSS_ORACLE_VARIANT(fitzhughnagumo), GOSS_ORACLE_VARIANT(lorentz),
       GOSS_ORACLE_VARIANT(fhn),            GOSS_ORACLE_VARIANT(beeler_reuter_1977),
       GOSS_ORACLE_VARIANT(oscilator),      GOSS_ORACLE_VARIANT(linear_decay),
+      GOSS_ORACLE_VARIANT(van_der_pol_dae), GOSS_ORACLE_VARIANT(markov_three_state),
   };
   return t;
 }
@@ -190,6 +205,16 @@ void fd_jacobian(const VariantOps& m, double* y, double t, const double* p, cons
     for (int j = 0; j < S; ++j) jac[j * S + i] = (f2[j] - f1[j]) / delta;
     y[i] = ysafe;
   }
+}
+
+// The model's ODE::compute_jacobian: a model class that overrides it (the
+// reference's hand-written test models) is called; otherwise the forward
+// differences above.
+void ode_jacobian(const VariantOps& m, double* y, double t, const double* p, const double* c, double* jac,
+                  double* f1, double* f2)
+{
+  if (m.ext && m.ext->compute_jacobian) m.ext->compute_jacobian(m.ext->ctx, y, t, p, jac);
+  else fd_jacobian(m, y, t, p, c, jac, f1, f2);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,17 +425,17 @@ inline double l2norm(const double* v, int S)
 }
 
 // ImplicitODESolver::compute_factorized_jacobian (ImplicitODESolver.cpp:92-105):
-// FD Jacobian at y, scaled by -dt*alpha, identity added (all states are
-// differential), LU factorized in place.
+// the model's Jacobian at y, scaled by -dt*alpha, the diagonal mass matrix
+// added (add_mass_matrix, :258-262), LU factorized in place.
 void ie_factorized_jacobian(const VariantOps& m, ImplicitState& st, double* y, double t, double dt, double alpha,
                             const double* p, const double* c, Work& w)
 {
   const int S = m.S;
-  fd_jacobian(m, y, t, p, c, st.jac.data(), w.ff1.data(), w.ff2.data());
+  ode_jacobian(m, y, t, p, c, st.jac.data(), w.ff1.data(), w.ff2.data());
   const double scale = -dt * alpha;
   for (int i = 0; i < S; ++i)
     for (int j = 0; j < S; ++j) st.jac[i * S + j] *= scale;
-  for (int i = 0; i < S; ++i) st.jac[i * S + i] += 1.0 * 1.0;
+  for (int i = 0; i < S; ++i) st.jac[i * S + i] += 1.0 * m.mass(i);  // add_mass_matrix (:258-262)
   lu_factorize(S, st.jac.data());
   st.jac_comp += 1;
 }
@@ -432,7 +457,7 @@ bool newton_solve(const VariantOps& m, const SolverParams& sp, ImplicitState& st
   do {
     for (int i = 0; i < S; ++i) yz[i] = y0[i] + z[i];
     m.eval(yz, t, p, c, f1);
-    for (int i = 0; i < S; ++i) b[i] = -z[i] * 1.0 + dt * (prev[i] + alpha * f1[i]);
+    for (int i = 0; i < S; ++i) b[i] = -z[i] * m.mass(i) + dt * (prev[i] + alpha * f1[i]);
     residual = l2norm(b, S);
     if (relative_residual < sp.relative_tolerance) break;
     if (st.recompute_jacobian || always_recompute) {
@@ -611,7 +636,7 @@ struct EsdirkCoef {
 void es_ode_jacobian(const VariantOps& m, ImplicitState& st, double* y, double t, const double* p, const double* c,
                      Work& w)
 {
-  fd_jacobian(m, y, t, p, c, st.jac.data(), w.ff1.data(), w.ff2.data());
+  ode_jacobian(m, y, t, p, c, st.jac.data(), w.ff1.data(), w.ff2.data());
   st.jac_comp++;
   st.new_jacobian = true;
 }
@@ -624,7 +649,7 @@ void es_factorized(const VariantOps& m, const EsdirkCoef& k, ImplicitState& st, 
   const double scale = -dt * k.a22;
   for (int i = 0; i < S; ++i)
     for (int j = 0; j < S; ++j) st.jfac[i * S + j] *= scale;
-  for (int i = 0; i < S; ++i) st.jfac[i * S + i] += 1.0 * 1.0;
+  for (int i = 0; i < S; ++i) st.jfac[i * S + i] += 1.0 * m.mass(i);
   lu_factorize(S, st.jfac.data());
 }
 
@@ -792,7 +817,7 @@ void basic_implicit_euler_forward(const VariantOps& m, const SolverParams& sp, I
     for (int i = 0; i < S; ++i) prev[i] = y[i];
     while (!newton_converged && st.newton_iterations < sp.max_iterations) {
       m.eval(y, lt + ldt, p, c, f1);
-      for (int i = 0; i < S; ++i) b[i] = (y[i] - prev[i]) * 1.0 - dt * f1[i];
+      for (int i = 0; i < S; ++i) b[i] = (y[i] - prev[i]) * m.mass(i) - dt * f1[i];
       residual = l2norm(b, S);
       if (st.newton_iterations == 0) initial_residual = residual;
       relative_residual = residual / initial_residual;
@@ -1007,6 +1032,54 @@ void esdirk4o32_forward(const VariantOps& m, const SolverParams& sp, ImplicitSta
 // underflowed after repeated rejections of a non-finite error).
 constexpr long kRkf32MaxSteps = 1000000;
 
+// AdaptiveExplicitSolver::dtinit (AdaptiveExplicitSolver.cpp:67-166) with _itol = 0: an initial step from
+// the scaled norms of y0 and f0 and a difference estimate of the second derivative after one Euler step.
+// y1 and f1 are scratch.
+double rkf32_dtinit(const VariantOps& m, const SolverParams& sp, double t, const double* y0, double* y1,
+                    const double* f0, double* f1, double iord, const double* p, const double* c)
+{
+  const int S = m.S;
+  double dnf = 0.0, dny = 0.0, sk, dt, tmp;
+  for (int i = 0; i < S; ++i) {
+    sk = sp.atol + sp.rtol * std::fabs(y0[i]);
+    tmp = f0[i] / sk;
+    dnf += tmp * tmp;
+    tmp = y0[i] / sk;
+    dny += tmp * tmp;
+  }
+  if (dnf <= 1.0e-10 || dny <= 1.0e-10)
+    dt = 1.0e-6;
+  else
+    dt = 0.01 * std::sqrt(dny / dnf);
+  for (int i = 0; i < S; ++i) y1[i] = y0[i] + dt * f0[i];
+  m.eval(y1, t + dt, p, c, f1);
+  double der2 = 0.0;
+  for (int i = 0; i < S; ++i) {
+    sk = sp.atol + sp.rtol * std::fabs(y1[i]);
+    tmp = (f1[i] - f0[i]) / sk;
+    der2 += tmp * tmp;
+  }
+  der2 = std::sqrt(der2) / dt;
+  double der12;
+  if (std::fabs(der2) >= std::sqrt(dnf))
+    der12 = std::fabs(der2);
+  else
+    der12 = std::sqrt(dnf);
+  double dt1;
+  if (der12 <= 1.0e-15) {
+    if (std::fabs(dt) * 1.0e-3 < 1.0e-6)
+      dt1 = 1.0e-6;
+    else
+      dt1 = std::fabs(dt) * 1.0e-3;
+  } else {
+    dt1 = std::pow((0.01 / der12), (1.0 / iord));
+  }
+  if (100 * std::fabs(dt) < dt1)
+    return 100 * std::fabs(dt);
+  else
+    return dt1;
+}
+
 void rkf32_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& st, double* y_out, double t,
                    double interval, const double* p, const double* c, Work& w)
 {
@@ -1028,9 +1101,17 @@ void rkf32_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& s
   // `first` is set by reset() and never cleared (RKF32.cpp:126-131).
   m.eval(y, t, p, c, k1);
   st.rk_nfevals += 1;
-  // _ldt is AdaptiveExplicitSolver::_ldt, never negative (its dtinit branch,
-  // RKF32.cpp:134-138, is not reachable).
-  double dt = st.rk_ldt;
+  // _ldt is AdaptiveExplicitSolver::_ldt (0.1 after construction).  It turns negative when a step
+  // overshoots t_end (the first step of a call is _ldt, which may exceed the interval) and the next
+  // step, t_end - _t < 0, is stored as the guess (AdaptiveExplicitSolver.cpp:234-241); the next call
+  // then takes its first step from dtinit (RKF32.cpp:134-141).
+  double dt;
+  if (st.rk_ldt < 0) {
+    dt = rkf32_dtinit(m, sp, t, y, yn, k1, k2, iord, p, c);
+    st.rk_nfevals += 1;
+  } else {
+    dt = st.rk_ldt;
+  }
   long guard = 0;
   while (!reached_tend) {
     if (++guard > kRkf32MaxSteps) throw std::runtime_error("RKF32 did not reach the end of the interval");
@@ -1238,6 +1319,10 @@ void oracle_variant_linear_terms(int v, unsigned char* f)
 {
   std::memcpy(f, kVariants[v].linear_terms, kVariants[v].S);
 }
+void oracle_variant_differential_states(int v, unsigned char* f)
+{
+  for (int i = 0; i < kVariants[v].S; ++i) f[i] = kVariants[v].diff ? kVariants[v].diff[i] : 1;
+}
 void oracle_model_eval(int v, const double* y, double t, const double* p, double* out)
 {
   double c[kMaxConstants];
@@ -1308,6 +1393,7 @@ int oracle_register_external_model(const oracle_external_model* e)
   m.defaults = e->default_parameters;
   m.ic = e->initial_conditions;
   m.linear_terms = e->linear_terms;
+  m.diff = e->differential_states;
   m.pnames = e->parameter_names;
   m.K = 0;
   m.constants = [](const double*, double*) {};
@@ -1324,6 +1410,13 @@ int oracle_system_create(const char* variant, const char* solver, unsigned num_n
   const int kind = solver_kind(solver);
   if (kind < 0) { g_err = std::string("unknown solver ") + solver; return -1; }
   if (kVariants[v].K > kMaxConstants) { g_err = "model has too many constants"; return -1; }
+  // Solvers that refuse a DAE when the model is attached (RL1.cpp:54-56; GRL1.cpp:56-58, RKF32.cpp:75-77,
+  // ESDIRK4O32.cpp:72-74 and the other explicit solvers).
+  if (kVariants[v].is_dae() && kind == RL1) { g_err = "cannot integrate a DAE ode with Rush Larsen method."; return -1; }
+  if (kVariants[v].is_dae() && (kind <= GRL2 || kind == RKF32 || kind == ESDIRK4O32)) {
+    g_err = "cannot integrate a DAE ode with an explicit solver.";
+    return -1;
+  }
   auto* s = new oracle_system;
   s->m = &kVariants[v];
   s->kind = kind;

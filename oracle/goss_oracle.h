@@ -33,13 +33,19 @@ void oracle_variant_sizes(int v, int* sizes);
 void oracle_variant_defaults(int v, double* parameters, double* initial_conditions);
 void oracle_variant_field_indices(int v, int* field_state_indices, int* field_parameter_indices);
 void oracle_variant_linear_terms(int v, unsigned char* flags);
+/* ODE::differential_states (ODE.h:85-89): 0 for the algebraic states of a DAE. */
+void oracle_variant_differential_states(int v, unsigned char* flags);
 
 /* A model supplied at run time by another library (the reference-side
  * checker registers the reference's own gotran-generated classes this way):
  * sizes, metadata and its right-hand side, called with the full parameter
  * vector (P values, field parameters substituted per node).  The struct and
- * everything it points to must outlive the oracle's use of it.  Analytic
- * Jacobians and step adjoints are not available for such a model. */
+ * everything it points to must outlive the oracle's use of it.  Step adjoints
+ * are not available for such a model.  differential_states may be NULL (every
+ * state differential); compute_jacobian may be NULL (the solvers then form the
+ * forward-difference Jacobian of ODE::compute_jacobian, ODE.cpp:47-67), else it
+ * stands for a model class that overrides ODE::compute_jacobian, as the
+ * reference's hand-written test models do. */
 typedef struct oracle_external_model {
   const char* name;
   int num_states, num_parameters, num_field_states, num_field_parameters;
@@ -53,6 +59,8 @@ typedef struct oracle_external_model {
   void (*eval)(void* ctx, const double* states, double t, const double* params, double* values);
   void (*linearized_eval)(void* ctx, const double* states, double t, const double* params, double* linear,
                           double* rhs, int only_linear);
+  const unsigned char* differential_states;
+  void (*compute_jacobian)(void* ctx, double* states, double t, const double* params, double* jac);
 } oracle_external_model;
 /* Returns the new variant index (or the existing one for the same struct), -1 on error. */
 int oracle_register_external_model(const oracle_external_model* model);

@@ -22,6 +22,8 @@
 #include "beeler_reuter_1977_plugin.h"
 #include "oscilator_plugin.h"
 #include "linear_decay_plugin.h"
+#include "van_der_pol_dae_plugin.h"
+#include "markov_three_state_plugin.h"
 
 namespace {
 
@@ -39,6 +41,8 @@ std::shared_ptr<ParameterizedODE> make_model(const std::string& v)
   if (v == "beeler_reuter_1977") return std::make_shared<Beeler_reuter_1977>();
   if (v == "oscilator") return std::make_shared<Oscilator>();
   if (v == "linear_decay") return std::make_shared<Linear_decay>();
+  if (v == "van_der_pol_dae") return std::make_shared<Van_der_pol_dae>();
+  if (v == "markov_three_state") return std::make_shared<Markov_three_state>();
   throw std::runtime_error("unknown model variant " + v);
 }
 

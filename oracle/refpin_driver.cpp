@@ -5,12 +5,18 @@
 // The models are the generated headers the reference ships with its C++
 // tests -- cpp/tests/fhn.h, beeler_reuter_1977.h, Winslow.h (field states and
 // field parameters) and Panfilov.h (a field state) -- not this project's
-// code generator.  Two things are exported:
+// code generator -- and the reference's hand-written DAE test model
+// cpp/tests/van_der_Pol.h (a plain ODE with an algebraic state and its own
+// analytic compute_jacobian).  Three things are exported:
 //   * the ODESystemSolver C ABI of ref_system_abi.h over the reference
 //     solvers (cpp/goss/*.cpp), with these models;
 //   * each model's metadata and right-hand side as an oracle_external_model,
 //     so the oracle's solver restatement can run on the very same classes
-//     (oracle_register_external_model).
+//     (oracle_register_external_model);
+//   * refpin_ode_run: a reference solver advancing one ODE (the van der Pol
+//     DAE, which ODESystemSolver cannot take: it is not a ParameterizedODE)
+//     through ODESolver::forward, as the reference's DAE test does
+//     (cpp/tests/ode.cpp:56-78).
 // Comparing the two pins the oracle's solvers against the reference with no
 // code of this project inside the model.
 #include <memory>
@@ -24,6 +30,7 @@
 #include "tests/Winslow.h"
 #include "tests/beeler_reuter_1977.h"
 #include "tests/fhn.h"
+#include "tests/van_der_Pol.h"
 
 #include "goss_oracle.h"
 
@@ -135,6 +142,56 @@ ExternalModel* external(int k)
   return all[k].get();
 }
 
+// The van der Pol DAE (an ODE, no parameters): one instance per thread.
+goss::ODE& van_der_pol()
+{
+  thread_local std::shared_ptr<goss::ODE> m = std::make_shared<VanDerPol>();
+  return *m;
+}
+
+void vdp_eval(void*, const double* y, double t, const double*, double* out) { van_der_pol().eval(y, t, out); }
+void vdp_linearized(void*, const double* y, double t, const double*, double* lin, double* rhs, int only_linear)
+{
+  van_der_pol().linearized_eval(y, t, lin, rhs, only_linear != 0);
+}
+void vdp_jacobian(void*, double* y, double t, const double*, double* jac)
+{
+  van_der_pol().compute_jacobian(y, t, jac);
+}
+
+struct OdeModel {
+  std::vector<double> ic;
+  std::vector<unsigned char> linear, diff;
+  oracle_external_model ops{};
+};
+
+const oracle_external_model* van_der_pol_ops()
+{
+  static std::unique_ptr<OdeModel> em;
+  if (!em) {
+    em = std::make_unique<OdeModel>();
+    auto& m = van_der_pol();
+    const uint S = m.num_states();
+    DoubleVector ic;
+    m.get_ic(&ic);
+    em->ic.assign(ic.data.get(), ic.data.get() + S);
+    for (uint i = 0; i < S; ++i) {
+      em->linear.push_back((unsigned char)m.linear_term(i));
+      em->diff.push_back(m.differential_states()[i]);
+    }
+    oracle_external_model& o = em->ops;
+    o.name = "ref_van_der_pol";
+    o.num_states = int(S);
+    o.initial_conditions = em->ic.data();
+    o.linear_terms = em->linear.data();
+    o.differential_states = em->diff.data();
+    o.eval = vdp_eval;
+    o.linearized_eval = vdp_linearized;
+    o.compute_jacobian = vdp_jacobian;
+  }
+  return &em->ops;
+}
+
 }  // namespace
 
 #include "ref_system_abi.h"
@@ -148,6 +205,7 @@ const char* refpin_model_name(int k) { return (k >= 0 && k < kNumModels) ? kMode
 const oracle_external_model* refpin_model_ops(const char* name)
 {
   try {
+    if (name && std::string(name) == "ref_van_der_pol") return van_der_pol_ops();
     for (int k = 0; k < kNumModels; ++k)
       if (name && std::string(name) == kModelNames[k]) return &external(k)->ops;
     g_error = std::string("unknown model ") + (name ? name : "(null)");
@@ -155,6 +213,28 @@ const oracle_external_model* refpin_model_ops(const char* name)
     g_error = e.what();
   }
   return nullptr;
+}
+
+// One reference solver advancing the van der Pol DAE from y (in place) over
+// nsteps intervals of dt from t0, with solver tunables by name:
+// solver->forward(y, t, dt); t += dt (cpp/tests/ode.cpp:61-78).
+int refpin_ode_run(const char* solver, int n_params, const char* const* names, const double* values, double* y,
+                   double t0, double dt, int nsteps)
+{
+  try {
+    auto s = make_solver(solver);
+    for (int i = 0; i < n_params; ++i) set_solver_param(*s, names[i], values[i]);
+    s->attach(std::make_shared<VanDerPol>());
+    double t = t0;
+    for (int i = 0; i < nsteps; ++i) {
+      s->forward(y, t, dt);
+      t += dt;
+    }
+    return 0;
+  } catch (std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
 }
 
 }  // extern "C"

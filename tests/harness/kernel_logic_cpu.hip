@@ -23,6 +23,7 @@
 #include "fhn.h"
 #include "lorentz.h"
 #include "fitzhughnagumo.h"
+#include "markov_three_state.h"
 
 using namespace goss_hip;
 
@@ -49,23 +50,27 @@ GOSS_MODEL_TYPE(M_beeler_reuter_1977, goss_models::beeler_reuter_1977);
 GOSS_MODEL_TYPE(M_fhn, goss_models::fhn);
 GOSS_MODEL_TYPE(M_lorentz, goss_models::lorentz);
 GOSS_MODEL_TYPE(M_fitzhughnagumo, goss_models::fitzhughnagumo);
+GOSS_MODEL_TYPE(M_markov_three_state, goss_models::markov_three_state);
 
 // RKF32 per node with the device logic (goss_adaptive.h); counters are
 // nfevals, ndtsa, ndtsr summed over nodes.
 template <class M>
 int run_rkf32(unsigned N, const double* y0, const double* t, unsigned long n, const double* prm, double atol,
-              double rtol, double* out_states, long* counters)
+              double rtol, double* out_states, long* counters, const double* fp = nullptr)
 {
   constexpr int S = M::num_states;
+  constexpr int NFP = M::num_field_parameters;
   const SharedParamsOf<M, double> sp = make_shared_params<M, double>(prm);
-  const double fpv[1] = {0.0};
-  const DevAcc<M, double> dacc{sp, fpv};
   const AdaptiveParams ap{atol, rtol};
   for (int k = 0; k < 4; ++k) counters[k] = 0;
   std::vector<long> acc(16 * 4, 0);
   std::vector<int> failed(16, 0);
   parallel_nodes(N, [&](unsigned first, unsigned last, unsigned slot) {
   for (unsigned node = first; node < last; ++node) {
+    // field parameters of the node (node-major [N][NFP]); the model's defaults without fp
+    double fpv[NFP > 0 ? NFP : 1] = {0.0};
+    for (int j = 0; j < NFP; ++j) fpv[j] = fp ? fp[size_t(node) * NFP + j] : prm[M::field_parameter_indices[j]];
+    const DevAcc<M, double> dacc{sp, fpv};
     double ys[S], w[3 * S];
     for (int i = 0; i < S; ++i) ys[i] = y0[size_t(node) * S + i];
     const RVec<double> yv{ys, 1}, k1{w, 1}, yn{w + S, 1}, ep{w + 2 * S, 1};
@@ -272,5 +277,16 @@ extern "C" int kernel_logic_run(const char* variant, int solver, unsigned N, con
   if (v == "tp06_epi") return run<M_tp06_epi>(solver, N, y0, t, n, params, ip, out_states, counters);
   if (v == "beeler_reuter_1977") return run<M_beeler_reuter_1977>(solver, N, y0, t, n, params, ip, out_states, counters);
   if (v == "fhn") return run<M_fhn>(solver, N, y0, t, n, params, ip, out_states, counters);
+  return -2;
+}
+
+// RKF32 with per-node field parameters fp ([N][NFP]).
+extern "C" int kernel_logic_run_rkf32_fp(const char* variant, unsigned N, const double* y0, const double* fp,
+                                         const double* t, unsigned long n, const double* params, double atol,
+                                         double rtol, double* out_states, long* counters)
+{
+  const std::string v(variant);
+  if (v == "markov_three_state")
+    return run_rkf32<M_markov_three_state>(N, y0, t, n, params, atol, rtol, out_states, counters, fp);
   return -2;
 }

@@ -36,9 +36,14 @@ SOLVER_IDS = {"ImplicitEuler": 7, "ESDIRK23a": 8, "RKF32": 9, "ThetaSolver": 10,
 
 
 def _libs():
+    from goss_amd._lib import _share_the_hsa_runtime_with_torch
+
     for p in (DEV_LIB, HOST_LIB):
         if not os.path.exists(p):
             pytest.fail(f"{p} is not built (run __graft_entry__.build())")
+    # The host build links the HIP runtime too: torch's runtime must come first, or torch (which the
+    # tests after this one use) finds no device (goss_amd._lib explains why).
+    _share_the_hsa_runtime_with_torch()
     L = ctypes.CDLL(HOST_LIB)
     dp = ctypes.POINTER(ctypes.c_double)
     L.kernel_logic_run.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint, dp, dp, ctypes.c_ulong, dp,
@@ -122,3 +127,39 @@ def test_device_equals_host_bit_for_bit(variant, solver, N, dt, steps, stim, rel
     # Counters: implicit -- Jacobians, rejections, accepted steps (the first entry holds only the last
     # solve's Newton iterations); RKF32 -- nfevals, rejected, accepted steps.
     np.testing.assert_array_equal(dev_c[1:], host_c[1:])
+
+
+def test_rkf32_initial_step_from_dtinit_device_equals_host():
+    """RKF32 calls that start from AdaptiveExplicitSolver::dtinit (tests/markov_cases.py) on the Markov chain
+    with one field parameter per node.  The opening rate's time modulation is switched off (a_co = 0: the
+    product with sin is an exact zero on both sides, so sin's last bit, which the parity build does not
+    control, cannot enter)."""
+    from goss_amd.system import DeviceSystem
+    from markov_cases import DT, STEPS, finishing_nodes, markov_inputs
+
+    L = _libs()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.kernel_logic_run_rkf32_fp.argtypes = [ctypes.c_char_p, ctypes.c_uint, dp, dp, dp, ctypes.c_ulong, dp,
+                                            ctypes.c_double, ctypes.c_double, dp, ctypes.POINTER(ctypes.c_long)]
+    meta = json.load(open(META))["markov_three_state"]
+    names = meta["parameter_names"]
+    params = np.ascontiguousarray(meta["default_parameters"], dtype=np.float64)
+    params[names.index("a_co")] = 0.0
+    y0, fp = markov_inputs(1024, 35)
+    t = np.arange(STEPS + 1) * DT
+    good = finishing_nodes(y0, fp, t, {"a_co": 0.0})
+    assert len(good) >= 64
+    y0, fp = np.ascontiguousarray(y0[good]), np.ascontiguousarray(fp[good])
+    host = np.zeros_like(y0)
+    host_c = (ctypes.c_long * 4)()
+    assert L.kernel_logic_run_rkf32_fp(b"markov_three_state", len(good), _p(y0), _p(fp), _p(t), len(t), _p(params),
+                                       1e-5, 1e-8, _p(host), host_c) == 0
+    s = DeviceSystem("markov_three_state", "RKF32", len(good), "fp64", {}, library=DEV_LIB)
+    s.set_parameter("a_co", 0.0)
+    s.set_states(y0)
+    s.set_field_parameters(fp)
+    s.forward_times(t)
+    dev, dev_c = s.states(), s.solver_counters()
+    s.close()
+    np.testing.assert_array_equal(dev, host)
+    np.testing.assert_array_equal(dev_c[1:], np.array(list(host_c))[1:])

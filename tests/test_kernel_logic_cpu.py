@@ -142,6 +142,34 @@ def test_rkf32_kernel_logic_matches_oracle_bit_exact(variant, dt, nsteps, tol):
     np.testing.assert_array_equal(np.array(list(counters))[1:], o.counters()[1:])
 
 
+def test_rkf32_kernel_logic_initial_step_from_dtinit_matches_oracle():
+    """The Markov chain at intervals of 0.05 (tests/markov_cases.py): calls that start from
+    AdaptiveExplicitSolver::dtinit after an overshoot, with a field parameter per node."""
+    from markov_cases import DT, STEPS, finishing_nodes, markov_inputs
+
+    L = _harness()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.kernel_logic_run_rkf32_fp.argtypes = [ctypes.c_char_p, ctypes.c_uint, dp, dp, dp, ctypes.c_ulong, dp,
+                                            ctypes.c_double, ctypes.c_double, dp, ctypes.POINTER(ctypes.c_long)]
+    y0, fp = markov_inputs(256, 32)
+    t = np.arange(STEPS + 1) * DT
+    good = finishing_nodes(y0, fp, t)
+    assert len(good) >= 20
+    y0, fp = np.ascontiguousarray(y0[good]), np.ascontiguousarray(fp[good])
+    o = oracle.OracleSystem("markov_three_state", "RKF32", len(good), per_node_state=True)
+    o.set_states(y0)
+    o.set_field_parameters(fp)
+    o.forward_times(t)
+    params = np.ascontiguousarray(json.load(open(META))["markov_three_state"]["default_parameters"], dtype=np.float64)
+    out = np.zeros_like(y0)
+    counters = (ctypes.c_long * 4)()
+    rc = L.kernel_logic_run_rkf32_fp(b"markov_three_state", len(good), _p(y0), _p(fp), _p(t), len(t), _p(params),
+                                     1e-5, 1e-8, _p(out), counters)
+    assert rc == 0
+    np.testing.assert_array_equal(out, o.states())
+    np.testing.assert_array_equal(np.array(list(counters))[1:], o.counters()[1:])
+
+
 STEP_VJP_SOLVERS = {"GRL1": 5, "RL1": 3, "ExplicitEuler": 0, "RK4": 2, "RK2": 1, "RL2": 4, "GRL2": 6}
 
 

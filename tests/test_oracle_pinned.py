@@ -9,6 +9,11 @@ the model, and against what the mathematics fixes.
   reference's bit for bit, serially and with OpenMP threads, and the oracle's
   per-node solver state must equal one reference system per node.  The FD
   Jacobian (ODE.cpp:47-67) must equal ODE::compute_jacobian bit for bit.
+* The reference's DAE test model (cpp/tests/van_der_Pol.h: one algebraic
+  state, its own analytic compute_jacobian) advanced by the reference's
+  implicit solvers through ODESolver::forward equals the oracle's restatement
+  on the same class bit for bit; the explicit solvers refuse it with the
+  reference's messages.
 * Closed forms on y' = a*y + b: one backward-Euler step is
   (y0 + h*b) / (1 - h*a); ESDIRK23a's single-step amplification R(z) has a
   local error of order z^4 (a third-order method) and tends to 0 as z -> -inf
@@ -143,7 +148,7 @@ def test_oracle_per_node_state_equals_one_reference_system_per_node(model, solve
 
 
 @needs_refpin
-@pytest.mark.parametrize("model", oracle.REFERENCE_MODELS)
+@pytest.mark.parametrize("model", [m for m in oracle.REFERENCE_MODELS if m != "ref_van_der_pol"])
 def test_fd_jacobian_equals_reference_compute_jacobian(model):
     oracle.register_reference_models()
     v = oracle.Variant(model)
@@ -260,3 +265,71 @@ def test_basic_implicit_euler_internal_time_step_equals_reference():
             s.forward_times(t)
         assert np.all(np.isfinite(r.states()))
         assert np.array_equal(o.states(), r.states()), model
+
+
+@needs_refpin
+@pytest.mark.parametrize("solver,dt,steps", [("ThetaSolver", 0.01, 80), ("ThetaSolver", 0.001, 800),
+                                             ("ImplicitEuler", 0.01, 80), ("BasicImplicitEuler", 0.01, 80),
+                                             ("ESDIRK23a", 0.01, 20)])
+def test_dae_van_der_pol_oracle_equals_reference(solver, dt, steps):
+    """The reference's DAE test (cpp/tests/ode.cpp:171-182 runs ThetaSolver to t = 0.8 with dt 0.01 and
+    0.001): the mass matrix enters the Newton residual and the iteration matrix
+    (ImplicitODESolver.cpp:140,258-262; BasicImplicitEuler.cpp:53,123; ESDIRK23a.cpp:112).  ESDIRK23a's
+    stage equations leave the mass matrix out (ESDIRK23a.cpp:223-224), and on this model the reference
+    rejects every step and stays at the initial state; the oracle does the same."""
+    oracle.register_reference_models()
+    v = oracle.Variant("ref_van_der_pol")
+    assert v.differential_states == [1, 0]
+    y_ref = oracle.refpin_ode_run(solver, v.initial_conditions, 0.0, dt, steps)
+    o = oracle.OracleSystem("ref_van_der_pol", solver, 1)
+    o.set_states(v.initial_conditions[None, :])
+    t = 0.0
+    for _ in range(steps):
+        o.forward(t, dt)
+        t += dt
+    y = o.states()[0]
+    assert np.array_equal(y, y_ref), (y, y_ref)
+    if solver != "ESDIRK23a":
+        assert y[1] < 1.2  # the run has moved along the slow branch of the limit cycle (z from 2 down)
+        assert abs(y[0] - (y[1] ** 2 / 3 - 1) * y[1]) < 1e-12  # on the constraint
+    else:
+        assert np.array_equal(y, v.initial_conditions)
+
+
+@needs_refpin
+@pytest.mark.parametrize("solver,message", [
+    ("ExplicitEuler", "an explicit solver"), ("RK2", "an explicit solver"), ("RK4", "an explicit solver"),
+    ("RL1", "Rush Larsen method"), ("RL2", "an explicit solver"), ("GRL1", "an explicit solver"),
+    ("GRL2", "an explicit solver"), ("RKF32", "an explicit solver"), ("ESDIRK4O32", "an explicit solver")])
+def test_dae_refused_by_explicit_solvers_with_reference_messages(solver, message):
+    oracle.register_reference_models()
+    with pytest.raises(RuntimeError, match=f"cannot integrate a DAE ode with {message}") as ref:
+        oracle.refpin_ode_run(solver, [0.5, 2.0], 0.0, 0.01, 1)
+    with pytest.raises(RuntimeError) as orc:
+        oracle.OracleSystem("ref_van_der_pol", solver, 1)
+    assert str(orc.value) == str(ref.value)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build oracle/_ref not present")
+def test_rkf32_initial_step_from_dtinit_equals_reference():
+    """RKF32 calls that start from AdaptiveExplicitSolver::dtinit (AdaptiveExplicitSolver.cpp:67-166): after
+    an overshoot of the interval end, an accepted backward step leaves a negative step guess
+    (tests/markov_cases.py).  The oracle's per-node state equals one reference system per node, bit for
+    bit, on the generated Markov chain (the reference's solvers with the generated plugin)."""
+    from markov_cases import DT, STEPS, finishing_nodes, markov_inputs
+
+    y0, fp = markov_inputs(64, 32)
+    t = np.arange(STEPS + 1) * DT
+    good = finishing_nodes(y0, fp, t)[:12]
+    assert len(good) >= 6
+    o = oracle.OracleSystem("markov_three_state", "RKF32", len(good), per_node_state=True)
+    o.set_states(y0[good])
+    o.set_field_parameters(fp[good])
+    o.forward_times(t)
+    for k, n in enumerate(good):
+        r = oracle.RefSystem("markov_three_state", "RKF32", 1)
+        r.set_states(y0[n:n + 1])
+        r.set_field_parameters(fp[n:n + 1])
+        for i in range(STEPS):
+            r.forward(t[i], t[i + 1] - t[i])
+        assert np.array_equal(o.states()[k], r.states()[0]), n
