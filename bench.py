@@ -147,6 +147,24 @@ def timed_run(prec, N, y0, fp, t_warm, t_timed, spl, ev):
     return wall, call_ms, kernel_ms, launches
 
 
+def one_step_calls(N, y0, fp, dt, calls=100):
+    """One outer step per forward call, as an operator-splitting caller advances the ODE nodes once per PDE
+    step (dolfinutils.py:750-780): every call reads and writes all states, so the launch is bound by that
+    HBM traffic (212 B per node in FP32C, 328 B in FP64) as much as by the arithmetic."""
+    ev = HipEvents()
+    out = {"workload": f"TP06 M-cell GRL1, {N} nodes, one step of {dt} ms per forward call, {calls} calls"}
+    for prec, nbytes in (("fp32c", 212), ("fp64", 328)):
+        t_warm = np.arange(11) * dt
+        t_timed = t_warm[-1] + np.arange(calls + 1) * dt
+        wall, call_ms, kernel_ms, launches = timed_run(prec, N, y0, fp, t_warm, t_timed, 1, ev)
+        per_call_s = kernel_ms * 1e-3 / launches
+        gbs = nbytes * N / per_call_s / 1e9
+        out[prec] = {"us_per_call": round(per_call_s * 1e6, 2), "node_steps_per_s": N * calls / (call_ms * 1e-3),
+                     "algorithmic_bytes_per_call": nbytes * N, "achieved_GBps": round(gbs, 1),
+                     "hbm_frac": round(gbs / PEAK_HBM_GBS, 3)}
+    return out
+
+
 def accuracy_gate(y0, fp, dt, horizon_ms=GATE_HORIZON_MS, sample_ms=GATE_SAMPLE_MS, oracle_nodes=64):
     """FP32C against FP64 on every node over [0, horizon_ms]: V of all nodes is
     compared on the device every `sample_ms` (through the device field-state
@@ -639,6 +657,7 @@ def main():
     }
     if accuracy is not None:
         out["accuracy"] = accuracy
+    out["one_step_calls"] = leg("one_step_calls", one_step_calls, N, y0, fp, dt)
     if not args.no_adjoint:
         adj = leg("adjoint", adjoint_run, steps=args.adjoint_steps)
         out["adjoint"] = adj
