@@ -165,6 +165,53 @@ def one_step_calls(N, y0, fp, dt, calls=100):
     return out
 
 
+def solve_rows(N, y0, fp, dt, steps=250):
+    """ODESystemSolver::solve (ODESystemSolver.cpp:81-99) on the same nodes: V of every node after each step
+    (8 bytes per node and step) into pageable host memory (goss_hip_system_solve: the rows of a chunk of steps
+    are copied after the chunk), into pinned host memory (goss_hip_system_solve_async: the copy of one chunk
+    runs while the next chunk steps) and into device memory, beside forward_times, which writes no rows.
+    Wall time of each call, from its start to the end of its last copy."""
+    from goss_amd.system import DeviceSystem, pinned_array
+    import torch
+
+    t = np.arange(steps + 1) * dt
+    nbytes = steps * N * 8
+    dests = {"pageable": np.zeros((steps + 1, N)), "pinned": pinned_array((steps + 1, N)),
+             "device": torch.zeros((steps + 1, N), dtype=torch.float64, device="cuda")}
+    dests["pinned"][:] = 0.0
+    out = {"workload": f"TP06 M-cell GRL1 FP32C, {N} nodes, {steps} steps of {dt} ms, V after every step "
+                       f"({nbytes / 1e9:.2f} GB of rows)"}
+
+    def once(kind):
+        dev = DeviceSystem(VARIANT, "GRL1", N, "fp32c")
+        dev.set_states(y0)
+        dev.set_field_parameters(fp)
+        dev.forward_times(t[:3])  # first launch of the kernel outside the timing
+        dev.synchronize()
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        if kind == "forward_times":
+            dev.forward_times(t)
+        elif kind == "pageable":
+            dev.solve(t, dests[kind])
+        else:
+            dev.solve_async(t, dests[kind])
+        dev.synchronize()
+        ms = (time.perf_counter() - w0) * 1e3
+        dev.close()
+        return ms
+
+    for kind in ("forward_times", "pageable", "pinned", "device"):
+        once(kind)  # the destination's pages touched, the copy paths set up
+        ms = once(kind)
+        out[f"{kind}_ms"] = round(ms, 2)
+        if kind != "forward_times":
+            out[f"{kind}_rows_GBps"] = round(nbytes / (ms * 1e-3) / 1e9, 2)
+    out["pinned_over_forward_times"] = round(out["pinned_ms"] / out["forward_times_ms"], 3)
+    out["pageable_over_forward_times"] = round(out["pageable_ms"] / out["forward_times_ms"], 3)
+    return out
+
+
 def accuracy_gate(y0, fp, dt, horizon_ms=GATE_HORIZON_MS, sample_ms=GATE_SAMPLE_MS, oracle_nodes=64):
     """FP32C against FP64 on every node over [0, horizon_ms]: V of all nodes is
     compared on the device every `sample_ms` (through the device field-state
@@ -552,6 +599,9 @@ def main():
     ap.add_argument("--implicit-steps", type=int, default=10000, help="steps of the implicit workload")
     ap.add_argument("--implicit-only", action="store_true",
                     help="only run the implicit-solver workload and print its figures as one JSON line")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only run the solve workload (rows into pageable, pinned and device memory) and print "
+                         "its figures as one JSON line")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only run the mixed-population batch workload and print its figures as one JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -594,6 +644,9 @@ def main():
     t_all = np.arange(W + K + 1) * dt
     t_warm, t_timed = t_all[: W + 1], t_all[W:]
     y0, fp = inputs(N, args.seed)
+    if args.solve_only:
+        print(json.dumps(solve_rows(N, y0, fp, dt)), flush=True)
+        return 0
     ev = HipEvents()
     log(f"{N} nodes, warmup {W}, steps {K}, dt {dt}")
 
@@ -658,6 +711,7 @@ def main():
     if accuracy is not None:
         out["accuracy"] = accuracy
     out["one_step_calls"] = leg("one_step_calls", one_step_calls, N, y0, fp, dt)
+    out["solve"] = leg("solve", solve_rows, N, y0, fp, dt)
     if not args.no_adjoint:
         adj = leg("adjoint", adjoint_run, steps=args.adjoint_steps)
         out["adjoint"] = adj

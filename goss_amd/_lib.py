@@ -40,7 +40,8 @@ EXPORTS = (
     "goss_hip_lu_factorize", "goss_hip_forward_backward_subst",
     "goss_hip_solver_params_default", "goss_hip_solver_is_adaptive",
     "goss_hip_system_create", "goss_hip_system_destroy", "goss_hip_system_forward",
-    "goss_hip_system_forward_times", "goss_hip_system_forward_times_noalloc", "goss_hip_system_solve", "goss_hip_system_get_field_states",
+    "goss_hip_system_forward_times", "goss_hip_system_forward_times_noalloc", "goss_hip_system_solve", "goss_hip_system_solve_async", "goss_hip_host_alloc",
+    "goss_hip_host_free", "goss_hip_system_get_field_states",
     "goss_hip_system_set_field_states", "goss_hip_system_get_field_state_components",
     "goss_hip_system_set_field_state_components", "goss_hip_system_get_field_parameters",
     "goss_hip_system_set_field_parameters", "goss_hip_system_get_field_states_device",
@@ -184,6 +185,9 @@ def load(path: str) -> ctypes.CDLL:
     L.goss_hip_system_forward_times.argtypes = [V, _dp, ctypes.c_uint64]
     L.goss_hip_system_forward_times_noalloc.argtypes = [V, _dp, ctypes.c_uint64]
     L.goss_hip_system_solve.argtypes = [V, _dp, _dp, ctypes.c_uint64, ctypes.c_int]
+    L.goss_hip_system_solve_async.argtypes = [V, ctypes.c_void_p, _dp, ctypes.c_uint64, ctypes.c_int]
+    L.goss_hip_host_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    L.goss_hip_host_free.argtypes = [V]
     for fn in ("get_field_states", "set_field_states", "get_field_parameters", "set_field_parameters",
                "get_field_states_device", "set_field_states_device"):
         getattr(L, f"goss_hip_system_{fn}").argtypes = [V, ctypes.c_void_p, ctypes.c_int]

@@ -197,15 +197,36 @@ struct goss_hip_system {
   // field vector of dof_values doubles.
   int* d_dofs = nullptr;
   uint64_t dof_values = 0;
-  // Adjoint backward pipeline (segment recomputes one segment ahead of the reverse sweeps).
+  // A second stream of the system: adjoint segment recomputes one segment ahead of the reverse
+  // sweeps, and the row copies of solve_async.
   hipStream_t aux = nullptr;
   hipEvent_t adj_ev[5] = {};
+  hipEvent_t sol_ev[4] = {};  // solve_async: rows written into staging buffer b (0, 1), copied out of b (2, 3)
+  double* sol_stage = nullptr;  // solve_async: two device staging buffers
+  size_t sol_stage_cap = 0;
 
-  void ensure_adjoint_pipeline()
+  void ensure_aux()
   {
     if (aux) return;
     hip_check(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking), "hipStreamCreate");
     for (hipEvent_t& e : adj_ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    for (hipEvent_t& e : sol_ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  }
+  void ensure_adjoint_pipeline() { ensure_aux(); }
+
+  double* solve_stage(size_t n)
+  {
+    if (n > sol_stage_cap) {
+      if (sol_stage) {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (aux) hip_check(hipStreamSynchronize(aux), "hipStreamSynchronize");
+        hip_check(hipFree(sol_stage), "hipFree");
+      }
+      sol_stage = nullptr;
+      hip_check(hipMalloc(&sol_stage, n * sizeof(double)), "hipMalloc(solve staging)");
+      sol_stage_cap = n;
+    }
+    return sol_stage;
   }
 
   size_t elem() const { return prec == kFP64 ? sizeof(double) : sizeof(float); }
@@ -444,7 +465,10 @@ struct goss_hip_system {
     if (h_error) (void)hipHostFree(h_error);
     if (d_dofs) (void)hipFree(d_dofs);
     if (aux) (void)hipStreamSynchronize(aux);
+    if (sol_stage) (void)hipFree(sol_stage);
     for (hipEvent_t e : adj_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : sol_ev)
       if (e) (void)hipEventDestroy(e);
     if (aux) (void)hipStreamDestroy(aux);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1108,6 +1132,69 @@ int goss_hip_adjoint_backward(goss_hip_system* s, const double* t, uint64_t n, u
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Where solve writes its rows: pageable host memory (staged and copied synchronously), pinned host
+// memory or device memory (the pipelined path of solve_async).
+enum class RowsDst { kPageable, kPinned, kDevice };
+
+RowsDst classify_rows_dst(const void* p)
+{
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // an unknown (pageable) pointer: clear the error
+    return RowsDst::kPageable;
+  }
+  if (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) return RowsDst::kDevice;
+  if (a.type == hipMemoryTypeHost) return RowsDst::kPinned;
+  return RowsDst::kPageable;
+}
+
+// Steps of solve with rows going to pinned host or device memory, enqueued without waiting.
+void solve_pipelined(goss_hip_system* s, double* field_states, const RowsDst dst, const double* t, uint64_t n,
+                     int tangled)
+{
+  const size_t row = size_t(s->N) * s->v->NFS;
+  std::vector<double2> st(n - 1);
+  for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
+  s->check_steps(st.data(), st.size());
+  const double2* d = s->upload_steps(st.data(), st.size());
+  if (row == 0 || dst == RowsDst::kDevice) {
+    // Rows 1..n-1 straight from the kernels.
+    s->launch_steps(d, st.size(), row ? field_states + row : nullptr, tangled);
+    return;
+  }
+  // Pinned host memory: chunks of rows alternate between two device staging buffers; the copy of
+  // chunk i (second stream) overlaps the steps of chunk i + 1, and chunk i + 2 waits for it.
+  const size_t chunk = std::max<size_t>(1, (size_t(64) << 20) / (row * sizeof(double)));
+  s->ensure_aux();
+  double* stage = s->solve_stage(2 * std::min(chunk, st.size()) * row);
+  const size_t half = std::min(chunk, st.size()) * row;
+  int last = 0;
+  for (size_t off = 0, i = 0; off < st.size(); off += chunk, ++i) {
+    const int b = int(i & 1);
+    const size_t cnt = std::min(chunk, st.size() - off);
+    double* buf = stage + size_t(b) * half;
+    if (i >= 2) hip_check(hipStreamWaitEvent(s->stream, s->sol_ev[2 + b], 0), "hipStreamWaitEvent");
+    s->launch_steps(d + off, cnt, buf, tangled);
+    hip_check(hipEventRecord(s->sol_ev[b], s->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(s->aux, s->sol_ev[b], 0), "hipStreamWaitEvent");
+    hip_check(hipMemcpyAsync(field_states + (off + 1) * row, buf, cnt * row * sizeof(double), hipMemcpyDeviceToHost,
+                             s->aux),
+              "hipMemcpyAsync(D2H)");
+    hip_check(hipEventRecord(s->sol_ev[2 + b], s->aux), "hipEventRecord");
+    last = b;
+  }
+  // Later work on the system's stream (and its synchronize) follows the last copy.
+  hip_check(hipStreamWaitEvent(s->stream, s->sol_ev[2 + last], 0), "hipStreamWaitEvent");
+}
+
+}  // namespace
+
+extern "C" {
+
 int goss_hip_system_solve(goss_hip_system* s, double* field_states, const double* t, uint64_t n, int tangled)
 {
   return guarded([&] {
@@ -1115,7 +1202,14 @@ int goss_hip_system_solve(goss_hip_system* s, double* field_states, const double
     if (n < 2) return;
     s->use_device();
     const size_t row = size_t(s->N) * s->v->NFS;
-    // Device rows are produced in chunks bounded by a staging budget (~256 MiB).
+    const RowsDst dst = row ? classify_rows_dst(field_states) : RowsDst::kDevice;
+    if (dst != RowsDst::kPageable) {
+      solve_pipelined(s, field_states, dst, t, n, tangled);
+      hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+      return;
+    }
+    // Pageable host memory: device rows are produced in chunks bounded by a staging budget
+    // (~256 MiB) and copied out after each chunk.
     const size_t budget_rows = std::max<size_t>(1, (size_t(256) << 20) / std::max<size_t>(row * sizeof(double), 1));
     std::vector<double2> st(n - 1);
     for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
@@ -1128,6 +1222,36 @@ int goss_hip_system_solve(goss_hip_system* s, double* field_states, const double
       if (row) s->copy_out(field_states + (off + 1) * row, dev, cnt * row);
     }
     hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  });
+}
+
+int goss_hip_system_solve_async(goss_hip_system* s, double* field_states, const double* t, uint64_t n, int tangled)
+{
+  return guarded([&] {
+    check_sys(s);
+    if (n < 2) return;
+    s->use_device();
+    const size_t row = size_t(s->N) * s->v->NFS;
+    const RowsDst dst = row ? classify_rows_dst(field_states) : RowsDst::kDevice;
+    if (dst == RowsDst::kPageable)
+      throw Error(GOSS_HIP_EINVAL, "solve_async needs pinned host memory (goss_hip_host_alloc) or device memory");
+    solve_pipelined(s, field_states, dst, t, n, tangled);
+  });
+}
+
+int goss_hip_host_alloc(void** ptr, size_t bytes)
+{
+  return guarded([&] {
+    if (!ptr) throw Error(GOSS_HIP_EINVAL, "null pointer");
+    *ptr = nullptr;
+    hip_check(hipHostMalloc(ptr, std::max<size_t>(bytes, 1), hipHostMallocDefault), "hipHostMalloc");
+  });
+}
+
+int goss_hip_host_free(void* ptr)
+{
+  return guarded([&] {
+    if (ptr) hip_check(hipHostFree(ptr), "hipHostFree");
   });
 }
 

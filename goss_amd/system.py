@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes
+import weakref
 from typing import Dict, List, Optional, Sequence
 
 import numpy as np
@@ -17,6 +18,19 @@ def _f64(a) -> np.ndarray:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
+
+
+def pinned_array(shape, library: Optional[str] = None) -> np.ndarray:
+    """A float64 array in pinned host memory (goss_hip_host_alloc), freed when
+    the array and its views are gone: a destination for
+    DeviceSystem.solve_async."""
+    L = load(library) if library else lib()
+    n = int(np.prod(shape, dtype=np.int64))
+    p = ctypes.c_void_p()
+    check(L.goss_hip_host_alloc(ctypes.byref(p), max(n, 1) * 8), L)
+    buf = (ctypes.c_double * max(n, 1)).from_address(p.value)
+    weakref.finalize(buf, L.goss_hip_host_free, p.value)
+    return np.frombuffer(buf, dtype=np.float64)[:n].reshape(shape)
 
 
 def model_names() -> List[str]:
@@ -111,6 +125,27 @@ class DeviceSystem:
         if not (out.flags.c_contiguous and out.dtype == np.float64):
             raise ValueError("out must be a C-contiguous float64 array")
         check(self._L.goss_hip_system_solve(self._h, _ptr(out), _ptr(t), len(t), int(tangled)), self._L)
+
+    def solve_async(self, t, out, tangled: bool = True) -> None:
+        """solve into pinned host memory (pinned_array, a pinned torch tensor)
+        or device memory (a CUDA tensor, or a device address), returning once
+        the steps and the row copies are enqueued on the system's streams;
+        synchronize() waits for them.  Rows 1..len(t)-1 are written."""
+        t = _f64(t)
+        need = len(t) * self.num_nodes * self.model.num_field_states
+        if isinstance(out, int):
+            addr = out
+        elif isinstance(out, np.ndarray):
+            if not (out.flags.c_contiguous and out.dtype == np.float64 and out.size >= need):
+                raise ValueError(f"out must be a C-contiguous float64 array of at least {need} values")
+            addr = out.ctypes.data
+        elif hasattr(out, "data_ptr"):
+            if not (str(out.dtype) == "torch.float64" and out.is_contiguous() and out.numel() >= need):
+                raise ValueError(f"out must be a contiguous float64 tensor of at least {need} values")
+            addr = out.data_ptr()
+        else:
+            raise TypeError("out must be a pinned or device buffer (numpy array, torch tensor or address)")
+        check(self._L.goss_hip_system_solve_async(self._h, addr, _ptr(t), len(t), int(tangled)), self._L)
 
     def synchronize(self) -> None:
         check(self._L.goss_hip_system_synchronize(self._h), self._L)

@@ -190,6 +190,22 @@ int goss_hip_system_forward_times_noalloc(goss_hip_system* sys, const double* t,
  * rows 1..n-1 of field_states[n][...] receive the field states after each
  * step; row 0 is left to the caller (python/goss/systemsolver.py:130-132). */
 int goss_hip_system_solve(goss_hip_system* sys, double* field_states, const double* t, uint64_t n, int tangled);
+/* ODESystemSolver::solve without waiting.  The reference's solve is
+ * synchronous and writes to host memory; here field_states is pinned host
+ * memory (goss_hip_host_alloc, or any allocation HIP has pinned) or device
+ * memory, and the call returns once the steps and the copies of the rows are
+ * enqueued.  Pinned destinations get the rows in chunks: the copy of one chunk
+ * (on a second stream of the system) runs while the steps of the next chunk
+ * run, so the host link and the GPU work together.  Device destinations are
+ * written by the kernels directly.  The times t are read during the call.
+ * Work enqueued on the system afterwards, and goss_hip_system_synchronize,
+ * follow the copies.  goss_hip_system_solve takes the same path when
+ * field_states is pinned or device memory, and waits. */
+int goss_hip_system_solve_async(goss_hip_system* sys, double* field_states, const double* t, uint64_t n,
+                                int tangled);
+/* Pinned (page-locked) host memory for goss_hip_system_solve_async. */
+int goss_hip_host_alloc(void** ptr, size_t bytes);
+int goss_hip_host_free(void* ptr);
 
 /* The solver's public tunables after creation (GRL1::delta, ImplicitODESolver's
  * eta_0 ..., ODESolver::set_internal_time_step; ODESolver.h:119-128): the

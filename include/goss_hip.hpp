@@ -600,6 +600,14 @@ class ODESystemSolver {
     check(goss_hip_system_solve(sys_, field_states, t, num_timesteps, tangled_storage));
     after();
   }
+  // solve into pinned host memory (goss_hip_host_alloc) or device memory, returning once the steps and the
+  // row copies are enqueued; synchronize() waits for them.
+  void solve_async(double* field_states, const double* t, const ulong num_timesteps, bool tangled_storage = true)
+  {
+    before();
+    check(goss_hip_system_solve_async(sys_, field_states, t, num_timesteps, tangled_storage));
+    after();
+  }
   void get_field_state_components(double* component_field_states, uint num_components, const uint* components,
                                   bool tangled_storage = true) const
   {
