@@ -449,8 +449,10 @@ def implicit_run(N=262144, steps=10000, dt=0.1, seed=2027, fd64_steps=500):
     .ode's default amplitude is 0).  The Jacobian is formed from forward
     differences of the FP32 right-hand side, with the Newton iteration, LU and
     substitutions in FP64, over the full `steps`; the reference's FP64
-    forward-difference Jacobian runs over the first `fd64_steps` steps beside
-    it.  Wall time around the call and a synchronize, after a warm-up step."""
+    forward-difference Jacobian and the model's analytic Jacobian (a plugin
+    generated with generate_jacobian) run over the first `fd64_steps` steps
+    beside it.  Wall time around the call and a synchronize, after a warm-up
+    step."""
     from goss_amd.system import DeviceSystem, ModelVariant
 
     mv = ModelVariant("tp06_epi")
@@ -459,7 +461,7 @@ def implicit_run(N=262144, steps=10000, dt=0.1, seed=2027, fd64_steps=500):
     out = {"workload": f"TP06 epi, stimulus -52 pA/pF 1 ms at 10 ms, {N} nodes, dt {dt} ms"}
     for solver in ("ImplicitEuler", "ESDIRK23a"):
         base = {"relative_tolerance": 1e-6} if solver == "ImplicitEuler" else {}
-        for jac, n_steps in (("fd_fp32", steps), ("fd", fd64_steps)):
+        for jac, n_steps in (("fd_fp32", steps), ("fd", fd64_steps), ("analytic", fd64_steps)):
             t = np.arange(n_steps + 1) * dt
             s = DeviceSystem("tp06_epi", solver, N, "fp64", {**base, "jacobian": jac})
             s.set_parameter("stim_amplitude", 52.0)

@@ -29,14 +29,14 @@ SOLVERS = {
     "ESDIRK4O32": 12,
 }
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp32c": 2}
-JACOBIANS = {"fd": 0, "fd_fp32": 1}
+JACOBIANS = {"fd": 0, "fd_fp32": 1, "analytic": 2}
 
 # Functions whose C signatures this module binds (checked by the tests against
 # the exported symbols).
 EXPORTS = (
     "goss_hip_last_error", "goss_hip_device_count", "goss_hip_model_count", "goss_hip_model_name",
     "goss_hip_model_info_get", "goss_hip_model_load", "goss_hip_model_eval", "goss_hip_model_linearized_eval",
-    "goss_hip_model_compute_jacobian", "goss_hip_model_intermediates", "goss_hip_model_eval_monitored",
+    "goss_hip_model_compute_jacobian", "goss_hip_model_compute_analytic_jacobian", "goss_hip_model_intermediates", "goss_hip_model_eval_monitored",
     "goss_hip_lu_factorize", "goss_hip_forward_backward_subst",
     "goss_hip_solver_params_default", "goss_hip_solver_is_adaptive",
     "goss_hip_system_create", "goss_hip_system_destroy", "goss_hip_system_forward",
@@ -170,6 +170,7 @@ def load(path: str) -> ctypes.CDLL:
         L.goss_hip_model_linearized_eval.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, ctypes.c_uint64, ctypes.c_int,
                                                      _dp, _dp]
         L.goss_hip_model_compute_jacobian.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, ctypes.c_uint64, _dp]
+        L.goss_hip_model_compute_analytic_jacobian.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, ctypes.c_uint64, _dp]
         L.goss_hip_model_intermediates.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
                                                    ctypes.POINTER(ctypes.POINTER(ctypes.c_char_p))]
         L.goss_hip_model_eval_monitored.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),

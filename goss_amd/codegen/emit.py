@@ -613,6 +613,40 @@ def emit_model_header(var: Variant, with_jacobian: bool = True) -> Emitted:
         L.append("")
         census["jacobian"] = census_body([e for _, e in jinter + jstmts] + [e for row in hjac for e in row])
 
+        # The same entries over the structure of the Jacobian (column-major, the storage of the implicit
+        # kernels), each stored right after the last statement it reads, so that few temporaries stay live.
+        cols = [sorted(j for j in range(S) if i in struct_rows[j]) for i in range(S)]
+        slot = {}
+        for i in range(S):
+            for j in cols[i]:
+                slot[(j, i)] = len(slot)
+        for i in range(S):
+            for j in range(S):
+                if hjac[i][j] != 0 and (i, j) not in slot:
+                    raise ValueError(f"{var.name}: analytic Jacobian entry ({i}, {j}) outside the structure")
+        stmts = jinter + jstmts
+        pos = {n: k for k, (n, _) in enumerate(stmts)}
+        after: Dict[int, List[str]] = {}
+        for (i, j), k in sorted(slot.items(), key=lambda kv: kv[1]):
+            e = hjac[i][j]
+            last = max([pos[s.name] for s in e.free_symbols if s.name in pos], default=-1)
+            after.setdefault(last, []).append(f"{ind}J[{k}] = {pr.p(e) if e != 0 else 'real(0.0)'};")
+        L.append("// The analytic Jacobian over its structure: J[slot] for the entries of jac_cols / jac_col_start")
+        L.append("// (d values[j] / d states[i] at jac_col_start[i] + rank of j in jac_cols[i]).  `J` is a pointer")
+        L.append("// or any object whose J[k] = value stores entry k.")
+        L.append("template <typename real, typename PA, typename treal, typename JS>")
+        L.append("GOSS_HD GOSS_INLINE void compute_jacobian_structured(const real* __restrict__ states, const treal time,")
+        L.append("                                                     const PA& pa, const JS& J)")
+        L.append("{")
+        L.append(f"{ind}(void)time; (void)pa;")
+        L += g.bind(jused, ind)
+        L += after.get(-1, [])
+        for k, st in enumerate(stmts):
+            L += g.lines([st], ind)
+            L += after.get(k, [])
+        L.append("}")
+        L.append("")
+
     L.append("}}  // namespace goss_models::" + var.name)
     L.append("")
     return Emitted(header="\n".join(L), census=census, linear_terms=lin.linear_terms, jac_nnz=jac_nnz,
@@ -677,6 +711,7 @@ def emit_parameterized_ode(var: Variant, header_include: str) -> str:
     L.append(f"  {N}(const {N}& ode) : ParameterizedODE(ode)")
     L.append("  {")
     L.append(f"    for (int i = 0; i < {P}; ++i) _p[i] = ode._p[i];")
+    L.append("    analytic_jacobian = ode.analytic_jacobian;")
     L.append("    _bind();")
     L.append("  }")
     L.append("  // Parameters can change through _param_to_value at any time, so the")
@@ -700,6 +735,20 @@ def emit_parameterized_ode(var: Variant, header_include: str) -> str:
     L.append("    values->data.reset(new double[_num_states]);")
     L.append(f"    for (int i = 0; i < {S}; ++i) values->data[i] = {ns}::initial_conditions[i];")
     L.append("  }")
+    L.append("  // ODE::compute_jacobian (ODE.h:71): the analytic Jacobian when analytic_jacobian is set, as a plugin")
+    L.append("  // generated with generate_jacobian would have it (codegeneration.py:186, 622-641); otherwise the")
+    L.append("  // forward differences of the base class, as gotran's default plugin.")
+    L.append("  void compute_jacobian(double* states, double time, double* jac)")
+    L.append("  {")
+    L.append("    if (!analytic_jacobian) {")
+    L.append("      ODE::compute_jacobian(states, time, jac);")
+    L.append("      return;")
+    L.append("    }")
+    L.append("    double c[K];")
+    L.append(f"    {ns}::compute_constants<double>(Acc{{_p, nullptr}}, c);")
+    L.append(f"    {ns}::compute_jacobian<double>(states, time, Acc{{_p, c}}, jac);")
+    L.append("  }")
+    L.append("  bool analytic_jacobian = false;")
     L.append("  std::shared_ptr<ODE> copy() const")
     L.append(f"  {{ return std::make_shared<{N}>(*this); }}")
     L.append("  void eval_monitored(const double*, double, double*) const")

@@ -548,4 +548,30 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   jac[3] = c__5;
 }
 
+// The analytic Jacobian over its structure: J[slot] for the entries of jac_cols / jac_col_start
+// (d values[j] / d states[i] at jac_col_start[i] + rank of j in jac_cols[i]).  `J` is a pointer
+// or any object whose J[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JS>
+GOSS_HD GOSS_INLINE void compute_jacobian_structured(const real* __restrict__ states, const treal time,
+                                                     const PA& pa, const JS& J)
+{
+  (void)time; (void)pa;
+  const real v = states[0];
+  const real s = states[1];
+  const real b = pa.template get<1>();
+  const real v_peak = pa.template get<9>();
+  const real v_rest = pa.template get<10>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  J[1] = b;
+  J[3] = c__5;
+  const real dI_dv = c__2*s + c__3*(v - c__1)*(v_peak - v) + c__3*(v - v_rest)*(v_peak - v) + c__6*(v - c__1)*(v - v_rest);
+  J[0] = dI_dv;
+  const real dI_ds = c__2*(v - v_rest);
+  J[2] = dI_ds;
+}
+
 }}  // namespace goss_models::fhn

@@ -548,4 +548,30 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   jac[3] = c__5;
 }
 
+// The analytic Jacobian over its structure: J[slot] for the entries of jac_cols / jac_col_start
+// (d values[j] / d states[i] at jac_col_start[i] + rank of j in jac_cols[i]).  `J` is a pointer
+// or any object whose J[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JS>
+GOSS_HD GOSS_INLINE void compute_jacobian_structured(const real* __restrict__ states, const treal time,
+                                                     const PA& pa, const JS& J)
+{
+  (void)time; (void)pa;
+  const real V = states[0];
+  const real s = states[1];
+  const real V_peak = pa.template get<0>();
+  const real V_rest = pa.template get<1>();
+  const real b = pa.template get<3>();
+  const real c__1 = pa.template cst<1>();
+  const real c__2 = pa.template cst<2>();
+  const real c__3 = pa.template cst<3>();
+  const real c__5 = pa.template cst<5>();
+  const real c__6 = pa.template cst<6>();
+  J[1] = b;
+  J[3] = c__5;
+  const real dI_dV = c__2*s + c__3*(V - V_rest)*(V_peak - V) + c__3*(V - c__1)*(V_peak - V) + c__6*(V - V_rest)*(V - c__1);
+  J[0] = dI_dV;
+  const real dI_ds = c__2*(V - V_rest);
+  J[2] = dI_ds;
+}
+
 }}  // namespace goss_models::fitzhughnagumo

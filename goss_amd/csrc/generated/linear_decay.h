@@ -298,4 +298,16 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   jac[0] = a;
 }
 
+// The analytic Jacobian over its structure: J[slot] for the entries of jac_cols / jac_col_start
+// (d values[j] / d states[i] at jac_col_start[i] + rank of j in jac_cols[i]).  `J` is a pointer
+// or any object whose J[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JS>
+GOSS_HD GOSS_INLINE void compute_jacobian_structured(const real* __restrict__ states, const treal time,
+                                                     const PA& pa, const JS& J)
+{
+  (void)time; (void)pa;
+  const real a = pa.template get<0>();
+  J[0] = a;
+}
+
 }}  // namespace goss_models::linear_decay

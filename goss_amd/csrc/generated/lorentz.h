@@ -425,4 +425,30 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   jac[8] = -beta;
 }
 
+// The analytic Jacobian over its structure: J[slot] for the entries of jac_cols / jac_col_start
+// (d values[j] / d states[i] at jac_col_start[i] + rank of j in jac_cols[i]).  `J` is a pointer
+// or any object whose J[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JS>
+GOSS_HD GOSS_INLINE void compute_jacobian_structured(const real* __restrict__ states, const treal time,
+                                                     const PA& pa, const JS& J)
+{
+  (void)time; (void)pa;
+  const real x = states[0];
+  const real y = states[1];
+  const real z = states[2];
+  const real beta = pa.template get<0>();
+  const real rho = pa.template get<1>();
+  const real sigma = pa.template get<2>();
+  J[0] = -sigma;
+  J[2] = y;
+  J[3] = sigma;
+  J[4] = -real(1.0);
+  J[5] = x;
+  J[7] = -beta;
+  const real ddy_dt_dx = rho - z;
+  J[1] = ddy_dt_dx;
+  const real ddy_dt_dz = -x;
+  J[6] = ddy_dt_dz;
+}
+
 }}  // namespace goss_models::lorentz

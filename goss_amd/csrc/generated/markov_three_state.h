@@ -535,4 +535,29 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   jac[8] = -k_io;
 }
 
+// The analytic Jacobian over its structure: J[slot] for the entries of jac_cols / jac_col_start
+// (d values[j] / d states[i] at jac_col_start[i] + rank of j in jac_cols[i]).  `J` is a pointer
+// or any object whose J[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JS>
+GOSS_HD GOSS_INLINE void compute_jacobian_structured(const real* __restrict__ states, const treal time,
+                                                     const PA& pa, const JS& J)
+{
+  (void)time; (void)pa;
+  const real k_io = pa.template get<2>();
+  const real k_oc = pa.template get<3>();
+  const real k_oi = pa.template get<4>();
+  J[2] = k_oc;
+  J[5] = k_io;
+  J[6] = -k_io;
+  const treal k_co_t = pa.template getd<1>()*(treal(1.0) + pa.template getd<0>()*m_sin(time*pa.template getd<5>()));
+  const treal rates_C_O = k_co_t;
+  J[1] = real(rates_C_O);
+  const real rates_O_I = k_oi;
+  J[4] = rates_O_I;
+  const treal ddC_dt_dC = -rates_C_O;
+  J[0] = real(ddC_dt_dC);
+  const real ddO_dt_dO = -k_oc - rates_O_I;
+  J[3] = ddO_dt_dO;
+}
+
 }}  // namespace goss_models::markov_three_state

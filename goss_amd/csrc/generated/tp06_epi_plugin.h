@@ -36,6 +36,7 @@ public:
   Tp06_epi(const Tp06_epi& ode) : ParameterizedODE(ode)
   {
     for (int i = 0; i < 53; ++i) _p[i] = ode._p[i];
+    analytic_jacobian = ode.analytic_jacobian;
     _bind();
   }
   // Parameters can change through _param_to_value at any time, so the
@@ -59,6 +60,20 @@ public:
     values->data.reset(new double[_num_states]);
     for (int i = 0; i < 19; ++i) values->data[i] = goss_models::tp06_epi::initial_conditions[i];
   }
+  // ODE::compute_jacobian (ODE.h:71): the analytic Jacobian when analytic_jacobian is set, as a plugin
+  // generated with generate_jacobian would have it (codegeneration.py:186, 622-641); otherwise the
+  // forward differences of the base class, as gotran's default plugin.
+  void compute_jacobian(double* states, double time, double* jac)
+  {
+    if (!analytic_jacobian) {
+      ODE::compute_jacobian(states, time, jac);
+      return;
+    }
+    double c[K];
+    goss_models::tp06_epi::compute_constants<double>(Acc{_p, nullptr}, c);
+    goss_models::tp06_epi::compute_jacobian<double>(states, time, Acc{_p, c}, jac);
+  }
+  bool analytic_jacobian = false;
   std::shared_ptr<ODE> copy() const
   { return std::make_shared<Tp06_epi>(*this); }
   void eval_monitored(const double*, double, double*) const

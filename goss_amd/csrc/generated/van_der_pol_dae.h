@@ -310,4 +310,19 @@ GOSS_HD GOSS_INLINE void compute_jacobian(const real* __restrict__ states, const
   jac[3] = dalg_z_0_dz;
 }
 
+// The analytic Jacobian over its structure: J[slot] for the entries of jac_cols / jac_col_start
+// (d values[j] / d states[i] at jac_col_start[i] + rank of j in jac_cols[i]).  `J` is a pointer
+// or any object whose J[k] = value stores entry k.
+template <typename real, typename PA, typename treal, typename JS>
+GOSS_HD GOSS_INLINE void compute_jacobian_structured(const real* __restrict__ states, const treal time,
+                                                     const PA& pa, const JS& J)
+{
+  (void)time; (void)pa;
+  const real z = states[1];
+  J[0] = real(1.0);
+  J[1] = -real(1.0);
+  const real dalg_z_0_dz = real(1.0) - real(1.0)*m_sq(z);
+  J[2] = dalg_z_0_dz;
+}
+
 }}  // namespace goss_models::van_der_pol_dae

@@ -38,6 +38,7 @@ public:
   Van_der_pol_dae(const Van_der_pol_dae& ode) : ParameterizedODE(ode)
   {
     for (int i = 0; i < 0; ++i) _p[i] = ode._p[i];
+    analytic_jacobian = ode.analytic_jacobian;
     _bind();
   }
   // Parameters can change through _param_to_value at any time, so the
@@ -61,6 +62,20 @@ public:
     values->data.reset(new double[_num_states]);
     for (int i = 0; i < 2; ++i) values->data[i] = goss_models::van_der_pol_dae::initial_conditions[i];
   }
+  // ODE::compute_jacobian (ODE.h:71): the analytic Jacobian when analytic_jacobian is set, as a plugin
+  // generated with generate_jacobian would have it (codegeneration.py:186, 622-641); otherwise the
+  // forward differences of the base class, as gotran's default plugin.
+  void compute_jacobian(double* states, double time, double* jac)
+  {
+    if (!analytic_jacobian) {
+      ODE::compute_jacobian(states, time, jac);
+      return;
+    }
+    double c[K];
+    goss_models::van_der_pol_dae::compute_constants<double>(Acc{_p, nullptr}, c);
+    goss_models::van_der_pol_dae::compute_jacobian<double>(states, time, Acc{_p, c}, jac);
+  }
+  bool analytic_jacobian = false;
   std::shared_ptr<ODE> copy() const
   { return std::make_shared<Van_der_pol_dae>(*this); }
   void eval_monitored(const double*, double, double*) const

@@ -709,7 +709,7 @@ void model_eval(const char* variant, const double* params, const double* states,
   if (!v->model_eval) throw Error(GOSS_HIP_EUNSUPPORTED, "model evaluation not compiled for this variant");
   if (n == 0) return;
   const size_t S = size_t(v->S);
-  const size_t w0 = kind == 2 ? S * S : (kind == 3 ? size_t(mon.n) : S);
+  const size_t w0 = (kind == 2 || kind == 4) ? S * S : (kind == 3 ? size_t(mon.n) : S);
   if (kind == 3 && mon.n == 0) return;
   DevBuf y(n * S), t(n), o0(n * w0), o1(kind == 1 ? n * S : 0);
   y.in(states, n * S);
@@ -754,6 +754,12 @@ int goss_hip_model_compute_jacobian(const char* variant, const double* params, c
                                     const double* times, uint64_t n, double* jac)
 {
   return guarded([&] { model_eval(variant, params, states, times, n, 2, 0, jac, nullptr); });
+}
+
+int goss_hip_model_compute_analytic_jacobian(const char* variant, const double* params, const double* states,
+                                             const double* times, uint64_t n, double* jac)
+{
+  return guarded([&] { model_eval(variant, params, states, times, n, 4, 0, jac, nullptr); });
 }
 
 int goss_hip_model_intermediates(const char* variant, int* count, const char* const** names)

@@ -42,12 +42,13 @@ namespace goss_hip {
 enum JacobianMode : int {
   kJacobianFD = 0,      // ODE::compute_jacobian (ODE.cpp:47-67): FP64 forward differences
   kJacobianFDFP32 = 1,  // forward differences of the FP32 right-hand side, stored as FP64
-  kNumJacobianModes = 2
+  kJacobianAnalytic = 2,  // the generated analytic Jacobian (a plugin generated with generate_jacobian)
+  kNumJacobianModes = 3
 };
 
-// Accessor of the implicit kernels in kJacobianFDFP32 mode: forwards the
-// parameter reads of the FP64 model code to `acc` and carries the FP32
-// accessor for the Jacobian's evaluations.
+// Accessor of the implicit kernels in the kJacobianFDFP32 and kJacobianAnalytic
+// modes: forwards the parameter reads of the FP64 model code to `acc`, carries
+// the FP32 accessor for the FP32 Jacobian's evaluations and names the mode.
 template <class M, int JAC>
 struct JacAcc {
   static constexpr int jacobian_mode = JAC;
@@ -280,6 +281,14 @@ struct Implicit {
   {
     if constexpr (JacobianModeOf<A>::value == kJacobianFDFP32) {
       if (!fd_jacobian_f32(y, t, acc, f1, J)) fd_jacobian(y, t, acc, f1, J);
+    } else if constexpr (JacobianModeOf<A>::value == kJacobianAnalytic) {
+      // ODE::compute_jacobian overridden by the model (ODE.h:71; codegeneration.py:622-641), each entry
+      // stored over the structure as soon as it is computed.
+      (void)f1;
+      double ys[S];
+#pragma unroll
+      for (int j = 0; j < S; ++j) ys[j] = y[j];
+      M::template compute_jacobian_structured<double>(ys, t, acc.acc, J);
     } else {
       fd_jacobian(y, t, acc, f1, J);
     }

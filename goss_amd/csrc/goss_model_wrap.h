@@ -45,6 +45,11 @@
     {                                                                                                                \
       NSN::compute_jacobian<R>(y, t, a, jac);                                                                        \
     }                                                                                                                \
+    template <typename R, typename A, typename T, typename J>                                                        \
+    GOSS_HD GOSS_INLINE static void compute_jacobian_structured(const R* y, const T t, const A& a, const J& jac)     \
+    {                                                                                                                \
+      NSN::compute_jacobian_structured<R>(y, t, a, jac);                                                             \
+    }                                                                                                                \
     static constexpr int num_intermediates = NSN::num_intermediates;                                                 \
     template <typename R, typename A, typename T>                                                                    \
     GOSS_HD GOSS_INLINE static void eval_intermediates(const R* y, const T t, const A& a, R* out)                    \

@@ -45,6 +45,17 @@ hipError_t launch_implicit(const ImplicitArgs& a)
                      a.nouter, sp, spf, a.ip, a.B, a.N, a.ld, a.fo);
   return hipGetLastError();
 }
+
+// The five implicit solvers with one Jacobian mode.
+template <int JAC>
+void register_implicit(VariantInfo& v)
+{
+  v.implicit_launch[0][JAC] = &launch_implicit<kImplicitEuler, JAC>;
+  v.implicit_launch[1][JAC] = &launch_implicit<kESDIRK23a, JAC>;
+  v.implicit_launch[2][JAC] = &launch_implicit<kThetaSolver, JAC>;
+  v.implicit_launch[3][JAC] = &launch_implicit<kBasicImplicitEuler, JAC>;
+  v.implicit_launch[4][JAC] = &launch_implicit<kESDIRK4O32, JAC>;
+}
 #endif
 
 template <class R>
@@ -169,6 +180,9 @@ __global__ void __launch_bounds__(kBlock) model_eval_kernel(const double* __rest
       out0[p * S + i] = lin[i];
       out1[p * S + i] = f[i];
     }
+  } else if constexpr (KIND == 4) {
+    // ODE::compute_jacobian (ODE.h:71) overridden by the generated analytic Jacobian.
+    M::template compute_jacobian<double>(y, t, acc, out0 + p * S * S);
   } else if constexpr (KIND == 3) {
     // ParameterizedODE::eval_monitored (ParameterizedODE.h:90): the selected intermediates.
     double all[M::num_intermediates > 0 ? M::num_intermediates : 1];
@@ -212,6 +226,8 @@ hipError_t launch_model_eval(const ModelEvalArgs& a)
     GOSS_MODEL_EVAL(2);
   else if (a.kind == 3)
     GOSS_MODEL_EVAL(3);
+  else if (a.kind == 4)
+    GOSS_MODEL_EVAL(4);
   else
     return hipErrorInvalidValue;
 #undef GOSS_MODEL_EVAL
@@ -294,11 +310,8 @@ void fill_info(VariantInfo& v)
 void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
 {
   fill_info(v);
-  v.implicit_launch[0][kJacobianFD] = &launch_implicit<kImplicitEuler, kJacobianFD>;
-  v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
-  v.implicit_launch[2][kJacobianFD] = &launch_implicit<kThetaSolver, kJacobianFD>;
-  v.implicit_launch[3][kJacobianFD] = &launch_implicit<kBasicImplicitEuler, kJacobianFD>;
-  v.implicit_launch[4][kJacobianFD] = &launch_implicit<kESDIRK4O32, kJacobianFD>;
+  register_implicit<kJacobianFD>(v);
+  register_implicit<kJacobianAnalytic>(v);
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
 }
 #elif defined(GOSS_PREC_F64)
@@ -306,15 +319,12 @@ void GOSS_CAT3(goss_register_, GOSS_VARIANT, _f64)(VariantInfo& v)
 {
   fill_info(v);
   register_family<double, false>(v, kFP64);
-  v.implicit_launch[0][kJacobianFD] = &launch_implicit<kImplicitEuler, kJacobianFD>;
+  register_implicit<kJacobianFD>(v);
+  register_implicit<kJacobianAnalytic>(v);
+  // The FP32 Jacobian for ImplicitEuler, ESDIRK23a and ThetaSolver.
   v.implicit_launch[0][kJacobianFDFP32] = &launch_implicit<kImplicitEuler, kJacobianFDFP32>;
-  v.implicit_launch[1][kJacobianFD] = &launch_implicit<kESDIRK23a, kJacobianFD>;
   v.implicit_launch[1][kJacobianFDFP32] = &launch_implicit<kESDIRK23a, kJacobianFDFP32>;
-  v.implicit_launch[2][kJacobianFD] = &launch_implicit<kThetaSolver, kJacobianFD>;
   v.implicit_launch[2][kJacobianFDFP32] = &launch_implicit<kThetaSolver, kJacobianFDFP32>;
-  // BasicImplicitEuler and ESDIRK4O32 with the reference's Jacobian only.
-  v.implicit_launch[3][kJacobianFD] = &launch_implicit<kBasicImplicitEuler, kJacobianFD>;
-  v.implicit_launch[4][kJacobianFD] = &launch_implicit<kESDIRK4O32, kJacobianFD>;
   v.rkf32_launch[kFP64] = &launch_rkf32<double>;
   register_adjoint<double>(v, kFP64);
   v.model_eval = &launch_model_eval;

@@ -119,6 +119,9 @@ class ODE:
 
             self._info = jit.metadata(*self._pending)
         self._params = self._info.default_parameters.copy()
+        # code_params={"generate_jacobian": True} (codegeneration.py:186): the model's compute_jacobian is
+        # its analytic Jacobian, and the implicit solvers use it.
+        self._analytic_jacobian = bool((kwargs.get("code_params") or {}).get("generate_jacobian", False))
         self._monitored: List[int] = []
         monitored = kwargs.get("monitored") or []
         names = self._intermediate_names()
@@ -132,6 +135,7 @@ class ODE:
         self._pending = other._pending
         self._info = other._info
         self._params = other._params.copy()
+        self._analytic_jacobian = getattr(other, "_analytic_jacobian", False)
         self._monitored = list(other._monitored)
 
     def __repr__(self) -> str:
@@ -209,14 +213,21 @@ class ODE:
             raise RuntimeError("throw - Index out of range")
         return float(self.eval(y, time)[component])
 
+    @property
+    def has_analytic_jacobian(self) -> bool:
+        """Whether the model was made with code_params={"generate_jacobian": True}."""
+        return getattr(self, "_analytic_jacobian", False)
+
     def compute_jacobian(self, states, time: float) -> np.ndarray:
-        """Forward-difference Jacobian (ODE.cpp:47-67), [S, S]."""
+        """The Jacobian, [S, S]: forward differences (ODE.cpp:47-67), or the generated analytic
+        Jacobian for a model made with code_params={"generate_jacobian": True} (codegeneration.py:622-641)."""
         y = self._check_state_input(states)
         S = self.num_states
         jac = np.zeros((1, S * S))
         t = np.array([time], dtype=np.float64)
-        check(lib().goss_hip_model_compute_jacobian(self.device_variant().encode(), _p(self._params), _p(y[None, :].copy()),
-                                                    _p(t), 1, _p(jac)))
+        fn = (lib().goss_hip_model_compute_analytic_jacobian if self.has_analytic_jacobian
+              else lib().goss_hip_model_compute_jacobian)
+        check(fn(self.device_variant().encode(), _p(self._params), _p(y[None, :].copy()), _p(t), 1, _p(jac)))
         return jac.reshape(S, S)
 
     def linearized_eval(self, states, time: float, only_linear: bool = False) -> LinearizedEval:

@@ -133,6 +133,8 @@ class ODESolver(abc.ABC):
                 d[k] = self._params[k]
         if self.adaptive:
             d["atol"], d["rtol"] = self._atol, self._rtol
+        if isinstance(self, ImplicitODESolver) and self._ode is not None and self._ode.has_analytic_jacobian:
+            d["jacobian"] = "analytic"  # the model's ODE::compute_jacobian is its analytic Jacobian
         return d
 
     def _one_node(self):

@@ -74,7 +74,7 @@ def solver_params(solver: str, library: Optional[str] = None, **overrides) -> So
         if not hasattr(sp, k):
             raise KeyError(f"unknown solver parameter {k!r}")
         if k == "jacobian" and isinstance(v, str):
-            v = JACOBIANS[v]  # "fd" or "fd_fp32"
+            v = JACOBIANS[v]  # "fd", "fd_fp32" or "analytic"
         setattr(sp, k, type(getattr(sp, k))(v))
     return sp
 

@@ -47,8 +47,8 @@ typedef enum goss_hip_solver {
   GOSS_HIP_ESDIRK23A = 8,      /* ESDIRK23a.cpp:280-316 */
   GOSS_HIP_RKF32 = 9,          /* RKF32.cpp:105-222 */
   GOSS_HIP_THETA = 10,         /* ThetaSolver.cpp:48-155 */
-  GOSS_HIP_BASIC_IMPLICIT_EULER = 11, /* BasicImplicitEuler.cpp:62-181 (FP64, reference Jacobian) */
-  GOSS_HIP_ESDIRK4O32 = 12     /* ESDIRK4O32.cpp:92-298 (FP64, reference Jacobian) */
+  GOSS_HIP_BASIC_IMPLICIT_EULER = 11, /* BasicImplicitEuler.cpp:62-181 (FP64) */
+  GOSS_HIP_ESDIRK4O32 = 12     /* ESDIRK4O32.cpp:92-298 (FP64) */
 } goss_hip_solver;
 
 /* Arithmetic of the per-node work.  FP64 is the reference's arithmetic.
@@ -79,16 +79,23 @@ typedef struct goss_hip_solver_params {
   int max_iterations;
   int always_recompute_jacobian;
   int num_refinements_without_always_recomputing_jacobian;
-  /* ImplicitEuler / ESDIRK23a: how the Jacobian is formed.  FD is the
-   * reference's (ODE.cpp:47-67); FD_FP32 has no counterpart in the reference:
-   * forward differences of the model's FP32 right-hand side.  The Newton
-   * iteration, its residual, the LU and the substitutions are FP64 in both. */
+  /* Implicit solvers: how the Jacobian is formed (GOSS_HIP_JACOBIAN_*).  FD is
+   * the reference's default (ODE.cpp:47-67); ANALYTIC the generated analytic
+   * Jacobian; FD_FP32 (ImplicitEuler, ESDIRK23a, ThetaSolver) has no
+   * counterpart in the reference: forward differences of the model's FP32
+   * right-hand side.  The Newton iteration, its residual, the LU and the
+   * substitutions are FP64 in all. */
   int jacobian;
   double theta; /* ThetaSolver::theta (ThetaSolver.h:37) */
 } goss_hip_solver_params;
 
 #define GOSS_HIP_JACOBIAN_FD 0      /* forward differences of the FP64 right-hand side */
 #define GOSS_HIP_JACOBIAN_FD_FP32 1 /* forward differences of the FP32 right-hand side */
+/* The model's analytic Jacobian: ODE::compute_jacobian (ODE.h:71) as overridden
+ * by a plugin generated with generate_jacobian (codegeneration.py:186, 622-641).
+ * The generated entries are evaluated and stored over the Jacobian's structure.
+ * Every implicit solver. */
+#define GOSS_HIP_JACOBIAN_ANALYTIC 2
 
 /* Sizes and names of a generated model variant
  * (ParameterizedODE.h:56-156 getters). */
@@ -143,6 +150,10 @@ int goss_hip_model_linearized_eval(const char* variant, const double* params, co
                                    const double* times, uint64_t n, int only_linear, double* linear, double* rhs);
 int goss_hip_model_compute_jacobian(const char* variant, const double* params, const double* states,
                                     const double* times, uint64_t n, double* jac);
+/* ODE::compute_jacobian (ODE.h:71) of a plugin generated with generate_jacobian
+ * (codegeneration.py:622-641): the generated analytic Jacobian, [n][S][S]. */
+int goss_hip_model_compute_analytic_jacobian(const char* variant, const double* params, const double* states,
+                                             const double* times, uint64_t n, double* jac);
 /* The model's named intermediates (the values eval_monitored can select),
  * and ParameterizedODE::eval_monitored (ParameterizedODE.h:90-105) of the
  * intermediates listed in `monitored` (indices into those names; at most 64)

@@ -385,7 +385,8 @@ class ImplicitODESolver : public ODESolver {
   double max_relative_previous_residual = 0.01;
   bool always_recompute_jacobian = false;
   // How the Jacobian is formed: GOSS_HIP_JACOBIAN_FD, the reference's forward differences
-  // (ODE.cpp:47-67), or GOSS_HIP_JACOBIAN_FD_FP32 (no counterpart in the reference).
+  // (ODE.cpp:47-67), GOSS_HIP_JACOBIAN_ANALYTIC, the model's generated analytic Jacobian (a plugin
+  // generated with generate_jacobian), or GOSS_HIP_JACOBIAN_FD_FP32 (no counterpart in the reference).
   int jacobian = GOSS_HIP_JACOBIAN_FD;
   // Number of Jacobian evaluations so far (ImplicitODESolver.h:65-68).
   int num_jac_comp() const { return int(counter(1)); }

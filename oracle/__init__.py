@@ -374,7 +374,9 @@ class OracleSystem(_SystemBase):
 class RefSystem(_SystemBase):
     """The reference's ODESystemSolver compiled from its own sources."""
 
-    def __init__(self, variant, solver, num_nodes, solver_params=None, model_params=None):
+    def __init__(self, variant, solver, num_nodes, solver_params=None, model_params=None, analytic_jacobian=False):
+        """analytic_jacobian: the generated plugin overrides ODE::compute_jacobian with its analytic Jacobian,
+        as gotran's plugin does with generate_jacobian (codegeneration.py:186, 622-641)."""
         if variant in REFERENCE_MODELS:
             register_reference_models()  # the Variant metadata comes from the oracle's registry
         super().__init__(variant, solver, num_nodes)
@@ -382,7 +384,8 @@ class RefSystem(_SystemBase):
         n, names, vals = _names(solver_params or {})
         m, mnames, mvals = _names(model_params or {})
         self._h = ctypes.c_void_p()
-        rc = self._L.ref_system_create(variant.encode(), solver.encode(), self.num_nodes, n, names, _ptr(vals),
+        name = variant + (":analytic" if analytic_jacobian else "")
+        rc = self._L.ref_system_create(name.encode(), solver.encode(), self.num_nodes, n, names, _ptr(vals),
                                        m, mnames, _ptr(mvals), ctypes.byref(self._h))
         if rc != 0:
             raise RuntimeError(self._L.ref_last_error().decode())
@@ -411,12 +414,13 @@ class RefSystem(_SystemBase):
 class RefModel:
     """A reference-side goss::ParameterizedODE instance (generated plugin)."""
 
-    def __init__(self, variant: str):
+    def __init__(self, variant: str, analytic_jacobian: bool = False):
         if variant in REFERENCE_MODELS:
             register_reference_models()
         self._L = ref_lib(REFPIN_SO if variant in REFERENCE_MODELS else REF_SO)
         self._h = ctypes.c_void_p()
-        if self._L.ref_model_create(variant.encode(), ctypes.byref(self._h)) != 0:
+        name = variant + (":analytic" if analytic_jacobian else "")
+        if self._L.ref_model_create(name.encode(), ctypes.byref(self._h)) != 0:
             raise RuntimeError(self._L.ref_last_error().decode())
         self.variant = Variant(variant)
 

@@ -67,6 +67,9 @@ struct VariantOps {
   // right-hand side is called through these entries with the full parameter
   // vector; the generated entries above are then unused.
   const oracle_external_model* ext = nullptr;
+  // The generated analytic Jacobian stands for ODE::compute_jacobian, as in a plugin generated with
+  // generate_jacobian (codegeneration.py:186, 622-641).
+  bool analytic_jacobian = false;
 
   // The mass matrix's diagonal entry of state i, as the reference reads it
   // (_ode->differential_states()[i], an unsigned char).
@@ -208,12 +211,13 @@ void fd_jacobian(const VariantOps& m, double* y, double t, const double* p, cons
 }
 
 // The model's ODE::compute_jacobian: a model class that overrides it (the
-// reference's hand-written test models) is called; otherwise the forward
-// differences above.
+// reference's hand-written test models, or a generated model with its
+// analytic Jacobian) is called; otherwise the forward differences above.
 void ode_jacobian(const VariantOps& m, double* y, double t, const double* p, const double* c, double* jac,
                   double* f1, double* f2)
 {
   if (m.ext && m.ext->compute_jacobian) m.ext->compute_jacobian(m.ext->ctx, y, t, p, jac);
+  else if (m.analytic_jacobian) m.jacobian(y, t, p, c, jac);
   else fd_jacobian(m, y, t, p, c, jac, f1, f2);
 }
 
@@ -1171,6 +1175,8 @@ void rkf32_forward(const VariantOps& m, const SolverParams& sp, ImplicitState& s
 
 struct oracle_system {
   const VariantOps* m;
+  const VariantOps* base;  // the registered model; m points here or at `analytic`
+  VariantOps analytic;     // the model with its analytic Jacobian (solver parameter jacobian = 2)
   int kind;
   SolverParams sp;
   unsigned N;
@@ -1270,6 +1276,19 @@ int set_param(oracle_system* s, const std::string& name, double v)
   if (name == "atol") { s->sp.atol = v; return 0; }
   if (name == "rtol") { s->sp.rtol = v; return 0; }
   if (name == "theta") { s->sp.theta = v; return 0; }
+  if (name == "jacobian") {
+    // 0: forward differences (ODE.cpp:47-67); 2: the model's analytic Jacobian.
+    if (v == 0) { s->m = s->base; return 0; }
+    if (v == 2) {
+      if (s->base->ext) { g_err = "a model registered at run time brings its own compute_jacobian"; return -1; }
+      s->analytic = *s->base;
+      s->analytic.analytic_jacobian = true;
+      s->m = &s->analytic;
+      return 0;
+    }
+    g_err = "jacobian must be 0 (forward differences) or 2 (analytic)";
+    return -1;
+  }
   if (name == "num_refinements_without_always_recomputing_jacobian") {
     s->sp.num_refinements_without_always_recomputing_jacobian = int(v);
     return 0;
@@ -1419,6 +1438,7 @@ int oracle_system_create(const char* variant, const char* solver, unsigned num_n
   }
   auto* s = new oracle_system;
   s->m = &kVariants[v];
+  s->base = s->m;
   s->kind = kind;
   if (kind == ESDIRK23A) s->sp.min_dt = 0.001;  // ESDIRK23a.h:38
   s->N = num_nodes;

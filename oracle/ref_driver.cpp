@@ -29,21 +29,34 @@ namespace {
 
 using namespace goss;
 
-std::shared_ptr<ParameterizedODE> make_model(const std::string& v)
+template <class T>
+std::shared_ptr<ParameterizedODE> made(const bool analytic)
 {
-  if (v == "tp06_m") return std::make_shared<Tp06_m>();
-  if (v == "tp06_m_fp3") return std::make_shared<Tp06_m_fp3>();
-  if (v == "tp06_m_fp6") return std::make_shared<Tp06_m_fp6>();
-  if (v == "tp06_epi") return std::make_shared<Tp06_epi>();
-  if (v == "fitzhughnagumo") return std::make_shared<Fitzhughnagumo>();
-  if (v == "lorentz") return std::make_shared<Lorentz>();
-  if (v == "fhn") return std::make_shared<Fhn>();
-  if (v == "beeler_reuter_1977") return std::make_shared<Beeler_reuter_1977>();
-  if (v == "oscilator") return std::make_shared<Oscilator>();
-  if (v == "linear_decay") return std::make_shared<Linear_decay>();
-  if (v == "van_der_pol_dae") return std::make_shared<Van_der_pol_dae>();
-  if (v == "markov_three_state") return std::make_shared<Markov_three_state>();
-  throw std::runtime_error("unknown model variant " + v);
+  auto m = std::make_shared<T>();
+  m->analytic_jacobian = analytic;
+  return m;
+}
+
+// "<variant>" or "<variant>:analytic", the plugin with its analytic compute_jacobian (a plugin generated
+// with generate_jacobian, codegeneration.py:186, 622-641).
+std::shared_ptr<ParameterizedODE> make_model(const std::string& name)
+{
+  const std::string suffix = ":analytic";
+  const bool an = name.size() > suffix.size() && name.compare(name.size() - suffix.size(), suffix.size(), suffix) == 0;
+  const std::string v = an ? name.substr(0, name.size() - suffix.size()) : name;
+  if (v == "tp06_m") return made<Tp06_m>(an);
+  if (v == "tp06_m_fp3") return made<Tp06_m_fp3>(an);
+  if (v == "tp06_m_fp6") return made<Tp06_m_fp6>(an);
+  if (v == "tp06_epi") return made<Tp06_epi>(an);
+  if (v == "fitzhughnagumo") return made<Fitzhughnagumo>(an);
+  if (v == "lorentz") return made<Lorentz>(an);
+  if (v == "fhn") return made<Fhn>(an);
+  if (v == "beeler_reuter_1977") return made<Beeler_reuter_1977>(an);
+  if (v == "oscilator") return made<Oscilator>(an);
+  if (v == "linear_decay") return made<Linear_decay>(an);
+  if (v == "van_der_pol_dae") return made<Van_der_pol_dae>(an);
+  if (v == "markov_three_state") return made<Markov_three_state>(an);
+  throw std::runtime_error("unknown model variant " + name);
 }
 
 }  // namespace

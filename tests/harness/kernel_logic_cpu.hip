@@ -243,6 +243,24 @@ extern "C" int kernel_logic_run_fp32_jacobian(const char* variant, int solver, u
   return -2;
 }
 
+// The implicit solvers with the model's analytic Jacobian (compute_jacobian_structured, as the kernels
+// evaluate it), reference defaults of the tunables.
+extern "C" int kernel_logic_run_analytic_jacobian(const char* variant, int solver, unsigned N, const double* y0,
+                                                  const double* t, unsigned long n, const double* params,
+                                                  double* out_states, long* counters)
+{
+  const double min_dt = (solver == kESDIRK23a) ? 1e-3 : 1e-4;
+  ImplicitParams ip{-1.0, 1.0, 0.1, 1e-12, 0.01, min_dt, 30, 0, 2, 0.5, 1.0e-5, 1.0e-8};
+  const std::string v(variant);
+  constexpr int A = kJacobianAnalytic;
+  if (v == "tp06_m") return run<M_tp06_m, A>(solver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "tp06_epi") return run<M_tp06_epi, A>(solver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "beeler_reuter_1977")
+    return run<M_beeler_reuter_1977, A>(solver, N, y0, t, n, params, ip, out_states, counters);
+  if (v == "fhn") return run<M_fhn, A>(solver, N, y0, t, n, params, ip, out_states, counters);
+  return -2;
+}
+
 extern "C" int kernel_logic_run_theta(const char* variant, unsigned N, const double* y0, const double* t,
                                       unsigned long n, const double* params, double ldt_0, double theta,
                                       double* out_states, long* counters)
