@@ -20,6 +20,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/goss_hip.h"
@@ -204,6 +205,8 @@ struct goss_hip_system {
   hipEvent_t sol_ev[4] = {};  // solve_async: rows written into staging buffer b (0, 1), copied out of b (2, 3)
   double* sol_stage = nullptr;  // solve_async: two device staging buffers
   size_t sol_stage_cap = 0;
+  double* sol_bounce = nullptr;  // solve into pageable memory: two pinned host bounce buffers
+  size_t sol_bounce_cap = 0;
 
   void ensure_aux()
   {
@@ -227,6 +230,20 @@ struct goss_hip_system {
       sol_stage_cap = n;
     }
     return sol_stage;
+  }
+
+  double* solve_bounce(size_t n)
+  {
+    if (n > sol_bounce_cap) {
+      if (sol_bounce) {
+        if (aux) hip_check(hipStreamSynchronize(aux), "hipStreamSynchronize");
+        hip_check(hipHostFree(sol_bounce), "hipHostFree");
+      }
+      sol_bounce = nullptr;
+      hip_check(hipHostMalloc(&sol_bounce, n * sizeof(double), hipHostMallocDefault), "hipHostMalloc(bounce)");
+      sol_bounce_cap = n;
+    }
+    return sol_bounce;
   }
 
   size_t elem() const { return prec == kFP64 ? sizeof(double) : sizeof(float); }
@@ -466,6 +483,7 @@ struct goss_hip_system {
     if (d_dofs) (void)hipFree(d_dofs);
     if (aux) (void)hipStreamSynchronize(aux);
     if (sol_stage) (void)hipFree(sol_stage);
+    if (sol_bounce) (void)hipHostFree(sol_bounce);
     for (hipEvent_t e : adj_ev)
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : sol_ev)
@@ -1142,6 +1160,65 @@ int goss_hip_adjoint_backward(goss_hip_system* s, const double* t, uint64_t n, u
 
 namespace {
 
+// Copies bytes with up to 8 host threads (slices of at least 4 MiB).
+void parallel_copy(void* dst, const void* src, size_t bytes)
+{
+  const size_t slice = size_t(4) << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned T = unsigned(std::min<size_t>({size_t(8), size_t(hw), (bytes + slice - 1) / slice}));
+  if (T <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned k = 1; k < T; ++k) {
+    const size_t a = bytes * k / T, b = bytes * (k + 1) / T;
+    th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a); });
+  }
+  std::memcpy(dst, src, bytes / T);
+  for (auto& x : th) x.join();
+}
+
+// solve into pageable host memory.  Chunks of rows (64 MiB) alternate between two device staging
+// buffers and two pinned bounce buffers: while the host copies chunk i from its bounce buffer into
+// the destination, chunk i + 1 steps and its rows travel to the other bounce buffer.  Chunk i + 2 is
+// enqueued once chunk i is out of both of its buffers.
+void solve_pageable(goss_hip_system* s, double* field_states, const double* t, uint64_t n, int tangled)
+{
+  const size_t row = size_t(s->N) * s->v->NFS;
+  std::vector<double2> st(n - 1);
+  for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
+  s->check_steps(st.data(), st.size());
+  const double2* d = s->upload_steps(st.data(), st.size());
+  const size_t chunk = std::max<size_t>(1, (size_t(64) << 20) / (row * sizeof(double)));
+  const size_t half = std::min(chunk, st.size()) * row;
+  const size_t nch = (st.size() + chunk - 1) / chunk;
+  s->ensure_aux();
+  double* stage = s->solve_stage(2 * half);
+  double* bounce = s->solve_bounce(2 * half);
+  auto enqueue = [&](const size_t i) {
+    const int b = int(i & 1);
+    const size_t off = i * chunk, cnt = std::min(chunk, st.size() - off);
+    s->launch_steps(d + off, cnt, stage + size_t(b) * half, tangled);
+    hip_check(hipEventRecord(s->sol_ev[b], s->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(s->aux, s->sol_ev[b], 0), "hipStreamWaitEvent");
+    hip_check(hipMemcpyAsync(bounce + size_t(b) * half, stage + size_t(b) * half, cnt * row * sizeof(double),
+                             hipMemcpyDeviceToHost, s->aux),
+              "hipMemcpyAsync(D2H)");
+    hip_check(hipEventRecord(s->sol_ev[2 + b], s->aux), "hipEventRecord");
+  };
+  enqueue(0);
+  if (nch > 1) enqueue(1);
+  for (size_t i = 0; i < nch; ++i) {
+    const int b = int(i & 1);
+    const size_t off = i * chunk, cnt = std::min(chunk, st.size() - off);
+    hip_check(hipEventSynchronize(s->sol_ev[2 + b]), "hipEventSynchronize");
+    parallel_copy(field_states + (off + 1) * row, bounce + size_t(b) * half, cnt * row * sizeof(double));
+    if (i + 2 < nch) enqueue(i + 2);
+  }
+  hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+}
+
 // Where solve writes its rows: pageable host memory (staged and copied synchronously), pinned host
 // memory or device memory (the pipelined path of solve_async).
 enum class RowsDst { kPageable, kPinned, kDevice };
@@ -1214,20 +1291,7 @@ int goss_hip_system_solve(goss_hip_system* s, double* field_states, const double
       hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
       return;
     }
-    // Pageable host memory: device rows are produced in chunks bounded by a staging budget
-    // (~256 MiB) and copied out after each chunk.
-    const size_t budget_rows = std::max<size_t>(1, (size_t(256) << 20) / std::max<size_t>(row * sizeof(double), 1));
-    std::vector<double2> st(n - 1);
-    for (uint64_t i = 0; i + 1 < n; ++i) st[i] = make_double2(t[i], t[i + 1] - t[i]);
-    s->check_steps(st.data(), st.size());
-    const double2* d = s->upload_steps(st.data(), st.size());
-    for (size_t off = 0; off < st.size(); off += budget_rows) {
-      const size_t cnt = std::min(budget_rows, st.size() - off);
-      double* dev = s->stage(cnt * row);
-      s->launch_steps(d + off, cnt, row ? dev : nullptr, tangled);
-      if (row) s->copy_out(field_states + (off + 1) * row, dev, cnt * row);
-    }
-    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    solve_pageable(s, field_states, t, n, tangled);
   });
 }
 
