@@ -199,7 +199,10 @@ int goss_hip_system_forward_times(goss_hip_system* sys, const double* t, uint64_
 int goss_hip_system_forward_times_noalloc(goss_hip_system* sys, const double* t, uint64_t n);
 /* ODESystemSolver::solve(field_states, t, n, tangled) (ODESystemSolver.cpp:81-99):
  * rows 1..n-1 of field_states[n][...] receive the field states after each
- * step; row 0 is left to the caller (python/goss/systemsolver.py:130-132). */
+ * step; row 0 is left to the caller (python/goss/systemsolver.py:130-132).
+ * Rows travel in chunks of 64 MiB: into pageable host memory through two
+ * pinned bounce buffers of the system (held until it is destroyed), copied
+ * out by up to 8 host threads while the next chunk steps. */
 int goss_hip_system_solve(goss_hip_system* sys, double* field_states, const double* t, uint64_t n, int tangled);
 /* ODESystemSolver::solve without waiting.  The reference's solve is
  * synchronous and writes to host memory; here field_states is pinned host
