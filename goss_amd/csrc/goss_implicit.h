@@ -269,7 +269,9 @@ struct Implicit {
     return finite;
   }
 
-  // The Jacobian at y into J, as the accessor's mode says (f1 is overwritten).
+  // The Jacobian at y into J, as the accessor's mode says (f1 is overwritten).  Inlined: every
+  // solver's kernel evaluates an analytic Jacobian under its own register budget (a function shared
+  // by kernels of different budgets is compiled for the smallest).
   // In the FP32 mode a node whose FP32 right-hand side is not finite at any of
   // the difference points gets the FP64 forward-difference Jacobian instead:
   // FP32 resolves removable singularities of a model only to its own width
@@ -277,7 +279,8 @@ struct Implicit {
   // (V - 15) / (exp(c (V - 15)) - 1), is 0/0 in FP32 for |V - 15 mV| below
   // about 1e-6 mV), where FP64 is finite.
   template <class A>
-  GOSS_HD static void jacobian(const VecView& y, const double t, const A& acc, const VecView& f1, const VecView& J)
+  GOSS_HD static GOSS_INLINE void jacobian(const VecView& y, const double t, const A& acc, const VecView& f1,
+                                           const VecView& J)
   {
     if constexpr (JacobianModeOf<A>::value == kJacobianFDFP32) {
       if (!fd_jacobian_f32(y, t, acc, f1, J)) fd_jacobian(y, t, acc, f1, J);
@@ -1130,9 +1133,24 @@ GOSS_HD GOSS_INLINE NodeViews node_views(double* y, const ImplicitBuffers& B, in
 #ifndef GOSS_IMPLICIT_WAVES
 #define GOSS_IMPLICIT_WAVES 4
 #endif
+// With the analytic Jacobian, the solvers that form a Jacobian at most Newton iterations
+// (ImplicitEuler, ThetaSolver, BasicImplicitEuler) run 2 waves of 256 VGPRs: the evaluation's
+// temporaries spill less.  TP06 epi, 262,144 nodes, ImplicitEuler 1.04e8 node-steps/s with 4 waves,
+// 9.2e7 with 3, 1.16e8 with 2.  ESDIRK23a reuses its Jacobian and keeps 4 (3.43e7; 2.47e7 with 3,
+// 3.03e7 with 2).
+#ifndef GOSS_IMPLICIT_WAVES_ANALYTIC_NEWTON
+#define GOSS_IMPLICIT_WAVES_ANALYTIC_NEWTON 2
+#endif
+constexpr int implicit_waves(const int solver, const int jac)
+{
+  return (jac == kJacobianAnalytic && (solver == kImplicitEuler || solver == kThetaSolver ||
+                                       solver == kBasicImplicitEuler))
+             ? GOSS_IMPLICIT_WAVES_ANALYTIC_NEWTON
+             : GOSS_IMPLICIT_WAVES;
+}
 
 template <class M, int SOLVER, int JAC>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GOSS_IMPLICIT_WAVES)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(implicit_waves(SOLVER, JAC))))
 implicit_kernel(double* __restrict__ y, const double* __restrict__ fp, const double2* __restrict__ steps,
                 const int nouter, const SharedParamsOf<M, double> sp, const SharedParamsOf<M, float> spf,
                 const ImplicitParams ip, const ImplicitBuffers B, const unsigned N, const unsigned ld,
