@@ -6,7 +6,9 @@ CPU: the model metadata and solver objects work without a GPU, and building
 a system without one fails loudly (no CPU fallback).  GPU: a TP06 GRL1 run
 through the C++ classes (set_parameter on the model before construction,
 tangled field parameters, forward, solve, states) equals the same run through
-the Python binding bit for bit, and the FP64 oracle to 1e-10 relative.
+the Python binding bit for bit, and the FP64 oracle to 1e-10 relative; solve_async into pinned memory
+(goss_hip_host_alloc) equals solve bit for bit, and ImplicitEuler with the analytic Jacobian agrees
+with the forward differences to the Newton tolerance.
 """
 import os
 import subprocess
@@ -60,6 +62,8 @@ def test_cpp_run_equals_python_binding_and_oracle(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     # Tunables and model parameters changed between calls apply to the next call.
     assert "tunables and parameters changed between calls apply" in r.stdout
+    assert "solve_async into pinned memory equals solve" in r.stdout
+    assert "analytic Jacobian: max |dV| vs forward differences" in r.stdout
     raw = np.fromfile(out, dtype=np.float64)
     N, S, F = (int(x) for x in raw[:3])
     NFP = 3
