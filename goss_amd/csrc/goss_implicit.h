@@ -1137,7 +1137,7 @@ GOSS_HD GOSS_INLINE NodeViews node_views(double* y, const ImplicitBuffers& B, in
 // (ImplicitEuler, ThetaSolver, BasicImplicitEuler) run 2 waves of 256 VGPRs: the evaluation's
 // temporaries spill less.  TP06 epi, 262,144 nodes, node-steps/s with 4 and 2 waves: ImplicitEuler
 // 1.24e8 / 1.33e8, ThetaSolver 1.07e8 / 1.11e8, BasicImplicitEuler 2.24e7 / 2.98e7.  ESDIRK23a reuses
-// its Jacobian and keeps 4.
+// its Jacobian and keeps 4 (3.83e7 / 3.46e7).
 #ifndef GOSS_IMPLICIT_WAVES_ANALYTIC_NEWTON
 #define GOSS_IMPLICIT_WAVES_ANALYTIC_NEWTON 2
 #endif
